@@ -1,0 +1,276 @@
+/*
+ * openvort_gpu.h -- C ABI of the MI355X vortex-filament engine.
+ *
+ * The engine evaluates the superfluid velocity of a tangle of quantised
+ * vortex filaments (Biot-Savart over straight segments, periodic image
+ * shells, local-induction term, mutual friction) and advances it with the
+ * reference's RK4 scheme, all on one GPU.  Tangle state uses the
+ * reference's slot layout so that the drop-in shims in openvort_dropin.h
+ * can pass struct tangle_state arrays straight through:
+ *
+ *   vnodes       double[3*n]  positions, x,y,z per slot
+ *                             (reference: struct vec3, include/vec3_math.h:16-18)
+ *   connections  int[2*n]     forward, reverse per slot, -1 = none
+ *                             (reference: struct neighbour_t, include/tangle.h:39-42)
+ *   status       int[2*n]     status, pin_wall per slot; status -1 = EMPTY,
+ *                             0 = FREE, 1 = PINNED, 2 = PINNED_SLIP
+ *                             (reference: struct node_status, include/boundaries.h:50-53)
+ *
+ * Every entry point returns OVG_OK (0) or a nonzero OVG_E* code; the text
+ * of the last error on the calling thread is returned by ovg_last_error().
+ * Nothing here prints, exits or falls back to the CPU: when no GPU or no
+ * gfx950 code object is available, ovg_create fails with OVG_EDEVICE.
+ *
+ * Threading: one host thread drives one context at a time.  Contexts are
+ * independent; each owns one HIP stream (default priority) and its device
+ * buffers.
+ */
+#ifndef OPENVORT_GPU_H
+#define OPENVORT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define OVG_OK           0
+#define OVG_EINVAL       1   /* bad argument, shape or state */
+#define OVG_ENOMEM       2   /* device or host allocation failed / cap exceeded */
+#define OVG_EDEVICE      3   /* HIP runtime error, no device, missing code object */
+#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. mirror walls) */
+#define OVG_ELIMIT       5   /* an internal fixed-size limit was exceeded */
+
+/* ---- node status and wall kinds (reference enums) -------------------- */
+#define OVG_EMPTY       (-1)  /* include/boundaries.h:50-53 */
+#define OVG_FREE         0
+#define OVG_PINNED       1
+#define OVG_PINNED_SLIP  2
+
+#define OVG_WALL_OPEN      0  /* include/boundaries.h:18-22 */
+#define OVG_WALL_PERIODIC  1
+#define OVG_WALL_MIRROR    2
+
+/* external velocity kinds, include/external_velocity.h:27-31 */
+#define OVG_FLOW_NONE        0
+#define OVG_FLOW_SIMPLE      1
+#define OVG_FLOW_DECREASING  2
+
+/* Biot-Savart evaluation method */
+#define OVG_METHOD_DIRECT  0  /* all segments, every image (src/tangle.c:652-687) */
+#define OVG_METHOD_BH      1  /* Barnes-Hut octree (src/octree.c:296-356)        */
+
+/* arithmetic of the pair interactions */
+#define OVG_PREC_FP64   0     /* every pair in FP64, reference operation order */
+#define OVG_PREC_MIXED  1     /* far-field pairs in FP32, per-point sums in FP64 */
+
+/* Barnes-Hut traversal organisation */
+#define OVG_TRAVERSE_POINT  0 /* one lane per (point, image), the reference's
+                                 per-point recursion made stackless             */
+#define OVG_TRAVERSE_GROUP  1 /* one wavefront per 64 Morton-adjacent points and
+                                 one image; same per-point acceptance test       */
+
+/* External (prescribed) flow, include/external_velocity.h:11-19. */
+typedef struct ovg_flow {
+    int32_t type;                 /* OVG_FLOW_*                               */
+    int32_t pad_;
+    double  strength;             /* cm/s                                     */
+    double  time_of_linear_decrease;
+    double  starting_strength;
+    double  direction[3];         /* normalised on every use, as the reference */
+} ovg_flow;
+
+/*
+ * Physics, geometry and method of one tangle.  Field names follow the
+ * reference globals of include/vortex_constants.h:11-60.
+ */
+typedef struct ovg_params {
+    double  kappa;                /* KAPPA, cm^2/s                            */
+    double  vortex_width;         /* VORTEX_WIDTH (a0), cm                    */
+    int32_t use_mutual_friction;
+    int32_t pad0_;
+    double  alpha, alpha_p;       /* mutual friction coefficients             */
+    ovg_flow vn, vs, vb;          /* normal, superfluid, boundary flows       */
+
+    double  box_lo[3];            /* domain_box.bottom_left_back              */
+    double  box_hi[3];            /* domain_box.top_right_front               */
+    int32_t wall[6];              /* LEFT RIGHT BACK FRONT DOWN UP, OVG_WALL_* */
+    int32_t n_images;             /* 0 (open) or 6/18/26 periodic shifts      */
+    int32_t images[26][3];        /* integer box shifts, reference order      */
+
+    int32_t method;               /* OVG_METHOD_*                             */
+    int32_t precision;            /* OVG_PREC_*                               */
+    double  theta;                /* BH_resolution (opening angle)            */
+    int32_t leaf_size;            /* points per Barnes-Hut bucket, >= 1; 1 is the
+                                     reference's split-until-one tree           */
+    int32_t traverse;             /* OVG_TRAVERSE_*                           */
+
+    double  rec_dist;             /* reconnection distance, cm                */
+    double  rec_angle;            /* reconnection_angle_cutoff, radians       */
+    double  freq_to_cutoff;       /* smoothing: fraction of modes removed     */
+    int32_t rk4_classic;          /* 0: reference stage 4 at x+dt/2*k3
+                                     (src/vortex_dynamics.c:90); 1: x+dt*k3    */
+    int32_t pad1_;
+    double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
+                                     than this (cm) stay FP64; 0 = none        */
+} ovg_params;
+
+/* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a
+ * periodic cube [0,D]^3 with the 26 periodic images of
+ * src/boundary_images.c:45-72.  vn/vs/vb are set to no flow. */
+void ovg_params_default(ovg_params *p, double D);
+
+/* Load the periodic image table with n = 6, 18 or 26 entries
+ * (src/boundary_images.c:13-72); n = 0 clears it. */
+int ovg_params_set_periodic_images(ovg_params *p, int n);
+
+const char *ovg_last_error(void);
+const char *ovg_version(void);
+
+/* sizeof(ovg_params) and sizeof(ovg_stats) as compiled into the library,
+ * for bindings that restate the structs (ctypes, FFI). */
+size_t ovg_params_sizeof(void);
+size_t ovg_stats_sizeof(void);
+
+/* ---- context --------------------------------------------------------- */
+typedef struct ovg_ctx ovg_ctx;
+
+/* Create a context on HIP device `device`. */
+int ovg_create(ovg_ctx **out, int device);
+int ovg_destroy(ovg_ctx *ctx);
+
+int ovg_set_params(ovg_ctx *ctx, const ovg_params *p);
+int ovg_get_params(const ovg_ctx *ctx, ovg_params *p);
+
+/*
+ * Upload a tangle of n slots.  Replaces positions, connections and status;
+ * velocities, tangents and normals become zero.  Re-upload whenever the
+ * host changed capacity or topology (the reference's expand_tangle
+ * reallocates, src/tangle.c:68-101).
+ */
+int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes,
+               const int *connections, const int *status);
+
+/* Upload only new positions (same n and topology). */
+int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes);
+
+/* Upload only connections (same n), e.g. after a host-side topology edit. */
+int ovg_upload_connections(ovg_ctx *ctx, const int *connections, const int *status);
+
+/* Copy device state back; any pointer may be NULL.  All are double[3*n]
+ * or int[2*n] in the layouts above. */
+int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *vs,
+                 double *tangents, double *normals);
+int ovg_download_connections(ovg_ctx *ctx, int *connections, int *status);
+
+int ovg_num_slots(const ovg_ctx *ctx);
+
+/* ---- hot path (the reference prototypes these replace are cited) ----- */
+
+/* update_tangents_normals, src/tangle.c:509-516 */
+int ovg_update_tangents_normals(ovg_ctx *ctx);
+
+/* update_velocities, src/tangle.c:733-749 (builds the tree when
+ * method == OVG_METHOD_BH). */
+int ovg_update_velocities(ovg_ctx *ctx, double t);
+
+/* update_tangle, include/tangle.h:114-118 */
+int ovg_update_tangle(ovg_ctx *ctx, double t);
+
+/* rk4_step, src/vortex_dynamics.c:50-119.  Uses the current vels as k1,
+ * exactly as the reference (the caller runs update_tangle first). */
+int ovg_rk4_step(ovg_ctx *ctx, double t, double dt);
+
+/* enforce_periodic_boundaries, src/tangle.c:793-814 */
+int ovg_enforce_periodic(ovg_ctx *ctx);
+
+/*
+ * Reconnection candidates, the predicate of src/vortex_dynamics.c:197-222
+ * before any reconnection of the pass: pairs (k,l), k<l, both FREE,
+ * l not a neighbour of k, |seg_pwrap(x_k,x_l)| <= rec_dist.  Written as
+ * k,l int pairs sorted by (k,l).  *n_pairs receives the total count even
+ * when it exceeds max_pairs (then only max_pairs are written and
+ * OVG_ELIMIT is returned).
+ */
+int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pairs);
+
+/*
+ * reconnect, src/vortex_dynamics.c:186-288: the candidate scan on the GPU,
+ * then the reference's greedy pass in (k,l) order with the angle test,
+ * the approach test on direct-sum velocities and the length test.  The
+ * updated connections stay on the device.
+ */
+int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections);
+
+/* smooth, src/vortex_smooth.c:459-462 (low_pass_filter per loop and
+ * coordinate, src/vortex_smooth.c:134-175,183-453). */
+int ovg_smooth(ovg_ctx *ctx);
+
+/*
+ * Direct-sum (tree == NULL) velocity at selected slots, the single-point
+ * update_velocity of src/tangle.c:600-726 as called by reconnect
+ * (src/vortex_dynamics.c:250-251).  vs and vels are double[3*m] (either may
+ * be NULL); tangents and normals are read from the device state.
+ */
+int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t,
+                    double *vs, double *vels);
+
+/* ---- diagnostics ----------------------------------------------------- */
+
+/* Line length sum |seg_pwrap(x_k, x_fwd(k))| over FREE slots with a
+ * forward neighbour, and the number of live points. */
+int ovg_line_length(ovg_ctx *ctx, double *length, int *n_points);
+
+/*
+ * Counters of the last ovg_update_velocities: pair interactions evaluated
+ * (segments and monopoles, all images), tree nodes, and the device time in
+ * milliseconds of the tree build and of the Biot-Savart kernels (HIP
+ * events on the context's stream).
+ */
+typedef struct ovg_stats {
+    double   interactions;        /* pair evaluations, FP64 + FP32           */
+    double   interactions_fp32;   /* the FP32 part of the above              */
+    int64_t  tree_nodes;
+    int32_t  tree_depth;
+    int32_t  n_points;
+    double   ms_tree;             /* tree build                              */
+    double   ms_bs;               /* Biot-Savart kernels                     */
+    double   ms_local;            /* stencil, LIA, friction, RK4 streams     */
+    int32_t  bs_launches;         /* Biot-Savart kernel launches measured    */
+    int32_t  pad_;
+} ovg_stats;
+
+int ovg_get_stats(ovg_ctx *ctx, ovg_stats *s);
+/* Enable/disable the per-call HIP-event timing (off by default). */
+int ovg_set_timing(ovg_ctx *ctx, int on);
+
+/* Wait for all work queued on the context's stream. */
+int ovg_synchronize(ovg_ctx *ctx);
+
+/* ---- seeded tangle generator (host code, no GPU) --------------------- */
+/*
+ * Rings with radius ~ U[r_min, r_max], centre ~ U[0,D)^3 and normal
+ * uniform on the sphere, n = floor(2 pi R / dxi) points each (the point
+ * count of add_loop, src/vortex_injection.c:52, with
+ * dl_min + dl_max = 2 dxi), positions wrapped into [0,D)^3.  Ring k uses
+ * the SplitMix64 stream of (seed, k).  Slots are filled contiguously from
+ * 0; the remaining capacity (n_cap) is EMPTY.  Returns the number of live
+ * points, or -1 when it would exceed n_cap.  Any output may be NULL to
+ * only count.
+ */
+int ovg_generate_rings(uint64_t seed, int n_rings, double D, double r_min,
+                       double r_max, double dxi, int n_cap, double *vnodes,
+                       int *connections, int *status);
+
+/* Number of rings needed to reach at least `target_points` with the same
+ * stream as ovg_generate_rings. */
+int ovg_rings_for_points(uint64_t seed, int target_points, double D,
+                         double r_min, double r_max, double dxi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENVORT_GPU_H */

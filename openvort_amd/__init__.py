@@ -1,0 +1,317 @@
+"""Python bindings (ctypes) of the MI355X vortex-filament engine.
+
+The product is the C ABI of ``include/openvort_gpu.h`` implemented by the
+in-tree ``openvort_amd/libopenvort_gpu.so`` (hand-written HIP for gfx950).
+This module only restates its structs and wraps its entry points with numpy
+arrays for tests and benchmarks.  Importing it fails loudly when the
+library has not been built; creating an :class:`Engine` fails loudly when
+no gfx950 GPU is present.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopenvort_gpu.so")
+DROPIN_PATH = os.path.join(_HERE, "libopenvort_dropin.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C openvort_amd` "
+        "(or __graft_entry__.build())")
+
+_lib = C.CDLL(LIB_PATH)
+
+# status codes (include/openvort_gpu.h)
+OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ELIMIT = 0, 1, 2, 3, 4, 5
+EMPTY, FREE, PINNED, PINNED_SLIP = -1, 0, 1, 2
+WALL_OPEN, WALL_PERIODIC, WALL_MIRROR = 0, 1, 2
+FLOW_NONE, FLOW_SIMPLE, FLOW_DECREASING = 0, 1, 2
+METHOD_DIRECT, METHOD_BH = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+TRAVERSE_POINT, TRAVERSE_GROUP = 0, 1
+
+
+class Flow(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad_", C.c_int32), ("strength", C.c_double),
+                ("time_of_linear_decrease", C.c_double),
+                ("starting_strength", C.c_double), ("direction", C.c_double * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("kappa", C.c_double), ("vortex_width", C.c_double),
+        ("use_mutual_friction", C.c_int32), ("pad0_", C.c_int32),
+        ("alpha", C.c_double), ("alpha_p", C.c_double),
+        ("vn", Flow), ("vs", Flow), ("vb", Flow),
+        ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
+        ("wall", C.c_int32 * 6), ("n_images", C.c_int32),
+        ("images", (C.c_int32 * 3) * 26),
+        ("method", C.c_int32), ("precision", C.c_int32), ("theta", C.c_double),
+        ("leaf_size", C.c_int32), ("traverse", C.c_int32),
+        ("rec_dist", C.c_double), ("rec_angle", C.c_double),
+        ("freq_to_cutoff", C.c_double), ("rk4_classic", C.c_int32),
+        ("pad1_", C.c_int32), ("mixed_near", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("interactions", C.c_double), ("interactions_fp32", C.c_double),
+                ("tree_nodes", C.c_int64), ("tree_depth", C.c_int32),
+                ("n_points", C.c_int32), ("ms_tree", C.c_double), ("ms_bs", C.c_double),
+                ("ms_local", C.c_double), ("bs_launches", C.c_int32), ("pad_", C.c_int32)]
+
+
+_P = C.POINTER
+_dp = _P(C.c_double)
+_ip = _P(C.c_int32)
+
+
+def _sig(name, res, *args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("ovg_last_error", C.c_char_p)
+_sig("ovg_version", C.c_char_p)
+_sig("ovg_params_sizeof", C.c_size_t)
+_sig("ovg_stats_sizeof", C.c_size_t)
+_sig("ovg_params_default", None, _P(Params), C.c_double)
+_sig("ovg_params_set_periodic_images", C.c_int, _P(Params), C.c_int)
+_sig("ovg_create", C.c_int, _P(C.c_void_p), C.c_int)
+_sig("ovg_destroy", C.c_int, C.c_void_p)
+_sig("ovg_set_params", C.c_int, C.c_void_p, _P(Params))
+_sig("ovg_get_params", C.c_int, C.c_void_p, _P(Params))
+_sig("ovg_upload", C.c_int, C.c_void_p, C.c_int, _dp, _ip, _ip)
+_sig("ovg_upload_positions", C.c_int, C.c_void_p, _dp)
+_sig("ovg_upload_connections", C.c_int, C.c_void_p, _ip, _ip)
+_sig("ovg_download", C.c_int, C.c_void_p, _dp, _dp, _dp, _dp, _dp)
+_sig("ovg_download_connections", C.c_int, C.c_void_p, _ip, _ip)
+_sig("ovg_num_slots", C.c_int, C.c_void_p)
+_sig("ovg_update_tangents_normals", C.c_int, C.c_void_p)
+_sig("ovg_update_velocities", C.c_int, C.c_void_p, C.c_double)
+_sig("ovg_update_tangle", C.c_int, C.c_void_p, C.c_double)
+_sig("ovg_rk4_step", C.c_int, C.c_void_p, C.c_double, C.c_double)
+_sig("ovg_enforce_periodic", C.c_int, C.c_void_p)
+_sig("ovg_reconnect_candidates", C.c_int, C.c_void_p, C.c_int, _ip, _ip)
+_sig("ovg_reconnect", C.c_int, C.c_void_p, C.c_double, _ip)
+_sig("ovg_smooth", C.c_int, C.c_void_p)
+_sig("ovg_velocity_at", C.c_int, C.c_void_p, C.c_int, _ip, C.c_double, _dp, _dp)
+_sig("ovg_line_length", C.c_int, C.c_void_p, _dp, _ip)
+_sig("ovg_get_stats", C.c_int, C.c_void_p, _P(Stats))
+_sig("ovg_set_timing", C.c_int, C.c_void_p, C.c_int)
+_sig("ovg_synchronize", C.c_int, C.c_void_p)
+_sig("ovg_generate_rings", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double,
+     C.c_double, C.c_int, _dp, _ip, _ip)
+_sig("ovg_rings_for_points", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
+     C.c_double, C.c_double)
+
+if _lib.ovg_params_sizeof() != C.sizeof(Params) or _lib.ovg_stats_sizeof() != C.sizeof(Stats):
+    raise ImportError("ctypes struct layout does not match libopenvort_gpu.so")
+
+
+class OvgError(RuntimeError):
+    def __init__(self, code, where):
+        msg = _lib.ovg_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed with code {code}: {msg}")
+        self.code = code
+
+
+def _check(code, where):
+    if code != OK:
+        raise OvgError(code, where)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def lib():
+    """The loaded ctypes CDLL of libopenvort_gpu.so."""
+    return _lib
+
+
+def default_params(D, **kw):
+    """Reference defaults (src/initial_conditions.c:12-91) for [0,D]^3 with
+    26 periodic images; keyword arguments override fields."""
+    p = Params()
+    _lib.ovg_params_default(C.byref(p), float(D))
+    for k, v in kw.items():
+        if k in ("vn", "vs", "vb"):
+            f = getattr(p, k)
+            typ, strength, direction = v
+            f.type = typ
+            f.strength = strength
+            for i in range(3):
+                f.direction[i] = direction[i]
+        elif not hasattr(p, k):
+            raise AttributeError(k)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def set_images(p, n):
+    _check(_lib.ovg_params_set_periodic_images(C.byref(p), n), "set_images")
+
+
+class Tangle:
+    """Host tangle in the reference slot layout (include/tangle.h:47-66)."""
+
+    def __init__(self, vnodes, conn, status):
+        self.vnodes = np.ascontiguousarray(vnodes, dtype=np.float64).reshape(-1, 3)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 2)
+        self.status = np.ascontiguousarray(status, dtype=np.int32).reshape(-1, 2)
+
+    @property
+    def n(self):
+        return self.vnodes.shape[0]
+
+    @property
+    def live(self):
+        return np.nonzero(self.status[:, 0] != EMPTY)[0]
+
+    def copy(self):
+        return Tangle(self.vnodes.copy(), self.conn.copy(), self.status.copy())
+
+
+def generate_rings(seed, n_rings, D, r_min, r_max, dxi, n_cap=None):
+    """Seeded ring tangle (ovg_generate_rings); n_cap defaults to the next
+    power of two at or above the point count, as the reference's doubling
+    capacity (src/tangle.c:248)."""
+    total = _lib.ovg_generate_rings(seed, n_rings, D, r_min, r_max, dxi, 0, None, None, None)
+    if total < 0:
+        raise OvgError(EINVAL, "generate_rings(count)")
+    if n_cap is None:
+        n_cap = 1
+        while n_cap < total:
+            n_cap *= 2
+    v = np.zeros((n_cap, 3))
+    c = np.zeros((n_cap, 2), dtype=np.int32)
+    s = np.zeros((n_cap, 2), dtype=np.int32)
+    got = _lib.ovg_generate_rings(seed, n_rings, D, r_min, r_max, dxi, n_cap, _d(v), _i(c), _i(s))
+    if got != total:
+        raise OvgError(ELIMIT, "generate_rings")
+    return Tangle(v, c, s)
+
+
+def rings_for_points(seed, target, D, r_min, r_max, dxi):
+    return _lib.ovg_rings_for_points(seed, target, D, r_min, r_max, dxi)
+
+
+class Engine:
+    """One device context (one HIP stream) holding one tangle."""
+
+    def __init__(self, params, device=0):
+        h = C.c_void_p()
+        _check(_lib.ovg_create(C.byref(h), device), "ovg_create")
+        self._h = h
+        self.set_params(params)
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            _lib.ovg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_params(self, p):
+        self.params = p
+        _check(_lib.ovg_set_params(self._h, C.byref(p)), "ovg_set_params")
+
+    def upload(self, t):
+        self.n = t.n
+        _check(_lib.ovg_upload(self._h, t.n, _d(t.vnodes), _i(t.conn), _i(t.status)), "ovg_upload")
+
+    def upload_positions(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(_lib.ovg_upload_positions(self._h, _d(v)), "ovg_upload_positions")
+
+    def download(self):
+        out = {k: np.zeros((self.n, 3)) for k in ("vnodes", "vels", "vs", "tangents", "normals")}
+        _check(_lib.ovg_download(self._h, _d(out["vnodes"]), _d(out["vels"]), _d(out["vs"]),
+                                 _d(out["tangents"]), _d(out["normals"])), "ovg_download")
+        return out
+
+    def download_connections(self):
+        c = np.zeros((self.n, 2), dtype=np.int32)
+        s = np.zeros((self.n, 2), dtype=np.int32)
+        _check(_lib.ovg_download_connections(self._h, _i(c), _i(s)), "ovg_download_connections")
+        return c, s
+
+    def update_tangents_normals(self):
+        _check(_lib.ovg_update_tangents_normals(self._h), "ovg_update_tangents_normals")
+
+    def update_velocities(self, t=0.0):
+        _check(_lib.ovg_update_velocities(self._h, t), "ovg_update_velocities")
+
+    def update_tangle(self, t=0.0):
+        _check(_lib.ovg_update_tangle(self._h, t), "ovg_update_tangle")
+
+    def rk4_step(self, t, dt):
+        _check(_lib.ovg_rk4_step(self._h, t, dt), "ovg_rk4_step")
+
+    def enforce_periodic(self):
+        _check(_lib.ovg_enforce_periodic(self._h), "ovg_enforce_periodic")
+
+    def reconnect_candidates(self, max_pairs=1 << 20):
+        pairs = np.zeros((max_pairs, 2), dtype=np.int32)
+        n = C.c_int32(0)
+        _check(_lib.ovg_reconnect_candidates(self._h, max_pairs, _i(pairs), C.byref(n)),
+               "ovg_reconnect_candidates")
+        return pairs[: n.value].copy()
+
+    def reconnect(self, t=0.0):
+        n = C.c_int32(0)
+        _check(_lib.ovg_reconnect(self._h, t, C.byref(n)), "ovg_reconnect")
+        return n.value
+
+    def smooth(self):
+        _check(_lib.ovg_smooth(self._h), "ovg_smooth")
+
+    def velocity_at(self, slots, t=0.0):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        vs = np.zeros((len(slots), 3))
+        vels = np.zeros((len(slots), 3))
+        _check(_lib.ovg_velocity_at(self._h, len(slots), _i(slots), t, _d(vs), _d(vels)),
+               "ovg_velocity_at")
+        return vs, vels
+
+    def line_length(self):
+        L = C.c_double(0)
+        n = C.c_int32(0)
+        _check(_lib.ovg_line_length(self._h, C.byref(L), C.byref(n)), "ovg_line_length")
+        return L.value, n.value
+
+    def stats(self):
+        s = Stats()
+        _check(_lib.ovg_get_stats(self._h, C.byref(s)), "ovg_get_stats")
+        return s
+
+    def set_timing(self, on):
+        _check(_lib.ovg_set_timing(self._h, int(bool(on))), "ovg_set_timing")
+
+    def synchronize(self):
+        _check(_lib.ovg_synchronize(self._h), "ovg_synchronize")

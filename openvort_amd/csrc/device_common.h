@@ -1,0 +1,139 @@
+// Device-side primitives shared by the kernels.  Everything that decides a
+// reference guard (seg_pwrap, the segment-field formula and its < 1e-8
+// tests) is written in the reference's operation order; the library is
+// compiled with -ffp-contract=off so nothing is fused behind our back.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ovg {
+
+constexpr int kMaxImages = 26;
+constexpr int kMaxLevel = 21;  // octree levels encoded in a 63-bit path key
+
+// Per-tangle constants, precomputed on the host exactly as the reference
+// computes them inside its loops.
+struct TangleParams {
+    double lo[3], hi[3];
+    double L[3];        // hi - lo                       (src/boundaries.c:61-63)
+    double halfL[3];    // L/2                            (src/boundaries.c:68-79)
+    int periodic[3];    // wall[LEFT], wall[BACK], wall[DOWN] are periodic
+    int wrap_face[6];   // wall[face] == periodic, for enforce_periodic
+    int n_images;
+    int pad0;
+    double Ls[kMaxImages][3];  // L * shift, the image offsets (src/tangle.c:378)
+    double f_bs;        // KAPPA / 4 / M_PI               (src/tangle.c:545)
+    double kappa, a0, sqrtE;
+    double evs[3];      // external superfluid flow at this call's t
+    double evn[3];      // external normal flow
+    double evb[3];      // boundary flow (PINNED nodes)
+    double alpha, alpha_p;
+    int use_mf;
+    int pad1;
+    double theta;
+    double crit_scale;  // 1/theta^2 for the squared acceptance test
+    int leaf_size;
+    int pad2;
+};
+
+struct Vec3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx,
+                                       double by, double bz)
+{
+    // src/vec3_math.c:91-96: ((0 + a0 b0) + a1 b1) + a2 b2; 0 + v == v
+    double o = ax * bx;
+    o = o + ay * by;
+    o = o + az * bz;
+    return o;
+}
+
+// seg_pwrap, src/boundaries.c:54-83.  r2 moves by whole periods until it is
+// within half a box of r1; the loops are capped (positions more than 64
+// boxes away mean the state is already corrupt; the caller's finite-state
+// check reports it).
+__device__ __forceinline__ void pwrap(const TangleParams &P, double r1x, double r1y,
+                                      double r1z, double &x, double &y, double &z)
+{
+    if (P.periodic[0]) {
+        int g = 0;
+        while ((x - r1x) > P.halfL[0] && g++ < 64) x -= P.L[0];
+        while ((x - r1x) < -P.halfL[0] && g++ < 128) x += P.L[0];
+    }
+    if (P.periodic[1]) {
+        int g = 0;
+        while ((y - r1y) > P.halfL[1] && g++ < 64) y -= P.L[1];
+        while ((y - r1y) < -P.halfL[1] && g++ < 128) y += P.L[1];
+    }
+    if (P.periodic[2]) {
+        int g = 0;
+        while ((z - r1z) > P.halfL[2] && g++ < 64) z -= P.L[2];
+        while ((z - r1z) < -P.halfL[2] && g++ < 128) z += P.L[2];
+    }
+}
+
+// The Biot-Savart field of a straight segment with end vectors R = r1 - r
+// and Rp = r2 - r, src/tangle.c:540-559 (and the monopole of
+// src/octree.c:327-342 with R = com - r, Rp = R + sum ds).  Reference
+// guards: |R| or |Rp| < 1e-8, and the normalised dot (R.Rp/|R|)/|Rp| <
+// 1e-8.  The two exact divisions of the second guard are only evaluated
+// when a cheap bracket cannot decide it: q = dot / (|R| |Rp|) differs from
+// the reference's (dot/|R|)/|Rp| by at most a few ulp, so a margin of
+// 2^-40 relative decides every other case identically.
+// Returns false when the pair contributes zero.
+__device__ __forceinline__ bool seg_field64(double Rx, double Ry, double Rz, double Px,
+                                            double Py, double Pz, double f_bs, double &vx,
+                                            double &vy, double &vz)
+{
+    double lR = sqrt(dot3(Rx, Ry, Rz, Rx, Ry, Rz));
+    double lP = sqrt(dot3(Px, Py, Pz, Px, Py, Pz));
+    double dot = dot3(Rx, Ry, Rz, Px, Py, Pz);
+    if (lR < 1e-8 || lP < 1e-8) return false;
+    double ab = lR * lP;
+    // bracket for ndot < 1e-8
+    const double lo_t = 1e-8 * (1.0 - 0x1.0p-40), hi_t = 1e-8 * (1.0 + 0x1.0p-40);
+    if (dot < lo_t * ab) return false;
+    if (!(dot > hi_t * ab)) {
+        double nd = dot / lR / lP;
+        if (nd < 1e-8) return false;
+    }
+    double denom = ab * (ab + dot);
+    double cx = Ry * Pz - Rz * Py;
+    double cy = Rz * Px - Rx * Pz;
+    double cz = Rx * Py - Ry * Px;
+    double m = f_bs * (lR + lP) / denom;
+    vx = cx * m;
+    vy = cy * m;
+    vz = cz * m;
+    return true;
+}
+
+// FP32 pair formula for the far field: R and Rp arrive already rounded to
+// FP32 from FP64 differences.  Same guards as the reference (thresholds in
+// FP32), native reciprocal square roots, one reciprocal.
+__device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float Px, float Py,
+                                            float Pz, float f_bs, float &vx, float &vy,
+                                            float &vz)
+{
+    float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
+    float p2 = Px * Px + Py * Py + Pz * Pz;
+    float dot = Rx * Px + Ry * Py + Rz * Pz;
+    float ia = __builtin_amdgcn_rsqf(r2);
+    float ib = __builtin_amdgcn_rsqf(p2);
+    float iab = ia * ib;
+    float nd = dot * iab;
+    bool ok = (r2 >= 1e-16f) && (p2 >= 1e-16f) && (nd >= 1e-8f);
+    // (|R|+|Rp|)/(|R||Rp|(|R||Rp| + dot)) = (ia + ib) / (1 + dot*iab) * iab
+    float m = f_bs * (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
+    m = ok ? m : 0.0f;
+    float cx = Ry * Pz - Rz * Py;
+    float cy = Rz * Px - Rx * Pz;
+    float cz = Rx * Py - Ry * Px;
+    vx = cx * m;
+    vy = cy * m;
+    vz = cz * m;
+}
+
+}  // namespace ovg

@@ -1,0 +1,744 @@
+// Host side of the C ABI: context, state upload/download, the parameter
+// block and the sequencing of the hot path (update_tangle, rk4_step,
+// enforce_periodic_boundaries).  Kernels live in k_*.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "engine_internal.h"
+
+struct ovg_ctx {
+    ovg::Ctx c;
+};
+
+namespace ovg {
+
+int DBuf::ensure(size_t n)
+{
+    if (n <= bytes) return OVG_OK;
+    if (p) {
+        hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return set_error(OVG_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return OVG_OK;
+}
+
+void DBuf::release()
+{
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ---- small stream-ordered helper kernels ---------------------------------
+__global__ void k_set_params(TangleParams p, TangleParams *dst) { *dst = p; }
+
+__global__ void k_aos_to_soa(int n, const double *__restrict__ aos, double *__restrict__ x,
+                             double *__restrict__ y, double *__restrict__ z)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = aos[3 * i];
+    y[i] = aos[3 * i + 1];
+    z[i] = aos[3 * i + 2];
+}
+
+__global__ void k_soa_to_aos(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, double *__restrict__ aos)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * i] = x[i];
+    aos[3 * i + 1] = y[i];
+    aos[3 * i + 2] = z[i];
+}
+
+__global__ void k_split_pairs(int n, const int *__restrict__ pairs, int *__restrict__ a,
+                              int *__restrict__ b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[i] = pairs[2 * i];
+    b[i] = pairs[2 * i + 1];
+}
+
+__global__ void k_join_pairs(int n, const int *__restrict__ a, const int *__restrict__ b,
+                             int *__restrict__ pairs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pairs[2 * i] = a[i];
+    pairs[2 * i + 1] = b[i];
+}
+
+__global__ void k_fill_zero(int n, double *__restrict__ a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+static inline int grid_for(int n, int bs = 256) { return (n + bs - 1) / bs; }
+
+void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos)
+{
+    if (c.n == 0) return;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c.n)), dim3(256), 0, c.stream, c.n, x, y, z,
+                       aos);
+}
+
+void launch_split_pairs(Ctx &c, const int *pairs)
+{
+    if (c.n == 0) return;
+    hipLaunchKernelGGL(k_split_pairs, dim3(grid_for(c.n)), dim3(256), 0, c.stream, c.n, pairs,
+                       c.fwd.as<int>(), c.rev.as<int>());
+}
+
+// reference get_v for one flow at time t (src/external_velocity.c:13-81)
+static void flow_at(const ovg_flow &f, double t, double out[3])
+{
+    out[0] = out[1] = out[2] = 0;
+    if (f.type == OVG_FLOW_NONE) return;
+    double str = f.strength;
+    if (f.type == OVG_FLOW_DECREASING) {
+        if (t < f.time_of_linear_decrease)
+            str = f.starting_strength -
+                  t * (f.starting_strength - f.strength) / f.time_of_linear_decrease;
+        else
+            str = f.strength;
+    }
+    double d[3] = {f.direction[0], f.direction[1], f.direction[2]};
+    if (!(d[0] == 0 && d[1] == 0 && d[2] == 0)) {
+        double l = 0;
+        for (int k = 0; k < 3; ++k) l += d[k] * d[k];
+        l = std::sqrt(l);
+        for (int k = 0; k < 3; ++k) d[k] = d[k] * (1 / l);
+    }
+    for (int k = 0; k < 3; ++k) out[k] = d[k] * str;
+}
+
+TangleParams host_tparams(const ovg_params &p, double t)
+{
+    TangleParams P;
+    std::memset(&P, 0, sizeof P);
+    for (int k = 0; k < 3; ++k) {
+        P.lo[k] = p.box_lo[k];
+        P.hi[k] = p.box_hi[k];
+        P.L[k] = p.box_hi[k] - p.box_lo[k];
+        P.halfL[k] = P.L[k] / 2;
+    }
+    P.periodic[0] = p.wall[0] == OVG_WALL_PERIODIC;
+    P.periodic[1] = p.wall[2] == OVG_WALL_PERIODIC;
+    P.periodic[2] = p.wall[4] == OVG_WALL_PERIODIC;
+    for (int f = 0; f < 6; ++f) P.wrap_face[f] = p.wall[f] == OVG_WALL_PERIODIC;
+    P.n_images = p.n_images;
+    for (int j = 0; j < p.n_images; ++j)
+        for (int k = 0; k < 3; ++k) P.Ls[j][k] = P.L[k] * p.images[j][k];
+    P.f_bs = p.kappa / 4 / M_PI;
+    P.kappa = p.kappa;
+    P.a0 = p.vortex_width;
+    P.sqrtE = std::sqrt(M_E);
+    flow_at(p.vs, t, P.evs);
+    flow_at(p.vn, t, P.evn);
+    flow_at(p.vb, t, P.evb);
+    P.alpha = p.alpha;
+    P.alpha_p = p.alpha_p;
+    P.use_mf = p.use_mutual_friction;
+    P.theta = p.theta;
+    P.crit_scale = 1.0 / (p.theta * p.theta);
+    P.leaf_size = p.leaf_size;
+    return P;
+}
+
+void push_params(Ctx &c, double t)
+{
+    TangleParams P = host_tparams(c.params, t);
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, c.stream, P,
+                       c.tparams.as<TangleParams>());
+}
+
+static int check_ready(const ovg_ctx *ctx, bool need_state = true)
+{
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    if (!ctx->c.have_params) return set_error(OVG_EINVAL, "ovg_set_params has not been called");
+    if (need_state && ctx->c.n == 0) return set_error(OVG_EINVAL, "no tangle uploaded");
+    return OVG_OK;
+}
+
+static double *dp(DBuf &b) { return b.as<double>(); }
+
+struct Triple {
+    double *v[3];
+};
+static Triple tri(DBuf *b)
+{
+    return Triple{{b[0].as<double>(), b[1].as<double>(), b[2].as<double>()}};
+}
+
+static void timer_begin(Ctx &c, int i)
+{
+    if (c.timing) hipEventRecord(c.ev[i], c.stream);
+}
+
+// Biot-Savart part of one evaluation at positions x: writes bs_prim,
+// bs_img by slot.
+static int eval_bs(Ctx &c, const double *const x[3])
+{
+    if (c.params.method == OVG_METHOD_BH) {
+        timer_begin(c, 0);
+        OVG_TRY(build_tree(c, x));
+        timer_begin(c, 1);
+        launch_bh(c, x);
+        timer_begin(c, 2);
+    } else {
+        timer_begin(c, 0);
+        launch_prep_segments(c, x);
+        timer_begin(c, 1);
+        Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
+        launch_direct(c, x, c.n_live, c.live.as<int>(), bp.v, bi.v, 0);
+        timer_begin(c, 2);
+    }
+    if (c.timing) {
+        hipEventSynchronize(c.ev[2]);
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, c.ev[0], c.ev[1]);
+        hipEventElapsedTime(&b, c.ev[1], c.ev[2]);
+        c.stats.ms_tree += a;
+        c.stats.ms_bs += b;
+        c.stats.bs_launches += 1;
+    }
+    return OVG_OK;
+}
+
+// Counters accumulate from one ovg_get_stats call to the next.
+static int fetch_stats(Ctx &c, ovg_stats *out)
+{
+    unsigned long long cnt[2] = {0, 0};
+    int tc[4] = {0, 0, 0, 0};
+    OVG_HIP(hipMemcpyAsync(cnt, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+    if (c.tree.counters.p)
+        OVG_HIP(hipMemcpyAsync(tc, c.tree.counters.p, 16, hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    *out = c.stats;
+    out->interactions += (double)cnt[0] + (double)cnt[1];
+    out->interactions_fp32 += (double)cnt[1];
+    out->tree_nodes = tc[1];
+    out->n_points = c.n_live;
+    std::memset(&c.stats, 0, sizeof c.stats);
+    hipMemsetAsync(c.counter64.p, 0, 16, c.stream);
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}
+
+}  // namespace ovg
+
+using namespace ovg;
+
+extern "C" int ovg_create(ovg_ctx **out, int device)
+{
+    clear_error();
+    if (!out) return set_error(OVG_EINVAL, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(OVG_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev)
+        return set_error(OVG_EINVAL, "device %d out of range (have %d)", device, ndev);
+    hipDeviceProp_t prop;
+    OVG_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(OVG_EDEVICE, "device %d is %s; this library is built for gfx950", device,
+                         prop.gcnArchName);
+    OVG_HIP(hipSetDevice(device));
+    ovg_ctx *ctx = new ovg_ctx();
+    Ctx &c = ctx->c;
+    c.device = device;
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return set_error(OVG_EDEVICE, "hipStreamCreate failed");
+    }
+    for (auto &e : c.ev) hipEventCreate(&e);
+    int r;
+    if ((r = c.tparams.ensure(sizeof(TangleParams))) || (r = c.flags.ensure(64)) ||
+        (r = c.counter64.ensure(64))) {
+        ovg_destroy(ctx);
+        return r;
+    }
+    hipMemsetAsync(c.flags.p, 0, 64, c.stream);
+    hipMemsetAsync(c.counter64.p, 0, 64, c.stream);
+    *out = ctx;
+    return OVG_OK;
+}
+
+extern "C" int ovg_destroy(ovg_ctx *ctx)
+{
+    if (!ctx) return OVG_OK;
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.seg_ids,
+                   &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
+                   &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf};
+    for (auto *b : all) b->release();
+    for (int q = 0; q < 3; ++q) {
+        c.x[q].release(); c.vels[q].release(); c.vs[q].release(); c.tan[q].release();
+        c.nor[q].release(); c.move[q].release(); c.k[q].release(); c.stan[q].release();
+        c.snor[q].release(); c.bs_prim[q].release(); c.bs_img[q].release();
+        c.xs[0][q].release(); c.xs[1][q].release();
+    }
+    for (auto &b : c.seg) b.release();
+    TreeBufs &T = c.tree;
+    DBuf *tb[] = {&T.keys[0], &T.keys[1], &T.vals[0], &T.vals[1], &T.hist, &T.delta,
+                  &T.counts, &T.base, &T.nodes, &T.meta, &T.level_hist, &T.level_list,
+                  &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
+                  &T.sqy, &T.sqz};
+    for (auto *b : tb) b->release();
+    for (auto &e : c.ev)
+        if (e) hipEventDestroy(e);
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete ctx;
+    return OVG_OK;
+}
+
+extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
+{
+    clear_error();
+    if (!ctx || !p) return set_error(OVG_EINVAL, "NULL argument");
+    for (int k = 0; k < 3; ++k)
+        if (!(p->box_hi[k] > p->box_lo[k]) || !std::isfinite(p->box_hi[k] - p->box_lo[k]))
+            return set_error(OVG_EINVAL, "box axis %d is empty or not finite", k);
+    for (int f = 0; f < 6; ++f) {
+        if (p->wall[f] == OVG_WALL_MIRROR)
+            return set_error(OVG_EUNSUPPORTED,
+                             "mirror walls are outside the engine's scope (periodic or open only)");
+        if (p->wall[f] != OVG_WALL_OPEN && p->wall[f] != OVG_WALL_PERIODIC)
+            return set_error(OVG_EINVAL, "wall[%d] = %d is not a wall kind", f, p->wall[f]);
+    }
+    if (p->n_images < 0 || p->n_images > kMaxImages)
+        return set_error(OVG_EINVAL, "n_images = %d out of range", p->n_images);
+    if (p->method != OVG_METHOD_DIRECT && p->method != OVG_METHOD_BH)
+        return set_error(OVG_EINVAL, "unknown method %d", p->method);
+    if (p->precision != OVG_PREC_FP64 && p->precision != OVG_PREC_MIXED)
+        return set_error(OVG_EINVAL, "unknown precision %d", p->precision);
+    if (p->method == OVG_METHOD_BH && !(p->theta > 0))
+        return set_error(OVG_EINVAL, "theta must be > 0");
+    if (p->leaf_size < 1 || p->leaf_size > 64)
+        return set_error(OVG_EINVAL, "leaf_size must be in [1, 64]");
+    if (p->traverse != OVG_TRAVERSE_POINT && p->traverse != OVG_TRAVERSE_GROUP)
+        return set_error(OVG_EINVAL, "unknown traversal %d", p->traverse);
+    if (p->traverse == OVG_TRAVERSE_POINT &&
+        (p->leaf_size != 1 || p->precision != OVG_PREC_FP64) && p->method == OVG_METHOD_BH)
+        return set_error(OVG_EINVAL,
+                         "point traversal is the reference's FP64 split-until-one tree: "
+                         "leaf_size 1 and FP64 only");
+    if (!(p->kappa > 0) || !(p->vortex_width > 0))
+        return set_error(OVG_EINVAL, "kappa and vortex_width must be > 0");
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    c.params = *p;
+    c.have_params = true;
+    push_params(c, 0.0);
+    return OVG_OK;
+}
+
+extern "C" int ovg_get_params(const ovg_ctx *ctx, ovg_params *p)
+{
+    if (!ctx || !p) return set_error(OVG_EINVAL, "NULL argument");
+    *p = ctx->c.params;
+    return OVG_OK;
+}
+
+// Validate topology: FREE slots must sit on closed loops with consistent
+// links; pinned nodes need mirror walls, which are out of scope.
+static int validate_topology(int n, const int *conn, const int *status)
+{
+    for (int k = 0; k < n; ++k) {
+        int s = status[2 * k];
+        if (s == OVG_EMPTY) continue;
+        if (s == OVG_PINNED || s == OVG_PINNED_SLIP)
+            return set_error(OVG_EUNSUPPORTED, "slot %d is pinned; wall pinning is out of scope", k);
+        if (s != OVG_FREE) return set_error(OVG_EINVAL, "slot %d has status %d", k, s);
+        int f = conn[2 * k], r = conn[2 * k + 1];
+        if (f < 0 || f >= n || r < 0 || r >= n)
+            return set_error(OVG_EUNSUPPORTED,
+                             "slot %d has an open end (forward %d, reverse %d): only closed "
+                             "loops are supported",
+                             k, f, r);
+        if (status[2 * f] == OVG_EMPTY || status[2 * r] == OVG_EMPTY)
+            return set_error(OVG_EINVAL, "slot %d links to an EMPTY slot", k);
+        if (conn[2 * f + 1] != k || conn[2 * r] != k)
+            return set_error(OVG_EINVAL, "slot %d: forward/reverse links are inconsistent", k);
+    }
+    return OVG_OK;
+}
+
+static int alloc_state(Ctx &c, int n)
+{
+    size_t b8 = sizeof(double) * (size_t)(n > 0 ? n : 1);
+    for (int q = 0; q < 3; ++q) {
+        DBuf *bufs[] = {&c.x[q], &c.vels[q], &c.vs[q], &c.tan[q], &c.nor[q], &c.move[q],
+                        &c.bs_prim[q], &c.bs_img[q], &c.xs[0][q], &c.xs[1][q]};
+        for (auto *b : bufs) OVG_TRY(b->ensure(b8));
+    }
+    for (auto &s : c.seg) OVG_TRY(s.ensure(b8));
+    OVG_TRY(c.seg_ids.ensure(sizeof(int2) * (size_t)(n > 0 ? n : 1)));
+    size_t b4 = sizeof(int) * (size_t)(n > 0 ? n : 1);
+    OVG_TRY(c.fwd.ensure(b4));
+    OVG_TRY(c.rev.ensure(b4));
+    OVG_TRY(c.status.ensure(b4));
+    OVG_TRY(c.live.ensure(b4));
+    OVG_TRY(c.src.ensure(b4));
+    return OVG_OK;
+}
+
+static int upload_positions(Ctx &c, const double *vnodes)
+{
+    int n = c.n;
+    DBuf &stage = c.partial;
+    OVG_TRY(stage.ensure(sizeof(double) * 3 * (size_t)n));
+    OVG_HIP(hipMemcpyAsync(stage.p, vnodes, sizeof(double) * 3 * (size_t)n,
+                           hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                       stage.as<double>(), dp(c.x[0]), dp(c.x[1]), dp(c.x[2]));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}
+
+static int upload_topology(Ctx &c, const int *conn, const int *status)
+{
+    int n = c.n;
+    c.topo_version++;
+    c.h_conn.assign(conn, conn + 2 * (size_t)n);
+    c.h_status.assign(status, status + 2 * (size_t)n);
+    std::vector<int> live;
+    live.reserve(n);
+    for (int k = 0; k < n; ++k)
+        if (status[2 * k] != OVG_EMPTY) live.push_back(k);
+    c.n_live = (int)live.size();
+    c.n_src = c.n_live;  // closed loops: every live slot starts a segment
+    DBuf &stage = c.partial;
+    OVG_TRY(stage.ensure(sizeof(int) * 2 * (size_t)n));
+    OVG_HIP(hipMemcpyAsync(stage.p, conn, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice,
+                           c.stream));
+    hipLaunchKernelGGL(k_split_pairs, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                       stage.as<int>(), c.fwd.as<int>(), c.rev.as<int>());
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> st(n);
+    for (int k = 0; k < n; ++k) st[k] = status[2 * k];
+    OVG_HIP(hipMemcpy(c.status.p, st.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    if (c.n_live) {
+        OVG_HIP(hipMemcpy(c.live.p, live.data(), sizeof(int) * (size_t)c.n_live,
+                          hipMemcpyHostToDevice));
+        OVG_HIP(hipMemcpy(c.src.p, live.data(), sizeof(int) * (size_t)c.n_live,
+                          hipMemcpyHostToDevice));
+    }
+    return OVG_OK;
+}
+
+extern "C" int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes, const int *connections,
+                          const int *status)
+{
+    clear_error();
+    if (!ctx || n < 0 || (n > 0 && (!vnodes || !connections || !status)))
+        return set_error(OVG_EINVAL, "bad arguments to ovg_upload");
+    OVG_TRY(validate_topology(n, connections, status));
+    for (int k = 0; k < n; ++k)
+        if (status[2 * k] != OVG_EMPTY)
+            for (int q = 0; q < 3; ++q)
+                if (!std::isfinite(vnodes[3 * k + q]))
+                    return set_error(OVG_EINVAL, "slot %d has a non-finite position", k);
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    OVG_TRY(alloc_state(c, n));
+    c.n = n;
+    OVG_TRY(upload_positions(c, vnodes));
+    OVG_TRY(upload_topology(c, connections, status));
+    for (int q = 0; q < 3; ++q) {
+        DBuf *bufs[] = {&c.vels[q], &c.vs[q], &c.tan[q], &c.nor[q]};
+        for (auto *b : bufs)
+            hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n > 0 ? n : 1)), dim3(256), 0, c.stream,
+                               n, b->as<double>());
+    }
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}
+
+extern "C" int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return upload_positions(c, vnodes);
+}
+
+extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, const int *status)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    OVG_TRY(validate_topology(c.n, connections, status));
+    hipSetDevice(c.device);
+    return upload_topology(c, connections, status);
+}
+
+static int check_flags(Ctx &c)
+{
+    int f = 0;
+    OVG_HIP(hipMemcpyAsync(&f, c.flags.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    if (f) {
+        hipMemsetAsync(c.flags.p, 0, sizeof(int), c.stream);
+        if (f & 2) return set_error(OVG_EINVAL, "non-finite positions in the tangle state");
+        return set_error(OVG_EINVAL, "a point left the box by more than 256 periods");
+    }
+    return OVG_OK;
+}
+
+extern "C" int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *vs,
+                            double *tangents, double *normals)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    int n = c.n;
+    OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)n));
+    double *outs[5] = {vnodes, vels, vs, tangents, normals};
+    DBuf *src[5] = {c.x, c.vels, c.vs, c.tan, c.nor};
+    for (int o = 0; o < 5; ++o) {
+        if (!outs[o]) continue;
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                           dp(src[o][0]), dp(src[o][1]), dp(src[o][2]), c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(outs[o], c.partial.p, sizeof(double) * 3 * (size_t)n,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    return check_flags(c);
+}
+
+extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *status)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    int n = c.n;
+    if (connections) {
+        OVG_TRY(c.partial.ensure(sizeof(int) * 2 * (size_t)n));
+        hipLaunchKernelGGL(k_join_pairs, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                           c.fwd.as<int>(), c.rev.as<int>(), c.partial.as<int>());
+        OVG_HIP(hipMemcpyAsync(connections, c.partial.p, sizeof(int) * 2 * (size_t)n,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (status) std::memcpy(status, c.h_status.data(), sizeof(int) * 2 * (size_t)n);
+    return OVG_OK;
+}
+
+extern "C" int ovg_num_slots(const ovg_ctx *ctx) { return ctx ? ctx->c.n : -1; }
+
+extern "C" int ovg_update_tangents_normals(ovg_ctx *ctx)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    Triple x = tri(c.x), t = tri(c.tan), nrm = tri(c.nor);
+    launch_stencil(c, x.v, t.v, nrm.v);
+    return OVG_OK;
+}
+
+static int velocities(Ctx &c, double t, bool own_tangents)
+{
+    push_params(c, t);
+    Triple x = tri(c.x);
+    OVG_TRY(eval_bs(c, x.v));
+    launch_compose(c, x.v, own_tangents, 0, 0.0);
+    return OVG_OK;
+}
+
+extern "C" int ovg_update_velocities(ovg_ctx *ctx, double t)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return velocities(c, t, false);
+}
+
+extern "C" int ovg_update_tangle(ovg_ctx *ctx, double t)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return velocities(c, t, true);
+}
+
+extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    double h4 = c.params.rk4_classic ? dt : dt / 2;
+    (void)h4;
+    // stage positions: xs[0] = x + dt/2 k1 (euler_step2, src/vortex_dynamics.c:76)
+    Triple k1 = tri(c.vels), xs0 = tri(c.xs[0]), xs1 = tri(c.xs[1]);
+    launch_stage_positions(c, k1.v, dt / 2, xs0.v);
+    double times[3] = {t + dt / 2, t + dt / 2, t + dt};
+    for (int s = 1; s <= 3; ++s) {
+        const double *const *xe = (s == 2) ? xs1.v : xs0.v;
+        push_params(c, times[s - 1]);
+        OVG_TRY(eval_bs(c, xe));
+        launch_compose(c, xe, true, s, dt);
+    }
+    return OVG_OK;
+}
+
+extern "C" int ovg_enforce_periodic(ovg_ctx *ctx)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    launch_wrap(c);
+    return OVG_OK;
+}
+
+extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, double *vs,
+                               double *vels)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    if (m < 0 || (m > 0 && !slots)) return set_error(OVG_EINVAL, "bad slot list");
+    for (int i = 0; i < m; ++i)
+        if (slots[i] < 0 || slots[i] >= c.n || c.h_status[2 * slots[i]] == OVG_EMPTY)
+            return set_error(OVG_EINVAL, "slot %d is not a live point", slots[i]);
+    if (m == 0) return OVG_OK;
+    hipSetDevice(c.device);
+    push_params(c, t);
+    Triple x = tri(c.x);
+    launch_prep_segments(c, x.v);
+    OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)m));
+    OVG_HIP(hipMemcpyAsync(c.rc_tmp.p, slots, sizeof(int) * (size_t)m, hipMemcpyHostToDevice,
+                           c.stream));
+    Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
+    ovg_params saved = c.params;
+    c.params.precision = OVG_PREC_FP64;
+    launch_direct(c, x.v, m, c.rc_tmp.as<int>(), bp.v, bi.v, 0);
+    c.params = saved;
+    // compose over the selected slots only
+    int saved_live = c.n_live;
+    DBuf saved_list = c.live;
+    c.live = c.rc_tmp;
+    c.n_live = m;
+    launch_compose(c, x.v, false, 0, 0.0);
+    c.live = saved_list;
+    c.n_live = saved_live;
+    std::vector<double> hv(3 * (size_t)c.n);
+    for (int o = 0; o < 2; ++o) {
+        double *out = o == 0 ? vs : vels;
+        if (!out) continue;
+        DBuf *src = o == 0 ? c.vs : c.vels;
+        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)c.n));
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c.n)), dim3(256), 0, c.stream, c.n,
+                           dp(src[0]), dp(src[1]), dp(src[2]), c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(hv.data(), c.partial.p, sizeof(double) * 3 * (size_t)c.n,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < m; ++i)
+            for (int q = 0; q < 3; ++q) out[3 * i + q] = hv[3 * (size_t)slots[i] + q];
+    }
+    return OVG_OK;
+}
+
+extern "C" int ovg_line_length(ovg_ctx *ctx, double *length, int *n_points)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    int gs = grid_for(c.n_live);
+    std::vector<double> part(gs > 0 ? gs : 1, 0.0);
+    if (gs > 0) {
+        OVG_TRY(c.partial.ensure(sizeof(double) * (size_t)gs));
+        launch_line_length(c, c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(part.data(), c.partial.p, sizeof(double) * (size_t)gs,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    double L = 0;
+    for (int b = 0; b < gs; ++b) L += part[b];
+    if (length) *length = L;
+    if (n_points) *n_points = c.n_live;
+    return OVG_OK;
+}
+
+extern "C" int ovg_get_stats(ovg_ctx *ctx, ovg_stats *s)
+{
+    clear_error();
+    if (!ctx || !s) return set_error(OVG_EINVAL, "NULL argument");
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return fetch_stats(c, s);
+}
+
+extern "C" int ovg_set_timing(ovg_ctx *ctx, int on)
+{
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    ctx->c.timing = on;
+    return OVG_OK;
+}
+
+extern "C" int ovg_synchronize(ovg_ctx *ctx)
+{
+    clear_error();
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return check_flags(c);
+}
+
+extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pairs)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    if (max_pairs < 0 || (max_pairs > 0 && !pairs))
+        return set_error(OVG_EINVAL, "bad pair buffer");
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return recon_candidates(c, max_pairs, pairs, n_pairs);
+}
+
+extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    int nrec = 0;
+    OVG_TRY(reconnect_pass(c, t, &nrec));
+    if (n_reconnections) *n_reconnections = nrec;
+    return OVG_OK;
+}
+
+extern "C" int ovg_smooth(ovg_ctx *ctx)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    return smooth_loops(c);
+}

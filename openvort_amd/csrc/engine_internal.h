@@ -1,0 +1,163 @@
+// Internal declarations shared by the host-side engine and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/openvort_gpu.h"
+#include "device_common.h"
+
+namespace ovg {
+
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+#define OVG_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::ovg::set_error(OVG_EDEVICE, "%s failed: %s (%s:%d)", #call,        \
+                                    hipGetErrorString(e_), __FILE__, __LINE__);        \
+    } while (0)
+
+#define OVG_TRY(call)                 \
+    do {                              \
+        int r_ = (call);              \
+        if (r_ != OVG_OK) return r_;  \
+    } while (0)
+
+// Grow-only device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Octree node in DFS pre-order, 64 bytes so a wavefront-uniform node loads
+// with one scalar 64-byte load.
+struct Node {
+    double cx, cy, cz;   // centre of mass  (src/octree.c:264-269)
+    double qx, qy, qz;   // total circulation, sum of pwrapped ds (:272-282)
+    double Lm;           // largest side of the cell's box (:303-306)
+    int32_t count;       // points in the cell (N)
+    int32_t skip;        // DFS index of the next node outside this subtree
+};
+static_assert(sizeof(Node) == 64, "Node must be 64 bytes");
+
+// Per-node build bookkeeping (separate from the 64-byte traversal record).
+struct NodeMeta {
+    int32_t start;       // first sorted point
+    int32_t level;       // cell level (0 = root box)
+    int32_t pad0, pad1;
+};
+
+struct TreeBufs {
+    DBuf keys[2], vals[2];   // radix sort ping-pong (uint64 key, int32 point)
+    DBuf hist;               // radix histograms
+    DBuf delta;              // int8 LCP levels of adjacent sorted keys
+    DBuf counts, base;       // nodes starting at each sorted point, exclusive scan
+    DBuf nodes, meta;        // Node / NodeMeta
+    DBuf level_hist, level_list;
+    DBuf scan_tmp;
+    DBuf counters;           // int32: [0] n_in, [1] n_nodes, [2] fallback count,
+                             //        [3] max level, [4..] per-level counts
+    DBuf sorted_pts;         // sorted point (slot) ids, in-box first then dropped
+    DBuf sx, sy, sz;         // FP64 positions in sorted order
+    DBuf sqx, sqy, sqz;      // FP64 pwrapped segment vectors in sorted order
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ovg_params params{};
+    bool have_params = false;
+    int n = 0;            // slots
+    int n_live = 0;       // non-EMPTY slots
+    int n_src = 0;        // non-EMPTY slots with a forward neighbour
+    int timing = 0;
+
+    // state, SoA over slots
+    DBuf x[3], vels[3], vs[3], tan[3], nor[3];
+    DBuf fwd, rev, status;
+    DBuf live, src;                 // slot lists
+    // RK4 scratch
+    DBuf xs[2][3], move[3], k[3], stan[3], snor[3];
+    // Biot-Savart partials
+    DBuf bs_prim[3], bs_img[3], partial;
+    DBuf seg[6];                    // direct-sum sources: r1 xyz, pwrapped r2 xyz
+    DBuf seg_ids;                   // int2 (slot, fwd) per source
+    DBuf tparams;                   // TangleParams[1]
+    DBuf flags;                     // int32 device flags ([0] error bits)
+    DBuf counter64;                 // uint64 interaction counters
+    TreeBufs tree;
+    // reconnection / smoothing scratch
+    DBuf rc_cell, rc_sorted, rc_start, rc_pairs, rc_count, rc_tmp;
+    DBuf sm_buf;
+
+    std::vector<int> h_conn, h_status;  // host mirrors of topology
+    uint64_t topo_version = 1;          // bumped whenever links change
+    uint64_t loops_version = 0;         // topology the smoothing loop table matches
+    int n_loops = 0;
+    DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
+    ovg_stats stats{};
+    hipEvent_t ev[8] = {};
+};
+
+// ---- kernel launchers (one translation unit each) ------------------------
+// All take the context stream; none synchronises.
+
+// k_local.hip
+void launch_stencil(Ctx &c, const double *const x[3], double *const tan[3],
+                    double *const nor[3]);
+void launch_prep_segments(Ctx &c, const double *const x[3]);
+void launch_compose(Ctx &c, const double *const x[3], bool own_tangents, int stage,
+                    double dt);
+void launch_stage_positions(Ctx &c, const double *const kv[3], double h,
+                            double *const out[3]);
+void launch_wrap(Ctx &c);
+void launch_line_length(Ctx &c, double *d_out);
+void launch_check_finite(Ctx &c, const double *const x[3]);
+
+// k_direct.hip
+void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
+                   double *const out_prim[3], double *const out_img[3], int chunks);
+
+// k_tree.hip
+int build_tree(Ctx &c, const double *const x[3]);
+
+// k_bh.hip
+void launch_bh(Ctx &c, const double *const x[3]);
+
+// k_recon.hip
+int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);
+
+// k_smooth.hip
+int smooth_loops(Ctx &c);
+
+// engine.hip helpers
+TangleParams host_tparams(const ovg_params &p, double t);
+void push_params(Ctx &c, double t);
+void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos);
+void launch_split_pairs(Ctx &c, const int *pairs);
+
+// k_recon.hip: candidate scan + greedy pass; links updated on the device
+int reconnect_pass(Ctx &c, double t, int *n_rec);
+
+// k_tree.hip: stable LSD radix sort of c.tree.keys[0]/vals[0] over the
+// 8-bit digits at first_shift..last_shift (an even number of passes)
+void radix_sort(Ctx &c, int n, int first_shift, int last_shift);
+
+// scan helper (k_tree.hip): exclusive scan of n int32 values, in -> out,
+// writes the total to *d_total when non-null.
+void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
+                           DBuf &tmp);
+
+inline const TangleParams *dparams(const Ctx &c) { return c.tparams.as<TangleParams>(); }
+
+}  // namespace ovg

@@ -1,0 +1,154 @@
+// Host-side FP64 formulas for the sequential part of the reconnection pass
+// (src/vortex_dynamics.c:224-284): tangents of the two candidate points,
+// their velocity composition and the length test.  Same operation order as
+// the device kernels; the library is built with -ffp-contract=off.
+#pragma once
+#include <cmath>
+
+#include "device_common.h"
+
+namespace ovg {
+namespace host {
+
+inline double dot3(const double *u, const double *v)
+{
+    double o = u[0] * v[0];
+    o = o + u[1] * v[1];
+    o = o + u[2] * v[2];
+    return o;
+}
+inline double len3(const double *u) { return std::sqrt(dot3(u, u)); }
+inline double dist3(const double *u, const double *v)
+{
+    double d[3] = {u[0] - v[0], u[1] - v[1], u[2] - v[2]};
+    return len3(d);
+}
+inline void cross3(double *r, const double *u, const double *v)
+{
+    double a0 = u[1] * v[2] - u[2] * v[1];
+    double a1 = u[2] * v[0] - u[0] * v[2];
+    double a2 = u[0] * v[1] - u[1] * v[0];
+    r[0] = a0; r[1] = a1; r[2] = a2;
+}
+
+// seg_pwrap, src/boundaries.c:54-83
+inline void pwrap(const TangleParams &P, const double *r1, double *o)
+{
+    for (int a = 0; a < 3; ++a) {
+        if (!P.periodic[a]) continue;
+        int g = 0;
+        while ((o[a] - r1[a]) > P.halfL[a] && g++ < 64) o[a] -= P.L[a];
+        while ((o[a] - r1[a]) < -P.halfL[a] && g++ < 128) o[a] += P.L[a];
+    }
+}
+
+// segment_field body with R = r1 - r, Rp = r2 - r (src/tangle.c:540-559)
+inline bool seg_field(const double *R, const double *Rp, double f_bs, double *v)
+{
+    double lR = len3(R), lP = len3(Rp);
+    double dot = dot3(R, Rp);
+    if (lR < 1e-8 || lP < 1e-8) return false;
+    if (dot / lR / lP < 1e-8) return false;
+    double denom = lR * lP * (lR * lP + dot);
+    double c[3];
+    cross3(c, R, Rp);
+    double m = f_bs * (lR + lP) / denom;
+    for (int q = 0; q < 3; ++q) v[q] = c[q] * m;
+    return true;
+}
+
+// Direct Biot-Savart of segment m (forward f) at point p (slot pk), all
+// images; the primary box skips the two LIA segments (src/tangle.c:658).
+inline void segment_all_images(const TangleParams &P, const double *X, int m, int f, int pk,
+                               double *acc)
+{
+    const double *xm = X + 3 * (size_t)m;
+    double r2[3] = {X[3 * (size_t)f], X[3 * (size_t)f + 1], X[3 * (size_t)f + 2]};
+    pwrap(P, xm, r2);
+    const double *r = X + 3 * (size_t)pk;
+    for (int im = -1; im < P.n_images; ++im) {
+        if (im < 0 && (m == pk || f == pk)) continue;
+        double q[3] = {r[0], r[1], r[2]};
+        if (im >= 0)
+            for (int a = 0; a < 3; ++a) q[a] = r[a] - P.Ls[im][a];
+        double R[3] = {xm[0] - q[0], xm[1] - q[1], xm[2] - q[2]};
+        double Rp[3] = {r2[0] - q[0], r2[1] - q[1], r2[2] - q[2]};
+        double v[3];
+        if (seg_field(R, Rp, P.f_bs, v))
+            for (int a = 0; a < 3; ++a) acc[a] += v[a];
+    }
+}
+
+// update_tangent_normal, src/tangle.c:423-502, with FREE links
+inline void tangent_normal(const TangleParams &P, const double *X, const int *conn, int k,
+                           double *t, double *nrm)
+{
+    int i1 = conn[2 * k], i2 = conn[2 * i1], im1 = conn[2 * k + 1], im2 = conn[2 * im1 + 1];
+    const double *s0 = X + 3 * (size_t)k, *s1 = X + 3 * (size_t)i1, *s2 = X + 3 * (size_t)i2;
+    const double *m1 = X + 3 * (size_t)im1, *m2 = X + 3 * (size_t)im2;
+    double w[4][3];
+    const double *ends[4] = {s2, s1, m1, m2};
+    for (int j = 0; j < 4; ++j) {
+        for (int a = 0; a < 3; ++a) w[j][a] = ends[j][a];
+        pwrap(P, s0, w[j]);
+    }
+    double w12[3] = {s2[0], s2[1], s2[2]}, wm12[3] = {m2[0], m2[1], m2[2]};
+    pwrap(P, s1, w12);
+    pwrap(P, m1, wm12);
+    double ds[4][3];
+    for (int j = 0; j < 4; ++j)
+        for (int a = 0; a < 3; ++a) ds[j][a] = w[j][a] - s0[a];
+    double d1 = dist3(s0, w[1]);
+    double d2 = d1 + dist3(s1, w12);
+    double dm1 = dist3(s0, w[2]);
+    double dm2 = dm1 + dist3(m1, wm12);
+    double dsd[4] = {d2 * (d2 - d1) * (dm1 + d2) * (dm2 + d2),
+                     d1 * (d2 - d1) * (dm1 + d1) * (dm2 + d1),
+                     dm1 * (dm1 + d1) * (dm1 + d2) * (dm2 - dm1),
+                     dm2 * (dm2 + d1) * (dm2 + d2) * (dm2 - dm1)};
+    double c1[4] = {-d1 * dm1 * dm2, d2 * dm1 * dm2, -d1 * d2 * dm2, d1 * d2 * dm1};
+    double c2[4] = {2 * ((dm1 - d1) * dm2 - d1 * dm1), -2 * ((dm1 - d2) * dm2 - d2 * dm1),
+                    2 * ((d2 + d1) * dm2 - d1 * d2), -2 * ((d2 + d1) * dm1 - d1 * d2)};
+    for (int i = 0; i < 3; ++i) {
+        double tt = 0, nn = 0;
+        for (int z = 0; z < 4; ++z) {
+            tt += c1[z] / dsd[z] * ds[z][i];
+            nn += c2[z] / dsd[z] * ds[z][i];
+        }
+        t[i] = tt;
+        nrm[i] = nn;
+    }
+}
+
+// update_velocity composition (src/tangle.c:617-716) given the Biot-Savart
+// sum bs at slot k.
+inline void compose(const TangleParams &P, const double *X, const int *conn, int k,
+                    const double *t, const double *nr, const double *bs, double *vs,
+                    double *vels)
+{
+    const double *p = X + 3 * (size_t)k;
+    const double *next = X + 3 * (size_t)conn[2 * k];
+    const double *prev = X + 3 * (size_t)conn[2 * k + 1];
+    double wf[3] = {next[0], next[1], next[2]}, wr[3] = {p[0], p[1], p[2]};
+    pwrap(P, p, wf);
+    pwrap(P, prev, wr);
+    double l_next = dist3(p, wf), l_prev = dist3(prev, wr);
+    double f = P.kappa * std::log(2 * std::sqrt(l_next * l_prev) / P.sqrtE / P.a0) / 4 / M_PI;
+    double c[3];
+    cross3(c, t, nr);
+    for (int a = 0; a < 3; ++a) vs[a] = c[a] * f;
+    for (int a = 0; a < 3; ++a) vs[a] = vs[a] + P.evs[a];
+    for (int a = 0; a < 3; ++a) vs[a] = vs[a] + bs[a];
+    for (int a = 0; a < 3; ++a) vels[a] = vs[a];
+    if (P.use_mf) {
+        double dv[3], m[3], nn[3];
+        for (int a = 0; a < 3; ++a) dv[a] = P.evn[a] - vs[a];
+        cross3(m, t, dv);
+        for (int a = 0; a < 3; ++a) vels[a] = vels[a] + m[a] * P.alpha;
+        cross3(nn, t, m);
+        for (int a = 0; a < 3; ++a) vels[a] = vels[a] + nn[a] * -P.alpha_p;
+    }
+}
+
+}  // namespace host
+}  // namespace ovg

@@ -1,0 +1,415 @@
+// Reconnection (src/vortex_dynamics.c:128-288).
+//
+// Candidate scan on the GPU: points are binned into a periodic grid whose
+// cells are at least rec_dist wide (at most 256 per axis), sorted by cell
+// with the radix sort of k_tree.hip, and each point k tests the points l > k
+// of its 27 neighbouring cells with the reference predicate: l not a
+// neighbour of k and |seg_pwrap(x_k, x_l)| <= rec_dist, in FP64 with the
+// reference's operation order, so the candidate list is bit-identical to
+// the reference's all-pairs scan (:197-222) before any reconnection.
+//
+// The greedy pass then runs on the host in (k, l) order, as the reference
+// (:224-284): points flagged by an earlier reconnection are skipped (their
+// links may have changed; every other point keeps its original links, so
+// the precomputed candidate list stays exact), tangents are recomputed with
+// the current links, the approach test uses direct-sum velocities, and the
+// length test decides the rewiring (do_reconnection, :128-175).  The
+// direct sums for all candidate endpoints come from one GPU launch; after
+// each rewiring the host adds the exact change of the two segments whose
+// forward link changed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine_internal.h"
+#include "host_phys.h"
+
+namespace ovg {
+
+struct Grid {
+    int nc[3];
+    double cs[3];
+};
+
+static Grid make_grid(const ovg_params &p)
+{
+    Grid g;
+    for (int a = 0; a < 3; ++a) {
+        double L = p.box_hi[a] - p.box_lo[a];
+        int n = (int)std::floor(L / p.rec_dist);
+        if (n > 256) n = 256;
+        if (n < 3) n = 1;  // too few cells to separate neighbours: one cell
+        g.nc[a] = n;
+        g.cs[a] = L / n;
+    }
+    return g;
+}
+
+__device__ __forceinline__ int cell_coord(double x, double lo, double cs, int nc)
+{
+    int c = (int)floor((x - lo) / cs);
+    c %= nc;
+    if (c < 0) c += nc;
+    return c;
+}
+
+__global__ void k_cell_keys(int n_live, const int *__restrict__ live, const double *__restrict__ X,
+                            const double *__restrict__ Y, const double *__restrict__ Z,
+                            const TangleParams *__restrict__ Pp, Grid g,
+                            uint64_t *__restrict__ keys, int *__restrict__ vals)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    const TangleParams &P = *Pp;
+    int k = live[i];
+    int cx = cell_coord(X[k], P.lo[0], g.cs[0], g.nc[0]);
+    int cy = cell_coord(Y[k], P.lo[1], g.cs[1], g.nc[1]);
+    int cz = cell_coord(Z[k], P.lo[2], g.cs[2], g.nc[2]);
+    keys[i] = ((uint64_t)cz * g.nc[1] + cy) * g.nc[0] + cx;
+    vals[i] = k;
+}
+
+__global__ void k_fill_int(int n, int *a, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_cell_ranges(int n_live, const uint64_t *__restrict__ keys,
+                              int *__restrict__ start, int *__restrict__ end)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    int c = (int)keys[i];
+    if (i == 0 || (int)keys[i - 1] != c) start[c] = i;
+    if (i == n_live - 1 || (int)keys[i + 1] != c) end[c] = i + 1;
+}
+
+// COUNT: counts[i] = candidates of point live-index i; else write them at
+// offs[i] sorted by l.
+template <bool COUNT>
+__global__ void k_candidates(int n_live, const int *__restrict__ live, const int *__restrict__ sorted,
+                             const uint64_t *__restrict__ keys_unused, const double *__restrict__ X,
+                             const double *__restrict__ Y, const double *__restrict__ Z,
+                             const int *__restrict__ fwd, const int *__restrict__ rev,
+                             const TangleParams *__restrict__ Pp, Grid g, double rec_dist,
+                             const int *__restrict__ start, const int *__restrict__ end,
+                             int *__restrict__ counts, const int *__restrict__ offs,
+                             int *__restrict__ out_k, int *__restrict__ out_l, int max_pairs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    const TangleParams &P = *Pp;
+    int k = live[i];
+    double xk = X[k], yk = Y[k], zk = Z[k];
+    int fk = fwd[k], rk = rev[k];
+    int cx = cell_coord(xk, P.lo[0], g.cs[0], g.nc[0]);
+    int cy = cell_coord(yk, P.lo[1], g.cs[1], g.nc[1]);
+    int cz = cell_coord(zk, P.lo[2], g.cs[2], g.nc[2]);
+    int rx0 = g.nc[0] >= 3 ? -1 : 0, rx1 = g.nc[0] >= 3 ? 1 : 0;
+    int ry0 = g.nc[1] >= 3 ? -1 : 0, ry1 = g.nc[1] >= 3 ? 1 : 0;
+    int rz0 = g.nc[2] >= 3 ? -1 : 0, rz1 = g.nc[2] >= 3 ? 1 : 0;
+    int cnt = 0;
+    int base = COUNT ? 0 : offs[i];
+    for (int dz = rz0; dz <= rz1; ++dz)
+        for (int dy = ry0; dy <= ry1; ++dy)
+            for (int dx = rx0; dx <= rx1; ++dx) {
+                int ax = (cx + dx + g.nc[0]) % g.nc[0];
+                int ay = (cy + dy + g.nc[1]) % g.nc[1];
+                int az = (cz + dz + g.nc[2]) % g.nc[2];
+                int c = (az * g.nc[1] + ay) * g.nc[0] + ax;
+                int s0 = start[c], s1 = end[c];
+                for (int s = s0; s < s1; ++s) {
+                    int l = sorted[s];
+                    if (l <= k || l == fk || l == rk) continue;
+                    double wx = X[l], wy = Y[l], wz = Z[l];
+                    pwrap(P, xk, yk, zk, wx, wy, wz);
+                    double ex = xk - wx, ey = yk - wy, ez = zk - wz;
+                    double dist = sqrt(dot3(ex, ey, ez, ex, ey, ez));
+                    if (dist > rec_dist) continue;
+                    if (!COUNT) {
+                        // insertion into this point's slice, sorted by l
+                        int pos = base + cnt;
+                        while (pos > base && out_l[pos - 1] > l) {
+                            if (pos < max_pairs) {
+                                out_l[pos] = out_l[pos - 1];
+                                out_k[pos] = k;
+                            }
+                            --pos;
+                        }
+                        if (pos < max_pairs) {
+                            out_l[pos] = l;
+                            out_k[pos] = k;
+                        }
+                    }
+                    ++cnt;
+                }
+            }
+    if (COUNT) counts[i] = cnt;
+}
+
+// Candidate pairs into c.rc_pairs (k array then l array); returns count.
+static int candidates_device(Ctx &c, int *n_out)
+{
+    int n = c.n_live;
+    *n_out = 0;
+    if (n == 0) return OVG_OK;
+    Grid g = make_grid(c.params);
+    int ncell = g.nc[0] * g.nc[1] * g.nc[2];
+    TreeBufs &T = c.tree;
+    for (int b = 0; b < 2; ++b) {
+        OVG_TRY(T.keys[b].ensure(sizeof(uint64_t) * (n + 1)));
+        OVG_TRY(T.vals[b].ensure(sizeof(int) * (n + 1)));
+    }
+    OVG_TRY(c.rc_start.ensure(sizeof(int) * 2 * (size_t)ncell));
+    OVG_TRY(c.rc_count.ensure(sizeof(int) * (n + 1)));
+    OVG_TRY(c.rc_cell.ensure(sizeof(int) * (n + 2)));
+    int bs = 256, gs = (n + bs - 1) / bs;
+    const double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
+    hipLaunchKernelGGL(k_cell_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), X, Y,
+                       Z, dparams(c), g, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
+    // sort by cell id (< 2^24): four 8-bit passes (an even count)
+    radix_sort(c, n, 0, 24);
+    int *start = c.rc_start.as<int>(), *end = start + ncell;
+    hipLaunchKernelGGL(k_fill_int, dim3((2 * ncell + 255) / 256), dim3(256), 0, c.stream,
+                       2 * ncell, start, 0);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), start, end);
+    hipLaunchKernelGGL(k_candidates<true>, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
+                       T.vals[0].as<int>(), T.keys[0].as<uint64_t>(), X, Y, Z, c.fwd.as<int>(),
+                       c.rev.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
+                       c.rc_count.as<int>(), nullptr, nullptr, nullptr, 0);
+    int *offs = c.rc_cell.as<int>();
+    launch_exclusive_scan(c, c.rc_count.as<int>(), offs, n, offs + n, c.rc_tmp);
+    int total = 0;
+    OVG_HIP(hipMemcpyAsync(&total, offs + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    OVG_TRY(c.rc_pairs.ensure(sizeof(int) * 2 * (size_t)(total + 1)));
+    int *pk = c.rc_pairs.as<int>(), *pl = pk + total;
+    hipLaunchKernelGGL(k_candidates<false>, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
+                       T.vals[0].as<int>(), T.keys[0].as<uint64_t>(), X, Y, Z, c.fwd.as<int>(),
+                       c.rev.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
+                       c.rc_count.as<int>(), offs, pk, pl, total);
+    *n_out = total;
+    return OVG_OK;
+}
+
+int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs)
+{
+    int total = 0;
+    OVG_TRY(candidates_device(c, &total));
+    if (n_pairs) *n_pairs = total;
+    int m = total < max_pairs ? total : max_pairs;
+    if (m > 0 && h_pairs) {
+        std::vector<int> kk(total), ll(total);
+        const int *pk = c.rc_pairs.as<int>();
+        OVG_HIP(hipMemcpyAsync(kk.data(), pk, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost,
+                               c.stream));
+        OVG_HIP(hipMemcpyAsync(ll.data(), pk + total, sizeof(int) * (size_t)total,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < m; ++i) {
+            h_pairs[2 * i] = kk[i];
+            h_pairs[2 * i + 1] = ll[i];
+        }
+    } else {
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (total > max_pairs)
+        return set_error(OVG_ELIMIT, "%d candidate pairs exceed max_pairs %d", total, max_pairs);
+    return OVG_OK;
+}
+
+}  // namespace ovg
+
+using namespace ovg;
+
+// do_reconnection, src/vortex_dynamics.c:128-175, on the host link mirror.
+// Returns 0 (no rewiring) or 1/2 for the two rewiring cases.
+static int do_reconnection(const TangleParams &P, const double *X, int *conn, int k, int l)
+{
+    int kf = conn[2 * k], kr = conn[2 * k + 1], lf = conn[2 * l], lr = conn[2 * l + 1];
+    auto plen = [&](int a, int b) {
+        double w[3] = {X[3 * (size_t)b], X[3 * (size_t)b + 1], X[3 * (size_t)b + 2]};
+        host::pwrap(P, X + 3 * (size_t)a, w);
+        return host::dist3(X + 3 * (size_t)a, w);
+    };
+    double l_kl = plen(k, l), l_kflr = plen(kf, lr), l_kkf = plen(k, kf), l_llr = plen(l, lr);
+    double l_krlf = plen(kr, lf), l_kkr = plen(k, kr), l_llf = plen(l, lf);
+    double cf1 = l_kl + l_kflr - l_kkf - l_llr;
+    double cf2 = l_kl + l_krlf - l_kkr - l_llf;
+    if (cf1 > 0 && cf2 > 0) return 0;
+    if (cf1 < cf2) {
+        conn[2 * k] = l;
+        conn[2 * l + 1] = k;
+        conn[2 * kf + 1] = lr;
+        conn[2 * lr] = kf;
+        return 1;
+    }
+    conn[2 * l] = k;
+    conn[2 * k + 1] = l;
+    conn[2 * lf + 1] = kr;
+    conn[2 * kr] = lf;
+    return 2;
+}
+
+namespace ovg {
+
+__global__ void k_gather3(int m, const int *__restrict__ idx, const double *__restrict__ a,
+                          const double *__restrict__ b, const double *__restrict__ c,
+                          double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int k = idx[i];
+    out[3 * i] = a[k];
+    out[3 * i + 1] = b[k];
+    out[3 * i + 2] = c[k];
+}
+
+int reconnect_pass(Ctx &c, double t, int *n_rec)
+{
+    *n_rec = 0;
+    int total = 0;
+    OVG_TRY(candidates_device(c, &total));
+    if (total == 0) return OVG_OK;
+    std::vector<int> ck(total), cl(total);
+    const int *pk = c.rc_pairs.as<int>();
+    OVG_HIP(hipMemcpyAsync(ck.data(), pk, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost,
+                           c.stream));
+    OVG_HIP(hipMemcpyAsync(cl.data(), pk + total, sizeof(int) * (size_t)total,
+                           hipMemcpyDeviceToHost, c.stream));
+    // positions (host copy, AoS)
+    int n = c.n;
+    std::vector<double> X(3 * (size_t)n);
+    {
+        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)n));
+        launch_soa_to_aos(c, c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>(),
+                          c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(X.data(), c.partial.p, sizeof(double) * 3 * (size_t)n,
+                               hipMemcpyDeviceToHost, c.stream));
+    }
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    TangleParams P = host_tparams(c.params, t);
+    // direct Biot-Savart at every candidate endpoint with the original links
+    std::vector<int> ends;
+    ends.reserve(2 * (size_t)total);
+    for (int i = 0; i < total; ++i) {
+        ends.push_back(ck[i]);
+        ends.push_back(cl[i]);
+    }
+    std::sort(ends.begin(), ends.end());
+    ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+    int m = (int)ends.size();
+    std::vector<int> where(n, -1);
+    for (int i = 0; i < m; ++i) where[ends[i]] = i;
+    std::vector<double> bsp(3 * (size_t)m), bsi(3 * (size_t)m);
+    {
+        OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)m));
+        OVG_HIP(hipMemcpyAsync(c.rc_tmp.p, ends.data(), sizeof(int) * (size_t)m,
+                               hipMemcpyHostToDevice, c.stream));
+        const double *xv[3] = {c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>()};
+        push_params(c, t);
+        launch_prep_segments(c, xv);
+        double *op[3] = {c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
+                         c.bs_prim[2].as<double>()};
+        double *oi[3] = {c.bs_img[0].as<double>(), c.bs_img[1].as<double>(),
+                         c.bs_img[2].as<double>()};
+        ovg_params saved = c.params;
+        c.params.precision = OVG_PREC_FP64;
+        launch_direct(c, xv, m, c.rc_tmp.as<int>(), op, oi, 0);
+        c.params = saved;
+        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)m));
+        hipLaunchKernelGGL(k_gather3, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,
+                           c.rc_tmp.as<int>(), op[0], op[1], op[2], c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(bsp.data(), c.partial.p, sizeof(double) * 3 * (size_t)m,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        hipLaunchKernelGGL(k_gather3, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,
+                           c.rc_tmp.as<int>(), oi[0], oi[1], oi[2], c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(bsi.data(), c.partial.p, sizeof(double) * 3 * (size_t)m,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    // greedy pass in (k, l) order
+    std::vector<int> &conn = c.h_conn;
+    std::vector<int> flag(n, 0);
+    struct Change {
+        int seg, old_f, new_f;
+    };
+    std::vector<Change> changes;
+    const double cutoff = std::cos(c.params.rec_angle);
+    auto velocity = [&](int p, double *tn, double *nr, double *vels) {
+        int e = where[p];
+        double bs[3];
+        for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
+        // exact change of every segment rewired so far
+        for (const Change &ch : changes) {
+            double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+            host::segment_all_images(P, X.data(), ch.seg, ch.old_f, p, vo);
+            host::segment_all_images(P, X.data(), ch.seg, ch.new_f, p, vn);
+            for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
+        }
+        double vs[3];
+        host::compose(P, X.data(), conn.data(), p, tn, nr, bs, vs, vels);
+    };
+    int nrec = 0;
+    int i = 0;
+    while (i < total) {
+        int k = ck[i];
+        int iend = i;
+        while (iend < total && ck[iend] == k) ++iend;
+        if (flag[k]) { i = iend; continue; }
+        for (int j = i; j < iend; ++j) {
+            int l = cl[j];
+            if (flag[l]) continue;
+            double w[3] = {X[3 * (size_t)l], X[3 * (size_t)l + 1], X[3 * (size_t)l + 2]};
+            host::pwrap(P, X.data() + 3 * (size_t)k, w);
+            double tk[3], nk[3], tl[3], nl[3];
+            host::tangent_normal(P, X.data(), conn.data(), k, tk, nk);
+            host::tangent_normal(P, X.data(), conn.data(), l, tl, nl);
+            if (host::len3(tk) < 0.5 || host::len3(tl) < 0.5) continue;
+            double calpha = host::dot3(tk, tl) / host::len3(tk) / host::len3(tl);
+            if (calpha > cutoff) continue;
+            double vk[3], vl[3];
+            velocity(k, tk, nk, vk);
+            velocity(l, tl, nl, vl);
+            double dx[3], dv[3];
+            for (int a = 0; a < 3; ++a) dx[a] = X[3 * (size_t)k + a] - w[a];
+            for (int a = 0; a < 3; ++a) dv[a] = vk[a] - vl[a];
+            if (host::dot3(dx, dv) > 0) continue;
+            int knext = conn[2 * k], kprev = conn[2 * k + 1];
+            int lnext = conn[2 * l], lprev = conn[2 * l + 1];
+            int kind = do_reconnection(P, X.data(), conn.data(), k, l);
+            if (!kind) continue;
+            if (kind == 1) {
+                changes.push_back({k, knext, l});
+                changes.push_back({lprev, l, knext});
+            } else {
+                changes.push_back({l, lnext, k});
+                changes.push_back({kprev, k, lnext});
+            }
+            flag[k]++; flag[conn[2 * k]]++; flag[conn[2 * k + 1]]++;
+            flag[knext]++; flag[kprev]++;
+            flag[l]++; flag[conn[2 * l]]++; flag[conn[2 * l + 1]]++;
+            flag[lnext]++; flag[lprev]++;
+            ++nrec;
+            break;
+        }
+        i = iend;
+    }
+    *n_rec = nrec;
+    if (nrec) {
+        // upload the new links
+        OVG_TRY(c.partial.ensure(sizeof(int) * 2 * (size_t)n));
+        OVG_HIP(hipMemcpyAsync(c.partial.p, conn.data(), sizeof(int) * 2 * (size_t)n,
+                               hipMemcpyHostToDevice, c.stream));
+        launch_split_pairs(c, c.partial.as<int>());
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        c.topo_version++;
+    }
+    return OVG_OK;
+}
+
+}  // namespace ovg

@@ -1,0 +1,569 @@
+// Barnes-Hut octree build (src/octree.c:26-286) on the GPU.
+//
+// 1. k_keys: each live point descends from the root box (= domain box,
+//    src/octree.c:150) through 21 levels using the reference's own child-box
+//    arithmetic (make_child_box, src/octree.c:165-232) and its first-match
+//    rule over the closed child boxes in the order BLF BLB BRF BRB TLF TLB
+//    TRF TRB (src/octree.c:116-127).  The child indices form a 63-bit path
+//    key, most significant level first, so key order is the reference's
+//    traversal order.  Points outside the root box get key ~0 (kept in the
+//    root's sums, absent from every child: src/octree.c:116-128).
+// 2. A stable LSD radix sort of (key, slot), 8-bit digits, written here:
+//    per-tile LDS histograms, one exclusive scan, then a scatter that ranks
+//    equal digits with 64-lane ballots.
+// 3. Compressed octree in DFS pre-order: the reference splits every cell
+//    with more than one point (src/octree.c:112) and only descends into
+//    nonempty children, so a chain of cells holding the same point set
+//    collapses into its deepest cell without changing any acceptance
+//    decision (Lm shrinks along the chain, com and circulation do not
+//    change).  For sorted keys with adjacent common-prefix levels delta[i],
+//    the nodes that start at point i are the distinct running minima of
+//    delta[i..] above delta[i-1], plus i's leaf; a node's DFS index is
+//    (nodes starting before i) + (rank among nodes starting at i) and its
+//    skip rope is the first node starting after its range.
+// 4. Sums (centre of mass, circulation) bottom-up by cell level: a node
+//    adds its children's sums, a leaf its own point.
+// HBM/latency-bound; no MFMA.
+#include "engine_internal.h"
+
+namespace ovg {
+
+constexpr uint64_t kDropKey = ~0ULL;
+
+// make_child_box, src/octree.c:165-232, exact arithmetic per child index.
+__device__ __forceinline__ void child_box(const double lo[3], const double hi[3], int idx,
+                                          double nlo[3], double nhi[3])
+{
+    double c[3], h[3];
+    for (int k = 0; k < 3; ++k) {
+        c[k] = lo[k] + hi[k];
+        c[k] = c[k] * 0.5;
+        h[k] = (hi[k] - lo[k]) / 2;
+    }
+    switch (idx) {
+    case 0:
+        nlo[0] = lo[0] + h[0]; nlo[1] = lo[1]; nlo[2] = lo[2];
+        nhi[0] = c[0] + h[0]; nhi[1] = c[1]; nhi[2] = c[2];
+        break;
+    case 1:
+        nlo[0] = lo[0]; nlo[1] = lo[1]; nlo[2] = lo[2];
+        nhi[0] = c[0]; nhi[1] = c[1]; nhi[2] = c[2];
+        break;
+    case 2:
+        nlo[0] = c[0]; nlo[1] = c[1]; nlo[2] = c[2] - h[2];
+        nhi[0] = hi[0]; nhi[1] = hi[1]; nhi[2] = hi[2] - h[2];
+        break;
+    case 3:
+        nlo[0] = lo[0]; nlo[1] = lo[1] + h[1]; nlo[2] = lo[2];
+        nhi[0] = c[0]; nhi[1] = c[1] + h[1]; nhi[2] = c[2];
+        break;
+    case 4:
+        nlo[0] = c[0]; nlo[1] = c[1] - h[1]; nlo[2] = c[2];
+        nhi[0] = hi[0]; nhi[1] = hi[1] - h[1]; nhi[2] = hi[2];
+        break;
+    case 5:
+        nlo[0] = lo[0]; nlo[1] = lo[1]; nlo[2] = lo[2] + h[2];
+        nhi[0] = c[0]; nhi[1] = c[1]; nhi[2] = c[2] + h[2];
+        break;
+    case 6:
+        nlo[0] = c[0]; nlo[1] = c[1]; nlo[2] = c[2];
+        nhi[0] = hi[0]; nhi[1] = hi[1]; nhi[2] = hi[2];
+        break;
+    default:
+        nlo[0] = c[0] - h[0]; nlo[1] = c[1]; nlo[2] = c[2];
+        nhi[0] = hi[0] - h[0]; nhi[1] = hi[1]; nhi[2] = hi[2];
+        break;
+    }
+}
+
+__device__ __forceinline__ bool in_closed(const double lo[3], const double hi[3], double x,
+                                          double y, double z)
+{
+    return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
+}
+
+__global__ void k_keys(int n_live, const int *__restrict__ live, const double *__restrict__ X,
+                       const double *__restrict__ Y, const double *__restrict__ Z,
+                       const TangleParams *__restrict__ Pp, uint64_t *__restrict__ keys,
+                       int *__restrict__ vals, int *__restrict__ counters)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    const TangleParams &P = *Pp;
+    int k = live[i];
+    double x = X[k], y = Y[k], z = Z[k];
+    double lo[3] = {P.lo[0], P.lo[1], P.lo[2]}, hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
+    uint64_t key = kDropKey;
+    if (in_closed(lo, hi, x, y, z)) {
+        key = 0;
+        int fallbacks = 0;
+        for (int lev = 1; lev <= kMaxLevel; ++lev) {
+            // fast path: the point is far (relative to the box) from every
+            // split plane, so exactly one child can hold it
+            double c0 = (lo[0] + hi[0]) * 0.5, c1 = (lo[1] + hi[1]) * 0.5,
+                   c2 = (lo[2] + hi[2]) * 0.5;
+            double m0 = (hi[0] - lo[0]) * 1e-6, m1 = (hi[1] - lo[1]) * 1e-6,
+                   m2 = (hi[2] - lo[2]) * 1e-6;
+            int digit = -1;
+            if (fabs(x - c0) > m0 && fabs(y - c1) > m1 && fabs(z - c2) > m2 &&
+                fabs(x - lo[0]) > m0 && fabs(x - hi[0]) > m0 && fabs(y - lo[1]) > m1 &&
+                fabs(y - hi[1]) > m1 && fabs(z - lo[2]) > m2 && fabs(z - hi[2]) > m2) {
+                int xh = x > c0, yh = y > c1, zh = z > c2;
+                digit = (xh ? 0 : 1) | (yh << 1) | (zh << 2);
+            } else {
+                double nlo[3], nhi[3];
+                for (int ch = 0; ch < 8; ++ch) {
+                    child_box(lo, hi, ch, nlo, nhi);
+                    if (in_closed(nlo, nhi, x, y, z)) { digit = ch; break; }
+                }
+                if (digit < 0) {  // within an ulp of a face both boxes miss
+                    int xh = x >= c0, yh = y > c1, zh = z > c2;
+                    digit = (xh ? 0 : 1) | (yh << 1) | (zh << 2);
+                    ++fallbacks;
+                }
+            }
+            double nlo[3], nhi[3];
+            child_box(lo, hi, digit, nlo, nhi);
+            for (int q = 0; q < 3; ++q) { lo[q] = nlo[q]; hi[q] = nhi[q]; }
+            key |= (uint64_t)digit << (63 - 3 * lev);
+        }
+        atomicAdd(&counters[0], 1);
+        if (fallbacks) atomicAdd(&counters[2], fallbacks);
+    }
+    keys[i] = key;
+    vals[i] = k;
+}
+
+// ---------------------------------------------------------------- scan ----
+constexpr int kScanThreads = 256, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
+
+__device__ int block_exclusive_scan(int v, int *sh, int &total)
+{
+    // Hillis-Steele over 256 threads in LDS
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        int a = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    int incl = sh[t];
+    total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// in and out may alias (in-place scan): every thread reads its items before
+// the block scan's barriers and writes them after.
+__global__ void k_scan_tile(const int *in, int *out, int n, int *__restrict__ tile_sums)
+{
+    __shared__ int sh[kScanThreads];
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = base + j < n ? in[base + j] : 0;
+        s += v[j];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, sh, total);
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the tile sums in place, total to *d_total
+__global__ void k_scan_top(int *__restrict__ sums, int m, int *__restrict__ d_total)
+{
+    __shared__ int sh[kScanThreads];
+    int carry = 0;
+    for (int b0 = 0; b0 < m; b0 += kScanThreads) {
+        int i = b0 + threadIdx.x;
+        int v = i < m ? sums[i] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, sh, total);
+        if (i < m) sums[i] = ex + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+__global__ void k_scan_add(int *out, int n, const int *__restrict__ sums)
+{
+    int base = blockIdx.x * kScanTile;
+    int add = sums[blockIdx.x];
+    for (int j = threadIdx.x; j < kScanTile; j += kScanThreads)
+        if (base + j < n) out[base + j] += add;
+}
+
+void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
+                           DBuf &tmp)
+{
+    if (n <= 0) {
+        if (d_total) hipMemsetAsync(d_total, 0, sizeof(int), c.stream);
+        return;
+    }
+    int tiles = (n + kScanTile - 1) / kScanTile;
+    tmp.ensure(sizeof(int) * (size_t)tiles);
+    hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_in, d_out,
+                       n, tmp.as<int>());
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, c.stream, tmp.as<int>(),
+                       tiles, d_total);
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_out, n,
+                       tmp.as<int>());
+}
+
+// ---------------------------------------------------------- radix sort ----
+constexpr int kRsThreads = 256, kRsRounds = 8, kRsTile = kRsThreads * kRsRounds;
+
+__global__ void k_rs_hist(const uint64_t *__restrict__ keys, int n, int shift,
+                          int *__restrict__ hist, int nblocks)
+{
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsRounds; ++r) {
+        int i = base + r * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
+    int *__restrict__ vout, int n, int shift, const int *__restrict__ offs, int nblocks)
+{
+    __shared__ int running[256];
+    __shared__ int wcnt[kRsThreads / 64][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    running[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+    for (int q = 0; q < kRsThreads / 64; ++q) wcnt[q][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    int base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsRounds; ++r) {
+        int i = base + r * kRsThreads + threadIdx.x;
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0;
+        int val = valid ? vin[i] : 0;
+        int digit = valid ? (int)((key >> shift) & 255) : 0;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            uint64_t m = __ballot((digit >> b) & 1);
+            peers &= ((digit >> b) & 1) ? m : ~m;
+        }
+        int rank = __popcll(peers & below);
+        int leader = __ffsll((unsigned long long)peers) - 1;
+        if (valid && lane == leader) wcnt[w][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = running[digit] + rank;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][digit];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int q = 0; q < kRsThreads / 64; ++q) {
+            add += wcnt[q][threadIdx.x];
+            wcnt[q][threadIdx.x] = 0;
+        }
+        running[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// Sorts c.tree.keys[0]/vals[0] (n entries) in place (result back in [0]).
+void radix_sort(Ctx &c, int n, int first_shift, int last_shift)
+{
+    TreeBufs &T = c.tree;
+    int nblocks = (n + kRsTile - 1) / kRsTile;
+    T.hist.ensure(sizeof(int) * 256 * (size_t)nblocks);
+    int src = 0;
+    for (int shift = first_shift; shift <= last_shift; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c.stream,
+                           T.keys[src].as<uint64_t>(), n, shift, T.hist.as<int>(), nblocks);
+        launch_exclusive_scan(c, T.hist.as<int>(), T.hist.as<int>(), 256 * nblocks, nullptr,
+                              T.scan_tmp);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c.stream,
+                           T.keys[src].as<uint64_t>(), T.vals[src].as<int>(),
+                           T.keys[1 - src].as<uint64_t>(), T.vals[1 - src].as<int>(), n, shift,
+                           T.hist.as<int>(), nblocks);
+        src = 1 - src;
+    }
+    if (src != 0) {
+        hipMemcpyAsync(T.keys[0].p, T.keys[1].p, sizeof(uint64_t) * n,
+                       hipMemcpyDeviceToDevice, c.stream);
+        hipMemcpyAsync(T.vals[0].p, T.vals[1].p, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                       c.stream);
+    }
+}
+
+// ----------------------------------------------------------- topology ----
+__device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b)
+{
+    if (a == b) return kMaxLevel;
+    return (__clzll((long long)(a ^ b)) - 1) / 3;
+}
+
+// sorted gather: slot ids, positions and pwrapped segment vectors in sorted
+// order; delta[i] = common levels of sorted keys i and i+1 (in-box part).
+__global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
+                                const int *__restrict__ vals, const double *__restrict__ X,
+                                const double *__restrict__ Y, const double *__restrict__ Z,
+                                const int *__restrict__ fwd, const TangleParams *__restrict__ Pp,
+                                const int *__restrict__ counters, int *__restrict__ spts,
+                                double *__restrict__ sx, double *__restrict__ sy,
+                                double *__restrict__ sz, double *__restrict__ qx,
+                                double *__restrict__ qy, double *__restrict__ qz,
+                                int8_t *__restrict__ delta)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    const TangleParams &P = *Pp;
+    int n_in = counters[0];
+    int k = vals[i];
+    spts[i] = k;
+    double x = X[k], y = Y[k], z = Z[k];
+    sx[i] = x; sy[i] = y; sz[i] = z;
+    int f = fwd[k];
+    double ax = 0, ay = 0, az = 0;
+    if (f >= 0) {  // octree_update_means circulation term, src/octree.c:276-281
+        double bx = X[f], by = Y[f], bz = Z[f];
+        pwrap(P, x, y, z, bx, by, bz);
+        ax = bx - x; ay = by - y; az = bz - z;
+    }
+    qx[i] = ax; qy[i] = ay; qz[i] = az;
+    if (i + 1 < n_in) delta[i] = (int8_t)lcp_levels(keys[i], keys[i + 1]);
+}
+
+// last j in [i+1, n_in-1] with lcp(K[i], K[j]) >= v (at least i+1 qualifies)
+__device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int i, int n_in,
+                                         int v)
+{
+    uint64_t ki = keys[i];
+    int lo = i + 1, hi = n_in - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (lcp_levels(ki, keys[mid]) >= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Walk the nodes starting at sorted point i; EMIT=false only counts them.
+template <bool EMIT>
+__global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
+                        const int8_t *__restrict__ delta, const int *__restrict__ counters,
+                        int *__restrict__ counts, const int *__restrict__ base,
+                        Node *__restrict__ nodes, NodeMeta *__restrict__ meta,
+                        const TangleParams *__restrict__ Pp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_live) return;
+    int n_in = counters[0];
+    if (i >= n_in) {  // dropped points start no node
+        if (!EMIT) counts[i] = i == 0 ? 1 : 0;
+        if (EMIT && i == 0) {
+            // no in-box point at all: the root is the only node
+            const TangleParams &P = *Pp;
+            Node r{};
+            r.count = n_live;
+            r.skip = base[n_live];
+            double Lx = P.hi[0] - P.lo[0], Ly = P.hi[1] - P.lo[1], Lz = P.hi[2] - P.lo[2];
+            r.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+            nodes[0] = r;
+            NodeMeta m{};
+            m.start = 0; m.level = 0;
+            meta[0] = m;
+        }
+        return;
+    }
+    int dl = i == 0 ? -1 : delta[i - 1];
+    int dr = i == n_in - 1 ? -1 : delta[i];
+    int cnt = 0;
+    int d = EMIT ? base[i] : 0;
+    int total = EMIT ? base[n_live] : 0;
+    const TangleParams &P = *Pp;
+    uint64_t ki = keys[i];
+    auto emit = [&](int level, int end, int count) {
+        if (EMIT) {
+            Node nd{};
+            nd.count = count;
+            int nxt = end + 1;
+            nd.skip = nxt < n_in ? base[nxt] : total;
+            if (count >= 2 && level >= 1) {
+                // replay the reference's child boxes down to `level`
+                double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
+                double hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
+                int lv = level < kMaxLevel ? level : kMaxLevel;
+                for (int l = 1; l <= lv; ++l) {
+                    int dg = (int)((ki >> (63 - 3 * l)) & 7);
+                    double nlo[3], nhi[3];
+                    child_box(lo, hi, dg, nlo, nhi);
+                    for (int q = 0; q < 3; ++q) { lo[q] = nlo[q]; hi[q] = nhi[q]; }
+                }
+                double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
+                nd.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+            }
+            nodes[d] = nd;
+            NodeMeta m{};
+            m.start = i;
+            m.level = level;
+            meta[d] = m;
+            ++d;
+        }
+        ++cnt;
+    };
+    if (i == 0) {
+        // explicit root: the domain box with every live point (src/octree.c:141-150)
+        if (EMIT) {
+            Node r{};
+            r.count = n_live;
+            r.skip = total;
+            double Lx = P.hi[0] - P.lo[0], Ly = P.hi[1] - P.lo[1], Lz = P.hi[2] - P.lo[2];
+            r.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+            nodes[d] = r;
+            NodeMeta m{};
+            m.start = 0; m.level = 0;
+            meta[d] = m;
+            ++d;
+        }
+        ++cnt;
+    }
+    int v = dl + 1;
+    if (v < 1) v = 1;
+    while (v <= dr) {
+        int e = range_end(keys, i, n_in, v);
+        int w = lcp_levels(ki, keys[e]);
+        emit(w, e, e - i + 1);
+        v = w + 1;
+    }
+    int leaf = (dl > dr ? dl : dr) + 1;
+    if (leaf < 1) leaf = 1;
+    emit(leaf, i, 1);
+    if (!EMIT) counts[i] = cnt;
+}
+
+// Sums bottom-up for the nodes of one cell level: a leaf takes its point,
+// an internal node adds its children (d+1, then along skip ropes); the
+// root also adds the points outside the root box.  Sums stay in cx..qz
+// until k_finish_means turns cx..cz into centres of mass.
+__global__ void k_sums_level(int level, const int *__restrict__ counters,
+                             Node *__restrict__ nodes, const NodeMeta *__restrict__ meta,
+                             const double *__restrict__ sx, const double *__restrict__ sy,
+                             const double *__restrict__ sz, const double *__restrict__ qx,
+                             const double *__restrict__ qy, const double *__restrict__ qz,
+                             int n_live)
+{
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    int n_nodes = counters[1];
+    if (d >= n_nodes) return;
+    if (meta[d].level != level) return;
+    Node nd = nodes[d];
+    int n_in = counters[0];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    if (d == 0) {
+        for (int c = 1; c < nd.skip; c = nodes[c].skip) {
+            s[0] += nodes[c].cx; s[1] += nodes[c].cy; s[2] += nodes[c].cz;
+            s[3] += nodes[c].qx; s[4] += nodes[c].qy; s[5] += nodes[c].qz;
+        }
+        for (int j = n_in; j < n_live; ++j) {
+            s[0] += sx[j]; s[1] += sy[j]; s[2] += sz[j];
+            s[3] += qx[j]; s[4] += qy[j]; s[5] += qz[j];
+        }
+    } else if (nd.count == 1) {
+        int j = meta[d].start;
+        s[0] = sx[j]; s[1] = sy[j]; s[2] = sz[j];
+        s[3] = qx[j]; s[4] = qy[j]; s[5] = qz[j];
+    } else {
+        for (int c = d + 1; c < nd.skip; c = nodes[c].skip) {
+            s[0] += nodes[c].cx; s[1] += nodes[c].cy; s[2] += nodes[c].cz;
+            s[3] += nodes[c].qx; s[4] += nodes[c].qy; s[5] += nodes[c].qz;
+        }
+    }
+    nd.cx = s[0]; nd.cy = s[1]; nd.cz = s[2];
+    nd.qx = s[3]; nd.qy = s[4]; nd.qz = s[5];
+    nodes[d] = nd;
+}
+
+// com = sum * (1.0/N), src/octree.c:269; leaves keep x exactly.
+__global__ void k_finish_means(const int *__restrict__ counters, Node *__restrict__ nodes)
+{
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= counters[1]) return;
+    Node &nd = nodes[d];
+    if (nd.count > 1) {
+        double inv = 1.0 / nd.count;
+        nd.cx = nd.cx * inv;
+        nd.cy = nd.cy * inv;
+        nd.cz = nd.cz * inv;
+    }
+}
+
+__global__ void k_tree_counters(int n_live, const int *__restrict__ base, int *__restrict__ counters)
+{
+    counters[1] = base[n_live];
+}
+
+int build_tree(Ctx &c, const double *const x[3])
+{
+    TreeBufs &T = c.tree;
+    int n = c.n_live;
+    size_t cap_nodes = 2 * (size_t)n + 2;
+    for (int b = 0; b < 2; ++b) {
+        OVG_TRY(T.keys[b].ensure(sizeof(uint64_t) * (n + 1)));
+        OVG_TRY(T.vals[b].ensure(sizeof(int) * (n + 1)));
+    }
+    OVG_TRY(T.delta.ensure(n + 1));
+    OVG_TRY(T.counts.ensure(sizeof(int) * (n + 1)));
+    OVG_TRY(T.base.ensure(sizeof(int) * (n + 2)));
+    OVG_TRY(T.nodes.ensure(sizeof(Node) * cap_nodes));
+    OVG_TRY(T.meta.ensure(sizeof(NodeMeta) * cap_nodes));
+    OVG_TRY(T.counters.ensure(sizeof(int) * 64));
+    OVG_TRY(T.sorted_pts.ensure(sizeof(int) * (n + 1)));
+    DBuf *sv[6] = {&T.sx, &T.sy, &T.sz, &T.sqx, &T.sqy, &T.sqz};
+    for (auto *b : sv) OVG_TRY(b->ensure(sizeof(double) * (n + 1)));
+    hipMemsetAsync(T.counters.p, 0, sizeof(int) * 64, c.stream);
+    if (n == 0) return OVG_OK;
+    int bs = 256, gs = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
+                       x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
+                       T.counters.as<int>());
+    radix_sort(c, n, 0, 56);
+    hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), T.vals[0].as<int>(), x[0], x[1], x[2],
+                       c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
+                       T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
+                       T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
+                       T.delta.as<int8_t>());
+    hipLaunchKernelGGL(k_nodes<false>, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
+                       T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
+                       T.meta.as<NodeMeta>(), dparams(c));
+    // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
+    launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n, T.base.as<int>() + n,
+                          T.scan_tmp);
+    hipLaunchKernelGGL(k_nodes<true>, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
+                       T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
+                       T.meta.as<NodeMeta>(), dparams(c));
+    hipLaunchKernelGGL(k_tree_counters, dim3(1), dim3(1), 0, c.stream, n, T.base.as<int>(),
+                       T.counters.as<int>());
+    int gn = (int)((cap_nodes + bs - 1) / bs);
+    for (int level = kMaxLevel + 1; level >= 0; --level)
+        hipLaunchKernelGGL(k_sums_level, dim3(gn), dim3(bs), 0, c.stream, level,
+                           T.counters.as<int>(), T.nodes.as<Node>(), T.meta.as<NodeMeta>(),
+                           T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
+                           T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(), n);
+    hipLaunchKernelGGL(k_finish_means, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+                       T.nodes.as<Node>());
+    return OVG_OK;
+}
+
+}  // namespace ovg
