@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""RK4 point-steps/s of the vortex-filament hot path on one MI355X.
+
+Workload (default "c3"): 4000 seeded rings in the periodic cube D = 0.2 cm,
+radii U[0.005, 0.05] cm, point spacing 5e-4 cm (1.39M points), T = 1.65 K
+(alpha 0.11, alpha' 0.014), counterflow v_ns = 1.5 cm/s, Barnes-Hut theta 0.4
+with the 26 periodic images, dt = 5e-5 s.  One step follows the reference
+driver's order (src/main.c:127-138): reconnect, small-loop removal,
+smooth (every 10th step), periodic wrap, remesh (dl_min = dxi/2,
+dl_max = 1.5 dxi), small-loop removal, update_tangle, rk4_step (3 more
+evaluations), periodic wrap.  State stays on the device between steps; the timed region
+starts with the tangle resident in HBM.
+
+The fast path is mixed precision (FP32 far-field pairs, FP64 sums,
+positions, LIA and RK4) with the wavefront-group traversal and buckets of
+16; the plain path is the reference's FP64 split-until-one tree with
+per-point traversal, timed on the same tangle beside it.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 point-steps/s at 1.4M pts θ=0.4; p50/p99 step latency vs plain FP64"
+FLOPS_PER_PAIR = 44  # 41 add/sub/mul + 2 sqrt + 1 div per segment or monopole pair
+PEAK_FP32 = 157.3e12  # MI355X vector FP32, FLOP/s
+PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
+    """Run n_steps device-resident steps in the order of src/main.c:127-138;
+    returns per-step seconds."""
+    lat = []
+    tm = t0
+    for s in range(n_steps):
+        k = step0 + s
+        a = time.perf_counter()
+        if reconnect:
+            e.reconnect(tm)
+            e.eliminate_small_loops(5)
+        if smooth_every and k % smooth_every == 0:
+            e.smooth()
+        e.enforce_periodic()
+        if remesh:
+            e.remesh(*remesh)
+            e.eliminate_small_loops(5)
+        e.update_tangle(tm)
+        e.rk4_step(tm, dt)
+        e.enforce_periodic()
+        e.synchronize()
+        lat.append(time.perf_counter() - a)
+        tm += dt
+    return lat, tm
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v) * 1e3, q))
+
+
+def cpu_baseline(workload, rings, threads, steps):
+    """The reference (oracle/_ref/ovref_bench, built with the reference
+    Makefile's flags) on a bounded sample of the same workload."""
+    from oracle.ovoracle import REF_BENCH
+    from openvort_amd import configs
+    if not os.path.exists(REF_BENCH):
+        return None
+    t, p, dt = configs.make(workload, rings=rings)
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+        f.write(np.int32(t.n).tobytes())
+        f.write(bytes(p))
+        f.write(t.vnodes.tobytes())
+        f.write(t.conn.tobytes())
+        f.write(t.status.tobytes())
+        path = f.name
+    env = dict(os.environ, LSAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS=str(threads))
+    try:
+        out = subprocess.run([REF_BENCH, path, str(threads), str(steps), repr(dt), "1"],
+                             capture_output=True, text=True, env=env, timeout=600)
+    finally:
+        os.unlink(path)
+    if out.returncode != 0:
+        log(f"cpu baseline failed: {out.stderr[-400:]}")
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--traverse", default="group", choices=["group", "point"])
+    ap.add_argument("--rec-velocity", default="direct", choices=["direct", "tree"])
+    ap.add_argument("--smooth-every", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-rings", type=int, default=62)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}); see DESIGN.md for how it would shard.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import openvort_amd as ov
+    from openvort_amd import configs
+
+    prec = ov.PREC_MIXED if args.precision == "mixed" else ov.PREC_FP64
+    trav = ov.TRAVERSE_GROUP if args.traverse == "group" else ov.TRAVERSE_POINT
+    recv = ov.RECVEL_DIRECT if args.rec_velocity == "direct" else ov.RECVEL_TREE
+    t, p, dt = configs.make(args.workload, precision=prec, traverse=trav, leaf_size=args.leaf)
+    p.rec_velocity = recv
+    n_pts = len(t.live)
+    log(f"workload {args.workload}: {n_pts} points, {t.n} slots")
+
+    # ---- fast path ------------------------------------------------------
+    e = ov.Engine(p)
+    e.upload(t)
+    e.synchronize()
+    rb = configs.remesh_bounds(args.workload)
+    lat_w, tm = run_steps(e, args.warmup, 0.0, dt, 0, args.smooth_every, True, rb)
+    e.stats()  # reset counters
+    e.set_timing(True)
+    e.synchronize()
+    t0 = time.perf_counter()
+    lat, tm = run_steps(e, args.steps, tm, dt, args.warmup, args.smooth_every, True, rb)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    st = e.stats()
+    L, n_after = e.line_length()
+    e.close()
+    value = n_pts * args.steps / wall
+    log(f"fast: {wall / args.steps * 1e3:.1f} ms/step, {value:.4g} point-steps/s")
+
+    launches = max(st.bs_launches, 1)
+    bs_ms = st.ms_bs / launches
+    inter_per_launch = st.interactions / launches
+    achieved = inter_per_launch * FLOPS_PER_PAIR / (bs_ms * 1e-3) if bs_ms > 0 else 0.0
+    peak = PEAK_FP32 if prec == ov.PREC_MIXED else PEAK_FP64
+    interactions_per_s = st.interactions / wall
+
+    # ---- plain FP64 path ------------------------------------------------
+    plain = None
+    if args.plain_steps > 0:
+        q = configs.make(args.workload)[1]
+        q.method, q.precision, q.traverse, q.leaf_size = (ov.METHOD_BH, ov.PREC_FP64,
+                                                          ov.TRAVERSE_POINT, 1)
+        q.rec_velocity = recv
+        e2 = ov.Engine(q)
+        e2.upload(t)
+        e2.synchronize()
+        run_steps(e2, 1, 0.0, dt, 0, args.smooth_every, True, rb)
+        a = time.perf_counter()
+        lat2, _ = run_steps(e2, args.plain_steps, dt, dt, 1, args.smooth_every, True, rb)
+        w2 = time.perf_counter() - a
+        e2.close()
+        plain = {"value": n_pts * args.plain_steps / w2, "steps": args.plain_steps,
+                 "ms_per_step": w2 / args.plain_steps * 1e3,
+                 "p50_step_ms": pct(lat2, 50), "p99_step_ms": pct(lat2, 99),
+                 "path": "FP64, reference split-until-one tree, per-point traversal"}
+        log(f"plain FP64: {plain['ms_per_step']:.1f} ms/step")
+
+    # ---- CPU baseline ---------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        r = cpu_baseline(args.workload, args.cpu_rings, args.cpu_threads, args.cpu_steps)
+        if r:
+            cpu = {"value": r["point_steps_per_s"], "unit": "point-steps/s",
+                   "cores": r["threads"], "kind": "reference",
+                   "sample": (f"first {args.cpu_rings} rings of the {args.workload} stream "
+                              f"({r['points']} points, same box and parameters), "
+                              f"{r['steps']} full steps incl. the O(N^2) reconnect "
+                              f"({r['reconnect_s']:.1f} s of {r['seconds']:.1f} s); "
+                              f"reference src/ built with its Makefile flags "
+                              f"(-march=x86-64-v3)"),
+                   "seconds": r["seconds"]}
+            log(f"cpu: {cpu['value']:.4g} point-steps/s on {r['points']} points")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "point-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == ov.PREC_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": (f"{args.workload}: seeded ring tangle, {n_pts} points, "
+                                f"BH theta {p.theta}, {p.n_images} periodic images, "
+                                f"dt {dt}, reconnect every step, smooth every "
+                                f"{args.smooth_every} steps"),
+                   "points": n_pts, "precision": args.precision, "traverse": args.traverse,
+                   "leaf_size": args.leaf, "rec_velocity": args.rec_velocity},
+        "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
+        "interactions_per_s": interactions_per_s,
+        "direct_equivalent_interactions_per_s": 27.0 * n_pts * n_pts * 4 * args.steps / wall,
+        "line_length_cm": L,
+        "plain_fp64": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_bh_group<mixed>" if trav == ov.TRAVERSE_GROUP and
+                     prec == ov.PREC_MIXED else "k_bh", "avg_launch_ms": bs_ms,
+                     "pairs_per_launch": inter_per_launch,
+                     "fp32_pair_fraction": st.interactions_fp32 / max(st.interactions, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

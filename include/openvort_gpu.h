@@ -72,6 +72,10 @@ extern "C" {
 #define OVG_TRAVERSE_GROUP  1 /* one wavefront per 64 Morton-adjacent points and
                                  one image; same per-point acceptance test       */
 
+/* velocity source of the reconnection approach test */
+#define OVG_RECVEL_DIRECT  0
+#define OVG_RECVEL_TREE    1
+
 /* External (prescribed) flow, include/external_velocity.h:11-19. */
 typedef struct ovg_flow {
     int32_t type;                 /* OVG_FLOW_*                               */
@@ -112,7 +116,12 @@ typedef struct ovg_params {
     double  freq_to_cutoff;       /* smoothing: fraction of modes removed     */
     int32_t rk4_classic;          /* 0: reference stage 4 at x+dt/2*k3
                                      (src/vortex_dynamics.c:90); 1: x+dt*k3    */
-    int32_t pad1_;
+    int32_t rec_velocity;         /* velocities of the reconnection approach
+                                     test: OVG_RECVEL_DIRECT (the reference's
+                                     update_velocity with tree == NULL,
+                                     src/vortex_dynamics.c:250-251) or
+                                     OVG_RECVEL_TREE (this context's
+                                     Barnes-Hut evaluator)                     */
     double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
                                      than this (cm) stay FP64; 0 = none        */
 } ovg_params;
@@ -159,6 +168,15 @@ int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes);
 /* Upload only connections (same n), e.g. after a host-side topology edit. */
 int ovg_upload_connections(ovg_ctx *ctx, const int *connections, const int *status);
 
+/* Upload one per-slot double[3*n] field of the current tangle (same n):
+ * the velocities used as k1 by ovg_rk4_step, or the tangents and normals
+ * used by ovg_update_velocities / ovg_velocity_at. */
+#define OVG_FIELD_VELS      0
+#define OVG_FIELD_VS        1
+#define OVG_FIELD_TANGENTS  2
+#define OVG_FIELD_NORMALS   3
+int ovg_upload_field(ovg_ctx *ctx, int field, const double *values);
+
 /* Copy device state back; any pointer may be NULL.  All are double[3*n]
  * or int[2*n] in the layouts above. */
 int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *vs,
@@ -203,6 +221,22 @@ int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pai
  * updated connections stay on the device.
  */
 int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections);
+
+/* eliminate_small_loops, src/tangle.c:1063-1128, for closed loops: a loop
+ * of m points is deleted when m - 1 < cutoff (the reference's count;
+ * small_loop_cutoff = 5, src/initial_conditions.c:43).  Runs on the host
+ * link mirror and re-uploads the topology. */
+int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
+
+/* remesh, src/tangle.c:1018-1061: remove a point whose forward or reverse
+ * segment is shorter than min_dist when both together are shorter than
+ * max_dist; insert a point on the fitted circle (add_point, :261-335) after
+ * every point whose forward segment is longer than max_dist; wrap into the
+ * box if points were added.  New points take the slots the reference's
+ * allocator would (src/tangle.c:201-253), doubling the slot count when full
+ * (expand_tangle, :68-101); ovg_num_slots reports the new count.  Runs on
+ * the host copy of the state and re-uploads it. */
+int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
 
 /* smooth, src/vortex_smooth.c:459-462 (low_pass_filter per loop and
  * coordinate, src/vortex_smooth.c:134-175,183-453). */

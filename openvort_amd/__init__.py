@@ -31,6 +31,8 @@ FLOW_NONE, FLOW_SIMPLE, FLOW_DECREASING = 0, 1, 2
 METHOD_DIRECT, METHOD_BH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 TRAVERSE_POINT, TRAVERSE_GROUP = 0, 1
+FIELD_VELS, FIELD_VS, FIELD_TANGENTS, FIELD_NORMALS = 0, 1, 2, 3
+RECVEL_DIRECT, RECVEL_TREE = 0, 1
 
 
 class Flow(C.Structure):
@@ -52,7 +54,7 @@ class Params(C.Structure):
         ("leaf_size", C.c_int32), ("traverse", C.c_int32),
         ("rec_dist", C.c_double), ("rec_angle", C.c_double),
         ("freq_to_cutoff", C.c_double), ("rk4_classic", C.c_int32),
-        ("pad1_", C.c_int32), ("mixed_near", C.c_double),
+        ("rec_velocity", C.c_int32), ("mixed_near", C.c_double),
     ]
 
 
@@ -87,6 +89,7 @@ _sig("ovg_set_params", C.c_int, C.c_void_p, _P(Params))
 _sig("ovg_get_params", C.c_int, C.c_void_p, _P(Params))
 _sig("ovg_upload", C.c_int, C.c_void_p, C.c_int, _dp, _ip, _ip)
 _sig("ovg_upload_positions", C.c_int, C.c_void_p, _dp)
+_sig("ovg_upload_field", C.c_int, C.c_void_p, C.c_int, _dp)
 _sig("ovg_upload_connections", C.c_int, C.c_void_p, _ip, _ip)
 _sig("ovg_download", C.c_int, C.c_void_p, _dp, _dp, _dp, _dp, _dp)
 _sig("ovg_download_connections", C.c_int, C.c_void_p, _ip, _ip)
@@ -99,6 +102,8 @@ _sig("ovg_enforce_periodic", C.c_int, C.c_void_p)
 _sig("ovg_reconnect_candidates", C.c_int, C.c_void_p, C.c_int, _ip, _ip)
 _sig("ovg_reconnect", C.c_int, C.c_void_p, C.c_double, _ip)
 _sig("ovg_smooth", C.c_int, C.c_void_p)
+_sig("ovg_eliminate_small_loops", C.c_int, C.c_void_p, C.c_int, _ip)
+_sig("ovg_remesh", C.c_int, C.c_void_p, C.c_double, C.c_double, _ip, _ip)
 _sig("ovg_velocity_at", C.c_int, C.c_void_p, C.c_int, _ip, C.c_double, _dp, _dp)
 _sig("ovg_line_length", C.c_int, C.c_void_p, _dp, _ip)
 _sig("ovg_get_stats", C.c_int, C.c_void_p, _P(Stats))
@@ -249,6 +254,10 @@ class Engine:
         v = np.ascontiguousarray(v, dtype=np.float64)
         _check(_lib.ovg_upload_positions(self._h, _d(v)), "ovg_upload_positions")
 
+    def upload_field(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(_lib.ovg_upload_field(self._h, field, _d(v)), "ovg_upload_field")
+
     def download(self):
         out = {k: np.zeros((self.n, 3)) for k in ("vnodes", "vels", "vs", "tangents", "normals")}
         _check(_lib.ovg_download(self._h, _d(out["vnodes"]), _d(out["vels"]), _d(out["vs"]),
@@ -287,6 +296,18 @@ class Engine:
         n = C.c_int32(0)
         _check(_lib.ovg_reconnect(self._h, t, C.byref(n)), "ovg_reconnect")
         return n.value
+
+    def eliminate_small_loops(self, cutoff=5):
+        n = C.c_int32(0)
+        _check(_lib.ovg_eliminate_small_loops(self._h, cutoff, C.byref(n)),
+               "ovg_eliminate_small_loops")
+        return n.value
+
+    def remesh(self, min_dist, max_dist):
+        a, r = C.c_int32(0), C.c_int32(0)
+        _check(_lib.ovg_remesh(self._h, min_dist, max_dist, C.byref(a), C.byref(r)), "ovg_remesh")
+        self.n = _lib.ovg_num_slots(self._h)
+        return a.value, r.value
 
     def smooth(self):
         _check(_lib.ovg_smooth(self._h), "ovg_smooth")

@@ -61,3 +61,9 @@ def make(name, rings=None, **kw):
         kw.setdefault("method", ov.METHOD_DIRECT)
     p = params(D, dxi, ap, vns, **kw)
     return t, p, dt
+
+
+def remesh_bounds(name):
+    """(dl_min, dl_max) = (dxi/2, 1.5 dxi) of a config."""
+    dxi = CONFIGS[name][5]
+    return dxi / 2, 1.5 * dxi

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "engine_internal.h"
+#include "host_phys.h"
 
 struct ovg_ctx {
     ovg::Ctx c;
@@ -183,37 +184,67 @@ static Triple tri(DBuf *b)
     return Triple{{b[0].as<double>(), b[1].as<double>(), b[2].as<double>()}};
 }
 
-static void timer_begin(Ctx &c, int i)
+static hipEvent_t take_event(Ctx &c)
 {
-    if (c.timing) hipEventRecord(c.ev[i], c.stream);
+    hipEvent_t e = nullptr;
+    if (!c.ev_pool.empty()) {
+        e = c.ev_pool.back();
+        c.ev_pool.pop_back();
+    } else {
+        hipEventCreate(&e);
+    }
+    return e;
+}
+
+static void mark(Ctx &c, hipEvent_t *slot)
+{
+    if (!c.timing) return;
+    *slot = take_event(c);
+    hipEventRecord(*slot, c.stream);
+}
+
+// Sum the pending (start, tree, Biot-Savart) event triples into c.stats.
+static void resolve_events(Ctx &c)
+{
+    if (c.ev_pending.empty()) return;
+    hipStreamSynchronize(c.stream);
+    for (size_t i = 0; i + 2 < c.ev_pending.size(); i += 3) {
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, c.ev_pending[i], c.ev_pending[i + 1]);
+        hipEventElapsedTime(&b, c.ev_pending[i + 1], c.ev_pending[i + 2]);
+        c.stats.ms_tree += a;
+        c.stats.ms_bs += b;
+        c.stats.bs_launches += 1;
+    }
+    for (auto e : c.ev_pending) c.ev_pool.push_back(e);
+    c.ev_pending.clear();
 }
 
 // Biot-Savart part of one evaluation at positions x: writes bs_prim,
 // bs_img by slot.
-static int eval_bs(Ctx &c, const double *const x[3])
+int eval_bs(Ctx &c, const double *const x[3])
 {
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    mark(c, &e0);
     if (c.params.method == OVG_METHOD_BH) {
-        timer_begin(c, 0);
         OVG_TRY(build_tree(c, x));
-        timer_begin(c, 1);
+        mark(c, &e1);
         launch_bh(c, x);
-        timer_begin(c, 2);
     } else {
-        timer_begin(c, 0);
         launch_prep_segments(c, x);
-        timer_begin(c, 1);
+        mark(c, &e1);
         Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
         launch_direct(c, x, c.n_live, c.live.as<int>(), bp.v, bi.v, 0);
-        timer_begin(c, 2);
+        // every (target, segment, image) pair except the two LIA segments
+        double pairs = (double)c.n_live * c.n_src * (1 + c.params.n_images) - 2.0 * c.n_live;
+        c.stats.interactions += pairs;
+        if (c.params.precision == OVG_PREC_MIXED) c.stats.interactions_fp32 += pairs;
     }
+    mark(c, &e2);
     if (c.timing) {
-        hipEventSynchronize(c.ev[2]);
-        float a = 0, b = 0;
-        hipEventElapsedTime(&a, c.ev[0], c.ev[1]);
-        hipEventElapsedTime(&b, c.ev[1], c.ev[2]);
-        c.stats.ms_tree += a;
-        c.stats.ms_bs += b;
-        c.stats.bs_launches += 1;
+        c.ev_pending.push_back(e0);
+        c.ev_pending.push_back(e1);
+        c.ev_pending.push_back(e2);
     }
     return OVG_OK;
 }
@@ -221,6 +252,7 @@ static int eval_bs(Ctx &c, const double *const x[3])
 // Counters accumulate from one ovg_get_stats call to the next.
 static int fetch_stats(Ctx &c, ovg_stats *out)
 {
+    resolve_events(c);
     unsigned long long cnt[2] = {0, 0};
     int tc[4] = {0, 0, 0, 0};
     OVG_HIP(hipMemcpyAsync(cnt, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
@@ -284,7 +316,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
-    DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.seg_ids,
+    DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
                    &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf};
     for (auto *b : all) b->release();
@@ -303,6 +335,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     for (auto *b : tb) b->release();
     for (auto &e : c.ev)
         if (e) hipEventDestroy(e);
+    for (auto e : c.ev_pending) hipEventDestroy(e);
+    for (auto e : c.ev_pool) hipEventDestroy(e);
     if (c.stream) hipStreamDestroy(c.stream);
     delete ctx;
     return OVG_OK;
@@ -459,6 +493,7 @@ extern "C" int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes, const int *
     hipSetDevice(c.device);
     OVG_TRY(alloc_state(c, n));
     c.n = n;
+    c.next_empty = 0;
     OVG_TRY(upload_positions(c, vnodes));
     OVG_TRY(upload_topology(c, connections, status));
     for (int q = 0; q < 3; ++q) {
@@ -488,6 +523,31 @@ extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, cons
     OVG_TRY(validate_topology(c.n, connections, status));
     hipSetDevice(c.device);
     return upload_topology(c, connections, status);
+}
+
+extern "C" int ovg_upload_field(ovg_ctx *ctx, int field, const double *values)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    if (!values) return set_error(OVG_EINVAL, "values is NULL");
+    Ctx &c = ctx->c;
+    DBuf *dst;
+    switch (field) {
+    case OVG_FIELD_VELS: dst = c.vels; break;
+    case OVG_FIELD_VS: dst = c.vs; break;
+    case OVG_FIELD_TANGENTS: dst = c.tan; break;
+    case OVG_FIELD_NORMALS: dst = c.nor; break;
+    default: return set_error(OVG_EINVAL, "unknown field %d", field);
+    }
+    hipSetDevice(c.device);
+    int n = c.n;
+    OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)n));
+    OVG_HIP(hipMemcpyAsync(c.partial.p, values, sizeof(double) * 3 * (size_t)n,
+                           hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                       c.partial.as<double>(), dp(dst[0]), dp(dst[1]), dp(dst[2]));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
 }
 
 static int check_flags(Ctx &c)
@@ -741,4 +801,198 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return smooth_loops(c);
+}
+
+// eliminate_small_loops, src/tangle.c:1063-1128 (closed loops)
+extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    int n = c.n;
+    std::vector<int> &conn = c.h_conn, &st = c.h_status;
+    std::vector<char> visited(n, 0);
+    int removed = 0;
+    for (int k = 0; k < n; ++k) {
+        if (st[2 * k] == OVG_EMPTY || visited[k]) continue;
+        visited[k] = 1;
+        int loop = 0, here = k, next = conn[2 * k];
+        while (next != k) {
+            visited[here] = 1;
+            here = next;
+            next = conn[2 * here];
+            loop++;
+        }
+        if (loop < cutoff) {
+            next = here;
+            for (;;) {
+                int tmp = next;
+                next = conn[2 * next];
+                conn[2 * tmp] = -1;
+                conn[2 * tmp + 1] = -1;
+                st[2 * tmp] = OVG_EMPTY;
+                removed++;
+                if (next == here || next < 0) break;
+            }
+        }
+    }
+    if (n_removed) *n_removed = removed;
+    if (removed) {
+        hipSetDevice(c.device);
+        std::vector<int> cc(conn), ss(st);
+        OVG_TRY(upload_topology(c, cc.data(), ss.data()));
+    }
+    return OVG_OK;
+}
+
+// ---- remesh (src/tangle.c:1018-1061) on the host copy ---------------------
+namespace {
+
+struct HostTangle {
+    int N;
+    std::vector<double> X;
+    std::vector<int> conn, st;
+    int next_empty;
+
+    // search_next_empty_node, src/tangle.c:201-234 (a read past the last
+    // slot counts as not EMPTY)
+    int search()
+    {
+        if (next_empty > 0) {
+            if (next_empty < N && st[2 * next_empty] == OVG_EMPTY) {
+                int r = next_empty;
+                if (next_empty + 1 < N && st[2 * (next_empty + 1)] == OVG_EMPTY) ++next_empty;
+                else next_empty = -1;
+                return r;
+            }
+        }
+        for (int k = 0; k < N; ++k)
+            if (st[2 * k] == OVG_EMPTY) {
+                next_empty = (k + 1 < N && st[2 * (k + 1)] == OVG_EMPTY) ? k + 1 : -1;
+                return k;
+            }
+        return -1;
+    }
+    // get_next_empty_node :241-253 with expand_tangle :68-101
+    int next_empty_node()
+    {
+        int idx = search();
+        while (idx < 0 || st[2 * idx] != OVG_EMPTY) {
+            int old = N;
+            N = 2 * old;
+            X.resize(3 * (size_t)N, 0.0);
+            conn.resize(2 * (size_t)N, -1);
+            st.resize(2 * (size_t)N, -1);
+            next_empty = search();
+            idx = search();
+        }
+        return idx;
+    }
+};
+
+// add_point, src/tangle.c:261-335
+int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
+{
+    using namespace ovg::host;
+    int nw = h.next_empty_node();
+    h.st[2 * nw] = OVG_FREE;
+    h.st[2 * nw + 1] = -1;
+    int next = h.conn[2 * p];
+    double t0[3], n0[3], t1[3], n1[3];
+    tangent_normal(P, h.X.data(), h.conn.data(), p, t0, n0);
+    tangent_normal(P, h.X.data(), h.conn.data(), next, t1, n1);
+    double s0[3] = {h.X[3 * (size_t)p], h.X[3 * (size_t)p + 1], h.X[3 * (size_t)p + 2]};
+    double s1[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1], h.X[3 * (size_t)next + 2]};
+    pwrap(P, s0, s1);
+    double l = dist3(s0, s1);
+    double nv[3], nwp[3];
+    for (int c = 0; c < 3; ++c) nv[c] = n0[c] + n1[c];
+    for (int c = 0; c < 3; ++c) nv[c] = nv[c] * 0.5;
+    if (len3(nv) > 1e-5) {
+        double R = 1 / len3(nv);
+        double dR = R * R - l * l / 4;
+        double delta = dR > 0 ? R - std::sqrt(dR) : R;
+        double ln = len3(nv);
+        for (int c = 0; c < 3; ++c) nv[c] = nv[c] * (1 / ln);
+        for (int c = 0; c < 3; ++c) nv[c] = nv[c] * -1;
+        double a[3], b[3];
+        for (int c = 0; c < 3; ++c) a[c] = s0[c] + s1[c];
+        for (int c = 0; c < 3; ++c) a[c] = a[c] * 0.5;
+        for (int c = 0; c < 3; ++c) b[c] = nv[c] * delta;
+        for (int c = 0; c < 3; ++c) nwp[c] = a[c] + b[c];
+    } else {
+        for (int c = 0; c < 3; ++c) nwp[c] = s0[c] + s1[c];
+        for (int c = 0; c < 3; ++c) nwp[c] = nwp[c] * 0.5;
+    }
+    for (int c = 0; c < 3; ++c) h.X[3 * (size_t)nw + c] = nwp[c];
+    h.conn[2 * nw + 1] = p;
+    h.conn[2 * nw] = next;
+    h.conn[2 * p] = nw;
+    h.conn[2 * next + 1] = nw;
+    return nw;
+}
+
+}  // namespace
+
+extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added,
+                          int *n_removed)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    HostTangle h;
+    h.N = c.n;
+    h.X.resize(3 * (size_t)c.n);
+    OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)c.n));
+    launch_soa_to_aos(c, dp(c.x[0]), dp(c.x[1]), dp(c.x[2]), c.partial.as<double>());
+    OVG_HIP(hipMemcpyAsync(h.X.data(), c.partial.p, sizeof(double) * 3 * (size_t)c.n,
+                           hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    h.conn = c.h_conn;
+    h.st = c.h_status;
+    h.next_empty = c.next_empty;
+    TangleParams P = host_tparams(c.params, 0.0);
+    int added = 0, removed = 0;
+    for (int k = 0; k < h.N; ++k) {
+        if (h.st[2 * k] == OVG_EMPTY) continue;
+        int next = h.conn[2 * k], prev = h.conn[2 * k + 1];
+        double lf = 0, lr = 0;
+        const double *xk = &h.X[3 * (size_t)k];
+        if (next >= 0) {
+            double w[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1], h.X[3 * (size_t)next + 2]};
+            host::pwrap(P, xk, w);
+            lf = host::dist3(xk, w);
+        }
+        if (prev >= 0) {
+            double w[3] = {h.X[3 * (size_t)prev], h.X[3 * (size_t)prev + 1], h.X[3 * (size_t)prev + 2]};
+            host::pwrap(P, xk, w);
+            lr = host::dist3(xk, w);
+        }
+        if (next >= 0 && prev >= 0 && ((lf < min_dist || lr < min_dist) && (lf + lr) < max_dist)) {
+            int pv = h.conn[2 * k + 1], nx = h.conn[2 * k];  // remove_point :342-355
+            if (pv >= 0) h.conn[2 * pv] = nx;
+            if (nx >= 0) h.conn[2 * nx + 1] = pv;
+            h.conn[2 * k] = h.conn[2 * k + 1] = -1;
+            h.st[2 * k] = OVG_EMPTY;
+            ++removed;
+        }
+        if (next >= 0 && lf > max_dist) {
+            ++added;
+            add_point(h, P, k);
+        }
+    }
+    c.next_empty = h.next_empty;
+    if (n_added) *n_added = added;
+    if (n_removed) *n_removed = removed;
+    if (!added && !removed) return OVG_OK;
+    if (h.N != c.n) OVG_TRY(alloc_state(c, h.N));
+    c.n = h.N;
+    OVG_TRY(upload_positions(c, h.X.data()));
+    OVG_TRY(upload_topology(c, h.conn.data(), h.st.data()));
+    if (added) {
+        launch_wrap(c);  // enforce_periodic_boundaries after insertions (:1057-1060)
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    return OVG_OK;
 }

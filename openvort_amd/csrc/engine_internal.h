@@ -89,7 +89,7 @@ struct Ctx {
     // RK4 scratch
     DBuf xs[2][3], move[3], k[3], stan[3], snor[3];
     // Biot-Savart partials
-    DBuf bs_prim[3], bs_img[3], partial;
+    DBuf bs_prim[3], bs_img[3], partial, dpart;  // partial: host staging; dpart: direct chunks
     DBuf seg[6];                    // direct-sum sources: r1 xyz, pwrapped r2 xyz
     DBuf seg_ids;                   // int2 (slot, fwd) per source
     DBuf tparams;                   // TangleParams[1]
@@ -102,11 +102,18 @@ struct Ctx {
 
     std::vector<int> h_conn, h_status;  // host mirrors of topology
     uint64_t topo_version = 1;          // bumped whenever links change
+    int next_empty = 0;                 // the reference allocator's hint (tangle->next_empty)
     uint64_t loops_version = 0;         // topology the smoothing loop table matches
     int n_loops = 0;
     DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
+    std::vector<int> h_loop_off;
+    DBuf sm_fft, sm_big;                // FFT scratch and big-loop list
     ovg_stats stats{};
     hipEvent_t ev[8] = {};
+    // per-evaluation timing events, resolved when statistics are read so
+    // that timing adds no synchronisation to the timed work
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<hipEvent_t> ev_pending;  // triples: start, tree done, BS done
 };
 
 // ---- kernel launchers (one translation unit each) ------------------------
@@ -141,6 +148,9 @@ int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);
 int smooth_loops(Ctx &c);
 
 // engine.hip helpers
+// Biot-Savart (primary + images) of every live point at positions x with
+// the context's method, into c.bs_prim / c.bs_img.
+int eval_bs(Ctx &c, const double *const x[3]);
 TangleParams host_tparams(const ovg_params &p, double t);
 void push_params(Ctx &c, double t);
 void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos);

@@ -40,6 +40,9 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_point(
         int k = spts[j];
         double rx = X[k], ry = Y[k], rz = Z[k];
         double tx = 0, ty = 0, tz = 0;
+        // a non-finite point would open every node; its velocity is NaN
+        const bool finite = isfinite(rx) && isfinite(ry) && isfinite(rz);
+        if (!finite) n_nodes = 0, tx = ty = tz = __builtin_nan("");
         for (int im = -1; im < P.n_images; ++im) {
             double qx = rx, qy = ry, qz = rz;
             if (im >= 0) {
@@ -47,7 +50,7 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_point(
                 qy = ry - P.Ls[im][1];
                 qz = rz - P.Ls[im][2];
             }
-            double ax = 0, ay = 0, az = 0;
+            double ax = finite ? 0 : __builtin_nan(""), ay = ax, az = ax;
             int d = 0;
             while (d < n_nodes) {
                 const Node nd = nodes[d];
@@ -117,7 +120,10 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
     unsigned long long c64 = 0, c32 = 0;
     int k = valid ? spts[j] : 0;
     double rx = valid ? X[k] : 0, ry = valid ? Y[k] : 0, rz = valid ? Z[k] : 0;
-    double tx = 0, ty = 0, tz = 0;
+    // a non-finite point would open every node; it takes no part and gets NaN
+    const bool walk = valid && isfinite(rx) && isfinite(ry) && isfinite(rz);
+    const double nan0 = walk ? 0.0 : __builtin_nan("");
+    double tx = nan0, ty = nan0, tz = nan0;
     for (int im = -1; im < P.n_images; ++im) {
         double qx = rx, qy = ry, qz = rz;
         if (im >= 0) {
@@ -125,8 +131,8 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
             qy = ry - P.Ls[im][1];
             qz = rz - P.Ls[im][2];
         }
-        double ax = 0, ay = 0, az = 0;
-        int p = valid ? 0 : 0x7fffffff;
+        double ax = nan0, ay = nan0, az = nan0;
+        int p = walk ? 0 : 0x7fffffff;
         for (;;) {
             int d = __builtin_amdgcn_readfirstlane(wave_min(p));
             if (d >= n_nodes) break;

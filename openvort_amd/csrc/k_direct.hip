@@ -136,7 +136,7 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
     if (chunk_len == 0) chunk_len = kDirBlock;
     chunks = (c.n_src + chunk_len - 1) / chunk_len;
     if (chunks < 1) chunks = 1;
-    c.partial.ensure(sizeof(double) * 6 * (size_t)n_tgt * chunks);
+    if (c.dpart.ensure(sizeof(double) * 6 * (size_t)n_tgt * chunks) != OVG_OK) return;
     dim3 grid(tiles, chunks);
     bool mixed = c.params.precision == OVG_PREC_MIXED;
     if (mixed)
@@ -144,19 +144,17 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
                            d_tgt_slots, x[0], x[1], x[2], c.n_src, chunk_len,
                            c.seg[0].as<double>(), c.seg[1].as<double>(), c.seg[2].as<double>(),
                            c.seg[3].as<double>(), c.seg[4].as<double>(), c.seg[5].as<double>(),
-                           c.seg_ids.as<int2>(), dparams(c), c.partial.as<double>());
+                           c.seg_ids.as<int2>(), dparams(c), c.dpart.as<double>());
     else
         hipLaunchKernelGGL(k_direct<false>, grid, dim3(kDirBlock), 0, c.stream, n_tgt,
                            d_tgt_slots, x[0], x[1], x[2], c.n_src, chunk_len,
                            c.seg[0].as<double>(), c.seg[1].as<double>(), c.seg[2].as<double>(),
                            c.seg[3].as<double>(), c.seg[4].as<double>(), c.seg[5].as<double>(),
-                           c.seg_ids.as<int2>(), dparams(c), c.partial.as<double>());
+                           c.seg_ids.as<int2>(), dparams(c), c.dpart.as<double>());
     int bs = 256, gs = (n_tgt + bs - 1) / bs;
     hipLaunchKernelGGL(k_direct_reduce, dim3(gs), dim3(bs), 0, c.stream, n_tgt, chunks,
-                       d_tgt_slots, c.partial.as<double>(), out_prim[0], out_prim[1],
+                       d_tgt_slots, c.dpart.as<double>(), out_prim[0], out_prim[1],
                        out_prim[2], out_img[0], out_img[1], out_img[2]);
-    c.stats.interactions += (double)n_tgt * c.n_src * (1 + c.params.n_images);
-    if (mixed) c.stats.interactions_fp32 += (double)n_tgt * c.n_src * (1 + c.params.n_images);
 }
 
 }  // namespace ovg

@@ -311,15 +311,21 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
                                hipMemcpyHostToDevice, c.stream));
         const double *xv[3] = {c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>()};
         push_params(c, t);
-        launch_prep_segments(c, xv);
         double *op[3] = {c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                          c.bs_prim[2].as<double>()};
         double *oi[3] = {c.bs_img[0].as<double>(), c.bs_img[1].as<double>(),
                          c.bs_img[2].as<double>()};
-        ovg_params saved = c.params;
-        c.params.precision = OVG_PREC_FP64;
-        launch_direct(c, xv, m, c.rc_tmp.as<int>(), op, oi, 0);
-        c.params = saved;
+        if (c.params.rec_velocity == OVG_RECVEL_TREE) {
+            // the step's own evaluator at every point, read at the endpoints
+            OVG_TRY(eval_bs(c, xv));
+        } else {
+            // update_velocity(tree = NULL): FP64 direct sum at the endpoints
+            launch_prep_segments(c, xv);
+            ovg_params saved = c.params;
+            c.params.precision = OVG_PREC_FP64;
+            launch_direct(c, xv, m, c.rc_tmp.as<int>(), op, oi, 0);
+            c.params = saved;
+        }
         OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)m));
         hipLaunchKernelGGL(k_gather3, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,
                            c.rc_tmp.as<int>(), op[0], op[1], op[2], c.partial.as<double>());
@@ -340,12 +346,28 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     };
     std::vector<Change> changes;
     const double cutoff = std::cos(c.params.rec_angle);
+    // Direct mode applies the change of every rewired segment (exact); tree
+    // mode only those within 25 rec_dist of the point: a rewiring moves a
+    // segment pair of extent ~rec_dist, whose field beyond that radius is
+    // below 1e-4 of the local one and far below the tree's own error.
+    const bool exact = c.params.rec_velocity != OVG_RECVEL_TREE;
+    const double rc2 = (25 * c.params.rec_dist) * (25 * c.params.rec_dist);
+    auto near = [&](int a, int b) {
+        double d2 = 0;
+        for (int q = 0; q < 3; ++q) {
+            double d = X[3 * (size_t)a + q] - X[3 * (size_t)b + q];
+            if (P.periodic[q]) d -= P.L[q] * std::nearbyint(d / P.L[q]);
+            d2 += d * d;
+        }
+        return d2 < rc2;
+    };
     auto velocity = [&](int p, double *tn, double *nr, double *vels) {
         int e = where[p];
         double bs[3];
         for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
-        // exact change of every segment rewired so far
+        // change of the segments rewired so far
         for (const Change &ch : changes) {
+            if (!exact && !near(ch.seg, p)) continue;
             double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
             host::segment_all_images(P, X.data(), ch.seg, ch.old_f, p, vo);
             host::segment_all_images(P, X.data(), ch.seg, ch.new_f, p, vn);

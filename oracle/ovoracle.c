@@ -819,3 +819,162 @@ void ovo_bh_field(int n, const double *vnodes, const int *conn,
     }
     rtree_free(&tree);
 }
+
+/* eliminate_small_loops, src/tangle.c:1063-1128, closed loops: a loop is
+ * walked from its first unvisited slot counting m-1 steps for m points, and
+ * deleted when that count is below the cutoff.  Returns deleted points. */
+int ovo_eliminate_small_loops(int n, int *conn, int *status, int cutoff)
+{
+    char *visited = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    int removed = 0;
+    for (int k = 0; k < n; ++k) {
+        if (status[2 * k] == OVG_EMPTY || visited[k]) continue;
+        visited[k] = 1;
+        int loop = 0, here = k, next = conn[2 * k];
+        while (next != k) {
+            visited[here] = 1;
+            here = next;
+            next = conn[2 * here];
+            loop++;
+        }
+        if (loop < cutoff) {
+            next = here;
+            for (;;) {
+                int tmp = next;
+                next = conn[2 * next];
+                conn[2 * tmp] = -1;
+                conn[2 * tmp + 1] = -1;
+                status[2 * tmp] = OVG_EMPTY;
+                removed++;
+                if (next == here || next < 0) break;
+            }
+        }
+    }
+    free(visited);
+    return removed;
+}
+
+/* ---------------- remesh (src/tangle.c:1018-1061) ----------------------- */
+/* search_next_empty_node, src/tangle.c:201-234 (a read past the last slot
+ * counts as not EMPTY) */
+static int search_next_empty(int N, const int *status, int *next_empty)
+{
+    if (*next_empty > 0) {
+        if (*next_empty < N && status[2 * *next_empty] == OVG_EMPTY) {
+            int r = *next_empty;
+            if (*next_empty + 1 < N && status[2 * (*next_empty + 1)] == OVG_EMPTY)
+                ++*next_empty;
+            else
+                *next_empty = -1;
+            return r;
+        }
+    }
+    for (int k = 0; k < N; ++k) {
+        if (status[2 * k] == OVG_EMPTY) {
+            *next_empty = (k + 1 < N && status[2 * (k + 1)] == OVG_EMPTY) ? k + 1 : -1;
+            return k;
+        }
+    }
+    return -1;
+}
+
+/* get_next_empty_node :241-253 with expand_tangle :68-101 (doubling) */
+static int next_empty_node(int *N, int cap, int *conn, int *status, int *next_empty)
+{
+    int idx = search_next_empty(*N, status, next_empty);
+    while (idx < 0 || status[2 * idx] != OVG_EMPTY) {
+        int old = *N, nn = 2 * old;
+        if (nn > cap) return -2;
+        for (int k = old; k < nn; ++k) {
+            status[2 * k] = OVG_EMPTY;
+            status[2 * k + 1] = -1;
+            conn[2 * k] = conn[2 * k + 1] = -1;
+        }
+        *N = nn;
+        *next_empty = search_next_empty(*N, status, next_empty);
+        idx = search_next_empty(*N, status, next_empty);
+    }
+    return idx;
+}
+
+/* add_point, src/tangle.c:261-335 */
+static int add_point(int *N, int cap, double *vnodes, int *conn, int *status,
+                     int *next_empty, const ovg_params *p, int pt)
+{
+    int nw = next_empty_node(N, cap, conn, status, next_empty);
+    if (nw < 0) return nw;
+    status[2 * nw] = OVG_FREE;
+    status[2 * nw + 1] = -1;
+    int next = conn[2 * pt];
+    double t0[3], n0[3], t1[3], n1[3];
+    tangent_normal(vnodes, conn, p, pt, t0, n0);
+    tangent_normal(vnodes, conn, p, next, t1, n1);
+    double s0[3], s1[3];
+    memcpy(s0, vnodes + 3 * pt, 24);
+    pwrap(s0, vnodes + 3 * next, p, s1);
+    double l = dist3(s0, s1);
+    double nv[3], nwp[3];
+    for (int c = 0; c < 3; ++c) nv[c] = n0[c] + n1[c];
+    for (int c = 0; c < 3; ++c) nv[c] = nv[c] * 0.5;
+    if (len3(nv) > 1e-5) {
+        double R = 1 / len3(nv);
+        double dR = R * R - l * l / 4;
+        double delta = dR > 0 ? R - sqrt(dR) : R;
+        double ln = len3(nv);
+        for (int c = 0; c < 3; ++c) nv[c] = nv[c] * (1 / ln);
+        for (int c = 0; c < 3; ++c) nv[c] = nv[c] * -1;
+        double a[3], b[3];
+        for (int c = 0; c < 3; ++c) a[c] = s0[c] + s1[c];
+        for (int c = 0; c < 3; ++c) a[c] = a[c] * 0.5;
+        for (int c = 0; c < 3; ++c) b[c] = nv[c] * delta;
+        for (int c = 0; c < 3; ++c) nwp[c] = a[c] + b[c];
+    } else {
+        for (int c = 0; c < 3; ++c) nwp[c] = s0[c] + s1[c];
+        for (int c = 0; c < 3; ++c) nwp[c] = nwp[c] * 0.5;
+    }
+    memcpy(vnodes + 3 * nw, nwp, 24);
+    conn[2 * nw + 1] = pt;
+    conn[2 * nw] = next;
+    conn[2 * pt] = nw;
+    conn[2 * next + 1] = nw;
+    return nw;
+}
+
+/* remesh; *N is the reference's tangle->N (may double up to cap), arrays
+ * must hold cap slots.  Returns points added, -1 on capacity overflow;
+ * *removed receives the removed count. */
+int ovo_remesh(int *N, int cap, double *vnodes, int *conn, int *status,
+               int *next_empty, const ovg_params *p, double min_dist,
+               double max_dist, int *removed)
+{
+    int added = 0, rem = 0;
+    for (int k = 0; k < *N; ++k) {
+        if (status[2 * k] == OVG_EMPTY) continue;
+        int next = conn[2 * k], prev = conn[2 * k + 1];
+        double lf = 0, lr = 0, w[3];
+        if (next >= 0) {
+            pwrap(vnodes + 3 * k, vnodes + 3 * next, p, w);
+            lf = dist3(vnodes + 3 * k, w);
+        }
+        if (prev >= 0) {
+            pwrap(vnodes + 3 * k, vnodes + 3 * prev, p, w);
+            lr = dist3(vnodes + 3 * k, w);
+        }
+        if (next >= 0 && prev >= 0 && ((lf < min_dist || lr < min_dist) && (lf + lr) < max_dist)) {
+            /* remove_point, src/tangle.c:342-355 */
+            int pv = conn[2 * k + 1], nx = conn[2 * k];
+            if (pv >= 0) conn[2 * pv] = nx;
+            if (nx >= 0) conn[2 * nx + 1] = pv;
+            conn[2 * k] = conn[2 * k + 1] = -1;
+            status[2 * k] = OVG_EMPTY;
+            rem++;
+        }
+        if (next >= 0 && lf > max_dist) {
+            added++;
+            if (add_point(N, cap, vnodes, conn, status, next_empty, p, k) < 0) return -1;
+        }
+    }
+    if (added) ovo_enforce_periodic(*N, vnodes, status, p);
+    if (removed) *removed = rem;
+    return added;
+}

@@ -88,6 +88,17 @@ void ovo_bh_tree_stats(int n, const double *vnodes, const int *conn,
                        const int *status, const ovg_params *p,
                        long *nodes, int *depth, long *nonempty);
 
+/* eliminate_small_loops (closed loops); conn/status in place; returns the
+ * number of deleted points. */
+int ovo_eliminate_small_loops(int n, int *conn, int *status, int cutoff);
+
+/* remesh (src/tangle.c:1018-1061) with add_point/remove_point and the
+ * reference's slot allocation (next_empty hint, capacity doubling of *N up
+ * to cap).  Returns points added (-1 on overflow of cap). */
+int ovo_remesh(int *N, int cap, double *vnodes, int *conn, int *status,
+               int *next_empty, const ovg_params *p, double min_dist,
+               double max_dist, int *removed);
+
 /* octree_get_vs (reference tree) at m arbitrary points. */
 void ovo_bh_field(int n, const double *vnodes, const int *conn,
                   const int *status, const ovg_params *p, int m,

@@ -53,6 +53,9 @@ class Oracle:
         L.ovo_reconnect.argtypes = [C.c_int, _dp, _ip, _ip, P, C.c_double, C.c_int]
         L.ovo_line_length.argtypes = [C.c_int, _dp, _ip, _ip, P]
         L.ovo_line_length.restype = C.c_double
+        L.ovo_eliminate_small_loops.argtypes = [C.c_int, _ip, _ip, C.c_int]
+        L.ovo_remesh.argtypes = [C.POINTER(C.c_int), C.c_int, _dp, _ip, _ip, C.POINTER(C.c_int),
+                                 P, C.c_double, C.c_double, C.POINTER(C.c_int)]
         L.ovo_bh_field.argtypes = [C.c_int, _dp, _ip, _ip, P, C.c_int, _dp, _dp]
         L.ovo_bh_tree_stats.argtypes = [C.c_int, _dp, _ip, _ip, P, C.POINTER(C.c_long),
                                         C.POINTER(C.c_int), C.POINTER(C.c_long)]
@@ -120,6 +123,25 @@ class Oracle:
     def line_length(self, t: Tangle, p):
         return self.lib.ovo_line_length(t.n, _d(t.vnodes), _i(t.conn), _i(t.status), C.byref(p))
 
+    def eliminate_small_loops(self, t: Tangle, cutoff=5):
+        """In place on t.conn / t.status; returns the deleted point count."""
+        return self.lib.ovo_eliminate_small_loops(t.n, _i(t.conn), _i(t.status), cutoff)
+
+    def remesh(self, t: Tangle, p, min_dist, max_dist, next_empty=0, cap_factor=4):
+        """Returns (new Tangle, added, removed, next_empty); capacity doubles as
+        the reference's expand_tangle."""
+        cap = t.n * cap_factor
+        v = np.zeros((cap, 3)); v[:t.n] = t.vnodes
+        c = -np.ones((cap, 2), dtype=np.int32); c[:t.n] = t.conn
+        s = np.zeros((cap, 2), dtype=np.int32); s[:, 0] = -1; s[:, 1] = -1; s[:t.n] = t.status
+        N, ne, rem = C.c_int(t.n), C.c_int(next_empty), C.c_int(0)
+        added = self.lib.ovo_remesh(C.byref(N), cap, _d(v), _i(c), _i(s), C.byref(ne),
+                                    C.byref(p), min_dist, max_dist, C.byref(rem))
+        if added < 0:
+            raise RuntimeError("remesh capacity")
+        n = N.value
+        return Tangle(v[:n], c[:n], s[:n]), added, rem.value, ne.value
+
     def bh_field(self, t: Tangle, p, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         out = np.zeros_like(pts)
@@ -163,6 +185,10 @@ class RefLib:
         L.ref_reconnect.argtypes = [vp, C.c_double]
         L.ref_velocity_one.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
         L.ref_bh_field.argtypes = [vp, C.c_int, _dp, C.c_double, _dp]
+        L.ref_eliminate_small_loops.argtypes = [vp, C.c_int]
+        L.ref_remesh.argtypes = [vp, C.c_double, C.c_double]
+        L.ref_num_slots.argtypes = [vp]
+        L.ref_get_status.argtypes = [vp, _ip]
         L.ref_time_phases.argtypes = [vp, C.c_double, C.c_double, _dp]
 
     def image_table(self, n):
@@ -212,6 +238,18 @@ class RefTangle:
 
     def enforce_periodic(self):
         self.ref.lib.ref_enforce_periodic(self.h)
+
+    def eliminate_small_loops(self, cutoff=5):
+        self.ref.lib.ref_eliminate_small_loops(self.h, cutoff)
+
+    def remesh(self, min_dist, max_dist):
+        self.ref.lib.ref_remesh(self.h, min_dist, max_dist)
+        self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def status(self):
+        st = np.zeros((self.n, 2), dtype=np.int32)
+        self.ref.lib.ref_get_status(self.h, _i(st))
+        return st
 
     def velocity_one(self, k, time=0.0):
         vs, vels = np.zeros(3), np.zeros(3)

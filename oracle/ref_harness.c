@@ -160,6 +160,30 @@ void ref_get(void *h, double *vnodes, double *vels, double *vs, double *tan,
     }
 }
 
+extern int small_loop_cutoff;
+
+void ref_eliminate_small_loops(void *h, int cutoff)
+{
+    small_loop_cutoff = cutoff;
+    eliminate_small_loops((struct tangle_state *)h);
+}
+
+void ref_remesh(void *h, double min_dist, double max_dist)
+{
+    remesh((struct tangle_state *)h, min_dist, max_dist, 0.0);
+}
+
+int ref_num_slots(void *h) { return ((struct tangle_state *)h)->N; }
+
+void ref_get_status(void *h, int *st)
+{
+    struct tangle_state *t = (struct tangle_state *)h;
+    for (int k = 0; k < t->N; ++k) {
+        st[2 * k] = t->status[k].status;
+        st[2 * k + 1] = t->status[k].pin_wall;
+    }
+}
+
 void ref_update_tangents_normals(void *h) { update_tangents_normals((struct tangle_state *)h); }
 void ref_update_velocities(void *h, double t) { update_velocities((struct tangle_state *)h, t); }
 void ref_update_tangle(void *h, double t) { update_tangle((struct tangle_state *)h, t); }
