@@ -108,6 +108,7 @@ struct Ctx {
     DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
     std::vector<int> h_loop_off;
     DBuf sm_fft, sm_big;                // FFT scratch and big-loop list
+    DBuf sm_giant, sm_gstate;           // grid-wide FFT scratch for giant loops
     ovg_stats stats{};
     hipEvent_t ev[8] = {};
     // per-evaluation timing events, resolved when statistics are read so
@@ -140,6 +141,7 @@ int build_tree(Ctx &c, const double *const x[3]);
 
 // k_bh.hip
 void launch_bh(Ctx &c, const double *const x[3]);
+void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist);
 
 // k_recon.hip
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);

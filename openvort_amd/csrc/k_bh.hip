@@ -7,8 +7,8 @@
 // k_bh_point: one lane per target point, images in sequence, the
 //   reference's acceptance test Lm/d < theta in FP64 with d = |r - com|
 //   correctly rounded.  This is the plain FP64 path.
-// k_bh_group: one wavefront per 64 Morton-adjacent target points and all
-//   images.  Each lane keeps its own next node; the wave visits the minimum
+// k_bh_group<.., LOCKSTEP = true>: one wavefront per 64 Morton-adjacent
+//   target points and all images.  Each lane keeps its own next node; the wave visits the minimum
 //   over its lanes, loading that node once (scalar, 64 B) and letting only
 //   the lanes positioned there act on it.  Every lane therefore sees exactly
 //   the node sequence of its own per-point walk.  Acceptance uses
@@ -17,6 +17,8 @@
 //   segment by segment (buckets).  With OVG_PREC_MIXED internal-node
 //   monopoles and far segments run in FP32 (R formed in FP64, then rounded),
 //   segments nearer than mixed_near in FP64; per-point sums are FP64.
+// k_bh_group<.., LOCKSTEP = false>: the same decisions with every lane
+//   walking on its own, for scattered target lists (reconnection endpoints).
 // VALU-bound in the pair formula, latency-bound in the node walk.
 #include "engine_internal.h"
 
@@ -25,7 +27,8 @@ namespace ovg {
 constexpr int kBhBlock = 256;
 
 __global__ __launch_bounds__(kBhBlock) void k_bh_point(
-    int n_live, const int *__restrict__ spts, const double *__restrict__ X,
+    int n_live, const int *__restrict__ spts, const int *__restrict__ tlist,
+    const double *__restrict__ X,
     const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
     const int *__restrict__ counters, const TangleParams *__restrict__ Pp,
     double *__restrict__ px, double *__restrict__ py, double *__restrict__ pz,
@@ -37,7 +40,7 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_point(
     unsigned long long cnt = 0;
     if (j < n_live) {
         int n_nodes = counters[1];
-        int k = spts[j];
+        int k = tlist ? tlist[j] : spts[j];
         double rx = X[k], ry = Y[k], rz = Z[k];
         double tx = 0, ty = 0, tz = 0;
         // a non-finite point would open every node; its velocity is NaN
@@ -97,9 +100,12 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
-template <bool MIXED>
+// LOCKSTEP: the wave visits min(p) over its lanes (coherent Morton-ordered
+// targets); otherwise every lane walks alone (scattered target lists).
+template <bool MIXED, bool LOCKSTEP>
 __global__ __launch_bounds__(kBhBlock) void k_bh_group(
-    int n_live, const int *__restrict__ spts, const double *__restrict__ X,
+    int n_live, const int *__restrict__ spts, const int *__restrict__ tlist,
+    const double *__restrict__ X,
     const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
     const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
     const double *__restrict__ sy, const double *__restrict__ sz,
@@ -118,7 +124,7 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
     const double f64 = P.f_bs;
     const float f32 = (float)P.f_bs;
     unsigned long long c64 = 0, c32 = 0;
-    int k = valid ? spts[j] : 0;
+    int k = valid ? (tlist ? tlist[j] : spts[j]) : 0;
     double rx = valid ? X[k] : 0, ry = valid ? Y[k] : 0, rz = valid ? Z[k] : 0;
     // a non-finite point would open every node; it takes no part and gets NaN
     const bool walk = valid && isfinite(rx) && isfinite(ry) && isfinite(rz);
@@ -134,10 +140,16 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
         double ax = nan0, ay = nan0, az = nan0;
         int p = walk ? 0 : 0x7fffffff;
         for (;;) {
-            int d = __builtin_amdgcn_readfirstlane(wave_min(p));
-            if (d >= n_nodes) break;
+            int d;
+            if (LOCKSTEP) {
+                d = __builtin_amdgcn_readfirstlane(wave_min(p));
+                if (d >= n_nodes) break;
+            } else {
+                d = p;
+                if (d >= n_nodes) break;
+            }
             const Node nd = nodes[d];
-            if (p != d) continue;
+            if (LOCKSTEP && p != d) continue;
             if (nd.count == 0) { p = nd.skip; continue; }
             double ex = nd.cx - qx, ey = nd.cy - qy, ez = nd.cz - qz;
             double d2 = ex * ex + ey * ey + ez * ez;
@@ -219,15 +231,17 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
     }
 }
 
-void launch_bh(Ctx &c, const double *const x[3])
+// Traversal for every live point (tlist == nullptr, targets in Morton
+// order) or for the n_tgt slots of tlist.
+void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist)
 {
-    if (c.n_live == 0) return;
+    if (n_tgt == 0) return;
     TreeBufs &T = c.tree;
-    int gs = (c.n_live + kBhBlock - 1) / kBhBlock;
+    int gs = (n_tgt + kBhBlock - 1) / kBhBlock;
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     if (c.params.traverse == OVG_TRAVERSE_POINT) {
-        hipLaunchKernelGGL(k_bh_point, dim3(gs), dim3(kBhBlock), 0, c.stream, c.n_live,
-                           T.sorted_pts.as<int>(), x[0], x[1], x[2], T.nodes.as<Node>(),
+        hipLaunchKernelGGL(k_bh_point, dim3(gs), dim3(kBhBlock), 0, c.stream, n_tgt,
+                           T.sorted_pts.as<int>(), tlist, x[0], x[1], x[2], T.nodes.as<Node>(),
                            T.counters.as<int>(), dparams(c), c.bs_prim[0].as<double>(),
                            c.bs_prim[1].as<double>(), c.bs_prim[2].as<double>(),
                            c.bs_img[0].as<double>(), c.bs_img[1].as<double>(),
@@ -236,8 +250,8 @@ void launch_bh(Ctx &c, const double *const x[3])
     }
     double near2 = c.params.mixed_near * c.params.mixed_near;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(gs), dim3(kBhBlock), 0, c.stream, c.n_live,
-                           T.sorted_pts.as<int>(), x[0], x[1], x[2], T.nodes.as<Node>(),
+        hipLaunchKernelGGL(kern, dim3(gs), dim3(kBhBlock), 0, c.stream, n_tgt,
+                           T.sorted_pts.as<int>(), tlist, x[0], x[1], x[2], T.nodes.as<Node>(),
                            T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
                            T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
                            T.sqz.as<double>(), T.counters.as<int>(), dparams(c), near2,
@@ -245,8 +259,16 @@ void launch_bh(Ctx &c, const double *const x[3])
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
     };
-    if (c.params.precision == OVG_PREC_MIXED) go(k_bh_group<true>);
-    else go(k_bh_group<false>);
+    bool mixed = c.params.precision == OVG_PREC_MIXED;
+    if (tlist) {
+        if (mixed) go(k_bh_group<true, false>);
+        else go(k_bh_group<false, false>);
+    } else {
+        if (mixed) go(k_bh_group<true, true>);
+        else go(k_bh_group<false, true>);
+    }
 }
+
+void launch_bh(Ctx &c, const double *const x[3]) { launch_bh_targets(c, x, c.n_live, nullptr); }
 
 }  // namespace ovg

@@ -315,9 +315,10 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
                          c.bs_prim[2].as<double>()};
         double *oi[3] = {c.bs_img[0].as<double>(), c.bs_img[1].as<double>(),
                          c.bs_img[2].as<double>()};
-        if (c.params.rec_velocity == OVG_RECVEL_TREE) {
-            // the step's own evaluator at every point, read at the endpoints
-            OVG_TRY(eval_bs(c, xv));
+        if (c.params.rec_velocity == OVG_RECVEL_TREE && c.params.method == OVG_METHOD_BH) {
+            // the step's own Barnes-Hut evaluator, traversed for the endpoints only
+            OVG_TRY(build_tree(c, xv));
+            launch_bh_targets(c, xv, m, c.rc_tmp.as<int>());
         } else {
             // update_velocity(tree = NULL): FP64 direct sum at the endpoints
             launch_prep_segments(c, xv);

@@ -6,8 +6,9 @@
 //    rule over the closed child boxes in the order BLF BLB BRF BRB TLF TLB
 //    TRF TRB (src/octree.c:116-127).  The child indices form a 63-bit path
 //    key, most significant level first, so key order is the reference's
-//    traversal order.  Points outside the root box get key ~0 (kept in the
-//    root's sums, absent from every child: src/octree.c:116-128).
+//    traversal order.  Points outside the root box get bit 63 set (kept in
+//    the root's sums, absent from every child: src/octree.c:116-128), with a
+//    Morton code of their wrapped position below it for target coherence.
 // 2. A stable LSD radix sort of (key, slot), 8-bit digits, written here:
 //    per-tile LDS histograms, one exclusive scan, then a scatter that ranks
 //    equal digits with 64-lane ballots.
@@ -82,19 +83,46 @@ __device__ __forceinline__ bool in_closed(const double lo[3], const double hi[3]
     return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
 }
 
+// clamp != 0 (tuned modes): a point outside the root box is placed by its
+// position clamped onto the box, so it stays in the tree and in Morton order
+// among its neighbours; the reference's drop (src/octree.c:116-128) is kept
+// for the reference-tree mode only.
 __global__ void k_keys(int n_live, const int *__restrict__ live, const double *__restrict__ X,
                        const double *__restrict__ Y, const double *__restrict__ Z,
                        const TangleParams *__restrict__ Pp, uint64_t *__restrict__ keys,
-                       int *__restrict__ vals, int *__restrict__ counters)
+                       int *__restrict__ vals, int *__restrict__ counters, int clamp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
     const TangleParams &P = *Pp;
     int k = live[i];
     double x = X[k], y = Y[k], z = Z[k];
+    if (clamp) {
+        x = fmin(fmax(x, P.lo[0]), P.hi[0]);
+        y = fmin(fmax(y, P.lo[1]), P.hi[1]);
+        z = fmin(fmax(z, P.lo[2]), P.hi[2]);
+    }
     double lo[3] = {P.lo[0], P.lo[1], P.lo[2]}, hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
     uint64_t key = kDropKey;
-    if (in_closed(lo, hi, x, y, z)) {
+    if (!in_closed(lo, hi, x, y, z)) {
+        // outside the root box: in no child (src/octree.c:116-128).  Bit 63
+        // puts it after every in-box point; the remaining bits are a Morton
+        // code of its wrapped position so that the traversal waves of these
+        // targets stay spatially coherent.
+        double p3[3] = {x, y, z};
+        uint64_t m = 0;
+        unsigned q[3];
+        for (int a = 0; a < 3; ++a) {
+            double u = (p3[a] - P.lo[a]) / P.L[a];
+            u = u - floor(u);
+            if (!(u >= 0.0 && u < 1.0)) u = 0.0;  // non-finite positions
+            q[a] = (unsigned)(u * 1048576.0);
+            if (q[a] > 1048575u) q[a] = 1048575u;
+        }
+        for (int b = 19; b >= 0; --b)
+            for (int a = 0; a < 3; ++a) m = (m << 1) | ((q[a] >> b) & 1u);
+        key = (1ULL << 63) | m;
+    } else {
         key = 0;
         int fallbacks = 0;
         for (int lev = 1; lev <= kMaxLevel; ++lev) {
@@ -532,9 +560,10 @@ int build_tree(Ctx &c, const double *const x[3])
     hipMemsetAsync(T.counters.p, 0, sizeof(int) * 64, c.stream);
     if (n == 0) return OVG_OK;
     int bs = 256, gs = (n + bs - 1) / bs;
+    int clamp = c.params.traverse == OVG_TRAVERSE_GROUP;
     hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
                        x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
-                       T.counters.as<int>());
+                       T.counters.as<int>(), clamp);
     radix_sort(c, n, 0, 56);
     hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.vals[0].as<int>(), x[0], x[1], x[2],

@@ -332,3 +332,46 @@ def test_remesh_capacity_doubling(oracle):
     assert a == added and e.n == want.n > t.n
     assert np.array_equal(conn, want.conn) and np.array_equal(st, want.status)
     assert np.array_equal(o["vnodes"][want.live], want.vnodes[want.live])
+
+
+def test_smooth_giant_loops():
+    from oracle import smooth as S
+    D = 0.2
+    rng = np.random.default_rng(12)
+    n1, n2 = 70000, 80000
+    R = 0.06
+    ph = 2 * np.pi * np.arange(n1) / n1
+    ring = np.stack([0.1 + R * np.cos(ph), 0.1 + R * np.sin(ph), np.full(n1, 0.07)], 1)
+    ring += rng.normal(scale=2e-6, size=ring.shape)
+    line = spanning_line(n2, D, 2, amp=1e-5, seed=3)
+    x = np.concatenate([ring, line]) % D
+    conn = np.concatenate([
+        np.stack([(np.arange(n1) + 1) % n1, (np.arange(n1) - 1) % n1], 1),
+        n1 + np.stack([(np.arange(n2) + 1) % n2, (np.arange(n2) - 1) % n2], 1)])
+    st = np.zeros((n1 + n2, 2), dtype=np.int32)
+    st[:, 1] = -1
+    t = ov.Tangle(x, conn.astype(np.int32), st)
+    p = configs.params(D, 1e-3, configs.T16, 0.0)
+    say("smoothing: grid-wide FFT path (70000-point ring, 80000-point spanning line)")
+    with engine(t, p) as e:
+        e.smooth()
+        got = e.download()["vnodes"]
+    want = S.smooth_tangle(t.vnodes, t.conn, t.status, [0, 0, 0], [D, D, D], [1, 1, 1],
+                           p.freq_to_cutoff)
+    err = np.abs(got - want).max()
+    say(f"max |gpu - numpy| = {err:.2e}")
+    assert err < 1e-12
+
+
+def test_reconnect_tree_velocity_mode():
+    g = load_golden("c1_reconnect.npz")
+    t, p = golden_tangle(g)
+    p.method = ov.METHOD_BH
+    p.theta = 1e-9  # the tree opens to single segments: velocities equal the direct sum
+    p.rec_velocity = ov.RECVEL_TREE
+    say("reconnect with tree velocities at theta -> 0")
+    with engine(t, p) as e:
+        n = e.reconnect(0.0)
+        conn, _ = e.download_connections()
+    assert n == int(g["reconnections"])
+    assert np.array_equal(conn, g["conn_after"])
