@@ -12,9 +12,10 @@ evaluations), periodic wrap.  State stays on the device between steps; the timed
 starts with the tangle resident in HBM.
 
 The fast path is mixed precision (FP32 far-field pairs, FP64 sums,
-positions, LIA and RK4) with the wavefront-group traversal and buckets of
-16; the plain path is the reference's FP64 split-until-one tree with
-per-point traversal, timed on the same tangle beside it.
+positions, LIA and RK4) with the interaction-list traversal: 64 targets
+per wavefront, groups bounded by one octree cell, buckets of 8.  The plain
+path is the reference's FP64 split-until-one tree with per-point
+traversal, timed on the same tangle beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -67,6 +68,31 @@ def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
     return lat, tm
 
 
+def accuracy(ov, t, p_fast, p_plain, n_sample, seed=7):
+    """Velocity error of both paths on the initial tangle against the FP64
+    direct sum (ovg_velocity_at, the reference's update_velocity with
+    tree == NULL) at a seeded sample of points: relative L2 over the sample."""
+    live = t.live
+    rng = np.random.default_rng(seed)
+    sub = np.sort(rng.choice(live, size=min(n_sample, len(live)), replace=False))
+    out = {}
+    direct = None
+    for name, q in (("fast", p_fast), ("plain_fp64", p_plain)):
+        e = ov.Engine(q)
+        e.upload(t)
+        e.update_tangle(0.0)
+        v = e.download()["vels"][sub]
+        if direct is None:
+            direct = e.velocity_at(sub, 0.0)[1]
+        e.close()
+        out[name] = v
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"sample_points": int(len(sub)), "field": "vels",
+            "fast_vs_direct": rel(out["fast"], direct),
+            "plain_fp64_vs_direct": rel(out["plain_fp64"], direct),
+            "fast_vs_plain_fp64": rel(out["fast"], out["plain_fp64"])}
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v) * 1e3, q))
 
@@ -106,15 +132,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--leaf", type=int, default=16)
-    ap.add_argument("--traverse", default="group", choices=["group", "point"])
-    ap.add_argument("--rec-velocity", default="direct", choices=["direct", "tree"])
+    ap.add_argument("--leaf", type=int, default=8)
+    ap.add_argument("--traverse", default="list", choices=["list", "group", "point"])
+    ap.add_argument("--ppw", type=int, default=64, help="list traversal: points per wavefront")
+    ap.add_argument("--group-level", type=int, default=0,
+                    help="list traversal: octree level that bounds a group (0 = auto)")
+    ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])
     ap.add_argument("--smooth-every", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-rings", type=int, default=62)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-sample", type=int, default=2048,
+                    help="points of the direct-sum error check (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 launches) for "
+                         "calibrating the FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,16 +157,40 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        return
+
     import openvort_amd as ov
     from openvort_amd import configs
 
     prec = ov.PREC_MIXED if args.precision == "mixed" else ov.PREC_FP64
-    trav = ov.TRAVERSE_GROUP if args.traverse == "group" else ov.TRAVERSE_POINT
+    trav = {"list": ov.TRAVERSE_LIST, "group": ov.TRAVERSE_GROUP,
+            "point": ov.TRAVERSE_POINT}[args.traverse]
     recv = ov.RECVEL_DIRECT if args.rec_velocity == "direct" else ov.RECVEL_TREE
     t, p, dt = configs.make(args.workload, precision=prec, traverse=trav, leaf_size=args.leaf)
     p.rec_velocity = recv
+    p.points_per_wave = args.ppw
+    p.group_level = args.group_level
     n_pts = len(t.live)
     log(f"workload {args.workload}: {n_pts} points, {t.n} slots")
+
+    q_plain = configs.make(args.workload)[1]
+    q_plain.method, q_plain.precision, q_plain.traverse, q_plain.leaf_size = (
+        ov.METHOD_BH, ov.PREC_FP64, ov.TRAVERSE_POINT, 1)
+    q_plain.rec_velocity = recv
+
+    acc = None
+    if args.accuracy_sample > 0:
+        acc = accuracy(ov, t, p, q_plain, args.accuracy_sample)
+        log(f"accuracy vs direct: fast {acc['fast_vs_direct']:.3e}, "
+            f"plain {acc['plain_fp64_vs_direct']:.3e}")
 
     # ---- fast path ------------------------------------------------------
     e = ov.Engine(p)
@@ -163,11 +221,7 @@ def main():
     # ---- plain FP64 path ------------------------------------------------
     plain = None
     if args.plain_steps > 0:
-        q = configs.make(args.workload)[1]
-        q.method, q.precision, q.traverse, q.leaf_size = (ov.METHOD_BH, ov.PREC_FP64,
-                                                          ov.TRAVERSE_POINT, 1)
-        q.rec_velocity = recv
-        e2 = ov.Engine(q)
+        e2 = ov.Engine(q_plain)
         e2.upload(t)
         e2.synchronize()
         run_steps(e2, 1, 0.0, dt, 0, args.smooth_every, True, rb)
@@ -207,7 +261,8 @@ def main():
                                 f"dt {dt}, reconnect every step, smooth every "
                                 f"{args.smooth_every} steps"),
                    "points": n_pts, "precision": args.precision, "traverse": args.traverse,
-                   "leaf_size": args.leaf, "rec_velocity": args.rec_velocity},
+                   "leaf_size": args.leaf, "points_per_wave": args.ppw,
+                   "group_level": args.group_level, "rec_velocity": args.rec_velocity},
         "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
         "interactions_per_s": interactions_per_s,
         "direct_equivalent_interactions_per_s": 27.0 * n_pts * n_pts * 4 * args.steps / wall,
@@ -216,11 +271,14 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_bh_group<mixed>" if trav == ov.TRAVERSE_GROUP and
-                     prec == ov.PREC_MIXED else "k_bh", "avg_launch_ms": bs_ms,
+                     "kernel": {ov.TRAVERSE_LIST: "k_bh_list", ov.TRAVERSE_GROUP: "k_bh_group",
+                                ov.TRAVERSE_POINT: "k_bh_point"}[trav]
+                     + ("<mixed>" if prec == ov.PREC_MIXED else "<fp64>"),
+                     "avg_launch_ms": bs_ms,
                      "pairs_per_launch": inter_per_launch,
                      "fp32_pair_fraction": st.interactions_fp32 / max(st.interactions, 1)},
         "cpu_baseline": cpu,
+        "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
 

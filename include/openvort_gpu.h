@@ -71,6 +71,12 @@ extern "C" {
                                  per-point recursion made stackless             */
 #define OVG_TRAVERSE_GROUP  1 /* one wavefront per 64 Morton-adjacent points and
                                  one image; same per-point acceptance test       */
+#define OVG_TRAVERSE_LIST   2 /* one wavefront per points_per_wave Morton-adjacent
+                                 points: one tree walk per group and image with
+                                 the acceptance test applied to the group's
+                                 bounding box (never coarser than the per-point
+                                 test), into an interaction list in LDS that
+                                 every lane then evaluates                       */
 
 /* velocity source of the reconnection approach test */
 #define OVG_RECVEL_DIRECT  0
@@ -124,6 +130,12 @@ typedef struct ovg_params {
                                      Barnes-Hut evaluator)                     */
     double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
                                      than this (cm) stay FP64; 0 = none        */
+    int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 16, 32 or 64
+                                     (0 = 64)                                   */
+    int32_t group_level;          /* OVG_TRAVERSE_LIST: groups never span two
+                                     octree cells of this level (1..21); 0 =
+                                     the coarsest level whose cells are at most
+                                     8 * mixed_near wide (5 if mixed_near = 0)  */
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a

@@ -30,7 +30,7 @@ WALL_OPEN, WALL_PERIODIC, WALL_MIRROR = 0, 1, 2
 FLOW_NONE, FLOW_SIMPLE, FLOW_DECREASING = 0, 1, 2
 METHOD_DIRECT, METHOD_BH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-TRAVERSE_POINT, TRAVERSE_GROUP = 0, 1
+TRAVERSE_POINT, TRAVERSE_GROUP, TRAVERSE_LIST = 0, 1, 2
 FIELD_VELS, FIELD_VS, FIELD_TANGENTS, FIELD_NORMALS = 0, 1, 2, 3
 RECVEL_DIRECT, RECVEL_TREE = 0, 1
 
@@ -55,6 +55,7 @@ class Params(C.Structure):
         ("rec_dist", C.c_double), ("rec_angle", C.c_double),
         ("freq_to_cutoff", C.c_double), ("rk4_classic", C.c_int32),
         ("rec_velocity", C.c_int32), ("mixed_near", C.c_double),
+        ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
     ]
 
 

@@ -229,7 +229,7 @@ int eval_bs(Ctx &c, const double *const x[3])
     if (c.params.method == OVG_METHOD_BH) {
         OVG_TRY(build_tree(c, x));
         mark(c, &e1);
-        launch_bh(c, x);
+        OVG_TRY(launch_bh(c, x));
     } else {
         launch_prep_segments(c, x);
         mark(c, &e1);
@@ -318,7 +318,9 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     if (c.stream) hipStreamSynchronize(c.stream);
     DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
-                   &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf};
+                   &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf, &c.split_part,
+                   &c.loop_off, &c.loop_slots, &c.sm_fft, &c.sm_big, &c.sm_giant,
+                   &c.sm_gstate};
     for (auto *b : all) b->release();
     for (int q = 0; q < 3; ++q) {
         c.x[q].release(); c.vels[q].release(); c.vs[q].release(); c.tan[q].release();
@@ -331,7 +333,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     DBuf *tb[] = {&T.keys[0], &T.keys[1], &T.vals[0], &T.vals[1], &T.hist, &T.delta,
                   &T.counts, &T.base, &T.nodes, &T.meta, &T.level_hist, &T.level_list,
                   &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
-                  &T.sqy, &T.sqz};
+                  &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups};
     for (auto *b : tb) b->release();
     for (auto &e : c.ev)
         if (e) hipEventDestroy(e);
@@ -366,8 +368,14 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
         return set_error(OVG_EINVAL, "theta must be > 0");
     if (p->leaf_size < 1 || p->leaf_size > 64)
         return set_error(OVG_EINVAL, "leaf_size must be in [1, 64]");
-    if (p->traverse != OVG_TRAVERSE_POINT && p->traverse != OVG_TRAVERSE_GROUP)
+    if (p->traverse != OVG_TRAVERSE_POINT && p->traverse != OVG_TRAVERSE_GROUP &&
+        p->traverse != OVG_TRAVERSE_LIST)
         return set_error(OVG_EINVAL, "unknown traversal %d", p->traverse);
+    if (p->points_per_wave != 0 && p->points_per_wave != 16 && p->points_per_wave != 32 &&
+        p->points_per_wave != 64)
+        return set_error(OVG_EINVAL, "points_per_wave must be 0, 16, 32 or 64");
+    if (p->group_level < 0 || p->group_level > 21)
+        return set_error(OVG_EINVAL, "group_level must be in [0, 21]");
     if (p->traverse == OVG_TRAVERSE_POINT &&
         (p->leaf_size != 1 || p->precision != OVG_PREC_FP64) && p->method == OVG_METHOD_BH)
         return set_error(OVG_EINVAL,

@@ -70,7 +70,11 @@ struct TreeBufs {
     DBuf sorted_pts;         // sorted point (slot) ids, in-box first then dropped
     DBuf sx, sy, sz;         // FP64 positions in sorted order
     DBuf sqx, sqy, sqz;      // FP64 pwrapped segment vectors in sorted order
+    DBuf grp_a, grp_b, grp_c;  // target-group construction scratch (int, n + 1)
+    DBuf groups;             // int2 (first sorted point, count) per target group
 };
+// counters[kGroupCounter]: number of target groups of the list traversal
+constexpr int kGroupCounter = 40;
 
 struct Ctx {
     int device = 0;
@@ -90,6 +94,7 @@ struct Ctx {
     DBuf xs[2][3], move[3], k[3], stan[3], snor[3];
     // Biot-Savart partials
     DBuf bs_prim[3], bs_img[3], partial, dpart;  // partial: host staging; dpart: direct chunks
+    DBuf split_part;                // per-(copy, target) sums of scattered-target walks
     DBuf seg[6];                    // direct-sum sources: r1 xyz, pwrapped r2 xyz
     DBuf seg_ids;                   // int2 (slot, fwd) per source
     DBuf tparams;                   // TangleParams[1]
@@ -140,8 +145,9 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
 int build_tree(Ctx &c, const double *const x[3]);
 
 // k_bh.hip
-void launch_bh(Ctx &c, const double *const x[3]);
+int launch_bh(Ctx &c, const double *const x[3]);
 void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist);
+int launch_bh_list(Ctx &c, const double *const x[3]);
 
 // k_recon.hip
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);

@@ -100,116 +100,112 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
-// LOCKSTEP: the wave visits min(p) over its lanes (coherent Morton-ordered
-// targets); otherwise every lane walks alone (scattered target lists).
+// Read-only view of the tree for the walks below.
+struct TreeView {
+    const Node *nodes;
+    const NodeMeta *meta;
+    const double *sx, *sy, *sz, *sqx, *sqy, *sqz;
+    int n_nodes, leaf;
+    double cs, f64, near2;
+    float f32;
+};
+
+// One target copy q against the tree, adding into (ax, ay, az).  Each lane
+// sees the node sequence of the reference's per-point walk.  LOCKSTEP: the
+// wave visits min(p) over its lanes (coherent Morton-ordered targets);
+// otherwise every lane walks alone.  on == false: the lane takes no part.
 template <bool MIXED, bool LOCKSTEP>
-__global__ __launch_bounds__(kBhBlock) void k_bh_group(
-    int n_live, const int *__restrict__ spts, const int *__restrict__ tlist,
-    const double *__restrict__ X,
-    const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
-    const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
-    const double *__restrict__ sy, const double *__restrict__ sz,
-    const double *__restrict__ sqx, const double *__restrict__ sqy,
-    const double *__restrict__ sqz, const int *__restrict__ counters,
-    const TangleParams *__restrict__ Pp, double near2, double *__restrict__ px,
-    double *__restrict__ py, double *__restrict__ pz, double *__restrict__ ix,
-    double *__restrict__ iy, double *__restrict__ iz, unsigned long long *__restrict__ icount)
+__device__ __forceinline__ void tree_walk(const TreeView &T, bool on, double qx, double qy,
+                                          double qz, double &ax, double &ay, double &az,
+                                          unsigned long long &c64, unsigned long long &c32)
 {
-    const TangleParams &P = *Pp;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = j < n_live;
-    const int n_nodes = counters[1];
-    const int leaf = P.leaf_size;
-    const double cs = P.crit_scale;
-    const double f64 = P.f_bs;
-    const float f32 = (float)P.f_bs;
-    unsigned long long c64 = 0, c32 = 0;
-    int k = valid ? (tlist ? tlist[j] : spts[j]) : 0;
-    double rx = valid ? X[k] : 0, ry = valid ? Y[k] : 0, rz = valid ? Z[k] : 0;
-    // a non-finite point would open every node; it takes no part and gets NaN
-    const bool walk = valid && isfinite(rx) && isfinite(ry) && isfinite(rz);
-    const double nan0 = walk ? 0.0 : __builtin_nan("");
-    double tx = nan0, ty = nan0, tz = nan0;
-    for (int im = -1; im < P.n_images; ++im) {
-        double qx = rx, qy = ry, qz = rz;
-        if (im >= 0) {
-            qx = rx - P.Ls[im][0];
-            qy = ry - P.Ls[im][1];
-            qz = rz - P.Ls[im][2];
+    int p = on ? 0 : 0x7fffffff;
+    for (;;) {
+        int d;
+        if (LOCKSTEP) {
+            d = __builtin_amdgcn_readfirstlane(wave_min(p));
+            if (d >= T.n_nodes) break;
+        } else {
+            d = p;
+            if (d >= T.n_nodes) break;
         }
-        double ax = nan0, ay = nan0, az = nan0;
-        int p = walk ? 0 : 0x7fffffff;
-        for (;;) {
-            int d;
-            if (LOCKSTEP) {
-                d = __builtin_amdgcn_readfirstlane(wave_min(p));
-                if (d >= n_nodes) break;
+        const Node nd = T.nodes[d];
+        if (LOCKSTEP && p != d) continue;
+        if (nd.count == 0) { p = nd.skip; continue; }
+        double ex = nd.cx - qx, ey = nd.cy - qy, ez = nd.cz - qz;
+        double d2 = ex * ex + ey * ey + ez * ez;
+        if (d2 < 1e-16) { p = nd.skip; continue; }
+        bool leafnode = nd.count == 1;
+        if (leafnode || d2 > nd.Lm * nd.Lm * T.cs) {
+            // monopole / single segment: R = com - r, Rp = R + sum ds
+            if (MIXED && !(leafnode && d2 < T.near2)) {
+                float vx, vy, vz;
+                seg_field32((float)ex, (float)ey, (float)ez, (float)(ex + nd.qx),
+                            (float)(ey + nd.qy), (float)(ez + nd.qz), T.f32, vx, vy, vz);
+                ax += vx; ay += vy; az += vz;
+                ++c32;
             } else {
-                d = p;
-                if (d >= n_nodes) break;
+                double vx, vy, vz;
+                if (seg_field64(ex, ey, ez, ex + nd.qx, ey + nd.qy, ez + nd.qz, T.f64, vx, vy,
+                                vz)) {
+                    ax += vx; ay += vy; az += vz;
+                }
+                ++c64;
             }
-            const Node nd = nodes[d];
-            if (LOCKSTEP && p != d) continue;
-            if (nd.count == 0) { p = nd.skip; continue; }
-            double ex = nd.cx - qx, ey = nd.cy - qy, ez = nd.cz - qz;
-            double d2 = ex * ex + ey * ey + ez * ez;
-            if (d2 < 1e-16) { p = nd.skip; continue; }
-            bool leafnode = nd.count == 1;
-            if (leafnode || d2 > nd.Lm * nd.Lm * cs) {
-                // monopole / single segment: R = com - r, Rp = R + sum ds
-                if (MIXED && !(leafnode && d2 < near2)) {
+            p = nd.skip;
+        } else if (nd.count <= T.leaf) {
+            // bucket: every segment of the cell, as count-1 leaves would be
+            int s0 = T.meta[d].start, s1 = s0 + nd.count;
+            for (int s = s0; s < s1; ++s) {
+                double bx = T.sx[s] - qx, by = T.sy[s] - qy, bz = T.sz[s] - qz;
+                double b2 = bx * bx + by * by + bz * bz;
+                if (b2 < 1e-16) continue;
+                if (MIXED && !(b2 < T.near2)) {
                     float vx, vy, vz;
-                    seg_field32((float)ex, (float)ey, (float)ez, (float)(ex + nd.qx),
-                                (float)(ey + nd.qy), (float)(ez + nd.qz), f32, vx, vy, vz);
+                    seg_field32((float)bx, (float)by, (float)bz, (float)(bx + T.sqx[s]),
+                                (float)(by + T.sqy[s]), (float)(bz + T.sqz[s]), T.f32, vx, vy,
+                                vz);
                     ax += vx; ay += vy; az += vz;
                     ++c32;
                 } else {
                     double vx, vy, vz;
-                    if (seg_field64(ex, ey, ez, ex + nd.qx, ey + nd.qy, ez + nd.qz, f64, vx, vy,
-                                    vz)) {
+                    if (seg_field64(bx, by, bz, bx + T.sqx[s], by + T.sqy[s], bz + T.sqz[s],
+                                    T.f64, vx, vy, vz)) {
                         ax += vx; ay += vy; az += vz;
                     }
                     ++c64;
                 }
-                p = nd.skip;
-            } else if (nd.count <= leaf) {
-                // bucket: every segment of the cell, as count-1 leaves would be
-                int s0 = meta[d].start, s1 = s0 + nd.count;
-                for (int s = s0; s < s1; ++s) {
-                    double bx = sx[s] - qx, by = sy[s] - qy, bz = sz[s] - qz;
-                    double b2 = bx * bx + by * by + bz * bz;
-                    if (b2 < 1e-16) continue;
-                    if (MIXED && !(b2 < near2)) {
-                        float vx, vy, vz;
-                        seg_field32((float)bx, (float)by, (float)bz, (float)(bx + sqx[s]),
-                                    (float)(by + sqy[s]), (float)(bz + sqz[s]), f32, vx, vy, vz);
-                        ax += vx; ay += vy; az += vz;
-                        ++c32;
-                    } else {
-                        double vx, vy, vz;
-                        if (seg_field64(bx, by, bz, bx + sqx[s], by + sqy[s], bz + sqz[s], f64,
-                                        vx, vy, vz)) {
-                            ax += vx; ay += vy; az += vz;
-                        }
-                        ++c64;
-                    }
-                }
-                p = nd.skip;
-            } else {
-                p = d + 1;
             }
-        }
-        if (valid) {
-            if (im < 0) {
-                px[k] = ax; py[k] = ay; pz[k] = az;
-            } else {
-                tx += ax; ty += ay; tz += az;
-            }
+            p = nd.skip;
+        } else {
+            p = d + 1;
         }
     }
-    if (valid) {
-        ix[k] = tx; iy[k] = ty; iz[k] = tz;
-    }
+}
+
+__device__ __forceinline__ TreeView tree_view(const Node *nodes, const NodeMeta *meta,
+                                              const double *sx, const double *sy,
+                                              const double *sz, const double *sqx,
+                                              const double *sqy, const double *sqz,
+                                              const int *counters, const TangleParams &P,
+                                              double near2)
+{
+    TreeView T;
+    T.nodes = nodes; T.meta = meta;
+    T.sx = sx; T.sy = sy; T.sz = sz; T.sqx = sqx; T.sqy = sqy; T.sqz = sqz;
+    T.n_nodes = counters[1];
+    T.leaf = P.leaf_size;
+    T.cs = P.crit_scale;
+    T.f64 = P.f_bs;
+    T.f32 = (float)P.f_bs;
+    T.near2 = near2;
+    return T;
+}
+
+// Block-sum the two interaction counters into icount[0..1].
+__device__ __forceinline__ void add_counts(unsigned long long c64, unsigned long long c32,
+                                           unsigned long long *icount)
+{
     __shared__ unsigned long long red[2][kBhBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
         c64 += __shfl_xor(c64, off, 64);
@@ -229,6 +225,115 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
         if (s64) atomicAdd(icount, s64);
         if (s32) atomicAdd(icount + 1, s32);
     }
+}
+
+// One lane per target, all 27 copies in sequence: primary into p*, the sum
+// of the 26 images (in table order) into i*.
+template <bool MIXED, bool LOCKSTEP>
+__global__ __launch_bounds__(kBhBlock) void k_bh_group(
+    int n_live, const int *__restrict__ spts, const int *__restrict__ tlist,
+    const double *__restrict__ X,
+    const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
+    const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ sqx, const double *__restrict__ sqy,
+    const double *__restrict__ sqz, const int *__restrict__ counters,
+    const TangleParams *__restrict__ Pp, double near2, double *__restrict__ px,
+    double *__restrict__ py, double *__restrict__ pz, double *__restrict__ ix,
+    double *__restrict__ iy, double *__restrict__ iz, unsigned long long *__restrict__ icount)
+{
+    const TangleParams &P = *Pp;
+    const TreeView T = tree_view(nodes, meta, sx, sy, sz, sqx, sqy, sqz, counters, P, near2);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = j < n_live;
+    unsigned long long c64 = 0, c32 = 0;
+    int k = valid ? (tlist ? tlist[j] : spts[j]) : 0;
+    double rx = valid ? X[k] : 0, ry = valid ? Y[k] : 0, rz = valid ? Z[k] : 0;
+    // a non-finite point would open every node; it takes no part and gets NaN
+    const bool walk = valid && isfinite(rx) && isfinite(ry) && isfinite(rz);
+    const double nan0 = walk ? 0.0 : __builtin_nan("");
+    double tx = nan0, ty = nan0, tz = nan0;
+    for (int im = -1; im < P.n_images; ++im) {
+        double qx = rx, qy = ry, qz = rz;
+        if (im >= 0) {
+            qx = rx - P.Ls[im][0];
+            qy = ry - P.Ls[im][1];
+            qz = rz - P.Ls[im][2];
+        }
+        double ax = nan0, ay = nan0, az = nan0;
+        tree_walk<MIXED, LOCKSTEP>(T, walk, qx, qy, qz, ax, ay, az, c64, c32);
+        if (valid) {
+            if (im < 0) {
+                px[k] = ax; py[k] = ay; pz[k] = az;
+            } else {
+                tx += ax; ty += ay; tz += az;
+            }
+        }
+    }
+    if (valid) {
+        ix[k] = tx; iy[k] = ty; iz[k] = tz;
+    }
+    add_counts(c64, c32, icount);
+}
+
+// Scattered target lists (reconnection endpoints): one lane per (copy,
+// target), lane j = c * n_tgt + t with copy c = 0 the primary and c = 1..26
+// the images, each walking alone; part[c][t] holds the copy's sum.
+template <bool MIXED>
+__global__ __launch_bounds__(kBhBlock) void k_bh_split(
+    int n_tgt, int n_copies, const int *__restrict__ tlist, const double *__restrict__ X,
+    const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
+    const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ sqx, const double *__restrict__ sqy,
+    const double *__restrict__ sqz, const int *__restrict__ counters,
+    const TangleParams *__restrict__ Pp, double near2, double *__restrict__ part,
+    unsigned long long *__restrict__ icount)
+{
+    const TangleParams &P = *Pp;
+    const TreeView T = tree_view(nodes, meta, sx, sy, sz, sqx, sqy, sqz, counters, P, near2);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = j < n_tgt * n_copies;
+    unsigned long long c64 = 0, c32 = 0;
+    int c = valid ? j / n_tgt : 0, t = valid ? j - c * n_tgt : 0;
+    int k = valid ? tlist[t] : 0;
+    double qx = valid ? X[k] : 0, qy = valid ? Y[k] : 0, qz = valid ? Z[k] : 0;
+    const bool walk = valid && isfinite(qx) && isfinite(qy) && isfinite(qz);
+    if (c > 0) {
+        qx = qx - P.Ls[c - 1][0];
+        qy = qy - P.Ls[c - 1][1];
+        qz = qz - P.Ls[c - 1][2];
+    }
+    const double nan0 = walk ? 0.0 : __builtin_nan("");
+    double ax = nan0, ay = nan0, az = nan0;
+    tree_walk<MIXED, false>(T, walk, qx, qy, qz, ax, ay, az, c64, c32);
+    if (valid) {
+        size_t o = (size_t)3 * j;
+        part[o] = ax; part[o + 1] = ay; part[o + 2] = az;
+    }
+    add_counts(c64, c32, icount);
+}
+
+// Primary = part[0][t]; images = 0 + part[1][t] + ... + part[26][t], the
+// order of the sequential loop in k_bh_group.
+__global__ void k_bh_split_sum(int n_tgt, int n_copies, const int *__restrict__ tlist,
+                               const double *__restrict__ part, double *__restrict__ px,
+                               double *__restrict__ py, double *__restrict__ pz,
+                               double *__restrict__ ix, double *__restrict__ iy,
+                               double *__restrict__ iz)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tgt) return;
+    int k = tlist[t];
+    const double *q = part + (size_t)3 * t;
+    px[k] = q[0]; py[k] = q[1]; pz[k] = q[2];
+    double nan0 = (q[0] == q[0]) ? 0.0 : __builtin_nan("");
+    double tx = nan0, ty = nan0, tz = nan0;
+    for (int c = 1; c < n_copies; ++c) {
+        const double *r = part + (size_t)3 * ((size_t)c * n_tgt + t);
+        tx += r[0]; ty += r[1]; tz += r[2];
+    }
+    ix[k] = tx; iy[k] = ty; iz[k] = tz;
 }
 
 // Traversal for every live point (tlist == nullptr, targets in Morton
@@ -261,14 +366,46 @@ void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *t
     };
     bool mixed = c.params.precision == OVG_PREC_MIXED;
     if (tlist) {
-        if (mixed) go(k_bh_group<true, false>);
-        else go(k_bh_group<false, false>);
+        int n_copies = c.params.n_images + 1;
+        size_t lanes = (size_t)n_tgt * n_copies;
+        if (lanes > 0x7fffffff) {
+            // beyond int lane indices: one lane per target
+            if (mixed) go(k_bh_group<true, false>);
+            else go(k_bh_group<false, false>);
+            return;
+        }
+        if (c.split_part.ensure(lanes * 3 * sizeof(double)) != OVG_OK) {
+            if (mixed) go(k_bh_group<true, false>);
+            else go(k_bh_group<false, false>);
+            return;
+        }
+        int gl = (int)((lanes + kBhBlock - 1) / kBhBlock);
+        auto split = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(gl), dim3(kBhBlock), 0, c.stream, n_tgt, n_copies,
+                               tlist, x[0], x[1], x[2], T.nodes.as<Node>(),
+                               T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
+                               T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
+                               T.sqz.as<double>(), T.counters.as<int>(), dparams(c), near2,
+                               c.split_part.as<double>(), cnt);
+        };
+        if (mixed) split(k_bh_split<true>);
+        else split(k_bh_split<false>);
+        hipLaunchKernelGGL(k_bh_split_sum, dim3((n_tgt + 255) / 256), dim3(256), 0, c.stream,
+                           n_tgt, n_copies, tlist, c.split_part.as<double>(),
+                           c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
+                           c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
+                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>());
     } else {
         if (mixed) go(k_bh_group<true, true>);
         else go(k_bh_group<false, true>);
     }
 }
 
-void launch_bh(Ctx &c, const double *const x[3]) { launch_bh_targets(c, x, c.n_live, nullptr); }
+int launch_bh(Ctx &c, const double *const x[3])
+{
+    if (c.params.traverse == OVG_TRAVERSE_LIST) return launch_bh_list(c, x);
+    launch_bh_targets(c, x, c.n_live, nullptr);
+    return OVG_OK;
+}
 
 }  // namespace ovg

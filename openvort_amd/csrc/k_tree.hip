@@ -560,7 +560,7 @@ int build_tree(Ctx &c, const double *const x[3])
     hipMemsetAsync(T.counters.p, 0, sizeof(int) * 64, c.stream);
     if (n == 0) return OVG_OK;
     int bs = 256, gs = (n + bs - 1) / bs;
-    int clamp = c.params.traverse == OVG_TRAVERSE_GROUP;
+    int clamp = c.params.traverse != OVG_TRAVERSE_POINT;
     hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
                        x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
                        T.counters.as<int>(), clamp);

@@ -26,8 +26,11 @@ def say(msg):
 
 
 def params_from(blob):
+    """Params from a fixture's bytes; fields appended to the struct after the
+    fixture was written keep their zero defaults."""
     p = ov.Params()
-    ov.C.memmove(ov.C.byref(p), blob.tobytes(), ov.C.sizeof(p))
+    raw = blob.tobytes()
+    ov.C.memmove(ov.C.byref(p), raw, min(len(raw), ov.C.sizeof(p)))
     return p
 
 
@@ -143,6 +146,53 @@ def test_c2_mixed_precision_error(c2):
     assert s32.interactions_fp32 > 0.5 * s32.interactions
     # buckets evaluate their segments exactly: no worse than the reference tree
     assert trunc < trunc_ref
+
+
+@pytest.mark.parametrize("ppw", [16, 32, 64])
+def test_c2_list_traversal(c2, ppw):
+    g, t, p = c2
+    say(f"c2 BH list traversal, {ppw} points per wave, leaf 16: FP64 and mixed")
+    kw = dict(traverse=ov.TRAVERSE_LIST, leaf_size=16, points_per_wave=ppw)
+    o64, s64 = run_c2(t, p, **kw)
+    o32, s32 = run_c2(t, p, precision=ov.PREC_MIXED, **kw)
+    live, sub = t.live, g["sub"]
+    prec = rel_l2(o32["vs"][live], o64["vs"][live])
+    trunc = rel_l2(o64["vs"][sub], g["direct_vs"])
+    trunc32 = rel_l2(o32["vs"][sub], g["direct_vs"])
+    trunc_ref = rel_l2(g["bh_vs"], g["direct_vs"])
+    say(f"precision error {prec:.3e}, list-direct {trunc:.3e} (mixed {trunc32:.3e}), "
+        f"ref BH-direct {trunc_ref:.3e}, pairs/pt {s64.interactions / len(live):.0f}")
+    # the group test never accepts a node that the per-point test opens
+    assert trunc < trunc_ref
+    assert trunc32 < trunc_ref
+    assert prec < 1e-5
+    assert s32.interactions_fp32 > 0.5 * s32.interactions
+    assert s64.interactions_fp32 == 0
+
+
+def test_c2_list_small_theta_matches_point(c2):
+    """theta -> 0: both walks reduce to the same segment sum (every source
+    visited once), so a missing or repeated subtree shows at O(1e-3).  At
+    theta = 1e-6 a cell is accepted only beyond 1e6 times its size, i.e.
+    only cells of points closer than 3e-7 cm, which this tangle lacks;
+    summation order alone separates the two."""
+    g, t, p = c2
+    say("c2 list vs point traversal at theta 1e-6, FP64")
+    o_list, _ = run_c2(t, p, traverse=ov.TRAVERSE_LIST, leaf_size=16, theta=1e-6)
+    o_pt, _ = run_c2(t, p, theta=1e-6)
+    live = t.live
+    assert rel_l2(o_list["vs"][live], o_pt["vs"][live]) < 1e-11
+    assert rel_l2(o_list["vels"][live], o_pt["vels"][live]) < 1e-11
+
+
+def test_list_traversal_deterministic(c2):
+    g, t, p = c2
+    say("c2 list traversal twice, mixed")
+    kw = dict(traverse=ov.TRAVERSE_LIST, leaf_size=16, precision=ov.PREC_MIXED,
+              points_per_wave=32)
+    a, _ = run_c2(t, p, **kw)
+    b, _ = run_c2(t, p, **kw)
+    assert np.array_equal(a["vels"], b["vels"])
 
 
 def test_reconnect_candidates_bitexact(oracle):
