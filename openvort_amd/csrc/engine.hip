@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <queue>
 #include <vector>
 
 #include "engine_internal.h"
@@ -320,7 +321,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
                    &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf, &c.split_part,
                    &c.loop_off, &c.loop_slots, &c.sm_fft, &c.sm_big, &c.sm_giant,
-                   &c.sm_gstate};
+                   &c.sm_gstate, &c.topo_stage};
     for (auto *b : all) b->release();
     for (int q = 0; q < 3; ++q) {
         c.x[q].release(); c.vels[q].release(); c.vs[q].release(); c.tan[q].release();
@@ -335,6 +336,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
                   &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
                   &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups};
     for (auto *b : tb) b->release();
+    if (c.h_xpin) hipHostFree(c.h_xpin);
+    c.h_xpin = nullptr;
     for (auto &e : c.ev)
         if (e) hipEventDestroy(e);
     for (auto e : c.ev_pending) hipEventDestroy(e);
@@ -458,8 +461,19 @@ static int upload_topology(Ctx &c, const int *conn, const int *status)
 {
     int n = c.n;
     c.topo_version++;
-    c.h_conn.assign(conn, conn + 2 * (size_t)n);
-    c.h_status.assign(status, status + 2 * (size_t)n);
+    if (conn != c.h_conn.data()) c.h_conn.assign(conn, conn + 2 * (size_t)n);
+    if (status != c.h_status.data()) c.h_status.assign(status, status + 2 * (size_t)n);
+    // the whole state is on the device now; every loop is new to the
+    // small-loop pass
+    c.dirty.clear();
+    c.dirty_mark.assign((size_t)n, 0);
+    c.new_pos.clear();
+    c.new_xyz.clear();
+    c.loop_dirty.clear();
+    c.loop_dirty_mark.assign((size_t)n, 0);
+    c.loops_full_scan = true;
+    c.rc_where.assign((size_t)n, -1);
+    c.rc_flag.assign((size_t)n, 0);
     std::vector<int> live;
     live.reserve(n);
     for (int k = 0; k < n; ++k)
@@ -811,7 +825,13 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
     return smooth_loops(c);
 }
 
-// eliminate_small_loops, src/tangle.c:1063-1128 (closed loops)
+// eliminate_small_loops, src/tangle.c:1063-1128 (closed loops).  Loops are
+// disjoint and each is deleted exactly when it has fewer than cutoff
+// points, so the result does not depend on the visiting order.  A loop can
+// only fall below the cutoff through a topology change, so after the first
+// full pass (and while the cutoff does not grow) only the loops through
+// slots touched since the previous pass are walked, each for at most
+// cutoff steps.
 extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed)
 {
     clear_error();
@@ -819,64 +839,97 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
     Ctx &c = ctx->c;
     int n = c.n;
     std::vector<int> &conn = c.h_conn, &st = c.h_status;
-    std::vector<char> visited(n, 0);
     int removed = 0;
-    for (int k = 0; k < n; ++k) {
-        if (st[2 * k] == OVG_EMPTY || visited[k]) continue;
-        visited[k] = 1;
-        int loop = 0, here = k, next = conn[2 * k];
-        while (next != k) {
-            visited[here] = 1;
-            here = next;
-            next = conn[2 * here];
-            loop++;
+    auto erase_loop = [&](int start) {
+        int next = start;
+        for (;;) {
+            int tmp = next;
+            next = conn[2 * (size_t)next];
+            conn[2 * (size_t)tmp] = -1;
+            conn[2 * (size_t)tmp + 1] = -1;
+            st[2 * (size_t)tmp] = OVG_EMPTY;
+            mark_dirty(c, tmp);
+            removed++;
+            if (next == start || next < 0) break;
         }
-        if (loop < cutoff) {
-            next = here;
-            for (;;) {
-                int tmp = next;
-                next = conn[2 * next];
-                conn[2 * tmp] = -1;
-                conn[2 * tmp + 1] = -1;
-                st[2 * tmp] = OVG_EMPTY;
-                removed++;
-                if (next == here || next < 0) break;
+    };
+    if (c.loops_full_scan || cutoff > c.small_cutoff_seen) {
+        std::vector<char> visited(n, 0);
+        for (int k = 0; k < n; ++k) {
+            if (st[2 * (size_t)k] == OVG_EMPTY || visited[k]) continue;
+            visited[k] = 1;
+            int loop = 0, here = k, next = conn[2 * (size_t)k];
+            while (next != k) {
+                visited[here] = 1;
+                here = next;
+                next = conn[2 * (size_t)here];
+                loop++;
             }
+            if (loop < cutoff) erase_loop(here);
+        }
+        c.loops_full_scan = false;
+        c.small_cutoff_seen = cutoff;
+    } else {
+        for (size_t i = 0; i < c.loop_dirty.size(); ++i) {
+            const int k = c.loop_dirty[i];
+            if (k >= n || st[2 * (size_t)k] == OVG_EMPTY) continue;
+            int loop = 0, here = k, next = conn[2 * (size_t)k];
+            while (next != k && loop < cutoff) {
+                here = next;
+                next = conn[2 * (size_t)here];
+                loop++;
+            }
+            if (next == k && loop < cutoff) erase_loop(k);
         }
     }
+    for (int k : c.loop_dirty)
+        if ((size_t)k < c.loop_dirty_mark.size()) c.loop_dirty_mark[k] = 0;
+    c.loop_dirty.clear();
     if (n_removed) *n_removed = removed;
     if (removed) {
         hipSetDevice(c.device);
-        std::vector<int> cc(conn), ss(st);
-        OVG_TRY(upload_topology(c, cc.data(), ss.data()));
+        OVG_TRY(commit_topology(c));
+        // the erased slots were marked for the next pass too; they are EMPTY
+        for (int k : c.loop_dirty)
+            if ((size_t)k < c.loop_dirty_mark.size()) c.loop_dirty_mark[k] = 0;
+        c.loop_dirty.clear();
     }
     return OVG_OK;
 }
 
-// ---- remesh (src/tangle.c:1018-1061) on the host copy ---------------------
+// ---- remesh (src/tangle.c:1018-1061) on the host mirrors ----------------
 namespace {
 
 struct HostTangle {
     int N;
-    std::vector<double> X;
-    std::vector<int> conn, st;
+    double *X;                   // AoS positions: the pinned mirror, or Xv
+    std::vector<double> Xv;      // owns the positions once the tangle grew
+    std::vector<int> &conn, &st; // the context's mirrors, edited in place
     int next_empty;
+    bool grown = false;
+
+    HostTangle(int n, const double *x, std::vector<int> &cn, std::vector<int> &s, int ne)
+        : N(n), X(const_cast<double *>(x)), conn(cn), st(s), next_empty(ne)
+    {
+    }
 
     // search_next_empty_node, src/tangle.c:201-234 (a read past the last
     // slot counts as not EMPTY)
     int search()
     {
         if (next_empty > 0) {
-            if (next_empty < N && st[2 * next_empty] == OVG_EMPTY) {
+            if (next_empty < N && st[2 * (size_t)next_empty] == OVG_EMPTY) {
                 int r = next_empty;
-                if (next_empty + 1 < N && st[2 * (next_empty + 1)] == OVG_EMPTY) ++next_empty;
-                else next_empty = -1;
+                if (next_empty + 1 < N && st[2 * (size_t)(next_empty + 1)] == OVG_EMPTY)
+                    ++next_empty;
+                else
+                    next_empty = -1;
                 return r;
             }
         }
         for (int k = 0; k < N; ++k)
-            if (st[2 * k] == OVG_EMPTY) {
-                next_empty = (k + 1 < N && st[2 * (k + 1)] == OVG_EMPTY) ? k + 1 : -1;
+            if (st[2 * (size_t)k] == OVG_EMPTY) {
+                next_empty = (k + 1 < N && st[2 * (size_t)(k + 1)] == OVG_EMPTY) ? k + 1 : -1;
                 return k;
             }
         return -1;
@@ -885,10 +938,15 @@ struct HostTangle {
     int next_empty_node()
     {
         int idx = search();
-        while (idx < 0 || st[2 * idx] != OVG_EMPTY) {
+        while (idx < 0 || st[2 * (size_t)idx] != OVG_EMPTY) {
             int old = N;
             N = 2 * old;
-            X.resize(3 * (size_t)N, 0.0);
+            if (!grown) {
+                Xv.assign(X, X + 3 * (size_t)old);
+                grown = true;
+            }
+            Xv.resize(3 * (size_t)N, 0.0);
+            X = Xv.data();
             conn.resize(2 * (size_t)N, -1);
             st.resize(2 * (size_t)N, -1);
             next_empty = search();
@@ -903,12 +961,12 @@ int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
 {
     using namespace ovg::host;
     int nw = h.next_empty_node();
-    h.st[2 * nw] = OVG_FREE;
-    h.st[2 * nw + 1] = -1;
-    int next = h.conn[2 * p];
+    h.st[2 * (size_t)nw] = OVG_FREE;
+    h.st[2 * (size_t)nw + 1] = -1;
+    int next = h.conn[2 * (size_t)p];
     double t0[3], n0[3], t1[3], n1[3];
-    tangent_normal(P, h.X.data(), h.conn.data(), p, t0, n0);
-    tangent_normal(P, h.X.data(), h.conn.data(), next, t1, n1);
+    tangent_normal(P, h.X, h.conn.data(), p, t0, n0);
+    tangent_normal(P, h.X, h.conn.data(), next, t1, n1);
     double s0[3] = {h.X[3 * (size_t)p], h.X[3 * (size_t)p + 1], h.X[3 * (size_t)p + 2]};
     double s1[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1], h.X[3 * (size_t)next + 2]};
     pwrap(P, s0, s1);
@@ -933,15 +991,22 @@ int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
         for (int c = 0; c < 3; ++c) nwp[c] = nwp[c] * 0.5;
     }
     for (int c = 0; c < 3; ++c) h.X[3 * (size_t)nw + c] = nwp[c];
-    h.conn[2 * nw + 1] = p;
-    h.conn[2 * nw] = next;
-    h.conn[2 * p] = nw;
-    h.conn[2 * next + 1] = nw;
+    h.conn[2 * (size_t)nw + 1] = p;
+    h.conn[2 * (size_t)nw] = next;
+    h.conn[2 * (size_t)p] = nw;
+    h.conn[2 * (size_t)next + 1] = nw;
     return nw;
 }
 
 }  // namespace
 
+// The reference visits every slot in index order (src/tangle.c:1021).  A
+// slot whose links are unchanged when the loop reaches it decides exactly as
+// it would with the links before the pass (positions of existing points do
+// not move), so the pass visits, in index order, the device's candidates
+// (that test with the links before the pass, widened by a margin) and every
+// slot its own edits reach ahead of the cursor; all other slots would do
+// nothing.
 extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added,
                           int *n_removed)
 {
@@ -949,55 +1014,91 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     OVG_TRY(check_ready(ctx));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    HostTangle h;
-    h.N = c.n;
-    h.X.resize(3 * (size_t)c.n);
-    OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)c.n));
-    launch_soa_to_aos(c, dp(c.x[0]), dp(c.x[1]), dp(c.x[2]), c.partial.as<double>());
-    OVG_HIP(hipMemcpyAsync(h.X.data(), c.partial.p, sizeof(double) * 3 * (size_t)c.n,
-                           hipMemcpyDeviceToHost, c.stream));
-    OVG_HIP(hipStreamSynchronize(c.stream));
-    h.conn = c.h_conn;
-    h.st = c.h_status;
-    h.next_empty = c.next_empty;
+    if (n_added) *n_added = 0;
+    if (n_removed) *n_removed = 0;
+    std::vector<int> cands;
+    OVG_TRY(remesh_candidates(c, min_dist, max_dist, cands));
+    if (cands.empty()) return OVG_OK;
+    const double *Xp = nullptr;
+    OVG_TRY(host_positions(c, &Xp));
+    HostTangle h(c.n, Xp, c.h_conn, c.h_status, c.next_empty);
     TangleParams P = host_tparams(c.params, 0.0);
     int added = 0, removed = 0;
-    for (int k = 0; k < h.N; ++k) {
-        if (h.st[2 * k] == OVG_EMPTY) continue;
-        int next = h.conn[2 * k], prev = h.conn[2 * k + 1];
+    std::priority_queue<int, std::vector<int>, std::greater<int>> ahead;
+    int cursor = -1;
+    size_t ci = 0;
+    std::vector<int> touched;
+    auto touch = [&](int sl) {
+        if (sl < 0) return;
+        touched.push_back(sl);
+        if (sl > cursor) ahead.push(sl);
+    };
+    std::vector<int> inserted;
+    for (;;) {
+        const int a = ci < cands.size() ? cands[ci] : INT32_MAX;
+        const int b = ahead.empty() ? INT32_MAX : ahead.top();
+        const int k = a < b ? a : b;
+        if (k == INT32_MAX) break;
+        if (a == k) ++ci;
+        while (!ahead.empty() && ahead.top() == k) ahead.pop();
+        if (k <= cursor) continue;
+        cursor = k;
+        if (k >= h.N || h.st[2 * (size_t)k] == OVG_EMPTY) continue;
+        int next = h.conn[2 * (size_t)k], prev = h.conn[2 * (size_t)k + 1];
         double lf = 0, lr = 0;
         const double *xk = &h.X[3 * (size_t)k];
         if (next >= 0) {
-            double w[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1], h.X[3 * (size_t)next + 2]};
+            double w[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1],
+                           h.X[3 * (size_t)next + 2]};
             host::pwrap(P, xk, w);
             lf = host::dist3(xk, w);
         }
         if (prev >= 0) {
-            double w[3] = {h.X[3 * (size_t)prev], h.X[3 * (size_t)prev + 1], h.X[3 * (size_t)prev + 2]};
+            double w[3] = {h.X[3 * (size_t)prev], h.X[3 * (size_t)prev + 1],
+                           h.X[3 * (size_t)prev + 2]};
             host::pwrap(P, xk, w);
             lr = host::dist3(xk, w);
         }
         if (next >= 0 && prev >= 0 && ((lf < min_dist || lr < min_dist) && (lf + lr) < max_dist)) {
-            int pv = h.conn[2 * k + 1], nx = h.conn[2 * k];  // remove_point :342-355
-            if (pv >= 0) h.conn[2 * pv] = nx;
-            if (nx >= 0) h.conn[2 * nx + 1] = pv;
-            h.conn[2 * k] = h.conn[2 * k + 1] = -1;
-            h.st[2 * k] = OVG_EMPTY;
+            int pv = h.conn[2 * (size_t)k + 1], nx = h.conn[2 * (size_t)k];  // remove_point :342-355
+            if (pv >= 0) h.conn[2 * (size_t)pv] = nx;
+            if (nx >= 0) h.conn[2 * (size_t)nx + 1] = pv;
+            h.conn[2 * (size_t)k] = h.conn[2 * (size_t)k + 1] = -1;
+            h.st[2 * (size_t)k] = OVG_EMPTY;
             ++removed;
+            touch(k);
+            touch(pv);
+            touch(nx);
         }
         if (next >= 0 && lf > max_dist) {
             ++added;
-            add_point(h, P, k);
+            const int nxt = h.conn[2 * (size_t)k];
+            const int nw = add_point(h, P, k);
+            inserted.push_back(nw);
+            touch(nw);
+            touch(k);
+            touch(nxt);
         }
     }
     c.next_empty = h.next_empty;
     if (n_added) *n_added = added;
     if (n_removed) *n_removed = removed;
     if (!added && !removed) return OVG_OK;
-    if (h.N != c.n) OVG_TRY(alloc_state(c, h.N));
-    c.n = h.N;
-    OVG_TRY(upload_positions(c, h.X.data()));
-    OVG_TRY(upload_topology(c, h.conn.data(), h.st.data()));
+    if (h.grown) {
+        // the tangle doubled: reallocate and upload the whole state
+        OVG_TRY(alloc_state(c, h.N));
+        c.n = h.N;
+        OVG_TRY(upload_positions(c, h.X));
+        OVG_TRY(upload_topology(c, c.h_conn.data(), c.h_status.data()));
+    } else {
+        for (int sl : touched) mark_dirty(c, sl);
+        for (int nw : inserted) {
+            if (c.h_status[2 * (size_t)nw] == OVG_EMPTY) continue;
+            c.new_pos.push_back(nw);
+            for (int q = 0; q < 3; ++q) c.new_xyz.push_back(h.X[3 * (size_t)nw + q]);
+        }
+        OVG_TRY(commit_topology(c));
+    }
     if (added) {
         launch_wrap(c);  // enforce_periodic_boundaries after insertions (:1057-1060)
         OVG_HIP(hipStreamSynchronize(c.stream));

@@ -107,6 +107,21 @@ struct Ctx {
 
     std::vector<int> h_conn, h_status;  // host mirrors of topology
     uint64_t topo_version = 1;          // bumped whenever links change
+    // Incremental topology: host passes record the slots they touch; a
+    // commit scatters just those slots to the device and rebuilds the live
+    // list there (k_topo.hip).
+    std::vector<int> dirty;             // slots with changed links/status
+    std::vector<char> dirty_mark;
+    std::vector<int> new_pos;           // slots with new positions (inserted points)
+    std::vector<double> new_xyz;        // their positions, 3 per slot
+    std::vector<int> loop_dirty;        // slots whose loop may have shrunk or split
+    std::vector<char> loop_dirty_mark;
+    bool loops_full_scan = true;        // next small-loop pass visits every loop
+    int small_cutoff_seen = 0;          // largest cutoff of a completed pass
+    double *h_xpin = nullptr;           // pinned AoS position mirror
+    size_t h_xpin_cap = 0;              // doubles
+    std::vector<int> rc_where, rc_flag; // reconnection scratch, kept -1 / 0
+    DBuf topo_stage;                    // device staging of commits
     int next_empty = 0;                 // the reference allocator's hint (tangle->next_empty)
     uint64_t loops_version = 0;         // topology the smoothing loop table matches
     int n_loops = 0;
@@ -147,7 +162,21 @@ int build_tree(Ctx &c, const double *const x[3]);
 // k_bh.hip
 int launch_bh(Ctx &c, const double *const x[3]);
 void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist);
+// k_list.hip
 int launch_bh_list(Ctx &c, const double *const x[3]);
+
+// k_topo.hip
+// Pinned host copy of the positions (AoS, c.n slots), valid until the next
+// device-side change of positions.
+int host_positions(Ctx &c, const double **X);
+// Record a slot whose links or status changed (also for the small-loop pass).
+void mark_dirty(Ctx &c, int slot);
+// Scatter the dirty slots' links/status and the new positions to the device,
+// rebuild the live list there and bump topo_version.
+int commit_topology(Ctx &c);
+// Remesh candidates: live slots whose original-link decision may remove or
+// insert (a superset of the slots the sequential pass acts on first).
+int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out);
 
 // k_recon.hip
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);

@@ -280,17 +280,10 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
                            c.stream));
     OVG_HIP(hipMemcpyAsync(cl.data(), pk + total, sizeof(int) * (size_t)total,
                            hipMemcpyDeviceToHost, c.stream));
-    // positions (host copy, AoS)
+    // positions (pinned host copy, AoS)
     int n = c.n;
-    std::vector<double> X(3 * (size_t)n);
-    {
-        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)n));
-        launch_soa_to_aos(c, c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>(),
-                          c.partial.as<double>());
-        OVG_HIP(hipMemcpyAsync(X.data(), c.partial.p, sizeof(double) * 3 * (size_t)n,
-                               hipMemcpyDeviceToHost, c.stream));
-    }
-    OVG_HIP(hipStreamSynchronize(c.stream));
+    const double *X = nullptr;
+    OVG_TRY(host_positions(c, &X));
     TangleParams P = host_tparams(c.params, t);
     // direct Biot-Savart at every candidate endpoint with the original links
     std::vector<int> ends;
@@ -302,7 +295,8 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     std::sort(ends.begin(), ends.end());
     ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
     int m = (int)ends.size();
-    std::vector<int> where(n, -1);
+    std::vector<int> &where = c.rc_where;  // all -1 between passes
+    if ((int)where.size() < n) where.assign((size_t)n, -1);
     for (int i = 0; i < m; ++i) where[ends[i]] = i;
     std::vector<double> bsp(3 * (size_t)m), bsi(3 * (size_t)m);
     {
@@ -341,7 +335,9 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     }
     // greedy pass in (k, l) order
     std::vector<int> &conn = c.h_conn;
-    std::vector<int> flag(n, 0);
+    std::vector<int> &flag = c.rc_flag;  // all 0 between passes
+    if ((int)flag.size() < n) flag.assign((size_t)n, 0);
+    std::vector<int> flagged;
     struct Change {
         int seg, old_f, new_f;
     };
@@ -370,12 +366,12 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
         for (const Change &ch : changes) {
             if (!exact && !near(ch.seg, p)) continue;
             double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
-            host::segment_all_images(P, X.data(), ch.seg, ch.old_f, p, vo);
-            host::segment_all_images(P, X.data(), ch.seg, ch.new_f, p, vn);
+            host::segment_all_images(P, X, ch.seg, ch.old_f, p, vo);
+            host::segment_all_images(P, X, ch.seg, ch.new_f, p, vn);
             for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
         }
         double vs[3];
-        host::compose(P, X.data(), conn.data(), p, tn, nr, bs, vs, vels);
+        host::compose(P, X, conn.data(), p, tn, nr, bs, vs, vels);
     };
     int nrec = 0;
     int i = 0;
@@ -388,10 +384,10 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
             int l = cl[j];
             if (flag[l]) continue;
             double w[3] = {X[3 * (size_t)l], X[3 * (size_t)l + 1], X[3 * (size_t)l + 2]};
-            host::pwrap(P, X.data() + 3 * (size_t)k, w);
+            host::pwrap(P, X + 3 * (size_t)k, w);
             double tk[3], nk[3], tl[3], nl[3];
-            host::tangent_normal(P, X.data(), conn.data(), k, tk, nk);
-            host::tangent_normal(P, X.data(), conn.data(), l, tl, nl);
+            host::tangent_normal(P, X, conn.data(), k, tk, nk);
+            host::tangent_normal(P, X, conn.data(), l, tl, nl);
             if (host::len3(tk) < 0.5 || host::len3(tl) < 0.5) continue;
             double calpha = host::dot3(tk, tl) / host::len3(tk) / host::len3(tl);
             if (calpha > cutoff) continue;
@@ -404,8 +400,9 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
             if (host::dot3(dx, dv) > 0) continue;
             int knext = conn[2 * k], kprev = conn[2 * k + 1];
             int lnext = conn[2 * l], lprev = conn[2 * l + 1];
-            int kind = do_reconnection(P, X.data(), conn.data(), k, l);
+            int kind = do_reconnection(P, X, conn.data(), k, l);
             if (!kind) continue;
+            for (int sl : {k, l, knext, kprev, lnext, lprev}) mark_dirty(c, sl);
             if (kind == 1) {
                 changes.push_back({k, knext, l});
                 changes.push_back({lprev, l, knext});
@@ -413,25 +410,20 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
                 changes.push_back({l, lnext, k});
                 changes.push_back({kprev, k, lnext});
             }
-            flag[k]++; flag[conn[2 * k]]++; flag[conn[2 * k + 1]]++;
-            flag[knext]++; flag[kprev]++;
-            flag[l]++; flag[conn[2 * l]]++; flag[conn[2 * l + 1]]++;
-            flag[lnext]++; flag[lprev]++;
+            for (int sl : {k, conn[2 * k], conn[2 * k + 1], knext, kprev, l, conn[2 * l],
+                           conn[2 * l + 1], lnext, lprev}) {
+                flag[sl]++;
+                flagged.push_back(sl);
+            }
             ++nrec;
             break;
         }
         i = iend;
     }
     *n_rec = nrec;
-    if (nrec) {
-        // upload the new links
-        OVG_TRY(c.partial.ensure(sizeof(int) * 2 * (size_t)n));
-        OVG_HIP(hipMemcpyAsync(c.partial.p, conn.data(), sizeof(int) * 2 * (size_t)n,
-                               hipMemcpyHostToDevice, c.stream));
-        launch_split_pairs(c, c.partial.as<int>());
-        OVG_HIP(hipStreamSynchronize(c.stream));
-        c.topo_version++;
-    }
+    for (int i = 0; i < m; ++i) where[ends[i]] = -1;
+    for (int sl : flagged) flag[sl] = 0;
+    if (nrec) OVG_TRY(commit_topology(c));  // the rewired links only
     return OVG_OK;
 }
 

@@ -1,0 +1,234 @@
+// Incremental topology maintenance between the host passes that follow the
+// reference's sequential semantics (remesh, reconnection, small-loop
+// removal) and the device state.
+//
+// A pass reads a pinned host copy of the positions, edits the host mirrors
+// of the links and status (Ctx::h_conn / h_status) and records every slot it
+// touches.  commit_topology then moves only those slots to the device (a
+// scatter of (slot, forward, reverse, status) records and of the inserted
+// points' positions) and rebuilds the ordered live-slot list on the device
+// with a flag / exclusive-scan / scatter pass.  remesh_candidates runs the
+// remesh test (src/tangle.c:1039-1051) with the links as they are before
+// the pass on every slot at once, so the host pass only visits candidates
+// and the slots its own edits reach.
+// HBM-bound, tiny: O(changed slots) transfers and O(slots) device passes.
+#include <cstring>
+
+#include "engine_internal.h"
+
+namespace ovg {
+
+int host_positions(Ctx &c, const double **X)
+{
+    const size_t need = 3 * (size_t)(c.n > 0 ? c.n : 1);
+    if (need > c.h_xpin_cap) {
+        if (c.h_xpin) hipHostFree(c.h_xpin);
+        c.h_xpin = nullptr;
+        c.h_xpin_cap = 0;
+        size_t cap = need + need / 4;
+        if (hipHostMalloc((void **)&c.h_xpin, sizeof(double) * cap, hipHostMallocDefault) !=
+            hipSuccess) {
+            c.h_xpin = nullptr;
+            return set_error(OVG_ENOMEM, "pinned host buffer of %zu doubles", cap);
+        }
+        c.h_xpin_cap = cap;
+    }
+    if (c.n > 0) {
+        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)c.n));
+        launch_soa_to_aos(c, c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>(),
+                          c.partial.as<double>());
+        OVG_HIP(hipMemcpyAsync(c.h_xpin, c.partial.p, sizeof(double) * 3 * (size_t)c.n,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    *X = c.h_xpin;
+    return OVG_OK;
+}
+
+void mark_dirty(Ctx &c, int slot)
+{
+    if (slot < 0) return;
+    if ((size_t)slot >= c.dirty_mark.size()) c.dirty_mark.resize((size_t)c.n, 0);
+    if ((size_t)slot >= c.loop_dirty_mark.size()) c.loop_dirty_mark.resize((size_t)c.n, 0);
+    if (!c.dirty_mark[slot]) {
+        c.dirty_mark[slot] = 1;
+        c.dirty.push_back(slot);
+    }
+    if (!c.loop_dirty_mark[slot]) {
+        c.loop_dirty_mark[slot] = 1;
+        c.loop_dirty.push_back(slot);
+    }
+}
+
+// (slot, forward, reverse, status) records
+__global__ void k_scatter_topo(int m, const int4 *__restrict__ rec, int *__restrict__ fwd,
+                               int *__restrict__ rev, int *__restrict__ status)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int4 r = rec[i];
+    fwd[r.x] = r.y;
+    rev[r.x] = r.z;
+    status[r.x] = r.w;
+}
+
+__global__ void k_scatter_pos(int m, const int *__restrict__ slot, const double *__restrict__ xyz,
+                              double *__restrict__ x, double *__restrict__ y,
+                              double *__restrict__ z)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int k = slot[i];
+    x[k] = xyz[3 * i];
+    y[k] = xyz[3 * i + 1];
+    z[k] = xyz[3 * i + 2];
+}
+
+__global__ void k_live_flags(int n, const int *__restrict__ status, int *__restrict__ flag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = status[i] != OVG_EMPTY;
+}
+
+__global__ void k_live_scatter(int n, const int *__restrict__ status,
+                               const int *__restrict__ pos, int *__restrict__ live,
+                               int *__restrict__ src)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || status[i] == OVG_EMPTY) return;
+    live[pos[i]] = i;
+    src[pos[i]] = i;
+}
+
+int commit_topology(Ctx &c)
+{
+    const int m = (int)c.dirty.size(), a = (int)c.new_pos.size();
+    if (m == 0 && a == 0) return OVG_OK;
+    const int n = c.n;
+    // host records, then one staging upload
+    const size_t rec_bytes = sizeof(int4) * (size_t)m;
+    const size_t pos_bytes = sizeof(int) * (size_t)a, xyz_bytes = sizeof(double) * 3 * (size_t)a;
+    const size_t off_pos = (rec_bytes + 255) & ~(size_t)255;
+    const size_t off_xyz = (off_pos + pos_bytes + 255) & ~(size_t)255;
+    const size_t total = off_xyz + xyz_bytes;
+    std::vector<unsigned char> h(total);
+    int4 *rec = reinterpret_cast<int4 *>(h.data());
+    int live = 0;
+    for (int i = 0; i < m; ++i) {
+        const int k = c.dirty[i];
+        rec[i] = make_int4(k, c.h_conn[2 * (size_t)k], c.h_conn[2 * (size_t)k + 1],
+                           c.h_status[2 * (size_t)k]);
+        c.dirty_mark[k] = 0;
+    }
+    if (a) {
+        std::memcpy(h.data() + off_pos, c.new_pos.data(), pos_bytes);
+        std::memcpy(h.data() + off_xyz, c.new_xyz.data(), xyz_bytes);
+    }
+    OVG_TRY(c.topo_stage.ensure(total));
+    OVG_HIP(hipMemcpyAsync(c.topo_stage.p, h.data(), total, hipMemcpyHostToDevice, c.stream));
+    unsigned char *d = c.topo_stage.as<unsigned char>();
+    if (m)
+        hipLaunchKernelGGL(k_scatter_topo, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,
+                           reinterpret_cast<const int4 *>(d), c.fwd.as<int>(), c.rev.as<int>(),
+                           c.status.as<int>());
+    if (a)
+        hipLaunchKernelGGL(k_scatter_pos, dim3((a + 255) / 256), dim3(256), 0, c.stream, a,
+                           reinterpret_cast<const int *>(d + off_pos),
+                           reinterpret_cast<const double *>(d + off_xyz), c.x[0].as<double>(),
+                           c.x[1].as<double>(), c.x[2].as<double>());
+    // ordered live list on the device
+    OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)(n + 1)));
+    int *flag = c.rc_tmp.as<int>();
+    hipLaunchKernelGGL(k_live_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.status.as<int>(), flag);
+    launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp);
+    hipLaunchKernelGGL(k_live_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.status.as<int>(), flag, c.live.as<int>(), c.src.as<int>());
+    OVG_HIP(hipMemcpyAsync(&live, flag + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    c.n_live = live;
+    c.n_src = live;  // closed loops: every live slot starts a segment
+    c.dirty.clear();
+    c.new_pos.clear();
+    c.new_xyz.clear();
+    c.topo_version++;
+    return OVG_OK;
+}
+
+// Remesh test of src/tangle.c:1039-1051 with the current links, widened by
+// a relative margin so that rounding differences between this evaluation
+// and the host pass can only add candidates.
+__global__ void k_remesh_flags(int n, const int *__restrict__ status, const int *__restrict__ fwd,
+                               const int *__restrict__ rev, const double *__restrict__ X,
+                               const double *__restrict__ Y, const double *__restrict__ Z,
+                               const TangleParams *__restrict__ Pp, double min_dist,
+                               double max_dist, int *__restrict__ flag)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int f = 0;
+    if (status[k] != OVG_EMPTY) {
+        const TangleParams &P = *Pp;
+        const int nx = fwd[k], pv = rev[k];
+        const double x = X[k], y = Y[k], z = Z[k];
+        double lf = 0, lr = 0;
+        if (nx >= 0) {
+            double a = X[nx], b = Y[nx], cz = Z[nx];
+            pwrap(P, x, y, z, a, b, cz);
+            a -= x; b -= y; cz -= z;
+            lf = sqrt(a * a + b * b + cz * cz);
+        }
+        if (pv >= 0) {
+            double a = X[pv], b = Y[pv], cz = Z[pv];
+            pwrap(P, x, y, z, a, b, cz);
+            a -= x; b -= y; cz -= z;
+            lr = sqrt(a * a + b * b + cz * cz);
+        }
+        const double up = 1.0 + 1e-9, dn = 1.0 - 1e-9;
+        const bool rm = nx >= 0 && pv >= 0 && (lf < min_dist * up || lr < min_dist * up) &&
+                        (lf + lr) < max_dist * up;
+        const bool ad = nx >= 0 && lf > max_dist * dn;
+        // a non-finite length decides nothing in the reference either way;
+        // let the host look at it
+        f = rm || ad || !(lf == lf) || !(lr == lr);
+    }
+    flag[k] = f;
+}
+
+__global__ void k_compact(int n, const int *__restrict__ flag, const int *__restrict__ pos,
+                          int *__restrict__ out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && flag[k]) out[pos[k]] = k;
+}
+
+int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out)
+{
+    const int n = c.n;
+    out.clear();
+    if (n == 0) return OVG_OK;
+    OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)(n + 1)));
+    OVG_TRY(c.rc_count.ensure(sizeof(int) * (size_t)(n + 1)));
+    OVG_TRY(c.rc_sorted.ensure(sizeof(int) * (size_t)(n + 1)));
+    int *flag = c.rc_tmp.as<int>(), *pos = c.rc_count.as<int>(), *lst = c.rc_sorted.as<int>();
+    push_params(c, 0.0);
+    hipLaunchKernelGGL(k_remesh_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.status.as<int>(), c.fwd.as<int>(), c.rev.as<int>(), c.x[0].as<double>(),
+                       c.x[1].as<double>(), c.x[2].as<double>(), dparams(c), min_dist, max_dist,
+                       flag);
+    launch_exclusive_scan(c, flag, pos, n, pos + n, c.tree.scan_tmp);
+    hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, flag, pos,
+                       lst);
+    int cnt = 0;
+    OVG_HIP(hipMemcpyAsync(&cnt, pos + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    out.resize(cnt);
+    if (cnt) {
+        OVG_HIP(hipMemcpyAsync(out.data(), lst, sizeof(int) * (size_t)cnt, hipMemcpyDeviceToHost,
+                               c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    return OVG_OK;
+}
+
+}  // namespace ovg

@@ -369,6 +369,45 @@ def test_remesh_golden():
     assert np.array_equal(o["vnodes"][live], g["x_after"][live])
 
 
+def test_topology_sequence_vs_oracle(oracle):
+    """remesh / small-loop removal / reconnection in sequence on one engine
+    (device candidates, host pass on the touched slots only, delta commits)
+    against the oracle's full sequential passes, bit-exact at every stage."""
+    t, p, _ = configs.make("c2", rings=60)
+    rng = np.random.default_rng(31)
+    live = t.live
+    t.vnodes[live] = (t.vnodes[live] + rng.normal(scale=2e-4, size=(len(live), 3))) % 0.1
+    p.method = ov.METHOD_DIRECT
+    p.rec_dist = 4e-4
+    say(f"topology sequence on {len(live)} jittered points")
+    want, ne = t.copy(), 0
+
+    def check(e, want, tag):
+        o = e.download()
+        conn, st = e.download_connections()
+        assert e.n == want.n, tag
+        assert np.array_equal(conn, want.conn), tag
+        assert np.array_equal(st, want.status), tag
+        lv = np.nonzero(st[:, 0] != -1)[0]
+        assert np.array_equal(o["vnodes"][lv], want.vnodes[lv]), tag
+
+    with engine(t, p) as e:
+        for step, (lo, hi) in enumerate(((4e-4, 1.2e-3), (5e-4, 1.0e-3))):
+            want, added, removed, ne = oracle.remesh(want, p, lo, hi, next_empty=ne)
+            a, r = e.remesh(lo, hi)
+            assert (a, r) == (added, removed) and added > 0 and removed > 0
+            check(e, want, f"remesh {step}")
+            n_small = oracle.eliminate_small_loops(want, 5)
+            assert e.eliminate_small_loops(5) == n_small
+            check(e, want, f"small loops {step}")
+            n_rec = oracle.reconnect(want, p)
+            assert e.reconnect(0.0) == n_rec
+            check(e, want, f"reconnect {step}")
+            n_small = oracle.eliminate_small_loops(want, 5)
+            assert e.eliminate_small_loops(5) == n_small
+            check(e, want, f"small loops after reconnect {step}")
+
+
 def test_remesh_capacity_doubling(oracle):
     t, p, _ = configs.make("c1", rings=40)
     seed, _, D, rmin, rmax, dxi = configs.CONFIGS["c1"][:6]
