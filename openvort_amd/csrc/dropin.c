@@ -63,11 +63,13 @@ static void read_env(void)
     if (g_configured) return;
     const char *s = getenv("OPENVORT_PRECISION");
     if (s && strcmp(s, "mixed") == 0) g_precision = OVG_PREC_MIXED;
-    s = getenv("OPENVORT_TRAVERSE");
-    if (s && strcmp(s, "group") == 0) g_traverse = OVG_TRAVERSE_GROUP;
     s = getenv("OPENVORT_LEAF");
     if (s) g_leaf = atoi(s);
-    if (g_precision != OVG_PREC_FP64 || g_leaf != 1) g_traverse = OVG_TRAVERSE_GROUP;
+    /* the point traversal is the reference's FP64 split-until-one walk */
+    if (g_precision != OVG_PREC_FP64 || g_leaf != 1) g_traverse = OVG_TRAVERSE_LIST;
+    s = getenv("OPENVORT_TRAVERSE");
+    if (s && strcmp(s, "group") == 0) g_traverse = OVG_TRAVERSE_GROUP;
+    if (s && strcmp(s, "list") == 0) g_traverse = OVG_TRAVERSE_LIST;
     g_configured = 1;
 }
 
