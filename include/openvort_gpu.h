@@ -237,7 +237,9 @@ int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections);
 /* eliminate_small_loops, src/tangle.c:1063-1128, for closed loops: a loop
  * of m points is deleted when m - 1 < cutoff (the reference's count;
  * small_loop_cutoff = 5, src/initial_conditions.c:43).  Runs on the host
- * link mirror and re-uploads the topology. */
+ * link mirror; after the first pass only loops through slots whose links
+ * changed since the previous pass are walked, and only the deleted slots
+ * are sent to the device. */
 int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
 
 /* remesh, src/tangle.c:1018-1061: remove a point whose forward or reverse
@@ -246,8 +248,11 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
  * every point whose forward segment is longer than max_dist; wrap into the
  * box if points were added.  New points take the slots the reference's
  * allocator would (src/tangle.c:201-253), doubling the slot count when full
- * (expand_tangle, :68-101); ovg_num_slots reports the new count.  Runs on
- * the host copy of the state and re-uploads it. */
+ * (expand_tangle, :68-101); ovg_num_slots reports the new count.  The
+ * device flags the slots whose test (with the links before the pass) can
+ * act; the host runs the reference's sequential pass over those and the
+ * slots its own edits reach, and sends only the changed slots back (the
+ * whole state when the slot count doubled). */
 int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
 
 /* smooth, src/vortex_smooth.c:459-462 (low_pass_filter per loop and
