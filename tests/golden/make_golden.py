@@ -66,6 +66,49 @@ def remesh_fixture(R):
          n_after=rt.n, x_after=after["vnodes"], conn_after=after["conn"], status_after=st_after)
 
 
+def line_length(x, conn, st, D):
+    """Sum over live slots of |seg_pwrap(x_k, x_fwd(k))| in a cube of side D."""
+    live = np.nonzero((st[:, 0] != -1) & (conn[:, 0] >= 0))[0]
+    d = x[conn[live, 0]] - x[live]
+    d -= D * np.round(d / D)
+    return float(np.linalg.norm(d, axis=1).sum())
+
+
+def c1_100_fixture(R):
+    # C1, direct sum with 26 images, 100 steps of the driver order
+    # (src/main.c:127-138) without smoothing (src/vortex_smooth.c needs
+    # FFTW): line length, live points and reconnections every 10 steps
+    t, p, dt = configs.make("c1")
+    D = float(p.box_hi[0] - p.box_lo[0])
+    dl_min, dl_max = configs.remesh_bounds("c1")
+    rt = R.tangle(t, p)
+    tm, recs = 0.0, 0
+    L, npts, nrec = [], [], []
+    for step in range(100):
+        recs += rt.reconnect(tm)
+        rt.eliminate_small_loops(5)
+        rt.enforce_periodic()
+        rt.remesh(dl_min, dl_max)
+        rt.eliminate_small_loops(5)
+        rt.update_tangle(tm)
+        rt.rk4_step(tm, dt)
+        rt.enforce_periodic()
+        tm += dt
+        if (step + 1) % 10 == 0:
+            o = rt.get()
+            st = rt.status()
+            L.append(line_length(o["vnodes"], o["conn"], st, D))
+            npts.append(int((st[:, 0] != -1).sum()))
+            nrec.append(recs)
+            print(f"step {step + 1}: L {L[-1]:.10f} points {npts[-1]} reconnections {recs}",
+                  flush=True)
+    o = rt.get()
+    save("c1_100steps.npz", config=np.array("c1"), rings=16, sha=np.array(state_hash(t)),
+         params=params_array(p), dt=dt, dl_min=dl_min, dl_max=dl_max,
+         line_length=np.array(L), points=np.array(npts), reconnections=np.array(nrec),
+         x_final=o["vnodes"], conn_final=o["conn"], status_final=rt.status())
+
+
 def main():
     R = RefLib()
     if len(sys.argv) > 1:
@@ -73,6 +116,7 @@ def main():
             globals()[name + "_fixture"](R)
         return
     remesh_fixture(R)
+    c1_100_fixture(R)
     # C1 (16 rings, ~2k points): direct sum with 26 images, full outputs
     t, p, dt = configs.make("c1")
     live = t.live

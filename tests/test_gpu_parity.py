@@ -408,6 +408,51 @@ def test_topology_sequence_vs_oracle(oracle):
             check(e, want, f"small loops after reconnect {step}")
 
 
+def line_length(x, conn, st, D):
+    live = np.nonzero((st[:, 0] != -1) & (conn[:, 0] >= 0))[0]
+    d = x[conn[live, 0]] - x[live]
+    d -= D * np.round(d / D)
+    return float(np.linalg.norm(d, axis=1).sum())
+
+
+def test_c1_100_steps_line_length():
+    """100 full steps (reconnect, small loops, wrap, remesh, small loops,
+    update_tangle, rk4_step, wrap) of the 16-ring tangle, direct FP64 sum,
+    against the reference: line length and line density (L / D^3) every 10
+    steps within 1e-9 relative, live points and reconnections exact, final
+    positions within 1e-9 cm."""
+    g = load_golden("c1_100steps.npz")
+    t, p = golden_tangle(g)
+    D = float(p.box_hi[0] - p.box_lo[0])
+    dt, lo, hi = float(g["dt"]), float(g["dl_min"]), float(g["dl_max"])
+    say("c1 direct FP64, 100 steps")
+    tm, recs = 0.0, 0
+    with engine(t, p) as e:
+        for step in range(100):
+            recs += e.reconnect(tm)
+            e.eliminate_small_loops(5)
+            e.enforce_periodic()
+            e.remesh(lo, hi)
+            e.eliminate_small_loops(5)
+            e.update_tangle(tm)
+            e.rk4_step(tm, dt)
+            e.enforce_periodic()
+            tm += dt
+            if (step + 1) % 10 == 0:
+                i = (step + 1) // 10 - 1
+                x = e.download()["vnodes"]
+                conn, st = e.download_connections()
+                L = line_length(x, conn, st, D)
+                assert int((st[:, 0] != -1).sum()) == int(g["points"][i]), step
+                assert recs == int(g["reconnections"][i]), step
+                assert abs(L - g["line_length"][i]) < 1e-9 * g["line_length"][i], step
+                dens, dens_ref = L / D ** 3, g["line_length"][i] / D ** 3
+                assert abs(dens - dens_ref) < 1e-9 * dens_ref, step
+    assert np.array_equal(conn, g["conn_final"]) and np.array_equal(st, g["status_final"])
+    lv = np.nonzero(st[:, 0] != -1)[0]
+    assert np.abs(x[lv] - g["x_final"][lv]).max() < 1e-9
+
+
 def test_remesh_capacity_doubling(oracle):
     t, p, _ = configs.make("c1", rings=40)
     seed, _, D, rmin, rmax, dxi = configs.CONFIGS["c1"][:6]
