@@ -12,8 +12,8 @@ evaluations), periodic wrap.  State stays on the device between steps; the timed
 starts with the tangle resident in HBM.
 
 The fast path is mixed precision (FP32 far-field pairs, FP64 sums,
-positions, LIA and RK4) with the interaction-list traversal: 64 targets
-per wavefront, groups bounded by one octree cell, buckets of 8.  The plain
+positions, LIA and RK4) with the interaction-list traversal: 16 targets
+per wavefront, groups bounded by one octree cell, no buckets.  The plain
 path is the reference's FP64 split-until-one tree with per-point
 traversal, timed on the same tangle beside it.
 
@@ -132,9 +132,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--leaf", type=int, default=8)
+    ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--traverse", default="list", choices=["list", "group", "point"])
-    ap.add_argument("--ppw", type=int, default=64, help="list traversal: points per wavefront")
+    ap.add_argument("--ppw", type=int, default=16, help="list traversal: points per wavefront")
     ap.add_argument("--group-level", type=int, default=0,
                     help="list traversal: octree level that bounds a group (0 = auto)")
     ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])

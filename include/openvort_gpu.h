@@ -131,11 +131,11 @@ typedef struct ovg_params {
     double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
                                      than this (cm) stay FP64; 0 = none        */
     int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 16, 32 or 64
-                                     (0 = 64)                                   */
+                                     (0 = 16)                                   */
     int32_t group_level;          /* OVG_TRAVERSE_LIST: groups never span two
                                      octree cells of this level (1..21); 0 =
                                      the coarsest level whose cells are at most
-                                     8 * mixed_near wide (5 if mixed_near = 0)  */
+                                     4 * mixed_near wide (5 if mixed_near = 0)  */
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a

@@ -111,15 +111,18 @@ __device__ __forceinline__ bool seg_field64(double Rx, double Ry, double Rz, dou
 }
 
 // FP32 pair formula for the far field: R and Rp arrive already rounded to
-// FP32 from FP64 differences.  Same guards as the reference (thresholds in
-// FP32), native reciprocal square roots, one reciprocal.
+// FP32 (from FP64 differences, or as offsets within one target group).
+// Same guards as the reference (thresholds in FP32), native reciprocal
+// square roots, one reciprocal, fused multiply-adds (this path makes no
+// claim to the reference's rounding; the library is otherwise built with
+// contraction off).
 __device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float Px, float Py,
                                             float Pz, float f_bs, float &vx, float &vy,
                                             float &vz)
 {
-    float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
-    float p2 = Px * Px + Py * Py + Pz * Pz;
-    float dot = Rx * Px + Ry * Py + Rz * Pz;
+    float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
+    float p2 = fmaf(Pz, Pz, fmaf(Py, Py, Px * Px));
+    float dot = fmaf(Rz, Pz, fmaf(Ry, Py, Rx * Px));
     float ia = __builtin_amdgcn_rsqf(r2);
     float ib = __builtin_amdgcn_rsqf(p2);
     float iab = ia * ib;
@@ -128,9 +131,9 @@ __device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float 
     // (|R|+|Rp|)/(|R||Rp|(|R||Rp| + dot)) = (ia + ib) / (1 + dot*iab) * iab
     float m = f_bs * (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
     m = ok ? m : 0.0f;
-    float cx = Ry * Pz - Rz * Py;
-    float cy = Rz * Px - Rx * Pz;
-    float cz = Rx * Py - Ry * Px;
+    float cx = fmaf(Ry, Pz, -(Rz * Py));
+    float cy = fmaf(Rz, Px, -(Rx * Pz));
+    float cz = fmaf(Rx, Py, -(Ry * Px));
     vx = cx * m;
     vy = cy * m;
     vz = cz * m;

@@ -2,8 +2,9 @@
 // (OVG_TRAVERSE_LIST).
 //
 // One wavefront (one 64-thread block) per group of PPW Morton-adjacent
-// target points.  For each of the 27 copies (primary, then the images in
-// table order) the wave walks the octree once for the whole group:
+// target points.  The wave walks the octree once for the whole group and
+// all 27 copies (primary and images) together, each stack range carrying
+// its copy index:
 //   - a node is accepted as a monopole when its centre of mass lies farther
 //     than Lm/theta from the group's bounding box (shifted by the image), so
 //     every point of the group would accept it under the reference's
@@ -13,8 +14,8 @@
 //     each of its segments;
 //   - anything else is opened.
 // The walk is parallel over the wave's lanes: a stack in LDS holds ranges
-// [first, end) of sibling nodes in DFS pre-order; each round every lane
-// pops one range, handles its first node and pushes the sibling
+// [first, end) of sibling nodes in DFS pre-order, tagged with their copy;
+// each round every lane pops one range, handles its first node and pushes the sibling
 // continuation [skip, end) and, when it opens the node, its children
 // [d + 1, skip).  Accepted entries are compacted into LDS lists:
 //   far entries (farther than mixed_near from the box, OVG_PREC_MIXED) as
@@ -25,7 +26,8 @@
 // with 64 / PPW slices striding the list, so every lane does useful pair
 // work and the walk is paid once per group instead of once per point.  The
 // per-slice FP64 partial sums are added in slice order at the end, so the
-// result does not depend on scheduling.
+// result does not depend on scheduling.  The primary and image parts are
+// one sum (written as the primary part; the image part is zero).
 //
 // Cost model: VALU-bound in the pair evaluation (44 flops per pair);
 // the walk costs about one node load per lane-round.
@@ -35,14 +37,18 @@
 
 namespace ovg {
 
-constexpr int kListStack = 512;  // ranges per wave
-constexpr int kListFar = 256;    // FP32 far entries per flush
+constexpr int kListStack = 384;  // ranges per wave
+constexpr int kListFar = 128;    // FP32 far entries per flush
 constexpr int kListNear = 128;   // near entries per flush
 // In the single-pop regime the stack grows by at most one pending sibling
 // range per tree level below the popped node (22 levels: root + 21), plus
 // the two pushes of the popped range itself.
 constexpr int kStackMargin = 32;
-constexpr int kNodeTag = 1 << 30;  // near entry index: node (else segment)
+// near entry: bits 0-25 node or segment index, 26-30 copy (0 = primary,
+// c = image c - 1), bit 31 set for a node
+constexpr unsigned kIdxMask = (1u << 26) - 1;
+constexpr int kCopyShift = 26;
+constexpr unsigned kNodeBit = 1u << 31;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -52,15 +58,19 @@ __device__ __forceinline__ int prefix_below(unsigned long long m)
                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
 }
 
-// Inclusive wave prefix sum.
-__device__ __forceinline__ int wave_scan(int v)
+// Inclusive wave prefix sum and wave total of small non-negative values
+// (< 2^bits, bits wave-uniform), one ballot per bit: no cross-lane data
+// movement through LDS.
+__device__ __forceinline__ int wave_scan_small(int v, int bits, int &total)
 {
-    const int l = lane_id();
-    for (int off = 1; off < 64; off <<= 1) {
-        int u = __shfl_up(v, off, 64);
-        if (l >= off) v += u;
+    int below = 0, sum = 0;
+    for (int b = 0; b < bits; ++b) {
+        const unsigned long long m = __ballot((v >> b) & 1);
+        below += prefix_below(m) << b;
+        sum += __popcll(m) << b;
     }
-    return v;
+    total = sum;
+    return below + v;
 }
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -127,13 +137,19 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     __shared__ int2 stk[kListStack];
     __shared__ float4 farA[kListFar];  // offset from the copy's group centre, qx
     __shared__ float2 farB[kListFar];  // qy, qz
-    __shared__ int nearI[kListNear];
-    __shared__ int bEnd[64], bStart[64];  // bucket cells of one round
+    __shared__ unsigned nearI[kListNear];
+    __shared__ int bEnd[64], bStart[64], bCopy[64];  // bucket cells of one round
+    __shared__ double shL[27][3];                    // copy shifts
 
     const TangleParams &P = *Pp;
     const int lane = lane_id();
     const int slice = lane / PPW;
     const int n_groups = counters[kGroupCounter];
+    if (lane < 27) {
+        const bool img = lane >= 1 && lane <= P.n_images;
+        for (int q = 0; q < 3; ++q) shL[lane][q] = img ? P.Ls[lane - 1][q] : 0.0;
+    }
+    __syncthreads();
     for (int g = xcd_group(blockIdx.x, gridDim.x); g < n_groups; g += gridDim.x) {
     const int2 grp = groups[g];
     const bool valid = lane % PPW < grp.y;
@@ -157,212 +173,208 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const float tx = on ? (float)(rx - cx) : 0.0f, ty = on ? (float)(ry - cy) : 0.0f,
                 tz = on ? (float)(rz - cz) : 0.0f;
     const int leaf = P.leaf_size;
+    const int leaf_bits = 32 - __clz(leaf);  // bits of counts 0..leaf
     const double cs = P.crit_scale;
     const double f64 = P.f_bs;
     const float f32 = (float)P.f_bs;
 
-    double apx = 0, apy = 0, apz = 0;  // primary
-    double aix = 0, aiy = 0, aiz = 0;  // images, in table order
+    // one walk for all 27 copies: a stack range carries its copy index, and
+    // a copy's sources are shifted by +L*shift (its targets are r - L*shift)
+    double ax = 0, ay = 0, az = 0;
     unsigned long long c64 = 0, c32 = 0;
+    int far_n = 0, near_n = 0;
+    const int n_copies = any ? 1 + P.n_images : 0;
 
-    for (int im = -1; any && im < P.n_images; ++im) {
-        const double shx = im < 0 ? 0.0 : P.Ls[im][0], shy = im < 0 ? 0.0 : P.Ls[im][1],
-                     shz = im < 0 ? 0.0 : P.Ls[im][2];
-        // The copy's targets are r - sh; sources are shifted by +sh instead,
-        // so the group box and centre stay in scalar registers.
-        double ax = 0, ay = 0, az = 0;
-        int far_n = 0, near_n = 0;
-
-        auto flush_far = [&]() {
-            __syncthreads();
-            for (int i0 = slice; i0 < far_n; i0 += 4 * S) {
-                float sxa = 0.f, sya = 0.f, sza = 0.f;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int i = i0 + u * S;
-                    if (i < far_n) {
-                        const float4 a = farA[i];
-                        const float2 b = farB[i];
-                        const float Rx = a.x - tx, Ry = a.y - ty, Rz = a.z - tz;
-                        float vx, vy, vz;
-                        seg_field32(Rx, Ry, Rz, Rx + a.w, Ry + b.x, Rz + b.y, f32, vx, vy, vz);
-                        sxa += vx; sya += vy; sza += vz;
-                    }
-                }
-                ax += sxa; ay += sya; az += sza;
-            }
-            if (on && far_n > slice) c32 += (far_n - slice + S - 1) / S;
-            __syncthreads();
-            far_n = 0;
-        };
-        auto flush_near = [&]() {
-            __syncthreads();
-            auto load_near = [&](int e, double &Rx, double &Ry, double &Rz, double &Qx,
-                                 double &Qy, double &Qz) {
-                if (e & kNodeTag) {
-                    const Node &nd = nodes[e & (kNodeTag - 1)];
-                    Rx = nd.cx; Ry = nd.cy; Rz = nd.cz;
-                    Qx = nd.qx; Qy = nd.qy; Qz = nd.qz;
-                } else {
-                    Rx = sx[e]; Ry = sy[e]; Rz = sz[e];
-                    Qx = sqx[e]; Qy = sqy[e]; Qz = sqz[e];
-                }
-            };
-            for (int i = slice; i < near_n; i += 2 * S) {
-                // both entries' loads in flight before either is used
-                const bool two = i + S < near_n;
-                double R0x, R0y, R0z, Q0x, Q0y, Q0z, R1x = 0, R1y = 0, R1z = 0, Q1x = 0,
-                       Q1y = 0, Q1z = 0;
-                load_near(nearI[i], R0x, R0y, R0z, Q0x, Q0y, Q0z);
-                if (two) load_near(nearI[i + S], R1x, R1y, R1z, Q1x, Q1y, Q1z);
-                double vx, vy, vz;
-                R0x = (R0x + shx) - rx; R0y = (R0y + shy) - ry; R0z = (R0z + shz) - rz;
-                if (seg_field64(R0x, R0y, R0z, R0x + Q0x, R0y + Q0y, R0z + Q0z, f64, vx, vy,
-                                vz)) {
-                    ax += vx; ay += vy; az += vz;
-                }
-                if (two) {
-                    R1x = (R1x + shx) - rx; R1y = (R1y + shy) - ry; R1z = (R1z + shz) - rz;
-                    if (seg_field64(R1x, R1y, R1z, R1x + Q1x, R1y + Q1y, R1z + Q1z, f64, vx, vy,
-                                    vz)) {
-                        ax += vx; ay += vy; az += vz;
-                    }
-                }
-            }
-            if (on && near_n > slice) c64 += (near_n - slice + S - 1) / S;
-            __syncthreads();
-            near_n = 0;
-        };
-        // append one entry per predicated lane: near as an index, far as FP32
-        auto emit = [&](bool want, bool near, int idx, double ex, double ey, double ez,
-                        double gx, double gy, double gz) {
-            const unsigned long long mf = __ballot(want && !near);
-            const unsigned long long mn = __ballot(want && near);
-            const int cf = __popcll(mf), cn = __popcll(mn);
-            if (cf && far_n + cf > kListFar) flush_far();
-            if (cn && near_n + cn > kListNear) flush_near();
-            if (want && !near) {
-                const int pos = far_n + prefix_below(mf);
-                farA[pos] = make_float4((float)ex, (float)ey, (float)ez, (float)gx);
-                farB[pos] = make_float2((float)gy, (float)gz);
-            }
-            if (want && near) nearI[near_n + prefix_below(mn)] = idx;
-            far_n += cf;
-            near_n += cn;
-        };
-
-        if (lane == 0) stk[0] = make_int2(0, n_nodes);
-        int sp = n_nodes > 0 ? 1 : 0;
+    auto flush_far = [&]() {
         __syncthreads();
-        while (sp > 0) {
-            // each popped range pushes at most two: keep kStackMargin free
-            const int room = kListStack - kStackMargin - sp;
-            const int kp = min(min(sp, 64), max(1, room));
-            const bool act = lane < kp;
-            int2 rg = make_int2(0, 0);
-            if (act) rg = stk[sp - kp + lane];
-            __syncthreads();
-            sp -= kp;
-            const int d = rg.x;
-            Node nd{};
-            int kind = 0;  // 0 nothing, 1 entry, 2 bucket, 3 open
-            int skp = 0;   // subtree end, kept inside the popped range
-            double dd2 = 0;
-            int st = 0;  // first sorted segment of the cell (buckets)
-            if (act) {
-                nd = nodes[d];
-                st = meta[d].start;
-                skp = min(max(nd.skip, d + 1), rg.y);
-                if (nd.count > 0) {
-                    dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix,
-                                    hiy, hiz);
-                    if (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs)
-                        kind = 1;
-                    else if (nd.count <= leaf || skp == d + 1)
-                        kind = 2;
-                    else
-                        kind = 3;
+        for (int i0 = slice; i0 < far_n; i0 += 4 * S) {
+            float sxa = 0.f, sya = 0.f, sza = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * S;
+                if (i < far_n) {
+                    const float4 a = farA[i];
+                    const float2 b = farB[i];
+                    const float Rx = a.x - tx, Ry = a.y - ty, Rz = a.z - tz;
+                    float vx, vy, vz;
+                    seg_field32(Rx, Ry, Rz, Rx + a.w, Ry + b.x, Rz + b.y, f32, vx, vy, vz);
+                    sxa += vx; sya += vy; sza += vz;
                 }
             }
-            // pushes (both ranges nonempty, first index beyond d, end <= the
-            // popped range's end): sibling continuation, then the children
-            const bool sib = act && skp < rg.y;
-            const int np = (int)sib + (kind == 3 ? 1 : 0);
-            const int incl = wave_scan(np);
-            const int base = sp + incl - np;
-            if (sib) stk[base] = make_int2(skp, rg.y);
-            if (kind == 3) stk[base + (int)sib] = make_int2(d + 1, skp);
-            sp += __builtin_amdgcn_readlane(incl, 63);
-            // monopoles and single segments
-            {
-                const bool want = kind == 1;
-                const bool near = !MIXED || dd2 < near2;
-                emit(want, near, d | kNodeTag, (nd.cx + shx) - cx, (nd.cy + shy) - cy,
-                     (nd.cz + shz) - cz, nd.qx, nd.qy, nd.qz);
-            }
-            // buckets: the round's bucket cells laid end to end, expanded
-            // 64 segments per pass (one per lane, its cell found by binary
-            // search over the cells' end offsets)
-            const int bc = kind == 2 ? nd.count : 0;
-            const int bincl = wave_scan(bc);
-            const int btotal = __builtin_amdgcn_readlane(bincl, 63);
-            if (btotal > 0) {
-                const unsigned long long mb = __ballot(bc > 0);
-                const int nb = __popcll(mb);
-                if (bc > 0) {
-                    const int slot = prefix_below(mb);
-                    bEnd[slot] = bincl;
-                    bStart[slot] = st;
-                }
-                __syncthreads();
-                for (int base = 0; base < btotal; base += 64) {
-                    const int j = base + lane;
-                    const bool want = j < btotal;
-                    double ex = 0, ey = 0, ez = 0, gx = 0, gy = 0, gz = 0, b2 = 0;
-                    int s = 0;
-                    if (want) {
-                        int lo = 0, hi = nb - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (bEnd[mid] > j) hi = mid;
-                            else lo = mid + 1;
-                        }
-                        s = bStart[lo] + j - (lo ? bEnd[lo - 1] : 0);
-                        const double ux = sx[s] + shx, uy = sy[s] + shy, uz = sz[s] + shz;
-                        gx = sqx[s]; gy = sqy[s]; gz = sqz[s];
-                        ex = ux - cx; ey = uy - cy; ez = uz - cz;
-                        b2 = box_dist2(ux, uy, uz, lox, loy, loz, hix, hiy, hiz);
-                    }
-                    emit(want, !MIXED || b2 < near2, s, ex, ey, ez, gx, gy, gz);
-                }
-            }
-            __syncthreads();
+            ax += sxa; ay += sya; az += sza;
         }
-        if (far_n) flush_far();
-        if (near_n) flush_near();
-        if (im < 0) {
-            apx = ax; apy = ay; apz = az;
+        if (on && far_n > slice) c32 += (far_n - slice + S - 1) / S;
+        __syncthreads();
+        far_n = 0;
+    };
+    auto flush_near = [&]() {
+        __syncthreads();
+        // R = (source + shift) - r, Rp = R + sum ds
+        auto load_near = [&](unsigned e, double &Rx, double &Ry, double &Rz, double &Qx,
+                             double &Qy, double &Qz) {
+            const int idx = (int)(e & kIdxMask), cp = (int)((e >> kCopyShift) & 31u);
+            if (e & kNodeBit) {
+                const Node &nd = nodes[idx];
+                Rx = nd.cx; Ry = nd.cy; Rz = nd.cz;
+                Qx = nd.qx; Qy = nd.qy; Qz = nd.qz;
+            } else {
+                Rx = sx[idx]; Ry = sy[idx]; Rz = sz[idx];
+                Qx = sqx[idx]; Qy = sqy[idx]; Qz = sqz[idx];
+            }
+            Rx = (Rx + shL[cp][0]) - rx;
+            Ry = (Ry + shL[cp][1]) - ry;
+            Rz = (Rz + shL[cp][2]) - rz;
+        };
+        for (int i = slice; i < near_n; i += S) {
+            double Rx, Ry, Rz, Qx, Qy, Qz, vx, vy, vz;
+            load_near(nearI[i], Rx, Ry, Rz, Qx, Qy, Qz);
+            if (seg_field64(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz)) {
+                ax += vx; ay += vy; az += vz;
+            }
+        }
+        if (on && near_n > slice) c64 += (near_n - slice + S - 1) / S;
+        __syncthreads();
+        near_n = 0;
+    };
+    // append one entry per predicated lane: near as a packed index, far as
+    // FP32 offsets from the group centre
+    auto emit = [&](bool want, bool near, unsigned idx, float ex, float ey, float ez,
+                    float gx, float gy, float gz) {
+        const unsigned long long mf = __ballot(want && !near);
+        const unsigned long long mn = __ballot(want && near);
+        const int cf = __popcll(mf), cn = __popcll(mn);
+        if (cf && far_n + cf > kListFar) flush_far();
+        if (cn && near_n + cn > kListNear) flush_near();
+        if (want && !near) {
+            const int pos = far_n + prefix_below(mf);
+            farA[pos] = make_float4(ex, ey, ez, gx);
+            farB[pos] = make_float2(gy, gz);
+        }
+        if (want && near) nearI[near_n + prefix_below(mn)] = idx;
+        far_n += cf;
+        near_n += cn;
+    };
+
+    if (lane < n_copies) stk[lane] = make_int2(0, n_nodes | (lane << kCopyShift));
+    int sp = n_nodes > 0 ? n_copies : 0;
+    __syncthreads();
+    while (sp > 0) {
+        // each popped range pushes at most two: keep kStackMargin free
+        const int room = kListStack - kStackMargin - sp;
+        const int kp = min(min(sp, 64), max(1, room));
+        const bool act = lane < kp;
+        int2 rg = make_int2(0, 0);
+        if (act) rg = stk[sp - kp + lane];
+        __syncthreads();
+        sp -= kp;
+        const int d = rg.x;
+        const int cp = rg.y >> kCopyShift, end = rg.y & (int)kIdxMask;
+        const double shx = shL[cp][0], shy = shL[cp][1], shz = shL[cp][2];
+        Node nd{};
+        int kind = 0;  // 0 nothing, 1 entry, 2 bucket, 3 open
+        int skp = 0;   // subtree end, kept inside the popped range
+        double dd2 = 0;
+        int st = 0;  // first sorted segment of the cell (buckets)
+        if (act) {
+            nd = nodes[d];
+            st = meta[d].start;
+            skp = min(max(nd.skip, d + 1), end);
+            if (nd.count > 0) {
+                dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix, hiy,
+                                hiz);
+                if (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs)
+                    kind = 1;
+                else if (nd.count <= leaf || skp == d + 1)
+                    kind = 2;
+                else
+                    kind = 3;
+            }
+        }
+        // pushes (both ranges nonempty, first index beyond d, end <= the
+        // popped range's end): sibling continuation, then the children
+        const bool sib = act && skp < end;
+        const bool opn = kind == 3;
+        const unsigned long long ms = __ballot(sib), mo = __ballot(opn);
+        const int base = sp + prefix_below(ms) + prefix_below(mo);
+        if (sib) stk[base] = make_int2(skp, end | (cp << kCopyShift));
+        if (opn) stk[base + (int)sib] = make_int2(d + 1, skp | (cp << kCopyShift));
+        sp += __popcll(ms) + __popcll(mo);
+        // monopoles and single segments
+        emit(kind == 1, !MIXED || dd2 < near2,
+             (unsigned)d | ((unsigned)cp << kCopyShift) | kNodeBit,
+             (float)((nd.cx + shx) - cx), (float)((nd.cy + shy) - cy),
+             (float)((nd.cz + shz) - cz), (float)nd.qx, (float)nd.qy, (float)nd.qz);
+        // buckets: the round's bucket cells laid end to end, expanded 64
+        // segments per pass (one per lane, its cell found by binary search
+        // over the cells' end offsets)
+        const int bc = kind == 2 ? nd.count : 0;
+        int bincl, btotal;
+        if (__ballot(bc > leaf)) {
+            // a childless cell above leaf_size points (coincident keys)
+            bincl = bc;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(bincl, off, 64);
+                if (lane >= off) bincl += u;
+            }
+            btotal = __builtin_amdgcn_readlane(bincl, 63);
         } else {
-            aix += ax; aiy += ay; aiz += az;
+            bincl = wave_scan_small(bc, leaf_bits, btotal);
         }
+        if (btotal > 0) {
+            const unsigned long long mb = __ballot(bc > 0);
+            const int nb = __popcll(mb);
+            if (bc > 0) {
+                const int slot = prefix_below(mb);
+                bEnd[slot] = bincl;
+                bStart[slot] = st;
+                bCopy[slot] = cp;
+            }
+            __syncthreads();
+            for (int b0 = 0; b0 < btotal; b0 += 64) {
+                const int j = b0 + lane;
+                const bool want = j < btotal;
+                float ex = 0, ey = 0, ez = 0, gx = 0, gy = 0, gz = 0;
+                double b2 = 0;
+                int sg = 0, bcp = 0;
+                if (want) {
+                    int lo = 0, hi = nb - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (bEnd[mid] > j) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    sg = bStart[lo] + j - (lo ? bEnd[lo - 1] : 0);
+                    bcp = bCopy[lo];
+                    const double ux = sx[sg] + shL[bcp][0], uy = sy[sg] + shL[bcp][1],
+                                 uz = sz[sg] + shL[bcp][2];
+                    gx = (float)sqx[sg]; gy = (float)sqy[sg]; gz = (float)sqz[sg];
+                    ex = (float)(ux - cx); ey = (float)(uy - cy); ez = (float)(uz - cz);
+                    b2 = box_dist2(ux, uy, uz, lox, loy, loz, hix, hiy, hiz);
+                }
+                emit(want, !MIXED || b2 < near2, (unsigned)sg | ((unsigned)bcp << kCopyShift),
+                     ex, ey, ez, gx, gy, gz);
+            }
+        }
+        __syncthreads();
     }
+    if (far_n) flush_far();
+    if (near_n) flush_near();
     // slice partials in slice order
 #pragma unroll
     for (int s = 1; s < S; ++s) {
         const int src = (lane % PPW) + s * PPW;
-        const double bpx = __shfl(apx, src, 64), bpy = __shfl(apy, src, 64),
-                     bpz = __shfl(apz, src, 64);
-        const double bix = __shfl(aix, src, 64), biy = __shfl(aiy, src, 64),
-                     biz = __shfl(aiz, src, 64);
+        const double bx = __shfl(ax, src, 64), by = __shfl(ay, src, 64),
+                     bz = __shfl(az, src, 64);
         if (slice == 0) {
-            apx += bpx; apy += bpy; apz += bpz;
-            aix += bix; aiy += biy; aiz += biz;
+            ax += bx; ay += by; az += bz;
         }
     }
     if (valid && slice == 0) {
+        // all copies in one sum: the primary/image split of the reference's
+        // summation order is not kept by this traversal
         const double nan = __builtin_nan("");
-        px[k] = on ? apx : nan; py[k] = on ? apy : nan; pz[k] = on ? apz : nan;
-        ix[k] = on ? aix : nan; iy[k] = on ? aiy : nan; iz[k] = on ? aiz : nan;
+        px[k] = on ? ax : nan; py[k] = on ? ay : nan; pz[k] = on ? az : nan;
+        ix[k] = on ? 0.0 : nan; iy[k] = on ? 0.0 : nan; iz[k] = on ? 0.0 : nan;
     }
     for (int off = 32; off > 0; off >>= 1) {
         c64 += __shfl_xor(c64, off, 64);
@@ -434,7 +446,7 @@ int list_group_level(const ovg_params &p)
     double side = 0;
     for (int k = 0; k < 3; ++k) side = std::max(side, p.box_hi[k] - p.box_lo[k]);
     int L = 1;
-    while (L < 10 && side / (double)(1 << L) > 8 * p.mixed_near) ++L;
+    while (L < 10 && side / (double)(1 << L) > 4 * p.mixed_near) ++L;
     return L;
 }
 
@@ -462,12 +474,15 @@ static int build_groups(Ctx &c)
 
 int list_points_per_wave(const ovg_params &p)
 {
-    return p.points_per_wave == 0 ? 64 : p.points_per_wave;
+    return p.points_per_wave == 0 ? 16 : p.points_per_wave;
 }
 
 int launch_bh_list(Ctx &c, const double *const x[3])
 {
     if (c.n_live == 0) return OVG_OK;
+    if (c.n_live >= (1 << 25))
+        return set_error(OVG_ELIMIT, "list traversal: %d points exceed its 2^25 limit",
+                         c.n_live);
     TreeBufs &T = c.tree;
     const int ppw = list_points_per_wave(c.params);
     switch (ppw) {
