@@ -334,7 +334,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     DBuf *tb[] = {&T.keys[0], &T.keys[1], &T.vals[0], &T.vals[1], &T.hist, &T.delta,
                   &T.counts, &T.base, &T.nodes, &T.meta, &T.level_hist, &T.level_list,
                   &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
-                  &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups};
+                  &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups,
+                  &T.rank, &T.sub_tgt, &T.sub_groups, &T.sub_count};
     for (auto *b : tb) b->release();
     if (c.h_xpin) hipHostFree(c.h_xpin);
     c.h_xpin = nullptr;

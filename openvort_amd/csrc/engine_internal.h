@@ -72,6 +72,8 @@ struct TreeBufs {
     DBuf sqx, sqy, sqz;      // FP64 pwrapped segment vectors in sorted order
     DBuf grp_a, grp_b, grp_c;  // target-group construction scratch (int, n + 1)
     DBuf groups;             // int2 (first sorted point, count) per target group
+    DBuf rank;               // sorted position of each live slot
+    DBuf sub_tgt, sub_groups, sub_count;  // list traversal of a target subset
 };
 // counters[kGroupCounter]: number of target groups of the list traversal
 constexpr int kGroupCounter = 40;
@@ -164,6 +166,9 @@ int launch_bh(Ctx &c, const double *const x[3]);
 void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist);
 // k_list.hip
 int launch_bh_list(Ctx &c, const double *const x[3]);
+// The list traversal for m live slots (device list) against the tree built
+// at x: grouped by tree order within level-L0 cells.
+int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d_slots);
 
 // k_topo.hip
 // Pinned host copy of the positions (AoS, c.n slots), valid until the next

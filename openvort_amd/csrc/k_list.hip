@@ -129,9 +129,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const double *__restrict__ sy, const double *__restrict__ sz,
     const double *__restrict__ sqx, const double *__restrict__ sqy,
     const double *__restrict__ sqz, const int *__restrict__ counters,
-    const TangleParams *__restrict__ Pp, double near2, double *__restrict__ px,
-    double *__restrict__ py, double *__restrict__ pz, double *__restrict__ ix,
-    double *__restrict__ iy, double *__restrict__ iz, unsigned long long *__restrict__ icount)
+    const int *__restrict__ n_groups_p, const TangleParams *__restrict__ Pp, double near2,
+    double *__restrict__ px, double *__restrict__ py, double *__restrict__ pz,
+    double *__restrict__ ix, double *__restrict__ iy, double *__restrict__ iz,
+    unsigned long long *__restrict__ icount)
 {
     constexpr int S = 64 / PPW;
     __shared__ int2 stk[kListStack];
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const TangleParams &P = *Pp;
     const int lane = lane_id();
     const int slice = lane / PPW;
-    const int n_groups = counters[kGroupCounter];
+    const int n_groups = *n_groups_p;
     if (lane < 27) {
         const bool img = lane >= 1 && lane <= P.n_images;
         for (int q = 0; q < 3; ++q) shL[lane][q] = img ? P.Ls[lane - 1][q] : 0.0;
@@ -501,7 +502,8 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                            T.sorted_pts.as<int>(), x[0], x[1], x[2], T.nodes.as<Node>(),
                            T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
                            T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                           T.sqz.as<double>(), T.counters.as<int>(), dparams(c), near2,
+                           T.sqz.as<double>(), T.counters.as<int>(),
+                           T.counters.as<int>() + kGroupCounter, dparams(c), near2,
                            c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
@@ -512,6 +514,91 @@ int launch_bh_list(Ctx &c, const double *const x[3])
     case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
     default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;
     }
+    return OVG_OK;
+}
+
+// ----------------------------------------------------- target subsets --
+__global__ void k_rank_key(int m, const int *__restrict__ slots, const int *__restrict__ rank,
+                           const uint64_t *__restrict__ keys, int *__restrict__ out_rank,
+                           uint64_t *__restrict__ out_key)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int r = rank[slots[i]];
+    out_rank[i] = r;
+    out_key[i] = keys[r];
+}
+
+// Same level of 3-bit digits as k_tree.hip's LCP levels.
+static int host_lcp_levels(uint64_t a, uint64_t b)
+{
+    if (a == b) return kMaxLevel;
+    return (__builtin_clzll(a ^ b) - 1) / 3;
+}
+
+int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d_slots)
+{
+    if (m <= 0) return OVG_OK;
+    TreeBufs &T = c.tree;
+    const int ppw = list_points_per_wave(c.params), L0 = list_group_level(c.params);
+    // tree order and keys of the targets, grouped on the host
+    OVG_TRY(T.sub_tgt.ensure((sizeof(int) + sizeof(uint64_t)) * (size_t)m + 64));
+    int *d_rank = T.sub_tgt.as<int>();
+    uint64_t *d_key = reinterpret_cast<uint64_t *>(
+        T.sub_tgt.as<unsigned char>() + ((sizeof(int) * (size_t)m + 15) & ~(size_t)15));
+    hipLaunchKernelGGL(k_rank_key, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, d_slots,
+                       T.rank.as<int>(), T.keys[0].as<uint64_t>(), d_rank, d_key);
+    std::vector<int> rk(m), sl(m);
+    std::vector<uint64_t> ky(m);
+    OVG_HIP(hipMemcpyAsync(rk.data(), d_rank, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost,
+                           c.stream));
+    OVG_HIP(hipMemcpyAsync(ky.data(), d_key, sizeof(uint64_t) * (size_t)m,
+                           hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipMemcpyAsync(sl.data(), d_slots, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost,
+                           c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> order(m);
+    for (int i = 0; i < m; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return rk[a] < rk[b]; });
+    std::vector<int> tgt(m);
+    std::vector<int2> grp;
+    for (int i = 0; i < m; ++i) {
+        tgt[i] = sl[order[i]];
+        const bool cut = i == 0 || grp.back().y == ppw ||
+                         host_lcp_levels(ky[order[i - 1]], ky[order[i]]) < L0;
+        if (cut) grp.push_back(make_int2(i, 1));
+        else grp.back().y++;
+    }
+    const int ng = (int)grp.size();
+    OVG_TRY(T.sub_groups.ensure(sizeof(int2) * (size_t)ng + sizeof(int) * (size_t)m + 64));
+    int2 *d_grp = T.sub_groups.as<int2>();
+    int *d_tgt = reinterpret_cast<int *>(d_grp + ng);
+    OVG_TRY(T.sub_count.ensure(sizeof(int)));
+    OVG_HIP(hipMemcpyAsync(d_grp, grp.data(), sizeof(int2) * (size_t)ng, hipMemcpyHostToDevice,
+                           c.stream));
+    OVG_HIP(hipMemcpyAsync(d_tgt, tgt.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice,
+                           c.stream));
+    OVG_HIP(hipMemcpyAsync(T.sub_count.p, &ng, sizeof(int), hipMemcpyHostToDevice, c.stream));
+    const double near2 = c.params.mixed_near * c.params.mixed_near;
+    unsigned long long *cnt = c.counter64.as<unsigned long long>();
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(ng), dim3(64), 0, c.stream, d_grp, d_tgt, x[0], x[1],
+                           x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
+                           T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
+                           T.sqy.as<double>(), T.sqz.as<double>(), T.counters.as<int>(),
+                           T.sub_count.as<int>(), dparams(c), near2,
+                           c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
+                           c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
+                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
+    };
+    const bool mixed = c.params.precision == OVG_PREC_MIXED;
+    switch (ppw) {
+    case 16: mixed ? go(k_bh_list<true, 16>) : go(k_bh_list<false, 16>); break;
+    case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
+    default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;
+    }
+    // the host vectors must outlive the copies above
+    OVG_HIP(hipStreamSynchronize(c.stream));
     return OVG_OK;
 }
 

@@ -312,7 +312,10 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
         if (c.params.rec_velocity == OVG_RECVEL_TREE && c.params.method == OVG_METHOD_BH) {
             // the step's own Barnes-Hut evaluator, traversed for the endpoints only
             OVG_TRY(build_tree(c, xv));
-            launch_bh_targets(c, xv, m, c.rc_tmp.as<int>());
+            if (c.params.traverse == OVG_TRAVERSE_LIST)
+                OVG_TRY(launch_bh_list_targets(c, xv, m, c.rc_tmp.as<int>()));
+            else
+                launch_bh_targets(c, xv, m, c.rc_tmp.as<int>());
         } else {
             // update_velocity(tree = NULL): FP64 direct sum at the endpoints
             launch_prep_segments(c, xv);

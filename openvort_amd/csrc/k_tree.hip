@@ -349,7 +349,7 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
                                 double *__restrict__ sx, double *__restrict__ sy,
                                 double *__restrict__ sz, double *__restrict__ qx,
                                 double *__restrict__ qy, double *__restrict__ qz,
-                                int8_t *__restrict__ delta)
+                                int8_t *__restrict__ delta, int *__restrict__ rank)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -357,6 +357,7 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
     int n_in = counters[0];
     int k = vals[i];
     spts[i] = k;
+    rank[k] = i;
     double x = X[k], y = Y[k], z = Z[k];
     sx[i] = x; sy[i] = y; sz[i] = z;
     int f = fwd[k];
@@ -555,6 +556,7 @@ int build_tree(Ctx &c, const double *const x[3])
     OVG_TRY(T.meta.ensure(sizeof(NodeMeta) * cap_nodes));
     OVG_TRY(T.counters.ensure(sizeof(int) * 64));
     OVG_TRY(T.sorted_pts.ensure(sizeof(int) * (n + 1)));
+    OVG_TRY(T.rank.ensure(sizeof(int) * ((size_t)c.n + 1)));
     DBuf *sv[6] = {&T.sx, &T.sy, &T.sz, &T.sqx, &T.sqy, &T.sqz};
     for (auto *b : sv) OVG_TRY(b->ensure(sizeof(double) * (n + 1)));
     hipMemsetAsync(T.counters.p, 0, sizeof(int) * 64, c.stream);
@@ -570,7 +572,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
                        T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
-                       T.delta.as<int8_t>());
+                       T.delta.as<int8_t>(), T.rank.as<int>());
     hipLaunchKernelGGL(k_nodes<false>, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),

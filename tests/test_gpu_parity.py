@@ -497,13 +497,15 @@ def test_smooth_giant_loops():
     assert err < 1e-12
 
 
-def test_reconnect_tree_velocity_mode():
+@pytest.mark.parametrize("traverse", [ov.TRAVERSE_POINT, ov.TRAVERSE_LIST])
+def test_reconnect_tree_velocity_mode(traverse):
     g = load_golden("c1_reconnect.npz")
     t, p = golden_tangle(g)
     p.method = ov.METHOD_BH
     p.theta = 1e-9  # the tree opens to single segments: velocities equal the direct sum
     p.rec_velocity = ov.RECVEL_TREE
-    say("reconnect with tree velocities at theta -> 0")
+    p.traverse = traverse
+    say(f"reconnect with tree velocities at theta -> 0, traversal {traverse}")
     with engine(t, p) as e:
         n = e.reconnect(0.0)
         conn, _ = e.download_connections()
