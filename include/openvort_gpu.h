@@ -196,6 +196,8 @@ int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *vs,
 int ovg_download_connections(ovg_ctx *ctx, int *connections, int *status);
 
 int ovg_num_slots(const ovg_ctx *ctx);
+/* Live (non-EMPTY) points of the current tangle. */
+int ovg_num_points(const ovg_ctx *ctx);
 
 /* ---- hot path (the reference prototypes these replace are cited) ----- */
 

@@ -629,6 +629,8 @@ extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *sta
 
 extern "C" int ovg_num_slots(const ovg_ctx *ctx) { return ctx ? ctx->c.n : -1; }
 
+extern "C" int ovg_num_points(const ovg_ctx *ctx) { return ctx ? ctx->c.n_live : 0; }
+
 extern "C" int ovg_update_tangents_normals(ovg_ctx *ctx)
 {
     clear_error();

@@ -26,6 +26,30 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
 
 
+def test_serve_symbols_exported_and_structs_match():
+    from openvort_amd import serve
+    txt = open(os.path.join(ROOT, "include", "openvort_serve.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b(ovs_[a-z0-9_]+)\s*\(", txt))
+    assert {"ovs_run", "ovs_job_bytes"} <= names
+    lib = C.CDLL(ov.LIB_PATH)
+    assert all(hasattr(lib, n) for n in names)
+    # field layouts restated in Python: 8-byte aligned, sizes as declared
+    assert C.sizeof(serve.Job) == 8 + 4 * 2 + 8 * 9 + 4 * 6 + 8
+    assert C.sizeof(serve.Report) % 8 == 0
+
+
+def test_c5_job_stream_is_seeded():
+    from openvort_amd import serve
+    a = serve.c5_jobs(7, 12)
+    b = serve.c5_jobs(7, 12)
+    assert [(j.seed, j.n_rings, j.steps, j.arrival_s) for j in a] == \
+        [(j.seed, j.n_rings, j.steps, j.arrival_s) for j in b]
+    arr = [j.arrival_s for j in a]
+    assert arr == sorted(arr) and arr[0] > 0
+    assert all(50 <= j.steps <= 500 and 0.1 <= j.D <= 0.2 for j in a)
+
+
 def test_dropin_exports_reference_prototypes():
     if not os.path.exists(ov.DROPIN_PATH):
         pytest.skip("libopenvort_dropin.so not built")
