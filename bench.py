@@ -93,6 +93,68 @@ def accuracy(ov, t, p_fast, p_plain, n_sample, seed=7):
             "fast_vs_plain_fp64": rel(out["fast"], out["plain_fp64"])}
 
 
+def serve_bench(args):
+    """The serving stream (or the batched ensemble) through the request loop:
+    fast path with several workers against the plain FP64 path one job at a
+    time (FIFO), both under saturation on the same seeded jobs; then the fast
+    path once more with the jobs' Poisson arrival times for latencies."""
+    from openvort_amd import serve
+    make = (lambda fast: serve.c5_jobs(n=args.jobs, steps_scale=args.steps_scale, fast=fast)) \
+        if args.workload == "c5" else \
+        (lambda fast: serve.c4_jobs(n=args.jobs, steps=max(1, int(round(args.steps_scale * 100))),
+                                    fast=fast))
+    fast_jobs = make(True)
+    pts = sum(ov_points(j) for j in fast_jobs)
+    log(f"{args.workload}: {len(fast_jobs)} jobs, {pts} points at admission")
+    rep, res = serve.run(fast_jobs, workers=args.workers, max_active=args.max_active,
+                         cap_bytes=args.hbm_cap, realtime=False)
+    log(f"fast: {rep['point_steps_per_s']:.4g} point-steps/s, {rep['wall_s']:.1f} s")
+    plain = None
+    if args.plain_steps > 0:
+        prep, _ = serve.run(make(False), workers=1, max_active=1, cap_bytes=args.hbm_cap,
+                            realtime=False)
+        plain = {"value": prep["point_steps_per_s"], "wall_s": prep["wall_s"],
+                 "p50_step_ms": prep["step_p50_ms"], "p99_step_ms": prep["step_p99_ms"],
+                 "path": "FP64, reference split-until-one tree, per-point traversal, "
+                         "one job at a time in arrival order"}
+        log(f"plain: {plain['value']:.4g} point-steps/s, {prep['wall_s']:.1f} s")
+    live = None
+    if args.workload == "c5":
+        lrep, _ = serve.run(fast_jobs, workers=args.workers, max_active=args.max_active,
+                            cap_bytes=args.hbm_cap, realtime=True)
+        live = {"arrivals": "Poisson, 20 jobs/s", "point_steps_per_s": lrep["point_steps_per_s"],
+                "p50_step_ms": lrep["step_p50_ms"], "p99_step_ms": lrep["step_p99_ms"],
+                "p50_job_s": lrep["job_p50_s"], "p99_job_s": lrep["job_p99_s"],
+                "wall_s": lrep["wall_s"]}
+    out = {
+        "metric": f"RK4 point-steps/s, {args.workload} job stream, batched mixed-precision BH "
+                  "serving vs plain FP64 one job at a time",
+        "value": rep["point_steps_per_s"], "unit": "point-steps/s", "n_gpus": 1,
+        "steps": rep["steps_total"], "warmup": 0, "ms_per_step": rep["step_p50_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": (f"{args.workload}: first {len(fast_jobs)} seeded jobs, steps "
+                                f"scaled by {args.steps_scale}, all admitted at t = 0"),
+                   "workers": args.workers, "max_active": args.max_active,
+                   "hbm_cap_bytes": args.hbm_cap},
+        "p50_step_ms": rep["step_p50_ms"], "p99_step_ms": rep["step_p99_ms"],
+        "p50_job_s": rep["job_p50_s"], "p99_job_s": rep["job_p99_s"],
+        "jobs_done": rep["jobs_done"], "jobs_failed": rep["jobs_failed"],
+        "peak_active": rep["peak_active"],
+        "plain_fp64": plain,
+        "speedup_vs_plain": rep["point_steps_per_s"] / plain["value"] if plain else None,
+        "with_arrivals": live,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def ov_points(job):
+    import openvort_amd as ov
+    return ov.lib().ovg_generate_rings(job.seed, job.n_rings, job.D, job.r_min, job.r_max,
+                                       job.dxi, 0, None, None, None)
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v) * 1e3, q))
 
@@ -146,6 +208,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-sample", type=int, default=2048,
                     help="points of the direct-sum error check (0 = skip)")
+    ap.add_argument("--jobs", type=int, default=24, help="c4/c5: number of jobs")
+    ap.add_argument("--steps-scale", type=float, default=0.05,
+                    help="c4/c5: fraction of each job's requested steps")
+    ap.add_argument("--workers", type=int, default=8, help="c4/c5: stepping threads")
+    ap.add_argument("--max-active", type=int, default=16, help="c4/c5: admitted jobs")
+    ap.add_argument("--hbm-cap", type=float, default=200e9, help="c4/c5: admission cap, bytes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -165,6 +233,10 @@ def main():
             torch.neg(a, out=b)
         torch.cuda.synchronize()
         log(f"copy probe: 5 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        return
+
+    if args.workload in ("c4", "c5"):
+        serve_bench(args)
         return
 
     import openvort_amd as ov
