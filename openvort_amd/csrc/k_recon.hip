@@ -361,17 +361,63 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
         }
         return d2 < rc2;
     };
+    // Tree mode: the rewired segments indexed on a periodic cell grid of
+    // cell side >= the cutoff, so a query visits its 27 neighbour cells;
+    // the matches are applied in the order of `changes`, as a full scan
+    // with the same predicate would.
+    int gn[3] = {0, 0, 0};
+    bool use_grid = !exact;
+    for (int q = 0; q < 3 && use_grid; ++q) {
+        const double h = 25 * c.params.rec_dist;
+        gn[q] = h > 0 ? (int)std::min(128.0, std::floor(P.L[q] / h)) : 0;
+        use_grid = gn[q] >= 3 && P.periodic[q];
+    }
+    std::vector<int> ghead, gnext, gtouched, found;
+    auto gcell = [&](int slot) {
+        int ci[3];
+        for (int q = 0; q < 3; ++q) {
+            const double u = (X[3 * (size_t)slot + q] - P.lo[q]) / P.L[q];
+            int v = (int)std::floor(u * gn[q]) % gn[q];
+            ci[q] = v < 0 ? v + gn[q] : v;
+        }
+        return (ci[2] * gn[1] + ci[1]) * gn[0] + ci[0];
+    };
+    if (use_grid) ghead.assign((size_t)gn[0] * gn[1] * gn[2], -1);
+    auto index_change = [&](int idx) {
+        if (!use_grid) return;
+        const int cell = gcell(changes[idx].seg);
+        if (ghead[cell] < 0) gtouched.push_back(cell);
+        gnext.push_back(ghead[cell]);
+        ghead[cell] = idx;
+    };
     auto velocity = [&](int p, double *tn, double *nr, double *vels) {
         int e = where[p];
         double bs[3];
         for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
-        // change of the segments rewired so far
-        for (const Change &ch : changes) {
-            if (!exact && !near(ch.seg, p)) continue;
+        auto apply = [&](const Change &ch) {
             double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
             host::segment_all_images(P, X, ch.seg, ch.old_f, p, vo);
             host::segment_all_images(P, X, ch.seg, ch.new_f, p, vn);
             for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
+        };
+        // change of the segments rewired so far
+        if (use_grid) {
+            found.clear();
+            const int pc = gcell(p);
+            const int px = pc % gn[0], py = (pc / gn[0]) % gn[1], pz = pc / (gn[0] * gn[1]);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int cx = (px + dx + gn[0]) % gn[0], cy = (py + dy + gn[1]) % gn[1],
+                                  cz = (pz + dz + gn[2]) % gn[2];
+                        for (int i = ghead[(cz * gn[1] + cy) * gn[0] + cx]; i >= 0; i = gnext[i])
+                            if (near(changes[i].seg, p)) found.push_back(i);
+                    }
+            std::sort(found.begin(), found.end());
+            for (int i : found) apply(changes[i]);
+        } else {
+            for (const Change &ch : changes)
+                if (exact || near(ch.seg, p)) apply(ch);
         }
         double vs[3];
         host::compose(P, X, conn.data(), p, tn, nr, bs, vs, vels);
@@ -408,10 +454,14 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
             for (int sl : {k, l, knext, kprev, lnext, lprev}) mark_dirty(c, sl);
             if (kind == 1) {
                 changes.push_back({k, knext, l});
+                index_change((int)changes.size() - 1);
                 changes.push_back({lprev, l, knext});
+                index_change((int)changes.size() - 1);
             } else {
                 changes.push_back({l, lnext, k});
+                index_change((int)changes.size() - 1);
                 changes.push_back({kprev, k, lnext});
+                index_change((int)changes.size() - 1);
             }
             for (int sl : {k, conn[2 * k], conn[2 * k + 1], knext, kprev, l, conn[2 * l],
                            conn[2 * l + 1], lnext, lprev}) {
