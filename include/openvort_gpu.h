@@ -302,6 +302,34 @@ int ovg_set_timing(ovg_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int ovg_synchronize(ovg_ctx *ctx);
 
+/* ---- tangle files (src/utils.c:68-199) ------------------------------- */
+/*
+ * One text line per live point, 18 tab-separated columns: loop index,
+ * position, vels, tangents, normals (3 each), slot, reverse, forward,
+ * status, pin_wall; loops in the slot order of their first point
+ * (save_tangle, src/utils.c:132-184).  digits = 15 writes the reference's
+ * %.15g byte for byte; 17 round-trips every double exactly.  vels,
+ * tangents, normals may be NULL (written as zeros).
+ */
+int ovg_write_tangle_file(const char *path, int n, const double *vnodes, const double *vels,
+                          const double *tangents, const double *normals,
+                          const int *connections, const int *status, int digits);
+/*
+ * Read a tangle file (load_tangle / load_point, src/utils.c:93-130,186-199)
+ * into slot arrays: each line's point goes to its slot; the slot count is
+ * 512 doubled until every slot id fits (create_tangle(512) and
+ * expand_tangle, src/main.c:85, src/utils.c:117-119); slots not in the file
+ * are EMPTY with zero vectors.  Call with all arrays NULL to get the slot
+ * count in *n_slots, then with arrays of that many slots.
+ */
+int ovg_read_tangle_file(const char *path, int n_cap, int *n_slots, double *vnodes,
+                         double *vels, double *tangents, double *normals, int *connections,
+                         int *status);
+/* The context's tangle to a file / a file into the context (replacing its
+ * tangle; vels, tangents and normals as in the file). */
+int ovg_save_tangle(ovg_ctx *ctx, const char *path, int digits);
+int ovg_load_tangle(ovg_ctx *ctx, const char *path);
+
 /* ---- seeded tangle generator (host code, no GPU) --------------------- */
 /*
  * Rings with radius ~ U[r_min, r_max], centre ~ U[0,D)^3 and normal

@@ -95,6 +95,12 @@ _sig("ovg_upload_connections", C.c_int, C.c_void_p, _ip, _ip)
 _sig("ovg_download", C.c_int, C.c_void_p, _dp, _dp, _dp, _dp, _dp)
 _sig("ovg_download_connections", C.c_int, C.c_void_p, _ip, _ip)
 _sig("ovg_num_slots", C.c_int, C.c_void_p)
+_sig("ovg_num_points", C.c_int, C.c_void_p)
+_sig("ovg_write_tangle_file", C.c_int, C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _ip, _ip,
+     C.c_int)
+_sig("ovg_read_tangle_file", C.c_int, C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip)
+_sig("ovg_save_tangle", C.c_int, C.c_void_p, C.c_char_p, C.c_int)
+_sig("ovg_load_tangle", C.c_int, C.c_void_p, C.c_char_p)
 _sig("ovg_update_tangents_normals", C.c_int, C.c_void_p)
 _sig("ovg_update_velocities", C.c_int, C.c_void_p, C.c_double)
 _sig("ovg_update_tangle", C.c_int, C.c_void_p, C.c_double)
@@ -321,6 +327,18 @@ class Engine:
                "ovg_velocity_at")
         return vs, vels
 
+    def save(self, path, digits=15):
+        """The tangle to a reference tangle file (src/utils.c:132-184)."""
+        _check(_lib.ovg_save_tangle(self._h, str(path).encode(), digits), "ovg_save_tangle")
+
+    def load(self, path):
+        """Replace the tangle by a reference tangle file's (src/utils.c:186-199)."""
+        _check(_lib.ovg_load_tangle(self._h, str(path).encode()), "ovg_load_tangle")
+        self.n = _lib.ovg_num_slots(self._h)
+
+    def num_points(self):
+        return _lib.ovg_num_points(self._h)
+
     def line_length(self):
         L = C.c_double(0)
         n = C.c_int32(0)
@@ -337,3 +355,31 @@ class Engine:
 
     def synchronize(self):
         _check(_lib.ovg_synchronize(self._h), "ovg_synchronize")
+
+
+def write_tangle_file(path, t, vels=None, tangents=None, normals=None, digits=15):
+    """Reference tangle file of slot arrays (save_tangle, src/utils.c:132-184):
+    digits 15 is the reference's %.15g, 17 round-trips exactly."""
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    vels, tangents, normals = c(vels), c(tangents), c(normals)
+    _check(_lib.ovg_write_tangle_file(str(path).encode(), t.n, _d(np.ascontiguousarray(t.vnodes)),
+                                      _d(vels), _d(tangents), _d(normals),
+                                      _i(np.ascontiguousarray(t.conn, dtype=np.int32)),
+                                      _i(np.ascontiguousarray(t.status, dtype=np.int32)), digits),
+           "ovg_write_tangle_file")
+
+
+def read_tangle_file(path):
+    """(Tangle, vels, tangents, normals) of a reference tangle file
+    (load_tangle into create_tangle(512), src/utils.c:93-130,186-199)."""
+    n = C.c_int(0)
+    _check(_lib.ovg_read_tangle_file(str(path).encode(), 0, C.byref(n), None, None, None, None,
+                                     None, None), "ovg_read_tangle_file")
+    n = n.value
+    x, v, tg, nr = (np.zeros((n, 3)) for _ in range(4))
+    conn = np.zeros((n, 2), dtype=np.int32)
+    st = np.zeros((n, 2), dtype=np.int32)
+    m = C.c_int(0)
+    _check(_lib.ovg_read_tangle_file(str(path).encode(), n, C.byref(m), _d(x), _d(v), _d(tg),
+                                     _d(nr), _i(conn), _i(st)), "ovg_read_tangle_file")
+    return Tangle(x, conn, st), v, tg, nr

@@ -188,6 +188,9 @@ class RefLib:
         L.ref_eliminate_small_loops.argtypes = [vp, C.c_int]
         L.ref_remesh.argtypes = [vp, C.c_double, C.c_double]
         L.ref_num_slots.argtypes = [vp]
+        L.ref_save_tangle.argtypes = [vp, C.c_char_p]
+        L.ref_load_tangle.argtypes = [C.c_char_p, C.c_void_p]
+        L.ref_load_tangle.restype = vp
         L.ref_get_status.argtypes = [vp, _ip]
         L.ref_time_phases.argtypes = [vp, C.c_double, C.c_double, _dp]
 
@@ -202,6 +205,14 @@ class RefLib:
         if not h:
             raise ValueError("reference has no such image table")
         return RefTangle(self, h, t.n)
+
+
+def ref_load(ref, path, p):
+    """The reference's load_tangle into create_tangle(512)."""
+    h = ref.lib.ref_load_tangle(str(path).encode(), C.byref(p))
+    if not h:
+        raise ValueError("reference has no such image table")
+    return RefTangle(ref, h, ref.lib.ref_num_slots(h))
 
 
 class RefTangle:
@@ -245,6 +256,10 @@ class RefTangle:
     def remesh(self, min_dist, max_dist):
         self.ref.lib.ref_remesh(self.h, min_dist, max_dist)
         self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def save(self, path):
+        """The reference's save_tangle (src/utils.c:132-184)."""
+        self.ref.lib.ref_save_tangle(self.h, str(path).encode())
 
     def status(self):
         st = np.zeros((self.n, 2), dtype=np.int32)

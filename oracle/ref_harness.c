@@ -19,6 +19,7 @@
 #include <math.h>
 
 #include "tangle.h"
+#include "utils.h"
 #include "vortex_constants.h"
 #include "vortex_dynamics.h"
 #include "external_velocity.h"
@@ -174,6 +175,20 @@ void ref_remesh(void *h, double min_dist, double max_dist)
 }
 
 int ref_num_slots(void *h) { return ((struct tangle_state *)h)->N; }
+
+/* save_tangle / load_tangle, src/utils.c:132-199; the load goes into a
+ * tangle made as src/main.c:84-86 does (create_tangle(512)) with the box
+ * and image table of p. */
+void ref_save_tangle(void *h, const char *path) { save_tangle(path, (struct tangle_state *)h); }
+
+void *ref_load_tangle(const char *path, const ovg_params *p)
+{
+    struct tangle_state *t = (struct tangle_state *)calloc(1, sizeof(*t));
+    create_tangle(t, 512);
+    if (set_box(t, p)) { free_tangle(t); free(t); return NULL; }
+    load_tangle(path, t);
+    return t;
+}
 
 void ref_get_status(void *h, int *st)
 {

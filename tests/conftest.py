@@ -30,3 +30,10 @@ def load_golden(name):
 def oracle():
     from oracle.ovoracle import Oracle
     return Oracle()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    """The compiled reference (oracle/_ref/libovref.so)."""
+    from oracle.ovoracle import RefLib
+    return RefLib()

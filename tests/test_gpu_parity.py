@@ -511,3 +511,25 @@ def test_reconnect_tree_velocity_mode(traverse):
         conn, _ = e.download_connections()
     assert n == int(g["reconnections"])
     assert np.array_equal(conn, g["conn_after"])
+
+
+def test_engine_save_load_round_trip(tmp_path):
+    t, p, dt = configs.make("c1", rings=8)
+    say("engine save (17 digits) / load round trip")
+    with engine(t, p) as e:
+        e.update_tangle(0.0)
+        e.save(tmp_path / "t.dat", digits=17)
+        o = e.download()
+        conn, st = e.download_connections()
+    q = ov.Params.from_buffer_copy(p)
+    with ov.Engine(q) as e2:
+        e2.load(tmp_path / "t.dat")
+        o2 = e2.download()
+        conn2, st2 = e2.download_connections()
+        live = np.nonzero(st[:, 0] != -1)[0]
+        assert np.array_equal(conn2[:len(conn)], conn) and np.array_equal(st2[:len(st)], st)
+        for k in ("vnodes", "vels", "tangents", "normals"):
+            assert np.array_equal(o2[k][live], o[k][live]), k
+        # the loaded state steps exactly as the original
+        e2.update_tangle(0.0)
+        assert np.array_equal(e2.download()["vels"][live], o["vels"][live])
