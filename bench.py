@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --ppw/--leaf/--group-level instead of tuning them first")
     ap.add_argument("--accuracy-sample", type=int, default=2048,
                     help="points of the direct-sum error check (0 = skip)")
     ap.add_argument("--jobs", type=int, default=24, help="c4/c5: number of jobs")
@@ -268,6 +270,12 @@ def main():
     e = ov.Engine(p)
     e.upload(t)
     e.synchronize()
+    tune = None
+    if trav == ov.TRAVERSE_LIST and not args.no_autotune:
+        a = time.perf_counter()
+        rows, best = e.autotune(repeats=2)
+        tune = dict(rows[best], seconds=time.perf_counter() - a, candidates=len(rows))
+        log(f"autotune: {tune}")
     rb = configs.remesh_bounds(args.workload)
     lat_w, tm = run_steps(e, args.warmup, 0.0, dt, 0, args.smooth_every, True, rb)
     e.stats()  # reset counters
@@ -333,8 +341,10 @@ def main():
                                 f"dt {dt}, reconnect every step, smooth every "
                                 f"{args.smooth_every} steps"),
                    "points": n_pts, "precision": args.precision, "traverse": args.traverse,
-                   "leaf_size": args.leaf, "points_per_wave": args.ppw,
-                   "group_level": args.group_level, "rec_velocity": args.rec_velocity},
+                   "leaf_size": tune["leaf_size"] if tune else args.leaf,
+                   "points_per_wave": tune["points_per_wave"] if tune else args.ppw,
+                   "group_level": tune["group_level"] if tune else args.group_level,
+                   "rec_velocity": args.rec_velocity, "autotune": tune},
         "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
         "interactions_per_s": interactions_per_s,
         "direct_equivalent_interactions_per_s": 27.0 * n_pts * n_pts * 4 * args.steps / wall,

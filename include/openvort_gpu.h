@@ -302,6 +302,27 @@ int ovg_set_timing(ovg_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int ovg_synchronize(ovg_ctx *ctx);
 
+/* ---- autotuning (no counterpart in the reference) --------------------- */
+/*
+ * Time the list traversal (OVG_TRAVERSE_LIST) on the context's current
+ * tangle for every combination of points_per_wave {16, 32, 64}, leaf_size
+ * {1, 2, 4, 8} and group_level {L-1, L, L+1} (L: the automatic level),
+ * one warm-up and `repeats` timed launches each on a tree built once, and
+ * keep the fastest in the context's parameters.  Results differ between
+ * configurations only within the traversal's truncation (each is never
+ * coarser than the reference's per-point acceptance).
+ */
+typedef struct ovg_tune_row {
+    int32_t points_per_wave, leaf_size, group_level, pad_;
+    double  ms;             /* average launch, HIP events               */
+    double  pairs;          /* pair interactions per launch            */
+} ovg_tune_row;
+typedef struct ovg_tune {
+    int32_t n_rows, best;
+    ovg_tune_row rows[36];
+} ovg_tune;
+int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out);
+
 /* ---- tangle files (src/utils.c:68-199) ------------------------------- */
 /*
  * One text line per live point, 18 tab-separated columns: loop index,

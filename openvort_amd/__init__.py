@@ -59,6 +59,16 @@ class Params(C.Structure):
     ]
 
 
+class TuneRow(C.Structure):
+    _fields_ = [("points_per_wave", C.c_int32), ("leaf_size", C.c_int32),
+                ("group_level", C.c_int32), ("pad_", C.c_int32), ("ms", C.c_double),
+                ("pairs", C.c_double)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("n_rows", C.c_int32), ("best", C.c_int32), ("rows", TuneRow * 36)]
+
+
 class Stats(C.Structure):
     _fields_ = [("interactions", C.c_double), ("interactions_fp32", C.c_double),
                 ("tree_nodes", C.c_int64), ("tree_depth", C.c_int32),
@@ -100,6 +110,7 @@ _sig("ovg_write_tangle_file", C.c_int, C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, 
      C.c_int)
 _sig("ovg_read_tangle_file", C.c_int, C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip)
 _sig("ovg_save_tangle", C.c_int, C.c_void_p, C.c_char_p, C.c_int)
+_sig("ovg_autotune", C.c_int, C.c_void_p, C.c_int, _P(Tune))
 _sig("ovg_load_tangle", C.c_int, C.c_void_p, C.c_char_p)
 _sig("ovg_update_tangents_normals", C.c_int, C.c_void_p)
 _sig("ovg_update_velocities", C.c_int, C.c_void_p, C.c_double)
@@ -335,6 +346,17 @@ class Engine:
         """Replace the tangle by a reference tangle file's (src/utils.c:186-199)."""
         _check(_lib.ovg_load_tangle(self._h, str(path).encode()), "ovg_load_tangle")
         self.n = _lib.ovg_num_slots(self._h)
+
+    def autotune(self, repeats=2):
+        """Time the list traversal over points per wave, leaf size and group
+        level on the current tangle and keep the fastest; returns (rows,
+        best index), rows as dicts."""
+        t = Tune()
+        _check(_lib.ovg_autotune(self._h, repeats, C.byref(t)), "ovg_autotune")
+        rows = [{"points_per_wave": r.points_per_wave, "leaf_size": r.leaf_size,
+                 "group_level": r.group_level, "ms": r.ms, "pairs": r.pairs}
+                for r in t.rows[:t.n_rows]]
+        return rows, t.best
 
     def num_points(self):
         return _lib.ovg_num_points(self._h)

@@ -1108,3 +1108,73 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     }
     return OVG_OK;
 }
+
+// ---- autotuning of the list traversal --------------------------------------
+extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    if (!out || repeats < 1) return set_error(OVG_EINVAL, "bad arguments to ovg_autotune");
+    Ctx &c = ctx->c;
+    if (c.params.method != OVG_METHOD_BH || c.params.traverse != OVG_TRAVERSE_LIST)
+        return set_error(OVG_EINVAL, "autotuning needs the Barnes-Hut list traversal");
+    hipSetDevice(c.device);
+    std::memset(out, 0, sizeof *out);
+    const ovg_params saved = c.params;
+    ovg_params auto_level = c.params;
+    auto_level.group_level = 0;
+    const int L = list_group_level(auto_level);
+    push_params(c, 0.0);
+    // the interaction counters are restored afterwards: tuning is not work
+    unsigned long long keep[2] = {0, 0};
+    OVG_HIP(hipMemcpyAsync(keep, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    Triple x = tri(c.x);
+    OVG_TRY(build_tree(c, x.v));
+    hipEvent_t e0 = take_event(c), e1 = take_event(c);
+    const int ppws[3] = {16, 32, 64}, leaves[4] = {1, 2, 4, 8};
+    double best = 1e300;
+    int n = 0;
+    for (int pw : ppws)
+        for (int lf : leaves)
+            for (int dl = -1; dl <= 1; ++dl) {
+                const int lvl = L + dl;
+                if (lvl < 1 || lvl > 21) continue;
+                c.params.points_per_wave = pw;
+                c.params.leaf_size = lf;
+                c.params.group_level = lvl;
+                push_params(c, 0.0);
+                OVG_TRY(launch_bh_list(c, x.v));  // warm-up
+                unsigned long long cnt0[2] = {0, 0}, cnt1[2] = {0, 0};
+                OVG_HIP(hipMemcpyAsync(cnt0, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+                OVG_HIP(hipEventRecord(e0, c.stream));
+                for (int r = 0; r < repeats; ++r) OVG_TRY(launch_bh_list(c, x.v));
+                OVG_HIP(hipEventRecord(e1, c.stream));
+                OVG_HIP(hipMemcpyAsync(cnt1, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+                OVG_HIP(hipStreamSynchronize(c.stream));
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                ovg_tune_row &row = out->rows[n];
+                row.points_per_wave = pw;
+                row.leaf_size = lf;
+                row.group_level = lvl;
+                row.ms = ms / repeats;
+                row.pairs = ((double)(cnt1[0] - cnt0[0]) + (double)(cnt1[1] - cnt0[1])) / repeats;
+                if (row.ms < best) {
+                    best = row.ms;
+                    out->best = n;
+                }
+                ++n;
+            }
+    out->n_rows = n;
+    c.ev_pool.push_back(e0);
+    c.ev_pool.push_back(e1);
+    c.params = saved;
+    c.params.points_per_wave = out->rows[out->best].points_per_wave;
+    c.params.leaf_size = out->rows[out->best].leaf_size;
+    c.params.group_level = out->rows[out->best].group_level;
+    push_params(c, 0.0);
+    OVG_HIP(hipMemcpyAsync(c.counter64.p, keep, 16, hipMemcpyHostToDevice, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}

@@ -166,6 +166,8 @@ int launch_bh(Ctx &c, const double *const x[3]);
 void launch_bh_targets(Ctx &c, const double *const x[3], int n_tgt, const int *tlist);
 // k_list.hip
 int launch_bh_list(Ctx &c, const double *const x[3]);
+int list_group_level(const ovg_params &p);    // the level that bounds a target group
+int list_points_per_wave(const ovg_params &p);
 // The list traversal for m live slots (device list) against the tree built
 // at x: grouped by tree order within level-L0 cells.
 int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d_slots);

@@ -533,3 +533,21 @@ def test_engine_save_load_round_trip(tmp_path):
         # the loaded state steps exactly as the original
         e2.update_tangle(0.0)
         assert np.array_equal(e2.download()["vels"][live], o["vels"][live])
+
+
+def test_autotune_picks_fastest_and_keeps_results_valid(c2):
+    g, t, p = c2
+    q = ov.Params.from_buffer_copy(p)
+    q.traverse, q.precision = ov.TRAVERSE_LIST, ov.PREC_MIXED
+    say("autotune of the list traversal on c2")
+    with engine(t, q) as e:
+        rows, best = e.autotune(repeats=1)
+        assert len(rows) == 36 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
+        assert rows[best]["ms"] == min(r["ms"] for r in rows)
+        e.update_tangle(0.0)
+        o = e.download()
+        s = e.stats()
+    # tuning launches are not counted as work; the evaluation ran the chosen setting
+    assert abs(s.interactions - rows[best]["pairs"]) <= 1e-9 * rows[best]["pairs"]
+    sub = g["sub"]
+    assert rel_l2(o["vs"][sub], g["direct_vs"]) < rel_l2(g["bh_vs"], g["direct_vs"])
