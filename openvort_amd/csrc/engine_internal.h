@@ -129,8 +129,10 @@ struct Ctx {
     int n_loops = 0;
     DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
     std::vector<int> h_loop_off;
-    DBuf sm_fft, sm_big;                // FFT scratch and big-loop list
-    DBuf sm_giant, sm_gstate;           // grid-wide FFT scratch for giant loops
+    DBuf sm_fft, sm_big;                // transform arena and big-loop list + chunk maps
+    DBuf sm_gstate;                     // per big loop state, per chunk wrap sums
+    DBuf sm_tw;                         // e^{-2 pi i k / 2^sm_tw_log}, k < 2^(sm_tw_log-1)
+    int sm_tw_log = 0;
     ovg_stats stats{};
     hipEvent_t ev[8] = {};
     // per-evaluation timing events, resolved when statistics are read so

@@ -1,6 +1,7 @@
 // Frequency-filter smoothing (smooth -> high_freq_cutoff -> low_pass_filter,
-// src/vortex_smooth.c:134-175,183-453,459-462), one workgroup per closed
-// loop.
+// src/vortex_smooth.c:134-175,183-453,459-462): one workgroup per loop up
+// to kSmallLoop points, one batched Bluestein transform over all longer
+// loops.
 //
 // Per loop (in slot order of its first slot, src/vortex_smooth.c:192-210)
 // the coordinates are unwrapped along the loop, each point within half a box
@@ -17,6 +18,9 @@
 //   y_j = [r_0 + 2 sum_k (r_k cos - i_k sin) + r_{N/2} (-1)^j] / N
 // restricted to the entries that survive.  Twiddles use exact integer
 // argument reduction (jk mod N) and sincospi.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "engine_internal.h"
@@ -25,8 +29,6 @@ namespace ovg {
 
 constexpr int kSmBlock = 256;
 constexpr int kSmallLoop = 1024;     // DFT of the kept band, 256 threads per loop
-constexpr int kMediumLoop = 65536;   // Bluestein FFT inside one 1024-thread workgroup
-constexpr int kFftBlock = 1024;
 
 // ---- shared pieces -----------------------------------------------------------
 
@@ -45,6 +47,16 @@ __device__ int block_scan_incl(int v, int *sh)
     int r = sh[t];
     __syncthreads();
     return r;
+}
+
+// Whole periods that seg_pwrap adds to the link difference d = x_j - x_{j-1}
+// along axis a (src/vortex_smooth.c:229-242).
+__device__ __forceinline__ int link_wraps(const TangleParams &P, int a, double d)
+{
+    int w = 0, g = 0;
+    while (d > P.halfL[a] && g++ < 64) { d -= P.L[a]; --w; }
+    while (d < -P.halfL[a] && g++ < 128) { d += P.L[a]; ++w; }
+    return w;
 }
 
 // Unwrap a loop's coordinates along its links: x'_0 = x_0 and x'_j is x_j
@@ -66,12 +78,7 @@ __device__ void block_unwrap(const TangleParams &P, const int *slots, int n, con
             double cur = 0;
             if (j < n) {
                 cur = src[a][slots[j]];
-                if (j > 0 && P.periodic[a]) {
-                    double d = cur - src[a][slots[j - 1]];
-                    int g = 0;
-                    while (d > P.halfL[a] && g++ < 64) { d -= P.L[a]; --w; }
-                    while (d < -P.halfL[a] && g++ < 128) { d += P.L[a]; ++w; }
-                }
+                if (j > 0 && P.periodic[a]) w = link_wraps(P, a, cur - src[a][slots[j - 1]]);
             }
             int incl = block_scan_incl(w, sh);
             int carry = *sh_carry;
@@ -227,99 +234,68 @@ __global__ __launch_bounds__(kSmBlock) void k_smooth(
     block_finish(xs, n, span, s_coef, s_x0, slots, X, Y, Z);
 }
 
-// ---- larger loops: Bluestein transform over radix-2 FFTs --------------------
+// ---- loops longer than kSmallLoop: one batched Bluestein transform ---------
 // U_k = sum_j u_j e^{-2 pi i jk/N} = c_k (a * b)_k with the chirp
-// c_j = e^{-pi i j^2/N}, a_j = u_j c_j, b_m = conj(c_m) (Bluestein 1970); the
-// cyclic convolution of length M = 2^ceil(log2(2N-1)) uses radix-2 FFTs.
-// The filter zeroes the same halfcomplex entries as the small-loop path and
-// y = conj(DFT(conj(U)))/N.
+// c_j = e^{-pi i j^2/N}, a_j = u_j c_j, b_m = conj(c_m) (Bluestein 1970).
+// The cyclic convolution of length M = 2^ceil(log2(2N-1)) is the inverse
+// transform of DIF(a) . DIF(b): the decimation-in-frequency forward transform
+// leaves its output in bit-reversed order, the product is taken in that
+// order and the decimation-in-time inverse returns to natural order, so no
+// bit-reversal pass is needed.  The filter zeroes the same halfcomplex
+// entries as the small-loop path and y = conj(DFT(conj(U)))/N.
+//
+// All loops longer than kSmallLoop are transformed together: the three
+// coordinates of every loop are signals of their own and every launch covers
+// every signal, a block finding its loop through a host-built chunk map.
+// Stages of span >= kTile run as grid-wide radix-4 passes (two stages per
+// pass); the stages of span < kTile run as one pass over kTile-element tiles
+// in LDS, which for the coordinate signals also takes the product with
+// DIF(b) and the first inverse stages.
 
-struct BigLoop {
-    int loop;       // index into the loop table
-    int logM;       // for the spanning length 2n-2 (an upper bound otherwise)
-    long long off;  // first double2 of this loop's scratch
+constexpr int kTileLog = 12;
+constexpr int kTile = 1 << kTileLog;
+constexpr int kChunk = 1024;       // points per block in the unwrap and the write-back
+constexpr int kPassBlock = 256;    // kTile/4 radix-4 butterflies per block, 4 per thread
+constexpr int kTileBlock = 512;
+// n > kSmallLoop gives 2N-1 > 2048, so M >= kTile: every signal is whole tiles
+static_assert(2 * (kSmallLoop + 1) - 1 > kTile / 2, "transforms shorter than a tile");
+
+struct BsLoop {
+    int loop;         // index into the loop table
+    int n;            // points
+    int logMmax;      // log2 of the convolution length for the spanning length 2n-2
+    int chunk0;       // first point chunk
+    int tile0;        // first tile of its 2^logMmax-element signals
+    int pad;
+    long long xs;     // first double of the unwrapped coordinates, 3 x n
+    long long chirp;  // first double2 of c_j, j < N
+    long long bhat;   // first double2 of b, then DIF(b): 2^logMmax
+    long long work;   // first double2 of the three coordinate signals, 3 x 2^logMmax
+};
+
+struct BsState {
+    int span;         // the loop spans the box: detrended and odd-extended
+    int N, logM;      // transform length, log2 of the convolution length
+    int kr, ki;       // kept band (kept_band)
+    int filt;         // 0: nothing is zeroed and the coordinates pass through
+    double coef[3], x0[3];
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
 
-// in place, within the workgroup; tw[k] = e^{-2 pi i k/M}, k < M/2
-__device__ void fft_radix2(double2 *a, int M, int logM, const double2 *tw, bool inverse)
+// e^{sign pi i (j^2 mod 2N)/N}: exact integer argument reduction
+__device__ __forceinline__ double2 chirp_at(long long j, int N, double sign)
 {
-    for (int i = threadIdx.x; i < M; i += blockDim.x) {
-        int j = (int)(__brev((unsigned)i) >> (32 - logM));
-        if (i < j) {
-            double2 t = a[i];
-            a[i] = a[j];
-            a[j] = t;
-        }
-    }
-    __syncthreads();
-    for (int st = 1; st <= logM; ++st) {
-        int half = 1 << (st - 1);
-        int stride = M >> st;
-        for (int b = threadIdx.x; b < M / 2; b += blockDim.x) {
-            int grp = b >> (st - 1), pos = b & (half - 1);
-            int i = grp * 2 * half + pos, j = i + half;
-            double2 w = tw[pos * stride];
-            if (inverse) w = conj2(w);
-            double2 t = cmul(w, a[j]);
-            double2 u = a[i];
-            a[i] = make_double2(u.x + t.x, u.y + t.y);
-            a[j] = make_double2(u.x - t.x, u.y - t.y);
-        }
-        __syncthreads();
-    }
-}
-
-__device__ void bluestein(double2 *U, int N, int M, int logM, const double2 *chirp,
-                          const double2 *Bhat, const double2 *tw, double2 *work)
-{
-    for (int j = threadIdx.x; j < M; j += blockDim.x)
-        work[j] = j < N ? cmul(U[j], chirp[j]) : make_double2(0, 0);
-    __syncthreads();
-    fft_radix2(work, M, logM, tw, false);
-    for (int j = threadIdx.x; j < M; j += blockDim.x) work[j] = cmul(work[j], Bhat[j]);
-    __syncthreads();
-    fft_radix2(work, M, logM, tw, true);
-    const double inv = 1.0 / M;
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
-        double2 v = cmul(work[k], chirp[k]);
-        U[k] = make_double2(v.x * inv, v.y * inv);
-    }
-    __syncthreads();
-}
-
-// twiddles, chirp (exact argument reduction) and the b sequence
-__device__ void bluestein_tables(int N, int M, double2 *tw, double2 *chirp, double2 *Bhat,
-                                 int tid, int nthreads)
-{
-    for (int k = tid; k < M / 2; k += nthreads) {
-        double sn, cs;
-        sincospi(-2.0 * (double)k / M, &sn, &cs);
-        tw[k] = make_double2(cs, sn);
-    }
-    for (int j = tid; j < N; j += nthreads) {
-        long long q = ((long long)j * j) % (2LL * N);
-        double sn, cs;
-        sincospi(-(double)q / N, &sn, &cs);
-        chirp[j] = make_double2(cs, sn);
-    }
-    for (int m = tid; m < M; m += nthreads) {
-        double2 v = make_double2(0, 0);
-        int mm = m < N ? m : (m > M - N ? M - m : -1);
-        if (mm >= 0) {
-            long long q = ((long long)mm * mm) % (2LL * N);
-            double sn, cs;
-            sincospi((double)q / N, &sn, &cs);  // conj(chirp)
-            v = make_double2(cs, sn);
-        }
-        Bhat[m] = v;
-    }
+    const long long q = (j * j) % (2LL * N);
+    double sn, cs;
+    sincospi(sign * (double)q / N, &sn, &cs);
+    return make_double2(cs, sn);
 }
 
 // keep the surviving halfcomplex entries (mirrored to k > N/2) and conjugate
@@ -331,240 +307,513 @@ __device__ __forceinline__ double2 filter_conj(double2 v, int k, int N, int kr, 
     return conj2(make_double2(keep_r ? v.x : 0.0, keep_i ? v.y : 0.0));
 }
 
-__global__ __launch_bounds__(kFftBlock) void k_smooth_fft(
-    int n_big, const BigLoop *__restrict__ big, const int *__restrict__ loop_off,
-    const int *__restrict__ loop_slots, double *__restrict__ X, double *__restrict__ Y,
-    double *__restrict__ Z, const TangleParams *__restrict__ Pp, double fcut,
-    double2 *__restrict__ scratch)
+// u_j of coordinate c: the unwrapped loop x, detrended and odd-extended to
+// length 2n-2 when it spans the box (src/vortex_smooth.c:248-279)
+__device__ __forceinline__ double bs_coord(const double *x, int n, const BsState &s, int c, int j)
 {
-    const BigLoop B = big[blockIdx.x];
-    const TangleParams &P = *Pp;
-    const int off = loop_off[B.loop];
-    const int n = loop_off[B.loop + 1] - off;
-    const int *slots = loop_slots + off;
-    // scratch: xs (3 x 2n doubles = 3n double2), tw (M/2), chirp (2n),
-    // Bhat (M), work (M), U (2n); M sized for the spanning length
-    const int Mmax = 1 << B.logM;
-    double2 *base = scratch + B.off;
-    double *xs[3] = {(double *)base, (double *)base + 2 * (size_t)n,
-                     (double *)base + 4 * (size_t)n};
-    double2 *tw = base + 3 * (size_t)n;
-    double2 *chirp = tw + Mmax / 2;
-    double2 *Bhat = chirp + 2 * (size_t)n;
-    double2 *work = Bhat + Mmax;
-    double2 *U = work + Mmax;
-    __shared__ int sh[kFftBlock];
-    __shared__ int s_carry, s_span;
-    __shared__ double s_coef[3], s_x0[3];
-    block_unwrap(P, slots, n, X, Y, Z, xs, sh, &s_carry);
-    if (threadIdx.x == 0) s_span = loop_spans(P, xs, n);
-    __syncthreads();
-    const bool span = s_span;
-    if (span) block_detrend(xs, n, s_coef, s_x0);
-    const int N = span ? 2 * n - 2 : n;
-    int kr, ki;
-    if (kept_band(N, fcut, kr, ki)) {
-        int logM = 0;
-        while ((1 << logM) < 2 * N - 1) ++logM;
-        const int M = 1 << logM;
-        bluestein_tables(N, M, tw, chirp, Bhat, threadIdx.x, blockDim.x);
-        __syncthreads();
-        fft_radix2(Bhat, M, logM, tw, false);
-        for (int c = 0; c < 3; ++c) {
-            for (int j = threadIdx.x; j < N; j += blockDim.x) U[j] = make_double2(xs[c][j], 0.0);
-            __syncthreads();
-            bluestein(U, N, M, logM, chirp, Bhat, tw, work);
-            for (int k = threadIdx.x; k < N; k += blockDim.x) U[k] = filter_conj(U[k], k, N, kr, ki);
-            __syncthreads();
-            bluestein(U, N, M, logM, chirp, Bhat, tw, work);
-            for (int j = threadIdx.x; j < n; j += blockDim.x) xs[c][j] = U[j].x / N;
-            __syncthreads();
+    if (!s.span) return x[j];
+    const int jj = j < n ? j : 2 * n - 2 - j;
+    const double v = (jj == 0 || jj == n - 1) ? 0.0 : x[jj] - s.coef[c] * jj - s.x0[c];
+    return j < n ? v : -v;
+}
+
+// signal sig of a loop: 0..2 the coordinates, 3 the b sequence
+__device__ __forceinline__ double2 *bs_signal(double2 *arena, const BsLoop &B, int sig)
+{
+    return arena + (sig == 3 ? B.bhat : B.work + ((long long)sig << B.logMmax));
+}
+
+// Inclusive scan of three ints per thread over the block (a multiple of 64
+// threads); tot receives the block totals.  sw: one row per wave.
+__device__ void block_scan3(int v[3], int (*sw)[3], int tot[3])
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int u = __shfl_up(v[a], off, 64);
+            if (lane >= off) v[a] += u;
         }
-    }
-    block_finish(xs, n, span, s_coef, s_x0, slots, X, Y, Z);
-}
-
-// ---- giant loops: the same transform with grid-wide FFT stages ------------
-struct GiantState {
-    int span;
-    int pad;
-    double coef[3], x0[3];
-};
-
-__global__ __launch_bounds__(kFftBlock) void k_giant_unwrap(
-    int loop, const int *__restrict__ loop_off, const int *__restrict__ loop_slots,
-    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
-    const TangleParams *__restrict__ Pp, double *__restrict__ xs0, GiantState *__restrict__ gs)
-{
-    const TangleParams &P = *Pp;
-    const int off = loop_off[loop];
-    const int n = loop_off[loop + 1] - off;
-    double *xs[3] = {xs0, xs0 + 2 * (size_t)n, xs0 + 4 * (size_t)n};
-    __shared__ int sh[kFftBlock];
-    __shared__ int s_carry, s_span;
-    __shared__ double s_coef[3], s_x0[3];
-    block_unwrap(P, loop_slots + off, n, X, Y, Z, xs, sh, &s_carry);
-    if (threadIdx.x == 0) s_span = loop_spans(P, xs, n);
+    if (lane == 63)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sw[wv][a] = v[a];
     __syncthreads();
-    if (s_span) block_detrend(xs, n, s_coef, s_x0);
-    if (threadIdx.x == 0) {
-        gs->span = s_span;
-        for (int c = 0; c < 3; ++c) {
-            gs->coef[c] = s_span ? s_coef[c] : 0;
-            gs->x0[c] = s_span ? s_x0[c] : 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int pre = 0, t = 0;
+        for (int k = 0; k < nw; ++k) {
+            const int s = sw[k][a];
+            pre += k < wv ? s : 0;
+            t += s;
         }
+        v[a] += pre;
+        tot[a] = t;
     }
+    __syncthreads();
 }
 
-__global__ void k_g_tables(int N, int M, double2 *tw, double2 *chirp, double2 *Bhat)
+// coordinates of point j of a loop and the periods of the link into it
+__device__ __forceinline__ void point_wraps(const TangleParams &P, const int *slots, int j, int n,
+                                            const double *const src[3], int w[3], double cur[3])
 {
-    bluestein_tables(N, M, tw, chirp, Bhat, blockIdx.x * blockDim.x + threadIdx.x,
-                     gridDim.x * blockDim.x);
-}
-
-__global__ void k_g_bitrev(double2 *a, int M, int logM)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    int j = (int)(__brev((unsigned)i) >> (32 - logM));
-    if (i < j) {
-        double2 t = a[i];
-        a[i] = a[j];
-        a[j] = t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        w[a] = 0;
+        cur[a] = 0;
     }
-}
-
-__global__ void k_g_stage(double2 *a, int M, int st, const double2 *tw, int inverse)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= M / 2) return;
-    int half = 1 << (st - 1);
-    int grp = b >> (st - 1), pos = b & (half - 1);
-    int i = grp * 2 * half + pos, j = i + half;
-    double2 w = tw[pos * (M >> st)];
-    if (inverse) w = conj2(w);
-    double2 t = cmul(w, a[j]);
-    double2 u = a[i];
-    a[i] = make_double2(u.x + t.x, u.y + t.y);
-    a[j] = make_double2(u.x - t.x, u.y - t.y);
-}
-
-// mode 0: work = U c (zero padded); 1: work *= Bhat; 2: U = work c / M
-__global__ void k_g_point(int mode, double2 *work, double2 *U, const double2 *chirp,
-                          const double2 *Bhat, int N, int M)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    if (mode == 0) {
-        work[j] = j < N ? cmul(U[j], chirp[j]) : make_double2(0, 0);
-    } else if (mode == 1) {
-        work[j] = cmul(work[j], Bhat[j]);
-    } else if (j < N) {
-        double2 v = cmul(work[j], chirp[j]);
-        U[j] = make_double2(v.x / M, v.y / M);
-    }
-}
-
-// mode 0: U = xs (real); 1: U = conj(filter(U)); 2: xs = Re U / N (j < n)
-__global__ void k_g_coord(int mode, double *xs, double2 *U, int n, int N, int kr, int ki)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mode == 0) {
-        if (j < N) U[j] = make_double2(xs[j], 0.0);
-    } else if (mode == 1) {
-        if (j < N) U[j] = filter_conj(U[j], j, N, kr, ki);
-    } else if (j < n) {
-        xs[j] = U[j].x / N;
-    }
-}
-
-__global__ void k_giant_finish(int loop, const int *__restrict__ loop_off,
-                               const int *__restrict__ loop_slots, double *__restrict__ xs0,
-                               const GiantState *__restrict__ gs, double *__restrict__ X,
-                               double *__restrict__ Y, double *__restrict__ Z)
-{
-    const int off = loop_off[loop];
-    const int n = loop_off[loop + 1] - off;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    double v[3];
-    for (int c = 0; c < 3; ++c) {
-        v[c] = xs0[2 * (size_t)n * c + j];
-        if (gs->span) v[c] = v[c] + gs->coef[c] * j + gs->x0[c];
+    const int s = slots[j];
+    const int s0 = j > 0 ? slots[j - 1] : s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cur[a] = src[a][s];
+        if (j > 0 && P.periodic[a]) w[a] = link_wraps(P, a, cur[a] - src[a][s0]);
     }
-    int m = loop_slots[off + j];
+}
+
+// per chunk of kChunk points: the sum of its links' periods per coordinate
+__global__ __launch_bounds__(kChunk) void k_bs_wraps(
+    const BsLoop *__restrict__ L, const int *__restrict__ chunk_loop,
+    const int *__restrict__ loop_off, const int *__restrict__ loop_slots,
+    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+    const TangleParams *__restrict__ Pp, int *__restrict__ csum)
+{
+    const BsLoop B = L[chunk_loop[blockIdx.x]];
+    const double *src[3] = {X, Y, Z};
+    const int j = (blockIdx.x - B.chunk0) * kChunk + threadIdx.x;
+    int w[3], tot[3];
+    double cur[3];
+    point_wraps(*Pp, loop_slots + loop_off[B.loop], j, B.n, src, w, cur);
+    __shared__ int sw[kChunk / 64][3];
+    block_scan3(w, sw, tot);
+    if (threadIdx.x == 0) {
+        csum[3 * blockIdx.x] = tot[0];
+        csum[3 * blockIdx.x + 1] = tot[1];
+        csum[3 * blockIdx.x + 2] = tot[2];
+    }
+}
+
+// exclusive prefix of the chunk sums along each loop, one block per loop
+__global__ __launch_bounds__(kChunk) void k_bs_carry(const BsLoop *__restrict__ L,
+                                                     const int *__restrict__ csum,
+                                                     int *__restrict__ carry)
+{
+    const BsLoop B = L[blockIdx.x];
+    const int nc = (B.n + kChunk - 1) / kChunk;
+    __shared__ int sw[kChunk / 64][3];
+    int run[3] = {0, 0, 0};
+    for (int base = 0; base < nc; base += kChunk) {
+        const int k = base + threadIdx.x;
+        int v[3], own[3], tot[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = own[a] = k < nc ? csum[3 * (B.chunk0 + k) + a] : 0;
+        block_scan3(v, sw, tot);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (k < nc) carry[3 * (B.chunk0 + k) + a] = run[a] + v[a] - own[a];
+            run[a] += tot[a];
+        }
+    }
+}
+
+// x'_j = x_j + (periods up to j) L: the unwrapped loop (block_unwrap)
+__global__ __launch_bounds__(kChunk) void k_bs_unwrap(
+    const BsLoop *__restrict__ L, const int *__restrict__ chunk_loop,
+    const int *__restrict__ loop_off, const int *__restrict__ loop_slots,
+    const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+    const TangleParams *__restrict__ Pp, const int *__restrict__ carry, double *__restrict__ arena)
+{
+    const BsLoop B = L[chunk_loop[blockIdx.x]];
+    const TangleParams &P = *Pp;
+    const double *src[3] = {X, Y, Z};
+    const int j = (blockIdx.x - B.chunk0) * kChunk + threadIdx.x;
+    int w[3], tot[3];
+    double cur[3];
+    point_wraps(P, loop_slots + loop_off[B.loop], j, B.n, src, w, cur);
+    __shared__ int sw[kChunk / 64][3];
+    block_scan3(w, sw, tot);
+    if (j >= B.n) return;
+    double *xs = arena + B.xs;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        xs[(size_t)a * B.n + j] = cur[a] + (double)(carry[3 * blockIdx.x + a] + w[a]) * P.L[a];
+}
+
+// span, detrend line, transform length and band of each loop
+// (loop_spans, block_detrend, kept_band)
+__global__ void k_bs_setup(int nb, const BsLoop *__restrict__ L, const double *__restrict__ arena,
+                           const TangleParams *__restrict__ Pp, double fcut,
+                           BsState *__restrict__ S)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const BsLoop B = L[b];
+    const TangleParams &P = *Pp;
+    const int n = B.n;
+    const double *xs = arena + B.xs;
+    double e[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) e[a] = xs[(size_t)a * n] - xs[(size_t)a * n + n - 1];
+    BsState s;
+    s.span = sqrt(dot3(e[0], e[1], e[2], e[0], e[1], e[2])) > (P.hi[0] - P.lo[0]) / 2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.coef[a] = s.span ? (xs[(size_t)a * n + n - 1] - xs[(size_t)a * n]) / (n - 1) : 0.0;
+        s.x0[a] = s.span ? xs[(size_t)a * n] : 0.0;
+    }
+    s.N = s.span ? 2 * n - 2 : n;
+    s.kr = s.ki = 0;
+    s.filt = kept_band(s.N, fcut, s.kr, s.ki);
+    int logM = 0;
+    while ((1 << logM) < 2 * s.N - 1) ++logM;
+    s.logM = logM;
+    S[b] = s;
+}
+
+// tw[k] = e^{-2 pi i k/2^logT}, k < 2^(logT-1)
+__global__ void k_bs_twiddles(int logT, double2 *__restrict__ tw)
+{
+    const int half = 1 << (logT - 1);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < half; k += gridDim.x * blockDim.x) {
+        double sn, cs;
+        sincospi(-(double)k / half, &sn, &cs);
+        tw[k] = make_double2(cs, sn);
+    }
+}
+
+// c_j (j < N), b_m (m < M) and the first convolution operands c_j u_j of the
+// three coordinates, zero padded to M; one tile per block
+__global__ __launch_bounds__(256) void k_bs_tables(const BsLoop *__restrict__ L,
+                                                   const BsState *__restrict__ S,
+                                                   const int *__restrict__ tile_loop,
+                                                   double2 *__restrict__ arena)
+{
+    const int b = tile_loop[blockIdx.x];
+    const BsState s = S[b];
+    const BsLoop B = L[b];
+    const int M = 1 << s.logM;
+    const int m0 = (blockIdx.x - B.tile0) << kTileLog;
+    if (!s.filt || m0 >= M) return;
+    const int n = B.n, N = s.N;
+    const double *xs = (const double *)arena + B.xs;
+    double2 *chirp = arena + B.chirp;
+    double2 *bq = bs_signal(arena, B, 3);
+    for (int m = m0 + threadIdx.x; m < m0 + kTile; m += blockDim.x) {
+        double2 cv = make_double2(0, 0), bv = make_double2(0, 0);
+        if (m < N) {
+            cv = chirp_at(m, N, -1.0);
+            chirp[m] = cv;
+            bv = conj2(cv);
+        } else if (m > M - N) {
+            bv = chirp_at(M - m, N, 1.0);
+        }
+        bq[m] = bv;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double2 v = make_double2(0, 0);
+            if (m < N) {
+                const double u = bs_coord(xs + (size_t)c * n, n, s, c, m);
+                v = make_double2(u * cv.x, u * cv.y);
+            }
+            bs_signal(arena, B, c)[m] = v;
+        }
+    }
+}
+
+// One grid-wide pass over the global stages (span >= kTile) of every signal:
+// stages 2*pass and 2*pass+1 as one radix-4 step, or the last stage alone
+// (radix 2) when a signal has an odd count.  Forward (DIF): stage s has span
+// M/2^(s+1); inverse (DIT): span kTile*2^s.  The twiddle of a span-h
+// butterfly at position p is e^{-+2 pi i p/(2h)} = tw[p << (logT-1-log2 h)].
+template <bool INV>
+__global__ __launch_bounds__(kPassBlock) void k_bs_pass(
+    const BsLoop *__restrict__ L, const BsState *__restrict__ S, const int *__restrict__ tile_loop,
+    double2 *__restrict__ arena, const double2 *__restrict__ tw, int logT, int sig0, int pass)
+{
+    const int b = tile_loop[blockIdx.x];
+    const int logM = S[b].logM;
+    if (!S[b].filt) return;
+    const BsLoop B = L[b];
+    const int tl = blockIdx.x - B.tile0;
+    const int G = logM - kTileLog;
+    const int s0 = 2 * pass;
+    if (s0 >= G || (tl << kTileLog) >= (1 << logM)) return;
+    double2 *a = bs_signal(arena, B, sig0 + blockIdx.y);
+    const int lh = INV ? kTileLog + s0 : logM - 1 - s0;  // log2 of the first stage's span
+    if (s0 + 1 < G) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int q = (tl << (kTileLog - 2)) + k * kPassBlock + threadIdx.x;
+            if (!INV) {
+                // spans h and h/2: elements i0 + {0, 1, 2, 3} h/2
+                const int hh = 1 << (lh - 1);
+                const int p = q & (hh - 1);
+                const int i0 = ((q >> (lh - 1)) << (lh + 1)) + p;
+                const double2 w1 = tw[p << (logT - 1 - lh)];
+                const double2 w2 = make_double2(w1.y, -w1.x);  // position p + h/2: w1 (-i)
+                const double2 w3 = tw[p << (logT - lh)];
+                const double2 a0 = a[i0], a1 = a[i0 + hh], a2 = a[i0 + 2 * hh], a3 = a[i0 + 3 * hh];
+                const double2 b0 = cadd(a0, a2), b2 = cmul(csub(a0, a2), w1);
+                const double2 b1 = cadd(a1, a3), b3 = cmul(csub(a1, a3), w2);
+                a[i0] = cadd(b0, b1);
+                a[i0 + hh] = cmul(csub(b0, b1), w3);
+                a[i0 + 2 * hh] = cadd(b2, b3);
+                a[i0 + 3 * hh] = cmul(csub(b2, b3), w3);
+            } else {
+                // spans h and 2h: elements i0 + {0, 1, 2, 3} h
+                const int h = 1 << lh;
+                const int p = q & (h - 1);
+                const int i0 = ((q >> lh) << (lh + 2)) + p;
+                const double2 W1 = conj2(tw[p << (logT - 1 - lh)]);
+                const double2 W2 = conj2(tw[p << (logT - 2 - lh)]);
+                const double2 W3 = make_double2(-W2.y, W2.x);  // position p + h of span 2h: W2 i
+                const double2 a0 = a[i0], a1 = a[i0 + h], a2 = a[i0 + 2 * h], a3 = a[i0 + 3 * h];
+                double2 t = cmul(W1, a1);
+                const double2 b0 = cadd(a0, t), b1 = csub(a0, t);
+                t = cmul(W1, a3);
+                const double2 b2 = cadd(a2, t), b3 = csub(a2, t);
+                t = cmul(W2, b2);
+                a[i0] = cadd(b0, t);
+                a[i0 + 2 * h] = csub(b0, t);
+                t = cmul(W3, b3);
+                a[i0 + h] = cadd(b1, t);
+                a[i0 + 3 * h] = csub(b1, t);
+            }
+        }
+    } else {
+        const int h = 1 << lh;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = (tl << (kTileLog - 1)) + k * kPassBlock + threadIdx.x;
+            const int p = q & (h - 1);
+            const int i = ((q >> lh) << (lh + 1)) + p;
+            const double2 w = tw[p << (logT - 1 - lh)];
+            const double2 u = a[i], v = a[i + h];
+            if (!INV) {
+                a[i] = cadd(u, v);
+                a[i + h] = cmul(csub(u, v), w);
+            } else {
+                const double2 t = cmul(conj2(w), v);
+                a[i] = cadd(u, t);
+                a[i + h] = csub(u, t);
+            }
+        }
+    }
+}
+
+// The stages of span < kTile of one tile, in LDS: forward only (the b
+// sequence), or forward, the product with DIF(b) and the inverse stages
+// of span < kTile (the coordinate signals).
+template <bool CONV>
+__global__ __launch_bounds__(kTileBlock) void k_bs_tile(
+    const BsLoop *__restrict__ L, const BsState *__restrict__ S, const int *__restrict__ tile_loop,
+    double2 *__restrict__ arena, const double2 *__restrict__ tw, int logT)
+{
+    const int b = tile_loop[blockIdx.x];
+    const int logM = S[b].logM;
+    if (!S[b].filt) return;
+    const BsLoop B = L[b];
+    const int tl = blockIdx.x - B.tile0;
+    if ((tl << kTileLog) >= (1 << logM)) return;
+    double2 *a = bs_signal(arena, B, CONV ? (int)blockIdx.y : 3) + ((size_t)tl << kTileLog);
+    __shared__ double2 sh[kTile];
+    for (int i = threadIdx.x; i < kTile; i += kTileBlock) sh[i] = a[i];
+    __syncthreads();
+    for (int lh = kTileLog - 1; lh >= 0; --lh) {
+        const int h = 1 << lh;
+        for (int q = threadIdx.x; q < kTile / 2; q += kTileBlock) {
+            const int p = q & (h - 1);
+            const int i = ((q >> lh) << (lh + 1)) + p;
+            const double2 w = tw[p << (logT - 1 - lh)];
+            const double2 u = sh[i], v = sh[i + h];
+            sh[i] = cadd(u, v);
+            sh[i + h] = cmul(csub(u, v), w);
+        }
+        __syncthreads();
+    }
+    if (CONV) {
+        const double2 *bh = bs_signal(arena, B, 3) + ((size_t)tl << kTileLog);
+        for (int i = threadIdx.x; i < kTile; i += kTileBlock) sh[i] = cmul(sh[i], bh[i]);
+        __syncthreads();
+        for (int lh = 0; lh < kTileLog; ++lh) {
+            const int h = 1 << lh;
+            for (int q = threadIdx.x; q < kTile / 2; q += kTileBlock) {
+                const int p = q & (h - 1);
+                const int i = ((q >> lh) << (lh + 1)) + p;
+                const double2 t = cmul(conj2(tw[p << (logT - 1 - lh)]), sh[i + h]);
+                const double2 u = sh[i];
+                sh[i] = cadd(u, t);
+                sh[i + h] = csub(u, t);
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < kTile; i += kTileBlock) a[i] = sh[i];
+}
+
+// Between the two transforms: U = c . conv / M, filtered and conjugated, and
+// times c again as the second convolution operand (zero padded).
+__global__ __launch_bounds__(256) void k_bs_mid(const BsLoop *__restrict__ L,
+                                                const BsState *__restrict__ S,
+                                                const int *__restrict__ tile_loop,
+                                                double2 *__restrict__ arena)
+{
+    const int b = tile_loop[blockIdx.x];
+    const BsState s = S[b];
+    const BsLoop B = L[b];
+    const int M = 1 << s.logM;
+    const int m0 = (blockIdx.x - B.tile0) << kTileLog;
+    if (!s.filt || m0 >= M) return;
+    double2 *a = bs_signal(arena, B, blockIdx.y);
+    const double2 *chirp = arena + B.chirp;
+    const double inv = 1.0 / M;
+    for (int m = m0 + threadIdx.x; m < m0 + kTile; m += blockDim.x) {
+        double2 v = make_double2(0, 0);
+        if (m < s.N) {
+            const double2 cv = chirp[m];
+            const double2 u = cmul(a[m], cv);
+            v = cmul(filter_conj(make_double2(u.x * inv, u.y * inv), m, s.N, s.kr, s.ki), cv);
+        }
+        a[m] = v;
+    }
+}
+
+// y_j = Re(c_j conv_j)/(M N), the detrend line added back, written along the
+// loop (src/vortex_smooth.c:286-309)
+__global__ __launch_bounds__(kChunk) void k_bs_finish(
+    const BsLoop *__restrict__ L, const BsState *__restrict__ S, const int *__restrict__ chunk_loop,
+    const int *__restrict__ loop_off, const int *__restrict__ loop_slots,
+    double2 *__restrict__ arena, double *__restrict__ X, double *__restrict__ Y,
+    double *__restrict__ Z)
+{
+    const int b = chunk_loop[blockIdx.x];
+    const BsLoop B = L[b];
+    const int j = (blockIdx.x - B.chunk0) * kChunk + threadIdx.x;
+    if (j >= B.n) return;
+    const BsState s = S[b];
+    const double *xs = (const double *)arena + B.xs;
+    const double inv = 1.0 / (1 << s.logM);
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double y;
+        if (s.filt) {
+            const double2 u = cmul(bs_signal(arena, B, c)[j], arena[B.chirp + j]);
+            y = u.x * inv / s.N;
+        } else {
+            y = bs_coord(xs + (size_t)c * B.n, B.n, s, c, j);
+        }
+        if (s.span) y = y + s.coef[c] * j + s.x0[c];
+        v[c] = y;
+    }
+    const int m = loop_slots[loop_off[B.loop] + j];
     X[m] = v[0];
     Y[m] = v[1];
     Z[m] = v[2];
 }
 
-static inline int g1(long long n) { return (int)((n + 255) / 256); }
+static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-static void fft_grid(Ctx &c, double2 *a, int M, int logM, const double2 *tw, int inverse)
+// Smooth the loops of the loop table listed in `big` (all longer than
+// kSmallLoop) with the batched transform.  Host vectors stay alive until
+// the caller's final stream synchronisation.
+static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hmeta)
 {
-    hipLaunchKernelGGL(k_g_bitrev, dim3(g1(M)), dim3(256), 0, c.stream, a, M, logM);
-    for (int st = 1; st <= logM; ++st)
-        hipLaunchKernelGGL(k_g_stage, dim3(g1(M / 2)), dim3(256), 0, c.stream, a, M, st, tw,
-                           inverse);
-}
-
-static void bluestein_grid(Ctx &c, double2 *U, int N, int M, int logM, const double2 *chirp,
-                           const double2 *Bhat, const double2 *tw, double2 *work)
-{
-    hipLaunchKernelGGL(k_g_point, dim3(g1(M)), dim3(256), 0, c.stream, 0, work, U, chirp, Bhat, N, M);
-    fft_grid(c, work, M, logM, tw, 0);
-    hipLaunchKernelGGL(k_g_point, dim3(g1(M)), dim3(256), 0, c.stream, 1, work, U, chirp, Bhat, N, M);
-    fft_grid(c, work, M, logM, tw, 1);
-    hipLaunchKernelGGL(k_g_point, dim3(g1(M)), dim3(256), 0, c.stream, 2, work, U, chirp, Bhat, N, M);
-}
-
-static int smooth_giant(Ctx &c, int loop, int n)
-{
-    int Nmax = 2 * n - 2, logMmax = 0;
-    while ((1LL << logMmax) < 2LL * Nmax - 1) ++logMmax;
-    long long Mmax = 1LL << logMmax;
-    size_t need = sizeof(double2) * (size_t)(3LL * n + Mmax / 2 + 2LL * n + 2 * Mmax + 2LL * n + 8);
-    OVG_TRY(c.sm_giant.ensure(need));
-    OVG_TRY(c.sm_gstate.ensure(sizeof(GiantState)));
-    double2 *base = c.sm_giant.as<double2>();
-    double *xs0 = (double *)base;
-    double2 *tw = base + 3 * (size_t)n;
-    double2 *chirp = tw + Mmax / 2;
-    double2 *Bhat = chirp + 2 * (size_t)n;
-    double2 *work = Bhat + Mmax;
-    double2 *U = work + Mmax;
-    GiantState *gs = c.sm_gstate.as<GiantState>();
-    hipLaunchKernelGGL(k_giant_unwrap, dim3(1), dim3(kFftBlock), 0, c.stream, loop,
-                       c.loop_off.as<int>(), c.loop_slots.as<int>(), c.x[0].as<double>(),
-                       c.x[1].as<double>(), c.x[2].as<double>(), dparams(c), xs0, gs);
-    GiantState h;
-    OVG_HIP(hipMemcpyAsync(&h, gs, sizeof h, hipMemcpyDeviceToHost, c.stream));
-    OVG_HIP(hipStreamSynchronize(c.stream));
-    const int N = h.span ? 2 * n - 2 : n;
-    const int s = (int)(c.params.freq_to_cutoff * N / 2);
-    if (N > 2 * s + 1 && N >= 2) {
-        const int half = N / 2;
-        int kr = half - s;
-        int ki = (N % 2 == 0) ? half - s : half + 2 - s;
-        if (ki > half + 1) ki = half + 1;
-        if (N % 2 == 0 && ki > half) ki = half;
+    const int nb = (int)big.size();
+    std::vector<BsLoop> L(nb);
+    std::vector<int> chunk_loop, tile_loop;
+    long long arena = 0;  // double2
+    int logMall = kTileLog;
+    auto al16 = [](long long v) { return (v + 15) & ~15LL; };
+    for (int i = 0; i < nb; ++i) {
+        const int lp = big[i];
+        const int n = c.h_loop_off[lp + 1] - c.h_loop_off[lp];
+        const long long Nmax = 2LL * n - 2;
         int logM = 0;
-        while ((1LL << logM) < 2LL * N - 1) ++logM;
-        int M = 1 << logM;
-        hipLaunchKernelGGL(k_g_tables, dim3(1024), dim3(256), 0, c.stream, N, M, tw, chirp, Bhat);
-        fft_grid(c, Bhat, M, logM, tw, 0);
-        for (int cc = 0; cc < 3; ++cc) {
-            double *xc = xs0 + 2 * (size_t)n * cc;
-            hipLaunchKernelGGL(k_g_coord, dim3(g1(N)), dim3(256), 0, c.stream, 0, xc, U, n, N, kr, ki);
-            bluestein_grid(c, U, N, M, logM, chirp, Bhat, tw, work);
-            hipLaunchKernelGGL(k_g_coord, dim3(g1(N)), dim3(256), 0, c.stream, 1, xc, U, n, N, kr, ki);
-            bluestein_grid(c, U, N, M, logM, chirp, Bhat, tw, work);
-            hipLaunchKernelGGL(k_g_coord, dim3(g1(n)), dim3(256), 0, c.stream, 2, xc, U, n, N, kr, ki);
-        }
+        while ((1LL << logM) < 2 * Nmax - 1) ++logM;
+        if (logM > 30) return set_error(OVG_ELIMIT, "a loop of %d points is too long to smooth", n);
+        BsLoop &B = L[i];
+        B.loop = lp;
+        B.n = n;
+        B.logMmax = logM;
+        B.pad = 0;
+        B.chunk0 = (int)chunk_loop.size();
+        B.tile0 = (int)tile_loop.size();
+        chunk_loop.insert(chunk_loop.end(), (n + kChunk - 1) / kChunk, i);
+        tile_loop.insert(tile_loop.end(), (size_t)1 << (logM - kTileLog), i);
+        B.xs = 2 * arena;
+        arena = al16(arena + (3LL * n + 1) / 2);
+        B.chirp = arena;
+        arena = al16(arena + Nmax);
+        B.bhat = arena;
+        arena = al16(arena + (1LL << logM));
+        B.work = arena;
+        arena = al16(arena + (3LL << logM));
+        logMall = std::max(logMall, logM);
     }
-    hipLaunchKernelGGL(k_giant_finish, dim3(g1(n)), dim3(256), 0, c.stream, loop,
-                       c.loop_off.as<int>(), c.loop_slots.as<int>(), xs0, gs, c.x[0].as<double>(),
-                       c.x[1].as<double>(), c.x[2].as<double>());
+    if (tile_loop.size() > (size_t)INT32_MAX / 2)
+        return set_error(OVG_ELIMIT, "too many transform tiles to smooth");
+    const long long nch = (long long)chunk_loop.size(), nt = (long long)tile_loop.size();
+    auto al256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t offC = al256(sizeof(BsLoop) * nb), offT = offC + al256(sizeof(int) * nch);
+    const size_t meta = offT + sizeof(int) * nt;
+    const size_t offS = 0, offW = al256(sizeof(BsState) * nb), offK = offW + al256(12 * nch);
+    const size_t state = offK + 12 * nch;
+    OVG_TRY(c.sm_big.ensure(meta));
+    OVG_TRY(c.sm_gstate.ensure(state));
+    OVG_TRY(c.sm_fft.ensure(sizeof(double2) * (size_t)arena));
+    if (logMall > c.sm_tw_log) {
+        OVG_TRY(c.sm_tw.ensure(sizeof(double2) << (logMall - 1)));
+        c.sm_tw_log = logMall;
+        hipLaunchKernelGGL(k_bs_twiddles, dim3(nblocks(1LL << (logMall - 1), 256)), dim3(256), 0,
+                           c.stream, c.sm_tw_log, c.sm_tw.as<double2>());
+    }
+    hmeta.assign(meta, 0);
+    std::memcpy(hmeta.data(), L.data(), sizeof(BsLoop) * nb);
+    std::memcpy(hmeta.data() + offC, chunk_loop.data(), sizeof(int) * nch);
+    std::memcpy(hmeta.data() + offT, tile_loop.data(), sizeof(int) * nt);
+    OVG_HIP(hipMemcpyAsync(c.sm_big.p, hmeta.data(), meta, hipMemcpyHostToDevice, c.stream));
+
+    char *mb = c.sm_big.as<char>(), *sb = c.sm_gstate.as<char>();
+    const BsLoop *dL = (const BsLoop *)mb;
+    const int *dC = (const int *)(mb + offC), *dT = (const int *)(mb + offT);
+    BsState *dS = (BsState *)(sb + offS);
+    int *csum = (int *)(sb + offW), *carry = (int *)(sb + offK);
+    double2 *ar = c.sm_fft.as<double2>();
+    const double2 *tw = c.sm_tw.as<double2>();
+    const int logT = c.sm_tw_log;
+    const int *loff = c.loop_off.as<int>(), *lsl = c.loop_slots.as<int>();
+    double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
+    const int passes = (logMall - kTileLog + 1) / 2;
+    const dim3 one((unsigned)nt, 1), three((unsigned)nt, 3);
+
+    hipLaunchKernelGGL(k_bs_wraps, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dC, loff,
+                       lsl, X, Y, Z, dparams(c), csum);
+    hipLaunchKernelGGL(k_bs_carry, dim3(nb), dim3(kChunk), 0, c.stream, dL, csum, carry);
+    hipLaunchKernelGGL(k_bs_unwrap, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dC, loff,
+                       lsl, X, Y, Z, dparams(c), carry, (double *)ar);
+    hipLaunchKernelGGL(k_bs_setup, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, nb, dL,
+                       (const double *)ar, dparams(c), c.params.freq_to_cutoff, dS);
+    hipLaunchKernelGGL(k_bs_tables, one, dim3(256), 0, c.stream, dL, dS, dT, ar);
+    for (int q = 0; q < passes; ++q)
+        hipLaunchKernelGGL((k_bs_pass<false>), one, dim3(kPassBlock), 0, c.stream, dL, dS, dT, ar,
+                           tw, logT, 3, q);
+    hipLaunchKernelGGL((k_bs_tile<false>), one, dim3(kTileBlock), 0, c.stream, dL, dS, dT, ar, tw,
+                       logT);
+    for (int rep = 0; rep < 2; ++rep) {
+        for (int q = 0; q < passes; ++q)
+            hipLaunchKernelGGL((k_bs_pass<false>), three, dim3(kPassBlock), 0, c.stream, dL, dS,
+                               dT, ar, tw, logT, 0, q);
+        hipLaunchKernelGGL((k_bs_tile<true>), three, dim3(kTileBlock), 0, c.stream, dL, dS, dT,
+                           ar, tw, logT);
+        for (int q = 0; q < passes; ++q)
+            hipLaunchKernelGGL((k_bs_pass<true>), three, dim3(kPassBlock), 0, c.stream, dL, dS,
+                               dT, ar, tw, logT, 0, q);
+        if (rep == 0)
+            hipLaunchKernelGGL(k_bs_mid, three, dim3(256), 0, c.stream, dL, dS, dT, ar);
+    }
+    hipLaunchKernelGGL(k_bs_finish, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dS, dC,
+                       loff, lsl, ar, X, Y, Z);
     return OVG_OK;
 }
 
@@ -599,45 +848,23 @@ static int build_loops(Ctx &c)
     return OVG_OK;
 }
 
+
 int smooth_loops(Ctx &c)
 {
     OVG_TRY(build_loops(c));
     if (c.n_loops == 0) return OVG_OK;
     size_t need = sizeof(double) * ((size_t)15 * c.n_live + (size_t)6 * c.n_loops + 64);
     OVG_TRY(c.sm_buf.ensure(need));
-    std::vector<BigLoop> big;
-    std::vector<int> giant;
-    long long foff = 0;
-    for (int lp = 0; lp < c.n_loops; ++lp) {
-        int n = c.h_loop_off[lp + 1] - c.h_loop_off[lp];
-        if (n <= kSmallLoop) continue;
-        if (n > kMediumLoop) {
-            giant.push_back(lp);
-            continue;
-        }
-        int N = 2 * n - 2, logM = 0;
-        while ((1 << logM) < 2 * N - 1) ++logM;
-        int M = 1 << logM;
-        big.push_back(BigLoop{lp, logM, foff});
-        foff += 3LL * n + M / 2 + 2LL * n + 2LL * M + 2LL * n + 8;
-    }
+    std::vector<int> big;
+    for (int lp = 0; lp < c.n_loops; ++lp)
+        if (c.h_loop_off[lp + 1] - c.h_loop_off[lp] > kSmallLoop) big.push_back(lp);
     hipLaunchKernelGGL(k_smooth, dim3(c.n_loops), dim3(kSmBlock), 0, c.stream, c.n_loops,
                        c.loop_off.as<int>(), c.loop_slots.as<int>(), c.x[0].as<double>(),
                        c.x[1].as<double>(), c.x[2].as<double>(), dparams(c),
                        c.params.freq_to_cutoff, c.sm_buf.as<double>());
-    if (!big.empty()) {
-        OVG_TRY(c.sm_fft.ensure(sizeof(double2) * (size_t)foff));
-        OVG_TRY(c.sm_big.ensure(sizeof(BigLoop) * big.size()));
-        OVG_HIP(hipMemcpyAsync(c.sm_big.p, big.data(), sizeof(BigLoop) * big.size(),
-                               hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(k_smooth_fft, dim3((unsigned)big.size()), dim3(kFftBlock), 0,
-                           c.stream, (int)big.size(), c.sm_big.as<BigLoop>(),
-                           c.loop_off.as<int>(), c.loop_slots.as<int>(), c.x[0].as<double>(),
-                           c.x[1].as<double>(), c.x[2].as<double>(), dparams(c),
-                           c.params.freq_to_cutoff, c.sm_fft.as<double2>());
-    }
-    for (int lp : giant) OVG_TRY(smooth_giant(c, lp, c.h_loop_off[lp + 1] - c.h_loop_off[lp]));
-    OVG_HIP(hipStreamSynchronize(c.stream));  // `big` is a host vector
+    std::vector<char> hmeta;
+    if (!big.empty()) OVG_TRY(smooth_big(c, big, hmeta));
+    OVG_HIP(hipStreamSynchronize(c.stream));  // hmeta is a host vector
     return OVG_OK;
 }
 

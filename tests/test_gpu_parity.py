@@ -468,6 +468,54 @@ def test_remesh_capacity_doubling(oracle):
     assert np.array_equal(o["vnodes"][want.live], want.vnodes[want.live])
 
 
+def test_smooth_many_long_loops_scattered_slots():
+    """Thirty loops longer than 1024 points in one batched transform: rings
+    and box-spanning lines of assorted lengths (1025 up, transform lengths
+    4096 to 65536), points scattered over the slots in random order, and a
+    few short loops beside them."""
+    from oracle import smooth as S
+    D = 0.1
+    rng = np.random.default_rng(21)
+    sizes = [1025, 1026, 2047, 2049, 4000, 1500, 12000, 3333, 1100, 9000] + \
+        list(rng.integers(1025, 14000, size=20))
+    sizes += [40, 200, 1024]
+    pieces = []
+    for i, m in enumerate(sizes):
+        if i % 3 == 1:
+            pieces.append(spanning_line(m, D, i % 3, amp=5e-5, seed=100 + i))
+        else:
+            R = 0.005 + 0.03 * rng.random()
+            ph = 2 * np.pi * np.arange(m) / m
+            c = rng.random(3) * D
+            ring = np.stack([c[0] + R * np.cos(ph), c[1] + R * np.sin(ph), np.full(m, c[2])], 1)
+            pieces.append(ring + rng.normal(scale=2e-5, size=ring.shape))
+    n = sum(sizes)
+    perm = rng.permutation(n)          # slot of the k-th point in piece order
+    x = np.zeros((n, 3))
+    conn = np.zeros((n, 2), dtype=np.int32)
+    off = 0
+    for p_ in pieces:
+        m = len(p_)
+        sl = perm[off:off + m]
+        x[sl] = p_ % D
+        conn[sl, 0] = sl[(np.arange(m) + 1) % m]
+        conn[sl, 1] = sl[(np.arange(m) - 1) % m]
+        off += m
+    st = np.zeros((n, 2), dtype=np.int32)
+    st[:, 1] = -1
+    t = ov.Tangle(x, conn, st)
+    p = configs.params(D, 1e-3, configs.T16, 0.0)
+    say(f"smoothing: {len(sizes)} loops, {n} points in scattered slots")
+    with engine(t, p) as e:
+        e.smooth()
+        got = e.download()["vnodes"]
+    want = S.smooth_tangle(t.vnodes, t.conn, t.status, [0, 0, 0], [D, D, D], [1, 1, 1],
+                           p.freq_to_cutoff)
+    err = np.abs(got - want).max()
+    say(f"max |gpu - numpy| = {err:.2e}")
+    assert err < 1e-13
+
+
 def test_smooth_giant_loops():
     from oracle import smooth as S
     D = 0.2
