@@ -320,7 +320,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
                    &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf, &c.split_part,
-                   &c.loop_off, &c.loop_slots, &c.sm_fft, &c.sm_big, &c.sm_tw,
+                   &c.loop_off, &c.loop_slots, &c.lt_buf, &c.sm_fft, &c.sm_big, &c.sm_tw,
                    &c.sm_gstate, &c.topo_stage};
     for (auto *b : all) b->release();
     for (int q = 0; q < 3; ++q) {

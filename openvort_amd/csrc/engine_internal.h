@@ -128,6 +128,7 @@ struct Ctx {
     uint64_t loops_version = 0;         // topology the smoothing loop table matches
     int n_loops = 0;
     DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
+    DBuf lt_buf;                        // loop table construction scratch
     std::vector<int> h_loop_off;
     DBuf sm_fft, sm_big;                // transform arena and big-loop list + chunk maps
     DBuf sm_gstate;                     // per big loop state, per chunk wrap sums

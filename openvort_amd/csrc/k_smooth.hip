@@ -19,6 +19,7 @@
 // restricted to the entries that survive.  Twiddles use exact integer
 // argument reduction (jk mod N) and sincospi.
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -817,11 +818,90 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
     return OVG_OK;
 }
 
-// Loop table from the host link mirror: loops in order of their smallest
-// slot, each listed forward from it (src/vortex_smooth.c:192-242).
-static int build_loops(Ctx &c)
+// ---- loop table ------------------------------------------------------------
+// Loops in order of their smallest slot, each listed forward from it
+// (src/vortex_smooth.c:192-242).  On the device: min-label pointer jumping
+// over the forward links gives every slot the smallest slot of its loop (a
+// window of 2^r links per round: ceil(log2 n_live) rounds cover any loop),
+// list ranking (Wyllie) on each loop cut open at that slot gives every
+// slot's distance d to it, so its position is (len - d) mod len; heads are
+// numbered by an exclusive scan in slot order and every slot is scattered
+// to its place.  Open chains (a live forward link < 0) take the host walk.
+
+__global__ void k_lt_init(int n, const int *__restrict__ fwd, const int *__restrict__ status,
+                          int *__restrict__ lab, int *__restrict__ nx, int *__restrict__ open)
 {
-    if (c.loops_version == c.topo_version) return OVG_OK;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool live = status[i] != OVG_EMPTY;  // one status word per slot
+    const int f = fwd[i];
+    if (live && f < 0) *open = 1;
+    lab[i] = live ? i : INT_MAX;
+    nx[i] = (live && f >= 0) ? f : i;
+}
+
+__global__ void k_lt_minjump(int n, const int *__restrict__ lab, const int *__restrict__ nx,
+                             int *__restrict__ lab2, int *__restrict__ nx2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = nx[i];
+    lab2[i] = min(lab[i], lab[j]);
+    nx2[i] = nx[j];
+}
+
+// the head (smallest slot) ends its list; dead slots are lists of their own
+__global__ void k_lt_rank_init(int n, const int *__restrict__ fwd, const int *__restrict__ lab,
+                               int *__restrict__ nx, int *__restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool end = lab[i] == INT_MAX || lab[i] == i || fwd[i] < 0;
+    nx[i] = end ? i : fwd[i];
+    d[i] = end ? 0 : 1;
+}
+
+__global__ void k_lt_rankjump(int n, const int *__restrict__ nx, const int *__restrict__ d,
+                              int *__restrict__ nx2, int *__restrict__ d2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = nx[i];
+    d2[i] = d[i] + d[j];
+    nx2[i] = nx[j];
+}
+
+// head flags and loop lengths at the heads (d of the head's successor + 1)
+__global__ void k_lt_heads(int n, const int *__restrict__ fwd, const int *__restrict__ lab,
+                           const int *__restrict__ d, int *__restrict__ flag, int *__restrict__ len)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool head = lab[i] == i;
+    flag[i] = head;
+    const int f = fwd[i];
+    len[i] = head ? (f >= 0 ? d[f] + 1 : 1) : 0;  // open chains are rebuilt on the host
+}
+
+// hidx: loop number of each head, hoff: first entry of each head's loop
+// (exclusive scans of flag and len); position (len - d) mod len
+__global__ void k_lt_scatter(int n, const int *__restrict__ lab, const int *__restrict__ d,
+                             const int *__restrict__ len, const int *__restrict__ hidx,
+                             const int *__restrict__ hoff, int *__restrict__ loop_off,
+                             int *__restrict__ slots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) loop_off[hidx[n]] = hoff[n];
+    if (i >= n || lab[i] == INT_MAX) return;
+    const int m = lab[i];
+    if (m == i) loop_off[hidx[i]] = hoff[i];
+    const int pos = m == i ? 0 : len[m] - d[i];
+    if (pos >= 0 && pos < len[m]) slots[hoff[m] + pos] = i;  // always, for consistent links
+}
+
+// Loop table from the host link mirror (chains with open ends).
+static int build_loops_host(Ctx &c)
+{
     int n = c.n;
     std::vector<int> off, slots;
     std::vector<char> seen(n, 0);
@@ -844,6 +924,63 @@ static int build_loops(Ctx &c)
     if (!slots.empty())
         OVG_HIP(hipMemcpy(c.loop_slots.p, slots.data(), sizeof(int) * slots.size(),
                           hipMemcpyHostToDevice));
+    c.loops_version = c.topo_version;
+    return OVG_OK;
+}
+
+static int build_loops(Ctx &c)
+{
+    if (c.loops_version == c.topo_version) return OVG_OK;
+    const int n = c.n;
+    if (n <= 0) return build_loops_host(c);
+    // lab, nx, lab2, nx2, len, flag (n + 1), len scan (n + 1), open flag
+    const size_t stride = ((size_t)n + 64) & ~(size_t)63;
+    OVG_TRY(c.lt_buf.ensure(sizeof(int) * (7 * stride + 64)));
+    int *A = c.lt_buf.as<int>();
+    int *lab = A, *nx = A + stride, *lab2 = A + 2 * stride, *nx2 = A + 3 * stride;
+    int *len = A + 4 * stride, *flag = A + 5 * stride, *hoff = A + 6 * stride;
+    int *open = A + 7 * stride;
+    const unsigned g = nblocks(n, 256);
+    OVG_HIP(hipMemsetAsync(open, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_lt_init, dim3(g), dim3(256), 0, c.stream, n, c.fwd.as<int>(),
+                       c.status.as<int>(), lab, nx, open);
+    int rounds = 0;
+    while ((1LL << rounds) < (long long)std::max(n, 2)) ++rounds;
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_lt_minjump, dim3(g), dim3(256), 0, c.stream, n, lab, nx, lab2, nx2);
+        std::swap(lab, lab2);
+        std::swap(nx, nx2);
+    }
+    // lab is final; nx, lab2, nx2 are free: ranking in nx/lab2 <-> nx2/len
+    int *d = lab2, *d2 = len;
+    hipLaunchKernelGGL(k_lt_rank_init, dim3(g), dim3(256), 0, c.stream, n, c.fwd.as<int>(), lab,
+                       nx, d);
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_lt_rankjump, dim3(g), dim3(256), 0, c.stream, n, nx, d, nx2, d2);
+        std::swap(nx, nx2);
+        std::swap(d, d2);
+    }
+    int *L = d2;  // the free one of lab2 / len
+    hipLaunchKernelGGL(k_lt_heads, dim3(g), dim3(256), 0, c.stream, n, c.fwd.as<int>(), lab, d,
+                       flag, L);
+    launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp);
+    launch_exclusive_scan(c, L, hoff, n, hoff + n, c.tree.scan_tmp);
+    int hv[3] = {0, 0, 0};
+    OVG_HIP(hipMemcpyAsync(&hv[0], flag + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipMemcpyAsync(&hv[1], hoff + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipMemcpyAsync(&hv[2], open, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    if (hv[2]) return build_loops_host(c);
+    const int n_loops = hv[0];
+    OVG_TRY(c.loop_off.ensure(sizeof(int) * ((size_t)n_loops + 1)));
+    OVG_TRY(c.loop_slots.ensure(sizeof(int) * ((size_t)hv[1] + 1)));
+    hipLaunchKernelGGL(k_lt_scatter, dim3(g), dim3(256), 0, c.stream, n, lab, d, L, flag, hoff,
+                       c.loop_off.as<int>(), c.loop_slots.as<int>());
+    c.n_loops = n_loops;
+    c.h_loop_off.resize((size_t)n_loops + 1);
+    OVG_HIP(hipMemcpyAsync(c.h_loop_off.data(), c.loop_off.p, sizeof(int) * ((size_t)n_loops + 1),
+                           hipMemcpyDeviceToHost, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
     c.loops_version = c.topo_version;
     return OVG_OK;
 }

@@ -471,8 +471,8 @@ def test_remesh_capacity_doubling(oracle):
 def test_smooth_many_long_loops_scattered_slots():
     """Thirty loops longer than 1024 points in one batched transform: rings
     and box-spanning lines of assorted lengths (1025 up, transform lengths
-    4096 to 65536), points scattered over the slots in random order, and a
-    few short loops beside them."""
+    4096 to 65536), points scattered over the slots in random order with
+    empty slots between them, and a few short loops beside them."""
     from oracle import smooth as S
     D = 0.1
     rng = np.random.default_rng(21)
@@ -489,10 +489,10 @@ def test_smooth_many_long_loops_scattered_slots():
             c = rng.random(3) * D
             ring = np.stack([c[0] + R * np.cos(ph), c[1] + R * np.sin(ph), np.full(m, c[2])], 1)
             pieces.append(ring + rng.normal(scale=2e-5, size=ring.shape))
-    n = sum(sizes)
+    n = sum(sizes) + 7000              # 7000 empty slots among them
     perm = rng.permutation(n)          # slot of the k-th point in piece order
     x = np.zeros((n, 3))
-    conn = np.zeros((n, 2), dtype=np.int32)
+    conn = np.full((n, 2), -1, dtype=np.int32)
     off = 0
     for p_ in pieces:
         m = len(p_)
@@ -503,9 +503,10 @@ def test_smooth_many_long_loops_scattered_slots():
         off += m
     st = np.zeros((n, 2), dtype=np.int32)
     st[:, 1] = -1
+    st[perm[off:], 0] = -1             # EMPTY
     t = ov.Tangle(x, conn, st)
     p = configs.params(D, 1e-3, configs.T16, 0.0)
-    say(f"smoothing: {len(sizes)} loops, {n} points in scattered slots")
+    say(f"smoothing: {len(sizes)} loops, {off} points in {n} scattered slots")
     with engine(t, p) as e:
         e.smooth()
         got = e.download()["vnodes"]
