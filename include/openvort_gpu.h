@@ -130,8 +130,8 @@ typedef struct ovg_params {
                                      Barnes-Hut evaluator)                     */
     double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
                                      than this (cm) stay FP64; 0 = none        */
-    int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 16, 32 or 64
-                                     (0 = 16)                                   */
+    int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 8, 16, 32 or
+                                     64 (0 = 16)                                */
     int32_t group_level;          /* OVG_TRAVERSE_LIST: groups never span two
                                      octree cells of this level (1..21); 0 =
                                      the coarsest level whose cells are at most
@@ -305,7 +305,7 @@ int ovg_synchronize(ovg_ctx *ctx);
 /* ---- autotuning (no counterpart in the reference) --------------------- */
 /*
  * Time the list traversal (OVG_TRAVERSE_LIST) on the context's current
- * tangle for every combination of points_per_wave {16, 32, 64}, leaf_size
+ * tangle for every combination of points_per_wave {8, 16, 32, 64}, leaf_size
  * {1, 2, 4, 8} and group_level {L-1, L, L+1} (L: the automatic level),
  * one warm-up and `repeats` timed launches each on a tree built once, and
  * keep the fastest in the context's parameters.  Results differ between
@@ -319,7 +319,7 @@ typedef struct ovg_tune_row {
 } ovg_tune_row;
 typedef struct ovg_tune {
     int32_t n_rows, best;
-    ovg_tune_row rows[36];
+    ovg_tune_row rows[48];
 } ovg_tune;
 int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out);
 

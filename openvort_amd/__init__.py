@@ -66,7 +66,7 @@ class TuneRow(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("n_rows", C.c_int32), ("best", C.c_int32), ("rows", TuneRow * 36)]
+    _fields_ = [("n_rows", C.c_int32), ("best", C.c_int32), ("rows", TuneRow * 48)]
 
 
 class Stats(C.Structure):

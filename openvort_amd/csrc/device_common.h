@@ -139,4 +139,32 @@ __device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float 
     vz = cz * m;
 }
 
+// seg_field32 in terms of R and the segment vector Q = Rp - R, for far-field
+// entries whose |Q|^2 = q2 is stored with them: R x Rp = R x Q,
+// R.Rp = r2 + R.Q and |Rp|^2 = r2 + 2 R.Q + q2 (far entries lie several
+// segment lengths away, so the last form loses nothing to cancellation).
+// The three guards are folded into one comparison of the smallest of r2,
+// |Rp|^2 and 1e-8 times the normalised dot against 1e-16.  Adds the field
+// without the factor f_bs to (ax, ay, az); the caller applies f_bs to the
+// sum.
+__device__ __forceinline__ void seg_field32_far(float Rx, float Ry, float Rz, float Qx, float Qy,
+                                                float Qz, float q2, float &ax, float &ay,
+                                                float &az)
+{
+    const float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
+    const float rq = fmaf(Rz, Qz, fmaf(Ry, Qy, Rx * Qx));
+    const float dot = r2 + rq;
+    const float p2 = fmaf(2.0f, rq, r2 + q2);
+    const float ia = __builtin_amdgcn_rsqf(r2);
+    const float ib = __builtin_amdgcn_rsqf(p2);
+    const float iab = ia * ib;
+    const float nd = dot * iab;
+    const bool ok = fminf(fminf(r2, p2), nd * 1e-8f) >= 1e-16f;
+    float m = (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
+    m = ok ? m : 0.0f;
+    ax = fmaf(fmaf(Ry, Qz, -(Rz * Qy)), m, ax);
+    ay = fmaf(fmaf(Rz, Qx, -(Rx * Qz)), m, ay);
+    az = fmaf(fmaf(Rx, Qy, -(Ry * Qx)), m, az);
+}
+
 }  // namespace ovg

@@ -375,9 +375,9 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
     if (p->traverse != OVG_TRAVERSE_POINT && p->traverse != OVG_TRAVERSE_GROUP &&
         p->traverse != OVG_TRAVERSE_LIST)
         return set_error(OVG_EINVAL, "unknown traversal %d", p->traverse);
-    if (p->points_per_wave != 0 && p->points_per_wave != 16 && p->points_per_wave != 32 &&
-        p->points_per_wave != 64)
-        return set_error(OVG_EINVAL, "points_per_wave must be 0, 16, 32 or 64");
+    if (p->points_per_wave != 0 && p->points_per_wave != 8 && p->points_per_wave != 16 &&
+        p->points_per_wave != 32 && p->points_per_wave != 64)
+        return set_error(OVG_EINVAL, "points_per_wave must be 0, 8, 16, 32 or 64");
     if (p->group_level < 0 || p->group_level > 21)
         return set_error(OVG_EINVAL, "group_level must be in [0, 21]");
     if (p->traverse == OVG_TRAVERSE_POINT &&
@@ -1132,7 +1132,7 @@ extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
     Triple x = tri(c.x);
     OVG_TRY(build_tree(c, x.v));
     hipEvent_t e0 = take_event(c), e1 = take_event(c);
-    const int ppws[3] = {16, 32, 64}, leaves[4] = {1, 2, 4, 8};
+    const int ppws[4] = {8, 16, 32, 64}, leaves[4] = {1, 2, 4, 8};
     double best = 1e300;
     int n = 0;
     for (int pw : ppws)

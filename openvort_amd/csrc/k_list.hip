@@ -137,7 +137,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     constexpr int S = 64 / PPW;
     __shared__ int2 stk[kListStack];
     __shared__ float4 farA[kListFar];  // offset from the copy's group centre, qx
-    __shared__ float2 farB[kListFar];  // qy, qz
+    __shared__ float4 farB[kListFar];  // qy, qz, |q|^2
     __shared__ unsigned nearI[kListNear];
     __shared__ int bEnd[64], bStart[64], bCopy[64];  // bucket cells of one round
     __shared__ double shL[27][3];                    // copy shifts
@@ -177,7 +177,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const int leaf_bits = 32 - __clz(leaf);  // bits of counts 0..leaf
     const double cs = P.crit_scale;
     const double f64 = P.f_bs;
-    const float f32 = (float)P.f_bs;
 
     // one walk for all 27 copies: a stack range carries its copy index, and
     // a copy's sources are shifted by +L*shift (its targets are r - L*shift)
@@ -188,22 +187,22 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
 
     auto flush_far = [&]() {
         __syncthreads();
+        float sxa = 0.f, sya = 0.f, sza = 0.f;
         for (int i0 = slice; i0 < far_n; i0 += 4 * S) {
-            float sxa = 0.f, sya = 0.f, sza = 0.f;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int i = i0 + u * S;
                 if (i < far_n) {
                     const float4 a = farA[i];
-                    const float2 b = farB[i];
-                    const float Rx = a.x - tx, Ry = a.y - ty, Rz = a.z - tz;
-                    float vx, vy, vz;
-                    seg_field32(Rx, Ry, Rz, Rx + a.w, Ry + b.x, Rz + b.y, f32, vx, vy, vz);
-                    sxa += vx; sya += vy; sza += vz;
+                    const float4 b = farB[i];
+                    seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, sxa, sya,
+                                    sza);
                 }
             }
-            ax += sxa; ay += sya; az += sza;
         }
+        ax += (double)sxa * f64;
+        ay += (double)sya * f64;
+        az += (double)sza * f64;
         if (on && far_n > slice) c32 += (far_n - slice + S - 1) / S;
         __syncthreads();
         far_n = 0;
@@ -249,7 +248,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (want && !near) {
             const int pos = far_n + prefix_below(mf);
             farA[pos] = make_float4(ex, ey, ez, gx);
-            farB[pos] = make_float2(gy, gz);
+            farB[pos] = make_float4(gy, gz, fmaf(gz, gz, fmaf(gy, gy, gx * gx)), 0.0f);
         }
         if (want && near) nearI[near_n + prefix_below(mn)] = idx;
         far_n += cf;
@@ -487,6 +486,7 @@ int launch_bh_list(Ctx &c, const double *const x[3])
     TreeBufs &T = c.tree;
     const int ppw = list_points_per_wave(c.params);
     switch (ppw) {
+    case 8: OVG_TRY(build_groups<8>(c)); break;
     case 16: OVG_TRY(build_groups<16>(c)); break;
     case 32: OVG_TRY(build_groups<32>(c)); break;
     default: OVG_TRY(build_groups<64>(c)); break;
@@ -510,6 +510,7 @@ int launch_bh_list(Ctx &c, const double *const x[3])
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
     switch (ppw) {
+    case 8: mixed ? go(k_bh_list<true, 8>) : go(k_bh_list<false, 8>); break;
     case 16: mixed ? go(k_bh_list<true, 16>) : go(k_bh_list<false, 16>); break;
     case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
     default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;
@@ -593,6 +594,7 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
     switch (ppw) {
+    case 8: mixed ? go(k_bh_list<true, 8>) : go(k_bh_list<false, 8>); break;
     case 16: mixed ? go(k_bh_list<true, 16>) : go(k_bh_list<false, 16>); break;
     case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
     default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;

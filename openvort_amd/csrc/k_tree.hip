@@ -77,6 +77,34 @@ __device__ __forceinline__ void child_box(const double lo[3], const double hi[3]
     }
 }
 
+// child_box without a branch on the child index: per axis each case takes
+// one of four forms, (lo, c), (lo+h, c+h), (c, hi) or (c-h, hi-h); the form
+// codes of the eight cases (2 bits per axis, 6 per case) sit in kChildForms.
+// The values are the same expressions as in child_box.
+constexpr uint64_t kChildForms =
+    (1ull << 0) | (0ull << 2) | (0ull << 4) |                 // 0: B A A
+    ((0ull | 0ull << 2 | 0ull << 4) << 6) |                   // 1: A A A
+    ((2ull | 2ull << 2 | 3ull << 4) << 12) |                  // 2: C C D
+    ((0ull | 1ull << 2 | 0ull << 4) << 18) |                  // 3: A B A
+    ((2ull | 3ull << 2 | 2ull << 4) << 24) |                  // 4: C D C
+    ((0ull | 0ull << 2 | 1ull << 4) << 30) |                  // 5: A A B
+    ((2ull | 2ull << 2 | 2ull << 4) << 36) |                  // 6: C C C
+    ((3ull | 2ull << 2 | 2ull << 4) << 42);                   // 7: D C C
+
+__device__ __forceinline__ void child_box_sel(double lo[3], double hi[3], int idx)
+{
+    for (int k = 0; k < 3; ++k) {
+        double c = lo[k] + hi[k];
+        c = c * 0.5;
+        const double h = (hi[k] - lo[k]) / 2;
+        const int f = (int)((kChildForms >> (6 * idx + 2 * k)) & 3);
+        const double nl = f == 0 ? lo[k] : f == 1 ? lo[k] + h : f == 2 ? c : c - h;
+        const double nh = f == 0 ? c : f == 1 ? c + h : f == 2 ? hi[k] : hi[k] - h;
+        lo[k] = nl;
+        hi[k] = nh;
+    }
+}
+
 __device__ __forceinline__ bool in_closed(const double lo[3], const double hi[3], double x,
                                           double y, double z)
 {
@@ -150,16 +178,17 @@ __global__ void k_keys(int n_live, const int *__restrict__ live, const double *_
                     ++fallbacks;
                 }
             }
-            double nlo[3], nhi[3];
-            child_box(lo, hi, digit, nlo, nhi);
-            for (int q = 0; q < 3; ++q) { lo[q] = nlo[q]; hi[q] = nhi[q]; }
+            child_box_sel(lo, hi, digit);
             key |= (uint64_t)digit << (63 - 3 * lev);
         }
-        atomicAdd(&counters[0], 1);
         if (fallbacks) atomicAdd(&counters[2], fallbacks);
     }
     keys[i] = key;
     vals[i] = k;
+    // in-box count: one atomic per wavefront (the lanes past n_live have left)
+    const unsigned long long inbox = __ballot(!(key >> 63));
+    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && inbox)
+        atomicAdd(&counters[0], __popcll(inbox));
 }
 
 // ---------------------------------------------------------------- scan ----

@@ -148,7 +148,7 @@ def test_c2_mixed_precision_error(c2):
     assert trunc < trunc_ref
 
 
-@pytest.mark.parametrize("ppw", [16, 32, 64])
+@pytest.mark.parametrize("ppw", [8, 16, 32, 64])
 def test_c2_list_traversal(c2, ppw):
     g, t, p = c2
     say(f"c2 BH list traversal, {ppw} points per wave, leaf 16: FP64 and mixed")
@@ -591,7 +591,7 @@ def test_autotune_picks_fastest_and_keeps_results_valid(c2):
     say("autotune of the list traversal on c2")
     with engine(t, q) as e:
         rows, best = e.autotune(repeats=1)
-        assert len(rows) == 36 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
+        assert len(rows) == 48 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
         assert rows[best]["ms"] == min(r["ms"] for r in rows)
         e.update_tangle(0.0)
         o = e.download()
