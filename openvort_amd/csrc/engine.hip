@@ -90,11 +90,17 @@ __global__ void k_fill_zero(int n, double *__restrict__ a)
 
 static inline int grid_for(int n, int bs = 256) { return (n + bs - 1) / bs; }
 
-void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos)
+void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos,
+                       hipStream_t stream)
 {
     if (c.n == 0) return;
-    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c.n)), dim3(256), 0, c.stream, c.n, x, y, z,
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c.n)), dim3(256), 0, stream, c.n, x, y, z,
                        aos);
+}
+
+void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos)
+{
+    launch_soa_to_aos(c, x, y, z, aos, c.stream);
 }
 
 void launch_split_pairs(Ctx &c, const int *pairs)
@@ -299,6 +305,13 @@ extern "C" int ovg_create(ovg_ctx **out, int device)
         return set_error(OVG_EDEVICE, "hipStreamCreate failed");
     }
     for (auto &e : c.ev) hipEventCreate(&e);
+    // position downloads overlap the context stream's work (default priority)
+    if (hipStreamCreateWithFlags(&c.copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c.xpin_from, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.xpin_done, hipEventDisableTiming) != hipSuccess) {
+        ovg_destroy(ctx);
+        return set_error(OVG_EDEVICE, "copy stream / events could not be created");
+    }
     int r;
     if ((r = c.tparams.ensure(sizeof(TangleParams))) || (r = c.flags.ensure(64)) ||
         (r = c.counter64.ensure(64))) {
@@ -317,11 +330,12 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
+    if (c.copy_stream) hipStreamSynchronize(c.copy_stream);
     DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
                    &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf, &c.split_part,
                    &c.loop_off, &c.loop_slots, &c.lt_buf, &c.sm_fft, &c.sm_big, &c.sm_tw,
-                   &c.sm_gstate, &c.topo_stage};
+                   &c.sm_gstate, &c.topo_stage, &c.xaos};
     for (auto *b : all) b->release();
     for (int q = 0; q < 3; ++q) {
         c.x[q].release(); c.vels[q].release(); c.vs[q].release(); c.tan[q].release();
@@ -343,6 +357,9 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
         if (e) hipEventDestroy(e);
     for (auto e : c.ev_pending) hipEventDestroy(e);
     for (auto e : c.ev_pool) hipEventDestroy(e);
+    if (c.xpin_from) hipEventDestroy(c.xpin_from);
+    if (c.xpin_done) hipEventDestroy(c.xpin_done);
+    if (c.copy_stream) hipStreamDestroy(c.copy_stream);
     if (c.stream) hipStreamDestroy(c.stream);
     delete ctx;
     return OVG_OK;
@@ -448,6 +465,7 @@ static int alloc_state(Ctx &c, int n)
 static int upload_positions(Ctx &c, const double *vnodes)
 {
     int n = c.n;
+    OVG_TRY(positions_change(c));
     DBuf &stage = c.partial;
     OVG_TRY(stage.ensure(sizeof(double) * 3 * (size_t)n));
     OVG_HIP(hipMemcpyAsync(stage.p, vnodes, sizeof(double) * 3 * (size_t)n,
@@ -677,6 +695,7 @@ extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
     hipSetDevice(c.device);
     double h4 = c.params.rk4_classic ? dt : dt / 2;
     (void)h4;
+    OVG_TRY(positions_change(c));  // the step moves every point
     // stage positions: xs[0] = x + dt/2 k1 (euler_step2, src/vortex_dynamics.c:76)
     Triple k1 = tri(c.vels), xs0 = tri(c.xs[0]), xs1 = tri(c.xs[1]);
     launch_stage_positions(c, k1.v, dt / 2, xs0.v);

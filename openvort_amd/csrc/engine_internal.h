@@ -105,6 +105,7 @@ struct Ctx {
     TreeBufs tree;
     // reconnection / smoothing scratch
     DBuf rc_cell, rc_sorted, rc_start, rc_pairs, rc_count, rc_tmp;
+    int rc_cap = 0;                 // int2 pairs that rc_pairs holds
     DBuf sm_buf;
 
     std::vector<int> h_conn, h_status;  // host mirrors of topology
@@ -122,6 +123,12 @@ struct Ctx {
     int small_cutoff_seen = 0;          // largest cutoff of a completed pass
     double *h_xpin = nullptr;           // pinned AoS position mirror
     size_t h_xpin_cap = 0;              // doubles
+    int xpin_state = 0;                 // mirror: 0 stale, 1 current, 2 current unless
+                                        // the wraps since moved points (flags[1])
+    bool xpin_pending = false;          // a copy into the mirror is in flight
+    hipStream_t copy_stream = nullptr;  // position downloads beside the main stream
+    hipEvent_t xpin_from = nullptr, xpin_done = nullptr;
+    DBuf xaos;                          // AoS staging of those downloads
     std::vector<int> rc_where, rc_flag; // reconnection scratch, kept -1 / 0
     DBuf topo_stage;                    // device staging of commits
     int next_empty = 0;                 // the reference allocator's hint (tangle->next_empty)
@@ -179,6 +186,12 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
 // Pinned host copy of the positions (AoS, c.n slots), valid until the next
 // device-side change of positions.
 int host_positions(Ctx &c, const double **X);
+// Start refreshing the pinned mirror on the copy stream behind the work
+// queued so far (no-op when it is current); host_positions waits for it.
+int host_positions_begin(Ctx &c);
+// Call before queueing any device write of positions: waits for a copy in
+// flight and marks the mirror stale.
+int positions_change(Ctx &c);
 // Record a slot whose links or status changed (also for the small-loop pass).
 void mark_dirty(Ctx &c, int slot);
 // Scatter the dirty slots' links/status and the new positions to the device,
@@ -201,6 +214,8 @@ int eval_bs(Ctx &c, const double *const x[3]);
 TangleParams host_tparams(const ovg_params &p, double t);
 void push_params(Ctx &c, double t);
 void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos);
+void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos,
+                       hipStream_t stream);
 void launch_split_pairs(Ctx &c, const int *pairs);
 
 // k_recon.hip: candidate scan + greedy pass; links updated on the device

@@ -343,6 +343,7 @@ __global__ void k_wrap(int n_live, const int *__restrict__ live, double *__restr
     const TangleParams &P = *Pp;
     int k = live[i];
     double p[3] = {X[k], Y[k], Z[k]};
+    int moved = 0;
     for (int it = 0; it < 256; ++it) {
         int face = -1;
         if (p[0] < P.lo[0]) face = 0;
@@ -356,15 +357,34 @@ __global__ void k_wrap(int n_live, const int *__restrict__ live, double *__restr
         int cc = face >> 1;
         if ((face & 1) == 0) p[cc] += P.L[cc];
         else p[cc] -= P.L[cc];
+        moved = 1;
         if (it == 255) atomicOr(flags, 1);
     }
     X[k] = p[0]; Y[k] = p[1]; Z[k] = p[2];
+    // flags[1]: points moved by this wrap (one atomic per wavefront)
+    const unsigned long long mv = __ballot(moved);
+    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && mv)
+        atomicAdd(flags + 1, __popcll(mv));
+}
+
+__global__ void k_zero_int(int *a)
+{
+    if (threadIdx.x == 0) *a = 0;
 }
 
 void launch_wrap(Ctx &c)
 {
     if (c.n_live == 0) return;
     int bs = 256, gs = (c.n_live + bs - 1) / bs;
+    // a current host position mirror stays current unless this wrap moves
+    // points (host_positions reads the count); a copy in flight finishes first
+    if (c.xpin_pending) {
+        hipEventSynchronize(c.xpin_done);
+        c.xpin_pending = false;
+        c.xpin_state = 1;
+    }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c.stream, c.flags.as<int>() + 1);
+    if (c.xpin_state != 0) c.xpin_state = 2;
     hipLaunchKernelGGL(k_wrap, dim3(gs), dim3(bs), 0, c.stream, c.n_live, c.live.as<int>(),
                        c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>(),
                        dparams(c), c.flags.as<int>());

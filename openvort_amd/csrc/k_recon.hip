@@ -86,21 +86,25 @@ __global__ void k_cell_ranges(int n_live, const uint64_t *__restrict__ keys,
     if (i == n_live - 1 || (int)keys[i + 1] != c) end[c] = i + 1;
 }
 
-// COUNT: counts[i] = candidates of point live-index i; else write them at
-// offs[i] sorted by l.
-template <bool COUNT>
-__global__ void k_candidates(int n_live, const int *__restrict__ live, const int *__restrict__ sorted,
-                             const uint64_t *__restrict__ keys_unused, const double *__restrict__ X,
+// Every pair (k, l > k) of the 27 neighbouring cells that passes the
+// reference predicate is appended at an atomic counter (order fixed later
+// by a sort); beyond cap only the count grows.  The distance test
+// dist = sqrt(e.e) > rec_dist is decided on e.e against rec_dist^2 with a
+// relative margin of 2^-40 (far wider than the roundings of e.e, rec_dist^2
+// and the square root) and evaluated exactly only inside that margin.
+__global__ void k_candidates(int n_live, const int *__restrict__ live,
+                             const int *__restrict__ sorted, const double *__restrict__ X,
                              const double *__restrict__ Y, const double *__restrict__ Z,
                              const int *__restrict__ fwd, const int *__restrict__ rev,
                              const TangleParams *__restrict__ Pp, Grid g, double rec_dist,
                              const int *__restrict__ start, const int *__restrict__ end,
-                             int *__restrict__ counts, const int *__restrict__ offs,
-                             int *__restrict__ out_k, int *__restrict__ out_l, int max_pairs)
+                             int *__restrict__ count, int2 *__restrict__ out, int cap)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
     const TangleParams &P = *Pp;
+    const double r2 = rec_dist * rec_dist;
+    const double far2 = r2 * (1.0 + 0x1.0p-40), near2 = r2 * (1.0 - 0x1.0p-40);
     int k = live[i];
     double xk = X[k], yk = Y[k], zk = Z[k];
     int fk = fwd[k], rk = rev[k];
@@ -110,8 +114,6 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live, const int
     int rx0 = g.nc[0] >= 3 ? -1 : 0, rx1 = g.nc[0] >= 3 ? 1 : 0;
     int ry0 = g.nc[1] >= 3 ? -1 : 0, ry1 = g.nc[1] >= 3 ? 1 : 0;
     int rz0 = g.nc[2] >= 3 ? -1 : 0, rz1 = g.nc[2] >= 3 ? 1 : 0;
-    int cnt = 0;
-    int base = COUNT ? 0 : offs[i];
     for (int dz = rz0; dz <= rz1; ++dz)
         for (int dy = ry0; dy <= ry1; ++dy)
             for (int dx = rx0; dx <= rx1; ++dx) {
@@ -126,34 +128,21 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live, const int
                     double wx = X[l], wy = Y[l], wz = Z[l];
                     pwrap(P, xk, yk, zk, wx, wy, wz);
                     double ex = xk - wx, ey = yk - wy, ez = zk - wz;
-                    double dist = sqrt(dot3(ex, ey, ez, ex, ey, ez));
-                    if (dist > rec_dist) continue;
-                    if (!COUNT) {
-                        // insertion into this point's slice, sorted by l
-                        int pos = base + cnt;
-                        while (pos > base && out_l[pos - 1] > l) {
-                            if (pos < max_pairs) {
-                                out_l[pos] = out_l[pos - 1];
-                                out_k[pos] = k;
-                            }
-                            --pos;
-                        }
-                        if (pos < max_pairs) {
-                            out_l[pos] = l;
-                            out_k[pos] = k;
-                        }
-                    }
-                    ++cnt;
+                    const double d2 = dot3(ex, ey, ez, ex, ey, ez);
+                    if (d2 > far2) continue;
+                    if (d2 >= near2 && sqrt(d2) > rec_dist) continue;
+                    const int pos = atomicAdd(count, 1);
+                    if (pos < cap) out[pos] = make_int2(k, l);
                 }
             }
-    if (COUNT) counts[i] = cnt;
 }
 
-// Candidate pairs into c.rc_pairs (k array then l array); returns count.
-static int candidates_device(Ctx &c, int *n_out)
+// Candidate pairs in the reference's scan order (k ascending, then l).
+static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
 {
     int n = c.n_live;
-    *n_out = 0;
+    ck.clear();
+    cl.clear();
     if (n == 0) return OVG_OK;
     Grid g = make_grid(c.params);
     int ncell = g.nc[0] * g.nc[1] * g.nc[2];
@@ -163,8 +152,9 @@ static int candidates_device(Ctx &c, int *n_out)
         OVG_TRY(T.vals[b].ensure(sizeof(int) * (n + 1)));
     }
     OVG_TRY(c.rc_start.ensure(sizeof(int) * 2 * (size_t)ncell));
-    OVG_TRY(c.rc_count.ensure(sizeof(int) * (n + 1)));
-    OVG_TRY(c.rc_cell.ensure(sizeof(int) * (n + 2)));
+    OVG_TRY(c.rc_count.ensure(sizeof(int) * 64));
+    if (c.rc_cap == 0) c.rc_cap = 1 << 16;
+    OVG_TRY(c.rc_pairs.ensure(sizeof(int2) * (size_t)c.rc_cap));
     int bs = 256, gs = (n + bs - 1) / bs;
     const double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
     hipLaunchKernelGGL(k_cell_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), X, Y,
@@ -176,46 +166,52 @@ static int candidates_device(Ctx &c, int *n_out)
                        2 * ncell, start, 0);
     hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), start, end);
-    hipLaunchKernelGGL(k_candidates<true>, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
-                       T.vals[0].as<int>(), T.keys[0].as<uint64_t>(), X, Y, Z, c.fwd.as<int>(),
-                       c.rev.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
-                       c.rc_count.as<int>(), nullptr, nullptr, nullptr, 0);
-    int *offs = c.rc_cell.as<int>();
-    launch_exclusive_scan(c, c.rc_count.as<int>(), offs, n, offs + n, c.rc_tmp);
     int total = 0;
-    OVG_HIP(hipMemcpyAsync(&total, offs + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    OVG_HIP(hipStreamSynchronize(c.stream));
-    OVG_TRY(c.rc_pairs.ensure(sizeof(int) * 2 * (size_t)(total + 1)));
-    int *pk = c.rc_pairs.as<int>(), *pl = pk + total;
-    hipLaunchKernelGGL(k_candidates<false>, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
-                       T.vals[0].as<int>(), T.keys[0].as<uint64_t>(), X, Y, Z, c.fwd.as<int>(),
-                       c.rev.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
-                       c.rc_count.as<int>(), offs, pk, pl, total);
-    *n_out = total;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, 1, c.rc_count.as<int>(),
+                           0);
+        hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
+                           T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
+                           dparams(c), g, c.params.rec_dist, start, end, c.rc_count.as<int>(),
+                           c.rc_pairs.as<int2>(), c.rc_cap);
+        OVG_HIP(hipMemcpyAsync(&total, c.rc_count.p, sizeof(int), hipMemcpyDeviceToHost,
+                               c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        if (total <= c.rc_cap) break;
+        // more candidates than room: grow and search again
+        c.rc_cap = total + total / 4;
+        OVG_TRY(c.rc_pairs.ensure(sizeof(int2) * (size_t)c.rc_cap));
+    }
+    std::vector<int2> pr((size_t)total);
+    if (total) {
+        OVG_HIP(hipMemcpyAsync(pr.data(), c.rc_pairs.p, sizeof(int2) * (size_t)total,
+                               hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    std::sort(pr.begin(), pr.end(), [](const int2 &a, const int2 &b) {
+        return a.x != b.x ? a.x < b.x : a.y < b.y;
+    });
+    ck.resize((size_t)total);
+    cl.resize((size_t)total);
+    for (int i = 0; i < total; ++i) {
+        ck[i] = pr[i].x;
+        cl[i] = pr[i].y;
+    }
     return OVG_OK;
 }
 
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs)
 {
-    int total = 0;
-    OVG_TRY(candidates_device(c, &total));
+    std::vector<int> ck, cl;
+    OVG_TRY(candidates_device(c, ck, cl));
+    const int total = (int)ck.size();
     if (n_pairs) *n_pairs = total;
     int m = total < max_pairs ? total : max_pairs;
-    if (m > 0 && h_pairs) {
-        std::vector<int> kk(total), ll(total);
-        const int *pk = c.rc_pairs.as<int>();
-        OVG_HIP(hipMemcpyAsync(kk.data(), pk, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost,
-                               c.stream));
-        OVG_HIP(hipMemcpyAsync(ll.data(), pk + total, sizeof(int) * (size_t)total,
-                               hipMemcpyDeviceToHost, c.stream));
-        OVG_HIP(hipStreamSynchronize(c.stream));
+    if (h_pairs)
         for (int i = 0; i < m; ++i) {
-            h_pairs[2 * i] = kk[i];
-            h_pairs[2 * i + 1] = ll[i];
+            h_pairs[2 * i] = ck[i];
+            h_pairs[2 * i + 1] = cl[i];
         }
-    } else {
-        OVG_HIP(hipStreamSynchronize(c.stream));
-    }
     if (total > max_pairs)
         return set_error(OVG_ELIMIT, "%d candidate pairs exceed max_pairs %d", total, max_pairs);
     return OVG_OK;
@@ -272,18 +268,14 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
 {
     *n_rec = 0;
     int total = 0;
-    OVG_TRY(candidates_device(c, &total));
+    // the host pass reads positions: their download overlaps the candidate
+    // search and the endpoint velocities
+    OVG_TRY(host_positions_begin(c));
+    std::vector<int> ck, cl;
+    OVG_TRY(candidates_device(c, ck, cl));
+    total = (int)ck.size();
     if (total == 0) return OVG_OK;
-    std::vector<int> ck(total), cl(total);
-    const int *pk = c.rc_pairs.as<int>();
-    OVG_HIP(hipMemcpyAsync(ck.data(), pk, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost,
-                           c.stream));
-    OVG_HIP(hipMemcpyAsync(cl.data(), pk + total, sizeof(int) * (size_t)total,
-                           hipMemcpyDeviceToHost, c.stream));
-    // positions (pinned host copy, AoS)
     int n = c.n;
-    const double *X = nullptr;
-    OVG_TRY(host_positions(c, &X));
     TangleParams P = host_tparams(c.params, t);
     // direct Biot-Savart at every candidate endpoint with the original links
     std::vector<int> ends;
@@ -336,6 +328,9 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
                                hipMemcpyDeviceToHost, c.stream));
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
+    // positions (pinned host copy, AoS)
+    const double *X = nullptr;
+    OVG_TRY(host_positions(c, &X));
     // greedy pass in (k, l) order
     std::vector<int> &conn = c.h_conn;
     std::vector<int> &flag = c.rc_flag;  // all 0 between passes
@@ -429,19 +424,24 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
         int iend = i;
         while (iend < total && ck[iend] == k) ++iend;
         if (flag[k]) { i = iend; continue; }
+        // k's links, tangent and velocity stay fixed until a rewiring, which
+        // ends this loop
+        double tk[3], nk[3], vk[3];
+        host::tangent_normal(P, X, conn.data(), k, tk, nk);
+        bool have_vk = false;
         for (int j = i; j < iend; ++j) {
             int l = cl[j];
             if (flag[l]) continue;
             double w[3] = {X[3 * (size_t)l], X[3 * (size_t)l + 1], X[3 * (size_t)l + 2]};
             host::pwrap(P, X + 3 * (size_t)k, w);
-            double tk[3], nk[3], tl[3], nl[3];
-            host::tangent_normal(P, X, conn.data(), k, tk, nk);
+            double tl[3], nl[3];
             host::tangent_normal(P, X, conn.data(), l, tl, nl);
             if (host::len3(tk) < 0.5 || host::len3(tl) < 0.5) continue;
             double calpha = host::dot3(tk, tl) / host::len3(tk) / host::len3(tl);
             if (calpha > cutoff) continue;
-            double vk[3], vl[3];
-            velocity(k, tk, nk, vk);
+            double vl[3];
+            if (!have_vk) velocity(k, tk, nk, vk);
+            have_vk = true;
             velocity(l, tl, nl, vl);
             double dx[3], dv[3];
             for (int a = 0; a < 3; ++a) dx[a] = X[3 * (size_t)k + a] - w[a];

@@ -990,6 +990,7 @@ int smooth_loops(Ctx &c)
 {
     OVG_TRY(build_loops(c));
     if (c.n_loops == 0) return OVG_OK;
+    OVG_TRY(positions_change(c));
     size_t need = sizeof(double) * ((size_t)15 * c.n_live + (size_t)6 * c.n_loops + 64);
     OVG_TRY(c.sm_buf.ensure(need));
     std::vector<int> big;

@@ -18,20 +18,72 @@
 
 namespace ovg {
 
-int host_positions(Ctx &c, const double **X)
+// Pinned mirror large enough for the current slots (its contents are lost
+// when it grows).
+static int mirror_capacity(Ctx &c)
 {
     const size_t need = 3 * (size_t)(c.n > 0 ? c.n : 1);
-    if (need > c.h_xpin_cap) {
-        if (c.h_xpin) hipHostFree(c.h_xpin);
+    if (need <= c.h_xpin_cap) return OVG_OK;
+    if (c.h_xpin) hipHostFree(c.h_xpin);
+    c.h_xpin = nullptr;
+    c.h_xpin_cap = 0;
+    c.xpin_state = 0;
+    size_t cap = need + need / 4;
+    if (hipHostMalloc((void **)&c.h_xpin, sizeof(double) * cap, hipHostMallocDefault) !=
+        hipSuccess) {
         c.h_xpin = nullptr;
-        c.h_xpin_cap = 0;
-        size_t cap = need + need / 4;
-        if (hipHostMalloc((void **)&c.h_xpin, sizeof(double) * cap, hipHostMallocDefault) !=
-            hipSuccess) {
-            c.h_xpin = nullptr;
-            return set_error(OVG_ENOMEM, "pinned host buffer of %zu doubles", cap);
-        }
-        c.h_xpin_cap = cap;
+        return set_error(OVG_ENOMEM, "pinned host buffer of %zu doubles", cap);
+    }
+    c.h_xpin_cap = cap;
+    return OVG_OK;
+}
+
+static int settle(Ctx &c)
+{
+    if (!c.xpin_pending) return OVG_OK;
+    c.xpin_pending = false;
+    OVG_HIP(hipEventSynchronize(c.xpin_done));
+    c.xpin_state = 1;
+    return OVG_OK;
+}
+
+int host_positions_begin(Ctx &c)
+{
+    if (c.xpin_pending || c.xpin_state == 1 || c.n <= 0) return OVG_OK;
+    OVG_TRY(mirror_capacity(c));
+    const size_t bytes = sizeof(double) * 3 * (size_t)c.n;
+    OVG_TRY(c.xaos.ensure(bytes));
+    OVG_HIP(hipEventRecord(c.xpin_from, c.stream));
+    OVG_HIP(hipStreamWaitEvent(c.copy_stream, c.xpin_from, 0));
+    launch_soa_to_aos(c, c.x[0].as<double>(), c.x[1].as<double>(), c.x[2].as<double>(),
+                      c.xaos.as<double>(), c.copy_stream);
+    OVG_HIP(hipMemcpyAsync(c.h_xpin, c.xaos.p, bytes, hipMemcpyDeviceToHost, c.copy_stream));
+    OVG_HIP(hipEventRecord(c.xpin_done, c.copy_stream));
+    c.xpin_pending = true;
+    return OVG_OK;
+}
+
+int positions_change(Ctx &c)
+{
+    OVG_TRY(settle(c));
+    c.xpin_state = 0;
+    return OVG_OK;
+}
+
+int host_positions(Ctx &c, const double **X)
+{
+    OVG_TRY(settle(c));
+    OVG_TRY(mirror_capacity(c));
+    if (c.xpin_state == 2) {
+        int moved = 0;
+        OVG_HIP(hipMemcpyAsync(&moved, c.flags.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost,
+                               c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        c.xpin_state = moved ? 0 : 1;
+    }
+    if (c.xpin_state == 1) {
+        *X = c.h_xpin;
+        return OVG_OK;
     }
     if (c.n > 0) {
         OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)c.n));
@@ -41,6 +93,7 @@ int host_positions(Ctx &c, const double **X)
                                hipMemcpyDeviceToHost, c.stream));
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
+    c.xpin_state = 1;
     *X = c.h_xpin;
     return OVG_OK;
 }
