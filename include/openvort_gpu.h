@@ -135,7 +135,7 @@ typedef struct ovg_params {
     int32_t group_level;          /* OVG_TRAVERSE_LIST: groups never span two
                                      octree cells of this level (1..21); 0 =
                                      the coarsest level whose cells are at most
-                                     4 * mixed_near wide (5 if mixed_near = 0)  */
+                                     8 * mixed_near wide (5 if mixed_near = 0)  */
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a

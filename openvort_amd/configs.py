@@ -38,7 +38,7 @@ def params(D, dxi, alpha_pair, v_ns, method=ov.METHOD_BH, precision=ov.PREC_FP64
     p.rec_dist = 0.8 * (dxi / 2)
     p.rec_angle = 5 * math.pi / 180.0
     p.freq_to_cutoff = 4.0 / 5
-    p.mixed_near = 4 * dxi
+    p.mixed_near = 2 * dxi
     return p
 
 

@@ -139,22 +139,24 @@ __device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float 
     vz = cz * m;
 }
 
-// seg_field32 in terms of R and the segment vector Q = Rp - R, for far-field
-// entries whose |Q|^2 = q2 is stored with them: R x Rp = R x Q,
-// R.Rp = r2 + R.Q and |Rp|^2 = r2 + 2 R.Q + q2 (far entries lie several
-// segment lengths away, so the last form loses nothing to cancellation).
-// The three guards are folded into one comparison of the smallest of r2,
-// |Rp|^2 and 1e-8 times the normalised dot against 1e-16.  Adds the field
-// without the factor f_bs to (ax, ay, az); the caller applies f_bs to the
-// sum.
+// seg_field32 for far-field entries given R and the segment vector
+// Q = Rp - R: Rp = R + Q is formed explicitly (a monopole's Q, the sum of
+// its cell's segments, can reach past the target, where |Rp|^2 expanded as
+// r2 + 2 R.Q + |Q|^2 would cancel).  The three guards are folded into one
+// comparison of the smallest of r2, |Rp|^2 and 1e-8 times the normalised
+// dot against 1e-16.  R x Rp = R x Q, and with R = a - t (a the entry's and
+// t the target's offset from a common centre) R x Q = a x Q - t x Q: the
+// entry carries A = a x Q, the pair adds m A to sa and m Q to sq, and the
+// caller forms sa - t x sq and applies f_bs once per batch of entries.
 __device__ __forceinline__ void seg_field32_far(float Rx, float Ry, float Rz, float Qx, float Qy,
-                                                float Qz, float q2, float &ax, float &ay,
-                                                float &az)
+                                                float Qz, float Ax, float Ay, float Az,
+                                                float &sax, float &say, float &saz, float &sqx,
+                                                float &sqy, float &sqz)
 {
+    const float Px = Rx + Qx, Py = Ry + Qy, Pz = Rz + Qz;
     const float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
-    const float rq = fmaf(Rz, Qz, fmaf(Ry, Qy, Rx * Qx));
-    const float dot = r2 + rq;
-    const float p2 = fmaf(2.0f, rq, r2 + q2);
+    const float p2 = fmaf(Pz, Pz, fmaf(Py, Py, Px * Px));
+    const float dot = fmaf(Rz, Pz, fmaf(Ry, Py, Rx * Px));
     const float ia = __builtin_amdgcn_rsqf(r2);
     const float ib = __builtin_amdgcn_rsqf(p2);
     const float iab = ia * ib;
@@ -162,9 +164,12 @@ __device__ __forceinline__ void seg_field32_far(float Rx, float Ry, float Rz, fl
     const bool ok = fminf(fminf(r2, p2), nd * 1e-8f) >= 1e-16f;
     float m = (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
     m = ok ? m : 0.0f;
-    ax = fmaf(fmaf(Ry, Qz, -(Rz * Qy)), m, ax);
-    ay = fmaf(fmaf(Rz, Qx, -(Rx * Qz)), m, ay);
-    az = fmaf(fmaf(Rx, Qy, -(Ry * Qx)), m, az);
+    sax = fmaf(Ax, m, sax);
+    say = fmaf(Ay, m, say);
+    saz = fmaf(Az, m, saz);
+    sqx = fmaf(Qx, m, sqx);
+    sqy = fmaf(Qy, m, sqy);
+    sqz = fmaf(Qz, m, sqz);
 }
 
 }  // namespace ovg

@@ -19,6 +19,7 @@ extern int use_mutual_friction;
 extern double alpha, alpha_p;
 extern double reconnection_angle_cutoff, rec_dist;
 extern double freq_to_cutoff;
+extern double global_dl_min, global_dl_max; /* include/vortex_constants.h:33-34 */
 extern int use_BH;
 extern double BH_resolution;
 
@@ -113,6 +114,8 @@ static ovg_ctx *prepare(const struct tangle_state *t)
     p.method = use_BH ? OVG_METHOD_BH : OVG_METHOD_DIRECT;
     p.theta = BH_resolution;
     p.precision = g_precision;
+    /* mixed precision: pairs within two of the largest point spacings stay FP64 */
+    p.mixed_near = 2 * global_dl_max;
     p.traverse = g_traverse;
     p.leaf_size = g_leaf;
     p.rec_dist = rec_dist;

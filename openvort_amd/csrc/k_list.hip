@@ -135,9 +135,11 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     unsigned long long *__restrict__ icount)
 {
     constexpr int S = 64 / PPW;
+    static_assert(kListFar % (4 * S) == 0, "far flushes run in whole rounds of 4 S entries");
     __shared__ int2 stk[kListStack];
-    __shared__ float4 farA[kListFar];  // offset from the copy's group centre, qx
-    __shared__ float4 farB[kListFar];  // qy, qz, |q|^2
+    __shared__ float4 farA[kListFar];  // offset a from the copy's group centre, qx
+    __shared__ float4 farB[kListFar];  // qy, qz, (a x q)_x, (a x q)_y
+    __shared__ float farC[kListFar];   // (a x q)_z
     __shared__ unsigned nearI[kListNear];
     __shared__ int bEnd[64], bStart[64], bCopy[64];  // bucket cells of one round
     __shared__ double shL[27][3];                    // copy shifts
@@ -186,23 +188,31 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const int n_copies = any ? 1 + P.n_images : 0;
 
     auto flush_far = [&]() {
+        // pad to whole rounds of 4 S entries with zero entries, which add
+        // exactly zero (a zero offset from the centre is rejected by the
+        // guard, any other gives a finite m times zero vectors)
+        const int fpad = (far_n + 4 * S - 1) / (4 * S) * (4 * S);
+        for (int i = far_n + lane; i < fpad; i += 64) {
+            farA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            farB[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            farC[i] = 0.f;
+        }
         __syncthreads();
-        float sxa = 0.f, sya = 0.f, sza = 0.f;
-        for (int i0 = slice; i0 < far_n; i0 += 4 * S) {
+        float sax = 0.f, say = 0.f, saz = 0.f, sqx = 0.f, sqy = 0.f, sqz = 0.f;
+        for (int i0 = slice; i0 < fpad; i0 += 4 * S) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int i = i0 + u * S;
-                if (i < far_n) {
-                    const float4 a = farA[i];
-                    const float4 b = farB[i];
-                    seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, sxa, sya,
-                                    sza);
-                }
+                const float4 a = farA[i];
+                const float4 b = farB[i];
+                seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i],
+                                sax, say, saz, sqx, sqy, sqz);
             }
         }
-        ax += (double)sxa * f64;
-        ay += (double)sya * f64;
-        az += (double)sza * f64;
+        // sum of m (a - t) x q = sum m a x q - t x sum m q
+        ax += (double)(sax - fmaf(ty, sqz, -(tz * sqy))) * f64;
+        ay += (double)(say - fmaf(tz, sqx, -(tx * sqz))) * f64;
+        az += (double)(saz - fmaf(tx, sqy, -(ty * sqx))) * f64;
         if (on && far_n > slice) c32 += (far_n - slice + S - 1) / S;
         __syncthreads();
         far_n = 0;
@@ -248,7 +258,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (want && !near) {
             const int pos = far_n + prefix_below(mf);
             farA[pos] = make_float4(ex, ey, ez, gx);
-            farB[pos] = make_float4(gy, gz, fmaf(gz, gz, fmaf(gy, gy, gx * gx)), 0.0f);
+            farB[pos] = make_float4(gy, gz, fmaf(ey, gz, -(ez * gy)), fmaf(ez, gx, -(ex * gz)));
+            farC[pos] = fmaf(ex, gy, -(ey * gx));
         }
         if (want && near) nearI[near_n + prefix_below(mn)] = idx;
         far_n += cf;
@@ -277,7 +288,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         int st = 0;  // first sorted segment of the cell (buckets)
         if (act) {
             nd = nodes[d];
-            st = meta[d].start;
             skp = min(max(nd.skip, d + 1), end);
             if (nd.count > 0) {
                 dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix, hiy,
@@ -289,6 +299,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                 else
                     kind = 3;
             }
+            if (kind == 2) st = meta[d].start;  // only buckets read their points
         }
         // pushes (both ranges nonempty, first index beyond d, end <= the
         // popped range's end): sibling continuation, then the children
@@ -446,7 +457,7 @@ int list_group_level(const ovg_params &p)
     double side = 0;
     for (int k = 0; k < 3; ++k) side = std::max(side, p.box_hi[k] - p.box_lo[k]);
     int L = 1;
-    while (L < 10 && side / (double)(1 << L) > 4 * p.mixed_near) ++L;
+    while (L < 10 && side / (double)(1 << L) > 8 * p.mixed_near) ++L;
     return L;
 }
 

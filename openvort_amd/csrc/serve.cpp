@@ -57,7 +57,7 @@ void job_params(const ovs_job &j, ovg_params &p)
     p.rec_dist = 0.8 * (j.dxi / 2);
     p.rec_angle = 5.0 * M_PI / 180.0;
     p.freq_to_cutoff = 0.8;
-    p.mixed_near = 4 * j.dxi;
+    p.mixed_near = 2 * j.dxi;
     p.rec_velocity = j.rec_velocity;
 }
 

@@ -165,7 +165,9 @@ def test_c2_list_traversal(c2, ppw):
     # the group test never accepts a node that the per-point test opens
     assert trunc < trunc_ref
     assert trunc32 < trunc_ref
-    assert prec < 1e-5
+    # FP32 far field: rounding only (a cancelling form of |Rp|^2 for long
+    # monopoles once showed here as 4e-6)
+    assert prec < 1e-6
     assert s32.interactions_fp32 > 0.5 * s32.interactions
     assert s64.interactions_fp32 == 0
 
