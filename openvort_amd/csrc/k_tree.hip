@@ -9,6 +9,9 @@
 //    traversal order.  Points outside the root box get bit 63 set (kept in
 //    the root's sums, absent from every child: src/octree.c:116-128), with a
 //    Morton code of their wrapped position below it for target coherence.
+//    The tuned traversals (group, list) clamp points into the box and take
+//    the same digits from integer Morton codes (morton_key).
+// 1b. Sums run level by level over nodes bucketed by level.
 // 2. A stable LSD radix sort of (key, slot), 8-bit digits, written here:
 //    per-tile LDS histograms, one exclusive scan, then a scatter that ranks
 //    equal digits with 64-lane ballots.
@@ -25,6 +28,8 @@
 // 4. Sums (centre of mass, circulation) bottom-up by cell level: a node
 //    adds its children's sums, a leaf its own point.
 // HBM/latency-bound; no MFMA.
+#include <algorithm>
+
 #include "engine_internal.h"
 
 namespace ovg {
@@ -111,6 +116,37 @@ __device__ __forceinline__ bool in_closed(const double lo[3], const double hi[3]
     return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
 }
 
+// Tuned modes: the cell of every level from the position quantised to 2^21
+// steps of the root box, as 3-bit digits in the reference's child numbering
+// (x upper half -> 0, y upper -> 2, z upper -> 4); these are the same dyadic
+// cells, except that a point on a split plane goes to the upper cell instead
+// of the reference's first match.
+__device__ __forceinline__ uint64_t spread3(uint64_t v)  // bit b -> bit 3b
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t morton_key(const TangleParams &P, double x, double y, double z)
+{
+    const double p3[3] = {x, y, z};
+    uint64_t q[3];
+    for (int a = 0; a < 3; ++a) {
+        double u = (p3[a] - P.lo[a]) / P.L[a] * 2097152.0;
+        u = fmin(fmax(u, 0.0), 2097151.0);
+        q[a] = (uint64_t)u;
+    }
+    q[0] = ~q[0] & 0x1fffff;
+    // level l's digit comes from coordinate bit b = 21 - l and sits at key
+    // bits 63 - 3l = 3b (x), 3b + 1 (y), 3b + 2 (z)
+    return spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+}
+
 // clamp != 0 (tuned modes): a point outside the root box is placed by its
 // position clamped onto the box, so it stays in the tree and in Morton order
 // among its neighbours; the reference's drop (src/octree.c:116-128) is kept
@@ -132,7 +168,9 @@ __global__ void k_keys(int n_live, const int *__restrict__ live, const double *_
     }
     double lo[3] = {P.lo[0], P.lo[1], P.lo[2]}, hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
     uint64_t key = kDropKey;
-    if (!in_closed(lo, hi, x, y, z)) {
+    if (clamp) {
+        key = morton_key(P, x, y, z);
+    } else if (!in_closed(lo, hi, x, y, z)) {
         // outside the root box: in no child (src/octree.c:116-128).  Bit 63
         // puts it after every in-box point; the remaining bits are a Morton
         // code of its wrapped position so that the traversal waves of these
@@ -185,10 +223,19 @@ __global__ void k_keys(int n_live, const int *__restrict__ live, const double *_
     }
     keys[i] = key;
     vals[i] = k;
-    // in-box count: one atomic per wavefront (the lanes past n_live have left)
-    const unsigned long long inbox = __ballot(!(key >> 63));
-    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && inbox)
-        atomicAdd(&counters[0], __popcll(inbox));
+}
+
+// counters[0] = in-box points: out-of-box keys have bit 63 set and sort
+// last, so this is the first sorted index with the bit (n_live if none).
+// Same-address atomics across the chip would serialise; this pass has none.
+__global__ void k_count_inbox(int n_live, const uint64_t *__restrict__ keys,
+                              int *__restrict__ counters)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_live) return;
+    const bool out = i < n_live && (keys[i] >> 63);
+    const bool prev_in = i == 0 || !(keys[i - 1] >> 63);
+    if ((out || i == n_live) && prev_in) counters[0] = i;
 }
 
 // ---------------------------------------------------------------- scan ----
@@ -420,7 +467,10 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const int8_t *__restrict__ delta, const int *__restrict__ counters,
                         int *__restrict__ counts, const int *__restrict__ base,
                         Node *__restrict__ nodes, NodeMeta *__restrict__ meta,
-                        const TangleParams *__restrict__ Pp)
+                        const TangleParams *__restrict__ Pp, const double *__restrict__ sx,
+                        const double *__restrict__ sy, const double *__restrict__ sz,
+                        const double *__restrict__ qx, const double *__restrict__ qy,
+                        const double *__restrict__ qz)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -460,14 +510,14 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                 double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
                 double hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
                 int lv = level < kMaxLevel ? level : kMaxLevel;
-                for (int l = 1; l <= lv; ++l) {
-                    int dg = (int)((ki >> (63 - 3 * l)) & 7);
-                    double nlo[3], nhi[3];
-                    child_box(lo, hi, dg, nlo, nhi);
-                    for (int q = 0; q < 3; ++q) { lo[q] = nlo[q]; hi[q] = nhi[q]; }
-                }
+                for (int l = 1; l <= lv; ++l)
+                    child_box_sel(lo, hi, (int)((ki >> (63 - 3 * l)) & 7));
                 double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
                 nd.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+            }
+            if (count == 1) {  // a leaf's sums are its point (octree_update_means)
+                nd.cx = sx[i]; nd.cy = sy[i]; nd.cz = sz[i];
+                nd.qx = qx[i]; nd.qy = qy[i]; nd.qz = qz[i];
             }
             nodes[d] = nd;
             NodeMeta m{};
@@ -508,21 +558,12 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
     if (!EMIT) counts[i] = cnt;
 }
 
-// Sums bottom-up for the nodes of one cell level: a leaf takes its point,
-// an internal node adds its children (d+1, then along skip ropes); the
-// root also adds the points outside the root box.  Sums stay in cx..qz
-// until k_finish_means turns cx..cz into centres of mass.
-__global__ void k_sums_level(int level, const int *__restrict__ counters,
-                             Node *__restrict__ nodes, const NodeMeta *__restrict__ meta,
-                             const double *__restrict__ sx, const double *__restrict__ sy,
-                             const double *__restrict__ sz, const double *__restrict__ qx,
-                             const double *__restrict__ qy, const double *__restrict__ qz,
-                             int n_live)
+__device__ void sum_node(int d, const int *__restrict__ counters, Node *__restrict__ nodes,
+                         const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
+                         const double *__restrict__ sy, const double *__restrict__ sz,
+                         const double *__restrict__ qx, const double *__restrict__ qy,
+                         const double *__restrict__ qz, int n_live)
 {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    int n_nodes = counters[1];
-    if (d >= n_nodes) return;
-    if (meta[d].level != level) return;
     Node nd = nodes[d];
     int n_in = counters[0];
     double s[6] = {0, 0, 0, 0, 0, 0};
@@ -550,6 +591,65 @@ __global__ void k_sums_level(int level, const int *__restrict__ counters,
     nodes[d] = nd;
 }
 
+
+// Internal nodes (and the root) bucketed by cell level, levels 0 ..
+// kMaxLevel + 1, without atomics: per-block counts in hist[level][block], an
+// exclusive scan over that array, then a scatter at block offset + rank in
+// the block.  The order inside a level is free (a level's nodes only read
+// deeper ones); leaves already hold their sums.
+constexpr int kLevels = kMaxLevel + 2;
+
+__device__ __forceinline__ int sum_level(const Node *__restrict__ nodes,
+                                         const NodeMeta *__restrict__ meta, int d)
+{
+    return (d == 0 || nodes[d].count > 1) ? meta[d].level : -1;
+}
+
+__global__ void k_level_count(const int *__restrict__ counters, const Node *__restrict__ nodes,
+                              const NodeMeta *__restrict__ meta, int *__restrict__ hist)
+{
+    __shared__ int cnt[kLevels];
+    if (threadIdx.x < kLevels) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = d < counters[1] ? sum_level(nodes, meta, d) : -1;
+    if (l >= 0) atomicAdd(&cnt[l], 1);
+    __syncthreads();
+    if (threadIdx.x < kLevels) hist[threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void k_level_scatter(const int *__restrict__ counters, const Node *__restrict__ nodes,
+                                const NodeMeta *__restrict__ meta,
+                                const int *__restrict__ offs, int *__restrict__ list)
+{
+    __shared__ int cnt[kLevels];
+    if (threadIdx.x < kLevels) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = d < counters[1] ? sum_level(nodes, meta, d) : -1;
+    if (l < 0) return;
+    list[offs[l * gridDim.x + blockIdx.x] + atomicAdd(&cnt[l], 1)] = d;
+}
+
+// Sums bottom-up for the nodes of one cell level: a leaf takes its point,
+// an internal node adds its children (d+1, then along skip ropes); the
+// root also adds the points outside the root box.  Sums stay in cx..qz
+// until k_finish_means turns cx..cz into centres of mass.  A fixed grid
+// strides over the level's bucket.
+__global__ void k_sums_level(int level, int nblk, const int *__restrict__ counters,
+                             const int *__restrict__ offs, const int *__restrict__ list,
+                             Node *__restrict__ nodes, const NodeMeta *__restrict__ meta,
+                             const double *__restrict__ sx, const double *__restrict__ sy,
+                             const double *__restrict__ sz, const double *__restrict__ qx,
+                             const double *__restrict__ qy, const double *__restrict__ qz,
+                             int n_live)
+{
+    const int b0 = offs[level * nblk],
+              b1 = level + 1 < kLevels ? offs[(level + 1) * nblk] : offs[kLevels * nblk];
+    for (int e = b0 + blockIdx.x * blockDim.x + threadIdx.x; e < b1;
+         e += gridDim.x * blockDim.x)
+        sum_node(list[e], counters, nodes, meta, sx, sy, sz, qx, qy, qz, n_live);
+}
 // com = sum * (1.0/N), src/octree.c:269; leaves keep x exactly.
 __global__ void k_finish_means(const int *__restrict__ counters, Node *__restrict__ nodes)
 {
@@ -562,6 +662,11 @@ __global__ void k_finish_means(const int *__restrict__ counters, Node *__restric
         nd.cy = nd.cy * inv;
         nd.cz = nd.cz * inv;
     }
+}
+
+__global__ void k_zero_ints(int *a, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
 }
 
 __global__ void k_tree_counters(int n_live, const int *__restrict__ base, int *__restrict__ counters)
@@ -583,6 +688,8 @@ int build_tree(Ctx &c, const double *const x[3])
     OVG_TRY(T.base.ensure(sizeof(int) * (n + 2)));
     OVG_TRY(T.nodes.ensure(sizeof(Node) * cap_nodes));
     OVG_TRY(T.meta.ensure(sizeof(NodeMeta) * cap_nodes));
+    OVG_TRY(T.level_hist.ensure(sizeof(int) * ((size_t)kLevels * ((cap_nodes + 255) / 256) + 1)));
+    OVG_TRY(T.level_list.ensure(sizeof(int) * cap_nodes));
     OVG_TRY(T.counters.ensure(sizeof(int) * 64));
     OVG_TRY(T.sorted_pts.ensure(sizeof(int) * (n + 1)));
     OVG_TRY(T.rank.ensure(sizeof(int) * ((size_t)c.n + 1)));
@@ -596,6 +703,8 @@ int build_tree(Ctx &c, const double *const x[3])
                        x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
                        T.counters.as<int>(), clamp);
     radix_sort(c, n, 0, 56);
+    hipLaunchKernelGGL(k_count_inbox, dim3((n + 1 + bs - 1) / bs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), T.counters.as<int>());
     hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.vals[0].as<int>(), x[0], x[1], x[2],
                        c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
@@ -605,22 +714,34 @@ int build_tree(Ctx &c, const double *const x[3])
     hipLaunchKernelGGL(k_nodes<false>, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
-                       T.meta.as<NodeMeta>(), dparams(c));
+                       T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
+                       T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
+                       T.sqz.as<double>());
     // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
     launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n, T.base.as<int>() + n,
                           T.scan_tmp);
     hipLaunchKernelGGL(k_nodes<true>, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
-                       T.meta.as<NodeMeta>(), dparams(c));
+                       T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
+                       T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
+                       T.sqz.as<double>());
     hipLaunchKernelGGL(k_tree_counters, dim3(1), dim3(1), 0, c.stream, n, T.base.as<int>(),
                        T.counters.as<int>());
     int gn = (int)((cap_nodes + bs - 1) / bs);
+    int *hist = T.level_hist.as<int>(), *list = T.level_list.as<int>();
+    hipLaunchKernelGGL(k_level_count, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+                       T.nodes.as<Node>(), T.meta.as<NodeMeta>(), hist);
+    launch_exclusive_scan(c, hist, hist, kLevels * gn, hist + kLevels * gn, T.scan_tmp);
+    hipLaunchKernelGGL(k_level_scatter, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+                       T.nodes.as<Node>(), T.meta.as<NodeMeta>(), hist, list);
+    const int gl = std::min(gn, 1024);
     for (int level = kMaxLevel + 1; level >= 0; --level)
-        hipLaunchKernelGGL(k_sums_level, dim3(gn), dim3(bs), 0, c.stream, level,
-                           T.counters.as<int>(), T.nodes.as<Node>(), T.meta.as<NodeMeta>(),
-                           T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
-                           T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(), n);
+        hipLaunchKernelGGL(k_sums_level, dim3(gl), dim3(bs), 0, c.stream, level, gn,
+                           T.counters.as<int>(), hist, list, T.nodes.as<Node>(),
+                           T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
+                           T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
+                           T.sqz.as<double>(), n);
     hipLaunchKernelGGL(k_finish_means, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
                        T.nodes.as<Node>());
     return OVG_OK;
