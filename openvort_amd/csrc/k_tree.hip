@@ -279,28 +279,25 @@ __global__ void k_scan_tile(const int *in, int *out, int n, int *__restrict__ ti
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-// one block: exclusive scan of the tile sums in place, total to *d_total
-__global__ void k_scan_top(int *__restrict__ sums, int m, int *__restrict__ d_total)
+// Each block adds the sum of the tile sums before it (a block reduction of
+// at most a few thousand values) to its tile; the last block also writes
+// the total.
+__global__ void k_scan_add(int *out, int n, const int *__restrict__ sums, int tiles,
+                           int *__restrict__ d_total)
 {
-    __shared__ int sh[kScanThreads];
-    int carry = 0;
-    for (int b0 = 0; b0 < m; b0 += kScanThreads) {
-        int i = b0 + threadIdx.x;
-        int v = i < m ? sums[i] : 0;
-        int total;
-        int ex = block_exclusive_scan(v, sh, total);
-        if (i < m) sums[i] = ex + carry;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && d_total) *d_total = carry;
-}
-
-__global__ void k_scan_add(int *out, int n, const int *__restrict__ sums)
-{
-    int base = blockIdx.x * kScanTile;
-    int add = sums[blockIdx.x];
+    __shared__ int red[kScanThreads / 64];
+    int pre = 0;
+    for (int j = threadIdx.x; j < (int)blockIdx.x; j += kScanThreads) pre += sums[j];
+    for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pre;
+    __syncthreads();
+    int add = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) add += red[w];
+    const int base = blockIdx.x * kScanTile;
     for (int j = threadIdx.x; j < kScanTile; j += kScanThreads)
         if (base + j < n) out[base + j] += add;
+    if (d_total && (int)blockIdx.x == tiles - 1 && threadIdx.x == 0)
+        *d_total = add + sums[blockIdx.x];
 }
 
 void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
@@ -314,10 +311,8 @@ void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_to
     tmp.ensure(sizeof(int) * (size_t)tiles);
     hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_in, d_out,
                        n, tmp.as<int>());
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, c.stream, tmp.as<int>(),
-                       tiles, d_total);
     hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_out, n,
-                       tmp.as<int>());
+                       tmp.as<int>(), tiles, d_total);
 }
 
 // ---------------------------------------------------------- radix sort ----
