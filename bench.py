@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])
     ap.add_argument("--smooth-every", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--reuse-steps", type=int, default=5,
+                    help="also time this many steps with reuse_lists = 1 (0 = skip)")
     ap.add_argument("--cpu-rings", type=int, default=62)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -298,6 +300,33 @@ def main():
     peak = PEAK_FP32 if prec == ov.PREC_MIXED else PEAK_FP64
     interactions_per_s = st.interactions / wall
 
+    # ---- the same path with the step's first lists reused ----------------
+    reuse = None
+    if trav == ov.TRAVERSE_LIST and args.reuse_steps > 0:
+        qr = ov.Params.from_buffer_copy(p)
+        if tune:
+            qr.points_per_wave, qr.leaf_size, qr.group_level = (
+                tune["points_per_wave"], tune["leaf_size"], tune["group_level"])
+        qr.reuse_lists = 1
+        e3 = ov.Engine(qr)
+        e3.upload(t)
+        e3.synchronize()
+        _, tm3 = run_steps(e3, args.warmup, 0.0, dt, 0, args.smooth_every, True, rb)
+        a = time.perf_counter()
+        lat3, _ = run_steps(e3, args.reuse_steps, tm3, dt, args.warmup, args.smooth_every, True,
+                            rb)
+        e3.synchronize()
+        w3 = time.perf_counter() - a
+        e3.close()
+        reuse = {"value": n_pts * args.reuse_steps / w3, "steps": args.reuse_steps,
+                 "ms_per_step": w3 / args.reuse_steps * 1e3,
+                 "p50_step_ms": pct(lat3, 50), "p99_step_ms": pct(lat3, 99),
+                 "path": "as the headline path, with reuse_lists = 1: the interaction "
+                         "lists of each step's first evaluation serve its RK4 stages "
+                         "(acceptance decided at the step's first stage; not the "
+                         "headline value)"}
+        log(f"reuse_lists: {reuse['ms_per_step']:.1f} ms/step")
+
     # ---- plain FP64 path ------------------------------------------------
     plain = None
     if args.plain_steps > 0:
@@ -351,6 +380,7 @@ def main():
         "line_length_cm": L,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "reuse_lists": reuse,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "kernel": {ov.TRAVERSE_LIST: "k_bh_list", ov.TRAVERSE_GROUP: "k_bh_group",

@@ -136,6 +136,15 @@ typedef struct ovg_params {
                                      octree cells of this level (1..21); 0 =
                                      the coarsest level whose cells are at most
                                      8 * mixed_near wide (5 if mixed_near = 0)  */
+    int32_t reuse_lists;          /* OVG_TRAVERSE_LIST: 1 = the interaction
+                                     lists of ovg_update_tangle's walk serve
+                                     the stages of the ovg_rk4_step that
+                                     follows it (same tree, refreshed at each
+                                     stage's positions; acceptance decided at
+                                     the step's first stage); 0 = every stage
+                                     walks its own tree (the reference's
+                                     build_tree per velocity evaluation)     */
+    int32_t pad_;
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a

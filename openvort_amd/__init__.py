@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("freq_to_cutoff", C.c_double), ("rk4_classic", C.c_int32),
         ("rec_velocity", C.c_int32), ("mixed_near", C.c_double),
         ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
+        ("reuse_lists", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

@@ -229,14 +229,33 @@ static void resolve_events(Ctx &c)
 
 // Biot-Savart part of one evaluation at positions x: writes bs_prim,
 // bs_img by slot.
-int eval_bs(Ctx &c, const double *const x[3])
+// phase: kEvalPlain, kEvalStep (update_tangle: a step's first evaluation,
+// which records its interaction lists when reuse_lists is set) or
+// kEvalStage (an RK4 stage of that step, which replays them).
+int eval_bs(Ctx &c, const double *const x[3], int phase)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     mark(c, &e0);
     if (c.params.method == OVG_METHOD_BH) {
-        OVG_TRY(build_tree(c, x));
+        const bool lists = c.params.traverse == OVG_TRAVERSE_LIST && c.params.reuse_lists;
+        if (lists && phase == kEvalStage && c.rec_ok && c.rec_tree == c.tree_id &&
+            c.rec_n == c.n_live) {
+            OVG_TRY(refit_tree(c, x));
+            c.list_mode = kReplay;
+        } else {
+            OVG_TRY(build_tree(c, x));
+            c.rec_ok = false;
+            c.list_mode = lists && phase == kEvalStep ? kRecord : kWalk;
+        }
         mark(c, &e1);
-        OVG_TRY(launch_bh(c, x));
+        const int r = launch_bh(c, x);
+        if (r == OVG_OK && c.list_mode == kRecord) {
+            c.rec_ok = true;
+            c.rec_tree = c.tree_id;
+            c.rec_n = c.n_live;
+        }
+        c.list_mode = kWalk;
+        OVG_TRY(r);
     } else {
         launch_prep_segments(c, x);
         mark(c, &e1);
@@ -397,6 +416,8 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
         return set_error(OVG_EINVAL, "points_per_wave must be 0, 8, 16, 32 or 64");
     if (p->group_level < 0 || p->group_level > 21)
         return set_error(OVG_EINVAL, "group_level must be in [0, 21]");
+    if (p->reuse_lists != 0 && p->reuse_lists != 1)
+        return set_error(OVG_EINVAL, "reuse_lists must be 0 or 1");
     if (p->traverse == OVG_TRAVERSE_POINT &&
         (p->leaf_size != 1 || p->precision != OVG_PREC_FP64) && p->method == OVG_METHOD_BH)
         return set_error(OVG_EINVAL,
@@ -465,6 +486,7 @@ static int alloc_state(Ctx &c, int n)
 static int upload_positions(Ctx &c, const double *vnodes)
 {
     int n = c.n;
+    c.rec_ok = false;
     OVG_TRY(positions_change(c));
     DBuf &stage = c.partial;
     OVG_TRY(stage.ensure(sizeof(double) * 3 * (size_t)n));
@@ -479,6 +501,7 @@ static int upload_positions(Ctx &c, const double *vnodes)
 static int upload_topology(Ctx &c, const int *conn, const int *status)
 {
     int n = c.n;
+    c.rec_ok = false;
     c.topo_version++;
     if (conn != c.h_conn.data()) c.h_conn.assign(conn, conn + 2 * (size_t)n);
     if (status != c.h_status.data()) c.h_status.assign(status, status + 2 * (size_t)n);
@@ -664,7 +687,7 @@ static int velocities(Ctx &c, double t, bool own_tangents)
 {
     push_params(c, t);
     Triple x = tri(c.x);
-    OVG_TRY(eval_bs(c, x.v));
+    OVG_TRY(eval_bs(c, x.v, own_tangents ? kEvalStep : kEvalPlain));
     launch_compose(c, x.v, own_tangents, 0, 0.0);
     return OVG_OK;
 }
@@ -703,9 +726,14 @@ extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
     for (int s = 1; s <= 3; ++s) {
         const double *const *xe = (s == 2) ? xs1.v : xs0.v;
         push_params(c, times[s - 1]);
-        OVG_TRY(eval_bs(c, xe));
+        const int r = eval_bs(c, xe, kEvalStage);
+        if (r != OVG_OK) {
+            c.rec_ok = false;
+            return r;
+        }
         launch_compose(c, xe, true, s, dt);
     }
+    c.rec_ok = false;  // recorded lists serve one step
     return OVG_OK;
 }
 

@@ -74,9 +74,15 @@ struct TreeBufs {
     DBuf groups;             // int2 (first sorted point, count) per target group
     DBuf rank;               // sorted position of each live slot
     DBuf sub_tgt, sub_groups, sub_count;  // list traversal of a target subset
+    DBuf rec, rec_count;     // recorded interaction lists (reuse_lists)
 };
 // counters[kGroupCounter]: number of target groups of the list traversal
 constexpr int kGroupCounter = 40;
+// list traversal launch modes: walk; walk and record each group's accepted
+// entries; replay the recorded entries at new positions (same tree)
+constexpr int kWalk = 0, kRecord = 1, kReplay = 2;
+// evaluation phases (eval_bs)
+constexpr int kEvalPlain = 0, kEvalStep = 1, kEvalStage = 2;
 
 struct Ctx {
     int device = 0;
@@ -103,6 +109,12 @@ struct Ctx {
     DBuf flags;                     // int32 device flags ([0] error bits)
     DBuf counter64;                 // uint64 interaction counters
     TreeBufs tree;
+    uint64_t tree_id = 0;           // bumped whenever the tree buffers are rebuilt or reused
+    int list_mode = 0;              // next list launch: walk / record / replay
+    bool rec_ok = false;            // tree.rec holds the lists of tree rec_tree
+    uint64_t rec_tree = 0;
+    int rec_n = 0;                  // live points when they were recorded
+    int rec_groups = 0;
     // reconnection / smoothing scratch
     DBuf rc_cell, rc_sorted, rc_start, rc_pairs, rc_count, rc_tmp;
     int rc_cap = 0;                 // int2 pairs that rc_pairs holds
@@ -170,6 +182,9 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
 
 // k_tree.hip
 int build_tree(Ctx &c, const double *const x[3]);
+// Same tree (keys, order, nodes, cell sizes) with positions, segment vectors
+// and node sums recomputed at x: the stages of one RK4 step.
+int refit_tree(Ctx &c, const double *const x[3]);
 
 // k_bh.hip
 int launch_bh(Ctx &c, const double *const x[3]);
@@ -210,7 +225,7 @@ int smooth_loops(Ctx &c);
 // engine.hip helpers
 // Biot-Savart (primary + images) of every live point at positions x with
 // the context's method, into c.bs_prim / c.bs_img.
-int eval_bs(Ctx &c, const double *const x[3]);
+int eval_bs(Ctx &c, const double *const x[3], int phase);
 TangleParams host_tparams(const ovg_params &p, double t);
 void push_params(Ctx &c, double t);
 void launch_soa_to_aos(Ctx &c, const double *x, const double *y, const double *z, double *aos);

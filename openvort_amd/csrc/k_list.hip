@@ -44,6 +44,7 @@ constexpr int kListNear = 128;   // near entries per flush
 // range per tree level below the popped node (22 levels: root + 21), plus
 // the two pushes of the popped range itself.
 constexpr int kStackMargin = 32;
+constexpr int kRecCap = 6144;    // recorded entries per group (about 3000 at theta 0.4)
 // near entry: bits 0-25 node or segment index, 26-30 copy (0 = primary,
 // c = image c - 1), bit 31 set for a node
 constexpr unsigned kIdxMask = (1u << 26) - 1;
@@ -121,8 +122,14 @@ __device__ __forceinline__ int xcd_group(int b, int nb)
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-template <bool MIXED, int PPW>
+// MODE: kWalk, kRecord (walk and keep every group's accepted entries, in
+// emission order, in rec[g * rec_cap ..], their count in rec_count[g], -1
+// when they do not fit), kReplay (a group with a record evaluates those
+// entries at the current positions instead of walking; the others walk).
+
+template <bool MIXED, int PPW, int MODE>
 __global__ __launch_bounds__(64, 4) void k_bh_list(
+    unsigned *__restrict__ rec, int *__restrict__ rec_count, int rec_cap, int rec_groups,
     const int2 *__restrict__ groups, const int *__restrict__ spts, const double *__restrict__ X,
     const double *__restrict__ Y, const double *__restrict__ Z, const Node *__restrict__ nodes,
     const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
@@ -186,6 +193,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     unsigned long long c64 = 0, c32 = 0;
     int far_n = 0, near_n = 0;
     const int n_copies = any ? 1 + P.n_images : 0;
+    const bool recorded = g < rec_groups;  // record / replay slot of this group
+    const size_t rec_base = (size_t)g * rec_cap;
+    int rec_n = 0;
+    bool rec_over = !recorded;
 
     auto flush_far = [&]() {
         // pad to whole rounds of 4 S entries with zero entries, which add
@@ -264,8 +275,45 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (want && near) nearI[near_n + prefix_below(mn)] = idx;
         far_n += cf;
         near_n += cn;
+        if (MODE == kRecord) {
+            const unsigned long long mw = mf | mn;
+            const int cw = cf + cn;
+            if (rec_n + cw <= rec_cap) {
+                if (want) rec[rec_base + rec_n + prefix_below(mw)] = idx;
+            } else {
+                rec_over = true;
+            }
+            rec_n += cw;
+        }
     };
 
+    const int rc = (MODE == kReplay && recorded) ? rec_count[g] : -1;
+    if (MODE == kReplay && rc >= 0) {
+        // the recorded entries at the current positions, 64 per round
+        for (int b0 = 0; b0 < rc; b0 += 64) {
+            const int j = b0 + lane;
+            const bool want = j < rc;
+            const unsigned e = want ? rec[rec_base + j] : 0u;
+            const int idx = (int)(e & kIdxMask), cp = (int)((e >> kCopyShift) & 31u);
+            float ex = 0, ey = 0, ez = 0, gx = 0, gy = 0, gz = 0;
+            double e2 = 0;
+            if (want) {
+                double ux, uy, uz;
+                if (e & kNodeBit) {
+                    const Node &nd = nodes[idx];
+                    ux = nd.cx; uy = nd.cy; uz = nd.cz;
+                    gx = (float)nd.qx; gy = (float)nd.qy; gz = (float)nd.qz;
+                } else {
+                    ux = sx[idx]; uy = sy[idx]; uz = sz[idx];
+                    gx = (float)sqx[idx]; gy = (float)sqy[idx]; gz = (float)sqz[idx];
+                }
+                ux += shL[cp][0]; uy += shL[cp][1]; uz += shL[cp][2];
+                ex = (float)(ux - cx); ey = (float)(uy - cy); ez = (float)(uz - cz);
+                e2 = box_dist2(ux, uy, uz, lox, loy, loz, hix, hiy, hiz);
+            }
+            emit(want, !MIXED || e2 < near2, e, ex, ey, ez, gx, gy, gz);
+        }
+    } else {
     if (lane < n_copies) stk[lane] = make_int2(0, n_nodes | (lane << kCopyShift));
     int sp = n_nodes > 0 ? n_copies : 0;
     __syncthreads();
@@ -368,6 +416,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         }
         __syncthreads();
     }
+    }  // walk
+    if (MODE == kRecord && recorded && lane == 0) rec_count[g] = rec_over ? -1 : rec_n;
     if (far_n) flush_far();
     if (near_n) flush_near();
     // slice partials in slice order
@@ -496,20 +546,38 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                          c.n_live);
     TreeBufs &T = c.tree;
     const int ppw = list_points_per_wave(c.params);
-    switch (ppw) {
-    case 8: OVG_TRY(build_groups<8>(c)); break;
-    case 16: OVG_TRY(build_groups<16>(c)); break;
-    case 32: OVG_TRY(build_groups<32>(c)); break;
-    default: OVG_TRY(build_groups<64>(c)); break;
+    const int mode = c.list_mode;
+    // a replay keeps the recorded run's groups (same tree, same order)
+    if (mode != kReplay) {
+        switch (ppw) {
+        case 8: OVG_TRY(build_groups<8>(c)); break;
+        case 16: OVG_TRY(build_groups<16>(c)); break;
+        case 32: OVG_TRY(build_groups<32>(c)); break;
+        default: OVG_TRY(build_groups<64>(c)); break;
+        }
     }
     // groups are counted on the device; blocks stride over them, so any
     // count is covered (typically 1.0-1.2 x n / ppw)
     const int full = (c.n_live + ppw - 1) / ppw;
     const int blocks = full + full / 4 + 8;
+    // records: one slot of kRecCap entries for each of the first `blocks`
+    // groups (a group beyond them, or one whose entries do not fit, walks)
+    int rec_groups = 0, rec_cap = 0;
+    if (mode == kRecord) {
+        OVG_TRY(T.rec.ensure(sizeof(unsigned) * (size_t)blocks * kRecCap));
+        OVG_TRY(T.rec_count.ensure(sizeof(int) * (size_t)blocks));
+        rec_groups = blocks;
+        rec_cap = kRecCap;
+        c.rec_groups = blocks;
+    } else if (mode == kReplay) {
+        rec_groups = c.rec_groups;
+        rec_cap = kRecCap;
+    }
     const double near2 = c.params.mixed_near * c.params.mixed_near;
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64), 0, c.stream, T.groups.as<int2>(),
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64), 0, c.stream, T.rec.as<unsigned>(),
+                           T.rec_count.as<int>(), rec_cap, rec_groups, T.groups.as<int2>(),
                            T.sorted_pts.as<int>(), x[0], x[1], x[2], T.nodes.as<Node>(),
                            T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
                            T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
@@ -520,12 +588,20 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
+    auto by_mode = [&](auto kw, auto kr, auto kp) {
+        if (mode == kRecord) go(kr);
+        else if (mode == kReplay) go(kp);
+        else go(kw);
+    };
+#define OVG_LIST_MODES(M, W) \
+    by_mode(k_bh_list<M, W, kWalk>, k_bh_list<M, W, kRecord>, k_bh_list<M, W, kReplay>)
     switch (ppw) {
-    case 8: mixed ? go(k_bh_list<true, 8>) : go(k_bh_list<false, 8>); break;
-    case 16: mixed ? go(k_bh_list<true, 16>) : go(k_bh_list<false, 16>); break;
-    case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
-    default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;
+    case 8: mixed ? OVG_LIST_MODES(true, 8) : OVG_LIST_MODES(false, 8); break;
+    case 16: mixed ? OVG_LIST_MODES(true, 16) : OVG_LIST_MODES(false, 16); break;
+    case 32: mixed ? OVG_LIST_MODES(true, 32) : OVG_LIST_MODES(false, 32); break;
+    default: mixed ? OVG_LIST_MODES(true, 64) : OVG_LIST_MODES(false, 64); break;
     }
+#undef OVG_LIST_MODES
     return OVG_OK;
 }
 
@@ -594,7 +670,8 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     const double near2 = c.params.mixed_near * c.params.mixed_near;
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ng), dim3(64), 0, c.stream, d_grp, d_tgt, x[0], x[1],
+        hipLaunchKernelGGL(kern, dim3(ng), dim3(64), 0, c.stream, nullptr, nullptr, 0, 0, d_grp,
+                           d_tgt, x[0], x[1],
                            x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                            T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
                            T.sqy.as<double>(), T.sqz.as<double>(), T.counters.as<int>(),
@@ -605,10 +682,10 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
     switch (ppw) {
-    case 8: mixed ? go(k_bh_list<true, 8>) : go(k_bh_list<false, 8>); break;
-    case 16: mixed ? go(k_bh_list<true, 16>) : go(k_bh_list<false, 16>); break;
-    case 32: mixed ? go(k_bh_list<true, 32>) : go(k_bh_list<false, 32>); break;
-    default: mixed ? go(k_bh_list<true, 64>) : go(k_bh_list<false, 64>); break;
+    case 8: mixed ? go(k_bh_list<true, 8, kWalk>) : go(k_bh_list<false, 8, kWalk>); break;
+    case 16: mixed ? go(k_bh_list<true, 16, kWalk>) : go(k_bh_list<false, 16, kWalk>); break;
+    case 32: mixed ? go(k_bh_list<true, 32, kWalk>) : go(k_bh_list<false, 32, kWalk>); break;
+    default: mixed ? go(k_bh_list<true, 64, kWalk>) : go(k_bh_list<false, 64, kWalk>); break;
     }
     // the host vectors must outlive the copies above
     OVG_HIP(hipStreamSynchronize(c.stream));

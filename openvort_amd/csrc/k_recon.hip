@@ -156,6 +156,7 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
     if (c.rc_cap == 0) c.rc_cap = 1 << 16;
     OVG_TRY(c.rc_pairs.ensure(sizeof(int2) * (size_t)c.rc_cap));
     int bs = 256, gs = (n + bs - 1) / bs;
+    ++c.tree_id;  // the sort below reuses the tree's key buffers
     const double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
     hipLaunchKernelGGL(k_cell_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), X, Y,
                        Z, dparams(c), g, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());

@@ -155,6 +155,7 @@ __global__ void k_live_scatter(int n, const int *__restrict__ status,
 
 int commit_topology(Ctx &c)
 {
+    c.rec_ok = false;  // recorded interaction lists follow the old links
     const int m = (int)c.dirty.size(), a = (int)c.new_pos.size();
     if (m == 0 && a == 0) return OVG_OK;
     const int n = c.n;

@@ -673,6 +673,7 @@ int build_tree(Ctx &c, const double *const x[3])
 {
     TreeBufs &T = c.tree;
     int n = c.n_live;
+    ++c.tree_id;
     size_t cap_nodes = 2 * (size_t)n + 2;
     for (int b = 0; b < 2; ++b) {
         OVG_TRY(T.keys[b].ensure(sizeof(uint64_t) * (n + 1)));
@@ -731,6 +732,52 @@ int build_tree(Ctx &c, const double *const x[3])
     hipLaunchKernelGGL(k_level_scatter, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
                        T.nodes.as<Node>(), T.meta.as<NodeMeta>(), hist, list);
     const int gl = std::min(gn, 1024);
+    for (int level = kMaxLevel + 1; level >= 0; --level)
+        hipLaunchKernelGGL(k_sums_level, dim3(gl), dim3(bs), 0, c.stream, level, gn,
+                           T.counters.as<int>(), hist, list, T.nodes.as<Node>(),
+                           T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
+                           T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
+                           T.sqz.as<double>(), n);
+    hipLaunchKernelGGL(k_finish_means, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+                       T.nodes.as<Node>());
+    return OVG_OK;
+}
+
+// leaves (count 1, not the root) take their point's position and segment
+__global__ void k_leaf_sums(const int *__restrict__ counters, Node *__restrict__ nodes,
+                            const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
+                            const double *__restrict__ sy, const double *__restrict__ sz,
+                            const double *__restrict__ qx, const double *__restrict__ qy,
+                            const double *__restrict__ qz)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d == 0 || d >= counters[1] || nodes[d].count != 1) return;
+    const int j = meta[d].start;
+    Node &nd = nodes[d];
+    nd.cx = sx[j]; nd.cy = sy[j]; nd.cz = sz[j];
+    nd.qx = qx[j]; nd.qy = qy[j]; nd.qz = qz[j];
+}
+
+int refit_tree(Ctx &c, const double *const x[3])
+{
+    TreeBufs &T = c.tree;
+    const int n = c.n_live;
+    if (n == 0) return OVG_OK;
+    const int bs = 256, gs = (n + bs - 1) / bs;
+    const size_t cap_nodes = 2 * (size_t)n + 2;
+    const int gn = (int)((cap_nodes + bs - 1) / bs);
+    hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), T.vals[0].as<int>(), x[0], x[1], x[2],
+                       c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
+                       T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
+                       T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
+                       T.delta.as<int8_t>(), T.rank.as<int>());
+    hipLaunchKernelGGL(k_leaf_sums, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+                       T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
+                       T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
+                       T.sqy.as<double>(), T.sqz.as<double>());
+    const int gl = std::min(gn, 1024);
+    int *hist = T.level_hist.as<int>(), *list = T.level_list.as<int>();
     for (int level = kMaxLevel + 1; level >= 0; --level)
         hipLaunchKernelGGL(k_sums_level, dim3(gl), dim3(bs), 0, c.stream, level, gn,
                            T.counters.as<int>(), hist, list, T.nodes.as<Node>(),

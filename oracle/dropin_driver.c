@@ -50,6 +50,17 @@ int main(int argc, char **argv)
     int n = 0;
     ovg_params p;
     if (!rd(&n, sizeof n, 1, f) || !rd(&p, sizeof p, 1, f)) return 2;
+    /* the file must hold exactly this build's ovg_params layout: a writer
+     * with another layout would hand the reference shifted links */
+    long here = ftell(f);
+    fseek(f, 0, SEEK_END);
+    long want = here + (long)n * (long)(3 * sizeof(double) + 4 * sizeof(int));
+    if (n < 0 || ftell(f) != want) {
+        fprintf(stderr, "%s: %ld bytes, expected %ld for %d points and a %zu-byte ovg_params\n",
+                argv[1], ftell(f), want, n, sizeof p);
+        return 2;
+    }
+    fseek(f, here, SEEK_SET);
     double *x = malloc(sizeof(double) * 3 * (size_t)n);
     int *conn = malloc(sizeof(int) * 2 * (size_t)n), *st = malloc(sizeof(int) * 2 * (size_t)n);
     if (!rd(x, sizeof(double), 3 * (size_t)n, f) || !rd(conn, sizeof(int), 2 * (size_t)n, f) ||

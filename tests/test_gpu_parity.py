@@ -197,6 +197,64 @@ def test_list_traversal_deterministic(c2):
     assert np.array_equal(a["vels"], b["vels"])
 
 
+def _list_params(p, reuse):
+    q = ov.Params.from_buffer_copy(p)
+    q.method = ov.METHOD_BH
+    q.traverse = ov.TRAVERSE_LIST
+    q.precision = ov.PREC_MIXED
+    q.leaf_size = 1
+    q.points_per_wave = 16
+    q.reuse_lists = reuse
+    return q
+
+
+def test_reuse_lists_replay_equals_walk_at_same_positions():
+    """With dt -> 0 the RK4 stages sit at the recorded positions, so the
+    replayed lists must give the walk's velocities (FP32 batch order aside);
+    the effective velocity (x_new - x)/dt resolves that to ~1e-7."""
+    t, p, _ = configs.make("c2")
+    live = t.live
+    dt = 1e-9
+    veff = {}
+    for reuse in (0, 1):
+        say(f"c2 rk4 step dt {dt}, reuse_lists {reuse}")
+        with engine(t, _list_params(p, reuse)) as e:
+            e.update_tangle(0.0)
+            e.rk4_step(0.0, dt)
+            veff[reuse] = (e.download()["vnodes"][live] - t.vnodes[live]) / dt
+    rel = np.linalg.norm(veff[1] - veff[0], axis=1) / np.linalg.norm(veff[0], axis=1)
+    say(f"replay vs walk: max {rel.max():.2e}, median {np.median(rel):.2e}")
+    assert rel.max() < 2e-5
+
+
+def test_reuse_lists_step_error():
+    """One RK4 step with the step's first lists reused at every stage: the
+    error against the FP64 direct-sum step stays that of a fresh walk per
+    stage (both are the tree's truncation)."""
+    t, p, dt = configs.make("c2")
+    live = t.live
+    q = ov.Params.from_buffer_copy(p)
+    q.method = ov.METHOD_DIRECT
+    q.precision = ov.PREC_FP64
+    say("c2 rk4 step, FP64 direct")
+    with engine(t, q) as e:
+        e.update_tangle(0.0)
+        e.rk4_step(0.0, dt)
+        xd = e.download()["vnodes"][live]
+    disp = np.linalg.norm(xd - t.vnodes[live])
+    xs = {}
+    for reuse in (0, 1):
+        with engine(t, _list_params(p, reuse)) as e:
+            e.update_tangle(0.0)
+            e.rk4_step(0.0, dt)
+            xs[reuse] = e.download()["vnodes"][live]
+    err = {r: np.linalg.norm(xs[r] - xd) / disp for r in (0, 1)}
+    diff = np.linalg.norm(xs[1] - xs[0]) / disp
+    say(f"step error / displacement: walk {err[0]:.4e}, reuse {err[1]:.4e}; reuse-walk {diff:.2e}")
+    assert err[1] < 1.02 * err[0]
+    assert diff < 0.05
+
+
 def test_reconnect_candidates_bitexact(oracle):
     g = load_golden("c1_reconnect.npz")
     t, p = golden_tangle(g)
