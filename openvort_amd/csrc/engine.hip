@@ -428,6 +428,7 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     c.params = *p;
+    c.rec_ok = false;  // recorded interaction lists belong to the old settings
     c.have_params = true;
     push_params(c, 0.0);
     return OVG_OK;

@@ -3,7 +3,8 @@
 // Candidate scan on the GPU: points are binned into a periodic grid whose
 // cells are at least rec_dist wide (at most 256 per axis), sorted by cell
 // with the radix sort of k_tree.hip, and each point k tests the points l > k
-// of its 27 neighbouring cells with the reference predicate: l not a
+// of its 27 neighbouring cells (9 rows, each one sorted range) with the
+// reference predicate: l not a
 // neighbour of k and |seg_pwrap(x_k, x_l)| <= rec_dist, in FP64 with the
 // reference's operation order, so the candidate list is bit-identical to
 // the reference's all-pairs scan (:197-222) before any reconnection.
@@ -18,6 +19,7 @@
 // each rewiring the host adds the exact change of the two segments whose
 // forward link changed.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -76,6 +78,8 @@ __global__ void k_fill_int(int n, int *a, int v)
     if (i < n) a[i] = v;
 }
 
+// start[c] / end[c]: the sorted range of cell c (start = INT_MAX, end = 0
+// for an empty cell, from the fills before)
 __global__ void k_cell_ranges(int n_live, const uint64_t *__restrict__ keys,
                               int *__restrict__ start, int *__restrict__ end)
 {
@@ -111,17 +115,35 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
     int cx = cell_coord(xk, P.lo[0], g.cs[0], g.nc[0]);
     int cy = cell_coord(yk, P.lo[1], g.cs[1], g.nc[1]);
     int cz = cell_coord(zk, P.lo[2], g.cs[2], g.nc[2]);
-    int rx0 = g.nc[0] >= 3 ? -1 : 0, rx1 = g.nc[0] >= 3 ? 1 : 0;
     int ry0 = g.nc[1] >= 3 ? -1 : 0, ry1 = g.nc[1] >= 3 ? 1 : 0;
     int rz0 = g.nc[2] >= 3 ? -1 : 0, rz1 = g.nc[2] >= 3 ? 1 : 0;
+    // cells x-1 .. x+1 of a row are consecutive ids (one range), or two
+    // ranges where the row wraps; a grid of fewer than 3 cells is one cell
+    const int nx = g.nc[0];
+    int xa0, xa1, xb0 = 0, xb1 = -1;  // cell x ranges [xa0, xa1], [xb0, xb1]
+    if (nx < 3) {
+        xa0 = xa1 = 0;
+    } else if (cx == 0) {
+        xa0 = 0; xa1 = 1; xb0 = xb1 = nx - 1;
+    } else if (cx == nx - 1) {
+        xa0 = nx - 2; xa1 = nx - 1; xb0 = xb1 = 0;
+    } else {
+        xa0 = cx - 1; xa1 = cx + 1;
+    }
     for (int dz = rz0; dz <= rz1; ++dz)
         for (int dy = ry0; dy <= ry1; ++dy)
-            for (int dx = rx0; dx <= rx1; ++dx) {
-                int ax = (cx + dx + g.nc[0]) % g.nc[0];
+            for (int part = 0; part < 2; ++part) {
+                const int x0 = part ? xb0 : xa0, x1 = part ? xb1 : xa1;
+                if (x1 < x0) continue;
                 int ay = (cy + dy + g.nc[1]) % g.nc[1];
                 int az = (cz + dz + g.nc[2]) % g.nc[2];
-                int c = (az * g.nc[1] + ay) * g.nc[0] + ax;
-                int s0 = start[c], s1 = end[c];
+                const int row = (az * g.nc[1] + ay) * nx;
+                // the cells x0..x1 of a row are consecutive in the sorted order
+                int s0 = INT_MAX, s1 = 0;
+                for (int x = x0; x <= x1; ++x) {
+                    s0 = min(s0, start[row + x]);
+                    s1 = max(s1, end[row + x]);
+                }
                 for (int s = s0; s < s1; ++s) {
                     int l = sorted[s];
                     if (l <= k || l == fk || l == rk) continue;
@@ -163,8 +185,10 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
     // sort by cell id (< 2^24): four 8-bit passes (an even count)
     radix_sort(c, n, 0, 24);
     int *start = c.rc_start.as<int>(), *end = start + ncell;
-    hipLaunchKernelGGL(k_fill_int, dim3((2 * ncell + 255) / 256), dim3(256), 0, c.stream,
-                       2 * ncell, start, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, ncell,
+                       start, INT_MAX);
+    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, ncell, end,
+                       0);
     hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), start, end);
     int total = 0;

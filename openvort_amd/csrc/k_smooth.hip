@@ -835,9 +835,10 @@ __global__ void k_lt_init(int n, const int *__restrict__ fwd, const int *__restr
     if (i >= n) return;
     const bool live = status[i] != OVG_EMPTY;  // one status word per slot
     const int f = fwd[i];
-    if (live && f < 0) *open = 1;
+    // open chains, and links into EMPTY or missing slots, take the host walk
+    if (live && (f < 0 || f >= n || status[f] == OVG_EMPTY)) *open = 1;
     lab[i] = live ? i : INT_MAX;
-    nx[i] = (live && f >= 0) ? f : i;
+    nx[i] = (live && f >= 0 && f < n) ? f : i;
 }
 
 __global__ void k_lt_minjump(int n, const int *__restrict__ lab, const int *__restrict__ nx,
@@ -856,7 +857,7 @@ __global__ void k_lt_rank_init(int n, const int *__restrict__ fwd, const int *__
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const bool end = lab[i] == INT_MAX || lab[i] == i || fwd[i] < 0;
+    const bool end = lab[i] == INT_MAX || lab[i] == i || fwd[i] < 0 || fwd[i] >= n;
     nx[i] = end ? i : fwd[i];
     d[i] = end ? 0 : 1;
 }
@@ -880,7 +881,7 @@ __global__ void k_lt_heads(int n, const int *__restrict__ fwd, const int *__rest
     const bool head = lab[i] == i;
     flag[i] = head;
     const int f = fwd[i];
-    len[i] = head ? (f >= 0 ? d[f] + 1 : 1) : 0;  // open chains are rebuilt on the host
+    len[i] = head ? (f >= 0 && f < n ? d[f] + 1 : 1) : 0;  // open chains: host walk
 }
 
 // hidx: loop number of each head, hoff: first entry of each head's loop

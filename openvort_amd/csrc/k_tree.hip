@@ -280,24 +280,55 @@ __global__ void k_scan_tile(const int *in, int *out, int n, int *__restrict__ ti
 }
 
 // Each block adds the sum of the tile sums before it (a block reduction of
-// at most a few thousand values) to its tile; the last block also writes
-// the total.
+// at most kScanDirect values) to its tile; the last block also writes the
+// total.  With more tiles the tile sums are first scanned in place (the
+// same scan, one level up) and each block adds its own entry.
+constexpr int kScanDirect = 2048;
+
 __global__ void k_scan_add(int *out, int n, const int *__restrict__ sums, int tiles,
-                           int *__restrict__ d_total)
+                           int *__restrict__ d_total, int scanned)
 {
     __shared__ int red[kScanThreads / 64];
-    int pre = 0;
-    for (int j = threadIdx.x; j < (int)blockIdx.x; j += kScanThreads) pre += sums[j];
-    for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pre;
-    __syncthreads();
     int add = 0;
-    for (int w = 0; w < kScanThreads / 64; ++w) add += red[w];
+    if (scanned) {
+        add = sums[blockIdx.x];
+    } else {
+        int pre = 0;
+        for (int j = threadIdx.x; j < (int)blockIdx.x; j += kScanThreads) pre += sums[j];
+        for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pre;
+        __syncthreads();
+        for (int w = 0; w < kScanThreads / 64; ++w) add += red[w];
+    }
     const int base = blockIdx.x * kScanTile;
     for (int j = threadIdx.x; j < kScanTile; j += kScanThreads)
         if (base + j < n) out[base + j] += add;
-    if (d_total && (int)blockIdx.x == tiles - 1 && threadIdx.x == 0)
+    if (!scanned && d_total && (int)blockIdx.x == tiles - 1 && threadIdx.x == 0)
         *d_total = add + sums[blockIdx.x];
+}
+
+// tmp holds the tile sums of every level (sized by scan_tmp_ints)
+static size_t scan_tmp_ints(int n)
+{
+    size_t need = 0;
+    for (int t = (n + kScanTile - 1) / kScanTile; t > 0; t = t > kScanDirect ? (t + kScanTile - 1) / kScanTile : 0)
+        need += (size_t)t;
+    return need + 1;
+}
+
+static void scan_level(Ctx &c, const int *d_in, int *d_out, int n, int *d_total, int *tmp)
+{
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_in, d_out,
+                       n, tmp);
+    if (tiles <= kScanDirect) {
+        hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_out, n,
+                           tmp, tiles, d_total, 0);
+    } else {
+        scan_level(c, tmp, tmp, tiles, d_total, tmp + tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_out, n,
+                           tmp, tiles, nullptr, 1);
+    }
 }
 
 void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
@@ -307,12 +338,8 @@ void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_to
         if (d_total) hipMemsetAsync(d_total, 0, sizeof(int), c.stream);
         return;
     }
-    int tiles = (n + kScanTile - 1) / kScanTile;
-    tmp.ensure(sizeof(int) * (size_t)tiles);
-    hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_in, d_out,
-                       n, tmp.as<int>());
-    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanThreads), 0, c.stream, d_out, n,
-                       tmp.as<int>(), tiles, d_total);
+    tmp.ensure(sizeof(int) * scan_tmp_ints(n));
+    scan_level(c, d_in, d_out, n, d_total, tmp.as<int>());
 }
 
 // ---------------------------------------------------------- radix sort ----
