@@ -23,8 +23,9 @@
 //   near entries (and every entry in FP64 mode) as node / segment indices,
 //     evaluated in FP64 with the reference's guards (src/tangle.c:549-552).
 // A full list is evaluated by all 64 lanes at once: lane = (slice, target)
-// with 64 / PPW slices striding the list, so every lane does useful pair
-// work and the walk is paid once per group instead of once per point.  The
+// with 64 / ny slices striding the list for a group of ny targets (groups
+// end at cell boundaries, so many hold fewer than PPW), so nearly every lane
+// does useful pair work and the walk is paid once per group.  The
 // per-slice FP64 partial sums are added in slice order at the end, so the
 // result does not depend on scheduling.  The primary and image parts are
 // one sum (written as the primary part; the image part is zero).
@@ -141,8 +142,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     double *__restrict__ ix, double *__restrict__ iy, double *__restrict__ iz,
     unsigned long long *__restrict__ icount)
 {
-    constexpr int S = 64 / PPW;
-    static_assert(kListFar % (4 * S) == 0, "far flushes run in whole rounds of 4 S entries");
+    static_assert(64 % PPW == 0 && 4 * 64 / PPW <= kListFar, "slices of a full group");
     __shared__ int2 stk[kListStack];
     __shared__ float4 farA[kListFar];  // offset a from the copy's group centre, qx
     __shared__ float4 farB[kListFar];  // qy, qz, (a x q)_x, (a x q)_y
@@ -153,7 +153,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
 
     const TangleParams &P = *Pp;
     const int lane = lane_id();
-    const int slice = lane / PPW;
     const int n_groups = *n_groups_p;
     if (lane < 27) {
         const bool img = lane >= 1 && lane <= P.n_images;
@@ -162,8 +161,15 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     __syncthreads();
     for (int g = xcd_group(blockIdx.x, gridDim.x); g < n_groups; g += gridDim.x) {
     const int2 grp = groups[g];
-    const bool valid = lane % PPW < grp.y;
-    const int t = grp.x + lane % PPW;
+    // lane = (slice, target) over the group's ny targets: S = 64 / ny slices
+    // (at most 32, so a flush round of 4 S entries fits the far list), the
+    // lanes from S * ny on idle.  A far flush holds whole rounds of 4 S.
+    const int ny = min(max(grp.y, 1), PPW);
+    const int S = min(64 / ny, 32);
+    const int far_cap = kListFar / (4 * S) * (4 * S);
+    const bool valid = grp.y > 0 && lane < S * ny;
+    const int slice = valid ? lane / ny : 0;
+    const int t = grp.x + lane % ny;
     const int k = valid ? spts[t] : 0;
     const double rx = valid ? X[k] : 0.0, ry = valid ? Y[k] : 0.0, rz = valid ? Z[k] : 0.0;
     const bool on = valid && isfinite(rx) && isfinite(ry) && isfinite(rz);
@@ -210,10 +216,11 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         }
         __syncthreads();
         float sax = 0.f, say = 0.f, saz = 0.f, sqx = 0.f, sqy = 0.f, sqz = 0.f;
-        for (int i0 = slice; i0 < fpad; i0 += 4 * S) {
+        // slice s takes entries 4s .. 4s + 3 of each round of 4 S
+        for (int i0 = 4 * slice; i0 < fpad; i0 += 4 * S) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int i = i0 + u * S;
+                const int i = i0 + u;
                 const float4 a = farA[i];
                 const float4 b = farB[i];
                 seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i],
@@ -224,7 +231,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         ax += (double)(sax - fmaf(ty, sqz, -(tz * sqy))) * f64;
         ay += (double)(say - fmaf(tz, sqx, -(tx * sqz))) * f64;
         az += (double)(saz - fmaf(tx, sqy, -(ty * sqx))) * f64;
-        if (on && far_n > slice) c32 += (far_n - slice + S - 1) / S;
+        if (on) {  // real entries of this slice: blocks of 4 with index = slice mod S
+            const int nf = far_n >> 2, rem = far_n & 3;
+            c32 += 4 * (nf > slice ? (nf - slice + S - 1) / S : 0) + (nf % S == slice ? rem : 0);
+        }
         __syncthreads();
         far_n = 0;
     };
@@ -264,7 +274,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         const unsigned long long mf = __ballot(want && !near);
         const unsigned long long mn = __ballot(want && near);
         const int cf = __popcll(mf), cn = __popcll(mn);
-        if (cf && far_n + cf > kListFar) flush_far();
+        if (cf && far_n + cf > far_cap) flush_far();
         if (cn && near_n + cn > kListNear) flush_near();
         if (want && !near) {
             const int pos = far_n + prefix_below(mf);
@@ -421,9 +431,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     if (far_n) flush_far();
     if (near_n) flush_near();
     // slice partials in slice order
-#pragma unroll
     for (int s = 1; s < S; ++s) {
-        const int src = (lane % PPW) + s * PPW;
+        const int src = lane % ny + s * ny;
         const double bx = __shfl(ax, src, 64), by = __shfl(ay, src, 64),
                      bz = __shfl(az, src, 64);
         if (slice == 0) {
