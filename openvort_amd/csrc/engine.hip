@@ -368,7 +368,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
                   &T.counts, &T.base, &T.nodes, &T.meta, &T.level_hist, &T.level_list,
                   &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
                   &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups,
-                  &T.rank, &T.sub_tgt, &T.sub_groups, &T.sub_count};
+                  &T.rank, &T.sub_tgt, &T.sub_groups, &T.sub_count, &T.rec,
+                  &T.rec_count, &T.fix, &T.fix_tiles};
     for (auto *b : tb) b->release();
     if (c.h_xpin) hipHostFree(c.h_xpin);
     c.h_xpin = nullptr;

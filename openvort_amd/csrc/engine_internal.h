@@ -75,6 +75,7 @@ struct TreeBufs {
     DBuf rank;               // sorted position of each live slot
     DBuf sub_tgt, sub_groups, sub_count;  // list traversal of a target subset
     DBuf rec, rec_count;     // recorded interaction lists (reuse_lists)
+    DBuf fix, fix_tiles;     // tuned trees: fixed-point prefix sums of the sorted points
 };
 // counters[kGroupCounter]: number of target groups of the list traversal
 constexpr int kGroupCounter = 40;

@@ -11,7 +11,6 @@
 //    Morton code of their wrapped position below it for target coherence.
 //    The tuned traversals (group, list) clamp points into the box and take
 //    the same digits from integer Morton codes (morton_key).
-// 1b. Sums run level by level over nodes bucketed by level.
 // 2. A stable LSD radix sort of (key, slot), 8-bit digits, written here:
 //    per-tile LDS histograms, one exclusive scan, then a scatter that ranks
 //    equal digits with 64-lane ballots.
@@ -25,10 +24,15 @@
 //    delta[i..] above delta[i-1], plus i's leaf; a node's DFS index is
 //    (nodes starting before i) + (rank among nodes starting at i) and its
 //    skip rope is the first node starting after its range.
-// 4. Sums (centre of mass, circulation) bottom-up by cell level: a node
-//    adds its children's sums, a leaf its own point.
+// 4. Sums (centre of mass, circulation).  Reference tree: bottom-up over
+//    nodes bucketed by cell level, a node adding its children's sums in the
+//    reference's order.  Tuned trees: a cell is a contiguous range of sorted
+//    points, so its sums are differences of exact fixed-point prefix sums,
+//    taken as the node is written (and again on a refit).  A leaf takes its
+//    own point in both.
 // HBM/latency-bound; no MFMA.
 #include <algorithm>
+#include <cmath>
 
 #include "engine_internal.h"
 
@@ -469,12 +473,24 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
     if (i + 1 < n_in) delta[i] = (int8_t)lcp_levels(keys[i], keys[i + 1]);
 }
 
-// last j in [i+1, n_in-1] with lcp(K[i], K[j]) >= v (at least i+1 qualifies)
+// last j in [i+1, n_in-1] with lcp(K[i], K[j]) >= v (at least i+1
+// qualifies).  The qualifying j form a run from i + 1, so a galloping search
+// (steps 1, 2, 4, .. then bisection of the last step) takes about
+// 2 log2(node size) key loads: most nodes hold a few points.
 __device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int i, int n_in,
                                          int v)
 {
-    uint64_t ki = keys[i];
+    const uint64_t ki = keys[i];
     int lo = i + 1, hi = n_in - 1;
+    for (int step = 1; lo < hi; step <<= 1) {
+        const int j = lo + step < hi ? lo + step : hi;
+        if (lcp_levels(ki, keys[j]) >= v) {
+            lo = j;
+        } else {
+            hi = j - 1;
+            break;
+        }
+    }
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (lcp_levels(ki, keys[mid]) >= v) lo = mid;
@@ -483,7 +499,188 @@ __device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int 
     return lo;
 }
 
+// ------------------------------------------- tuned trees: prefix sums ----
+// A cell of the tuned tree holds a contiguous range [a, b) of sorted points,
+// so its sums are differences of prefix sums over the sorted points.  They
+// are taken in 64-bit fixed point (units of 2^-shift cm, positions relative
+// to the box corner), which makes every difference exact and independent of
+// summation order.  Component 6 counts points whose values are not finite or
+// lie beyond the fixed-point range (more than 4 box lengths out); a cell
+// holding one gets NaN sums, as a floating-point sum would.
+constexpr int kFix = 7;
+struct Fix7 {
+    long long v[kFix];
+};
+constexpr int kFixThreads = 256, kFixItems = 4, kFixTile = kFixThreads * kFixItems;
+constexpr int kFixScanThreads = 1024;
+
+// Largest shift with n values of magnitude up to 4 Lmax summing below 2^62.
+static int fix_shift(int n, double Lmax)
+{
+    const double lim = std::ldexp(1.0, 62) / ((double)(n > 0 ? n : 1) * 4.0 * Lmax);
+    int k = (int)std::floor(std::log2(lim));
+    return std::max(-60, std::min(60, k));
+}
+
+__device__ __forceinline__ long long to_fix(double v, double scale, double lim, long long &bad)
+{
+    const double s = v * scale;
+    if (!(fabs(s) <= lim)) {
+        bad = 1;
+        return 0;
+    }
+    return __double2ll_rn(s);
+}
+
+// Wave-inclusive scan of kFix 64-bit values.
+__device__ __forceinline__ void wave_scan_fix(long long s[kFix], int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+        for (int f = 0; f < kFix; ++f) {
+            const long long u = __shfl_up(s[f], off, 64);
+            if (lane >= off) s[f] += u;
+        }
+}
+
+// P[i] (i <= n) = fixed-point sums of sorted points [tile start, i); T[t] =
+// the tile's total.  Tiles of kFixTile entries.
+__global__ __launch_bounds__(kFixThreads) void k_fix_tile(
+    int n, const double *__restrict__ sx, const double *__restrict__ sy,
+    const double *__restrict__ sz, const double *__restrict__ qx, const double *__restrict__ qy,
+    const double *__restrict__ qz, double lo0, double lo1, double lo2, double scale, double lim,
+    Fix7 *__restrict__ P, Fix7 *__restrict__ T)
+{
+    __shared__ long long wsum[kFixThreads / 64][kFix];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int base = blockIdx.x * kFixTile + threadIdx.x * kFixItems;
+    long long v[kFixItems][kFix], s[kFix];
+    for (int f = 0; f < kFix; ++f) s[f] = 0;
+#pragma unroll
+    for (int j = 0; j < kFixItems; ++j) {
+        const int i = base + j;
+        long long bad = 0;
+        if (i < n) {
+            v[j][0] = to_fix(sx[i] - lo0, scale, lim, bad);
+            v[j][1] = to_fix(sy[i] - lo1, scale, lim, bad);
+            v[j][2] = to_fix(sz[i] - lo2, scale, lim, bad);
+            v[j][3] = to_fix(qx[i], scale, lim, bad);
+            v[j][4] = to_fix(qy[i], scale, lim, bad);
+            v[j][5] = to_fix(qz[i], scale, lim, bad);
+        } else {
+            for (int f = 0; f < 6; ++f) v[j][f] = 0;
+        }
+        v[j][6] = bad;
+        for (int f = 0; f < kFix; ++f) s[f] += v[j][f];
+    }
+    long long incl[kFix];
+    for (int f = 0; f < kFix; ++f) incl[f] = s[f];
+    wave_scan_fix(incl, lane);
+    if (lane == 63)
+        for (int f = 0; f < kFix; ++f) wsum[w][f] = incl[f];
+    __syncthreads();
+    long long ex[kFix], tot[kFix];
+    for (int f = 0; f < kFix; ++f) {
+        long long pre = 0, all = 0;
+        for (int q = 0; q < kFixThreads / 64; ++q) {
+            if (q < w) pre += wsum[q][f];
+            all += wsum[q][f];
+        }
+        ex[f] = pre + incl[f] - s[f];
+        tot[f] = all;
+    }
+#pragma unroll
+    for (int j = 0; j < kFixItems; ++j) {
+        const int i = base + j;
+        if (i <= n)
+            for (int f = 0; f < kFix; ++f) P[i].v[f] = ex[f];
+        for (int f = 0; f < kFix; ++f) ex[f] += v[j][f];
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < kFix; ++f) T[blockIdx.x].v[f] = tot[f];
+}
+
+// Exclusive scan of the tile totals in place, one block: each thread takes a
+// run of consecutive tiles.
+__global__ __launch_bounds__(kFixScanThreads) void k_fix_tiles(int tiles, Fix7 *__restrict__ T)
+{
+    __shared__ long long wsum[kFixScanThreads / 64][kFix];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int per = (tiles + kFixScanThreads - 1) / kFixScanThreads;
+    const int t0 = threadIdx.x * per, t1 = min(t0 + per, tiles);
+    long long s[kFix];
+    for (int f = 0; f < kFix; ++f) s[f] = 0;
+    for (int t = t0; t < t1; ++t)
+        for (int f = 0; f < kFix; ++f) s[f] += T[t].v[f];
+    long long incl[kFix];
+    for (int f = 0; f < kFix; ++f) incl[f] = s[f];
+    wave_scan_fix(incl, lane);
+    if (lane == 63)
+        for (int f = 0; f < kFix; ++f) wsum[w][f] = incl[f];
+    __syncthreads();
+    long long ex[kFix];
+    for (int f = 0; f < kFix; ++f) {
+        long long pre = 0;
+        for (int q = 0; q < w; ++q) pre += wsum[q][f];
+        ex[f] = pre + incl[f] - s[f];
+    }
+    for (int t = t0; t < t1; ++t)
+        for (int f = 0; f < kFix; ++f) {
+            const long long u = T[t].v[f];
+            T[t].v[f] = ex[f];
+            ex[f] += u;
+        }
+}
+
+// Sums of sorted points [a, b) into a node: centre of mass and circulation.
+__device__ __forceinline__ void fix_node_sums(Node &nd, const Fix7 *__restrict__ P,
+                                              const Fix7 *__restrict__ T, int a, int b,
+                                              const double lo[3], double inv_scale)
+{
+    const Fix7 &pa = P[a], &pb = P[b];
+    const Fix7 &ta = T[a / kFixTile], &tb = T[b / kFixTile];
+    long long s[kFix];
+    for (int f = 0; f < kFix; ++f) s[f] = (pb.v[f] + tb.v[f]) - (pa.v[f] + ta.v[f]);
+    const double nan = __builtin_nan("");
+    const double inv = 1.0 / (double)(b - a);
+    const bool bad = s[6] != 0;
+    nd.cx = bad ? nan : lo[0] + (double)s[0] * inv_scale * inv;
+    nd.cy = bad ? nan : lo[1] + (double)s[1] * inv_scale * inv;
+    nd.cz = bad ? nan : lo[2] + (double)s[2] * inv_scale * inv;
+    nd.qx = bad ? nan : (double)s[3] * inv_scale;
+    nd.qy = bad ? nan : (double)s[4] * inv_scale;
+    nd.qz = bad ? nan : (double)s[5] * inv_scale;
+}
+
+// Fixed-point prefix sums of the sorted points (c.tree.sx .. sqz).
+static int fix_prefix(Ctx &c, int n, double &inv_scale)
+{
+    TreeBufs &T = c.tree;
+    const ovg_params &p = c.params;
+    double Lmax = 0;
+    for (int k = 0; k < 3; ++k) Lmax = std::max(Lmax, p.box_hi[k] - p.box_lo[k]);
+    const int shift = fix_shift(n, Lmax);
+    const double scale = std::ldexp(1.0, shift);
+    const double lim = std::ldexp(1.0, 62) / (double)std::max(n, 1);
+    inv_scale = std::ldexp(1.0, -shift);
+    const int tiles = (n + 1 + kFixTile - 1) / kFixTile;
+    OVG_TRY(T.fix.ensure(sizeof(Fix7) * ((size_t)n + 1)));
+    OVG_TRY(T.fix_tiles.ensure(sizeof(Fix7) * (size_t)tiles));
+    hipLaunchKernelGGL(k_fix_tile, dim3(tiles), dim3(kFixThreads), 0, c.stream, n,
+                       T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
+                       T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(), p.box_lo[0],
+                       p.box_lo[1], p.box_lo[2], scale, lim, T.fix.as<Fix7>(),
+                       T.fix_tiles.as<Fix7>());
+    hipLaunchKernelGGL(k_fix_tiles, dim3(1), dim3(kFixScanThreads), 0, c.stream, tiles,
+                       T.fix_tiles.as<Fix7>());
+    return OVG_OK;
+}
+
 // Walk the nodes starting at sorted point i; EMIT=false only counts them.
+// fixP != nullptr (tuned trees): a cell's Lm is the root's largest side
+// times 2^-level and its sums come from the prefix sums; otherwise Lm
+// replays the reference's child boxes and the sums run level by level
+// afterwards.  Leaves take their point either way.
 template <bool EMIT>
 __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const int8_t *__restrict__ delta, const int *__restrict__ counters,
@@ -492,7 +689,8 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const TangleParams *__restrict__ Pp, const double *__restrict__ sx,
                         const double *__restrict__ sy, const double *__restrict__ sz,
                         const double *__restrict__ qx, const double *__restrict__ qy,
-                        const double *__restrict__ qz)
+                        const double *__restrict__ qz, const Fix7 *__restrict__ fixP,
+                        const Fix7 *__restrict__ fixT, double inv_scale)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -521,6 +719,8 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
     int total = EMIT ? base[n_live] : 0;
     const TangleParams &P = *Pp;
     uint64_t ki = keys[i];
+    const double Lroot = fmax(fmax(P.hi[0] - P.lo[0], P.hi[1] - P.lo[1]), P.hi[2] - P.lo[2]);
+    const double plo[3] = {P.lo[0], P.lo[1], P.lo[2]};
     auto emit = [&](int level, int end, int count) {
         if (EMIT) {
             Node nd{};
@@ -528,14 +728,19 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
             int nxt = end + 1;
             nd.skip = nxt < n_in ? base[nxt] : total;
             if (count >= 2 && level >= 1) {
-                // replay the reference's child boxes down to `level`
-                double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
-                double hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
                 int lv = level < kMaxLevel ? level : kMaxLevel;
-                for (int l = 1; l <= lv; ++l)
-                    child_box_sel(lo, hi, (int)((ki >> (63 - 3 * l)) & 7));
-                double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
-                nd.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+                if (fixP) {
+                    nd.Lm = ldexp(Lroot, -lv);
+                    fix_node_sums(nd, fixP, fixT, i, i + count, plo, inv_scale);
+                } else {
+                    // replay the reference's child boxes down to `level`
+                    double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
+                    double hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
+                    for (int l = 1; l <= lv; ++l)
+                        child_box_sel(lo, hi, (int)((ki >> (63 - 3 * l)) & 7));
+                    double Lx = hi[0] - lo[0], Ly = hi[1] - lo[1], Lz = hi[2] - lo[2];
+                    nd.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+                }
             }
             if (count == 1) {  // a leaf's sums are its point (octree_update_means)
                 nd.cx = sx[i]; nd.cy = sy[i]; nd.cz = sz[i];
@@ -558,6 +763,7 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
             r.skip = total;
             double Lx = P.hi[0] - P.lo[0], Ly = P.hi[1] - P.lo[1], Lz = P.hi[2] - P.lo[2];
             r.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
+            if (fixP) fix_node_sums(r, fixP, fixT, 0, n_live, plo, inv_scale);
             nodes[d] = r;
             NodeMeta m{};
             m.start = 0; m.level = 0;
@@ -734,12 +940,18 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
                        T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
                        T.delta.as<int8_t>(), T.rank.as<int>());
+    // tuned trees (every point in the box): sums from prefix sums as the
+    // nodes are written
+    double inv_scale = 0;
+    if (clamp) OVG_TRY(fix_prefix(c, n, inv_scale));
+    const Fix7 *fixP = clamp ? T.fix.as<Fix7>() : nullptr;
+    const Fix7 *fixT = clamp ? T.fix_tiles.as<Fix7>() : nullptr;
     hipLaunchKernelGGL(k_nodes<false>, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>());
+                       T.sqz.as<double>(), fixP, fixT, inv_scale);
     // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
     launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n, T.base.as<int>() + n,
                           T.scan_tmp);
@@ -748,9 +960,11 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>());
+                       T.sqz.as<double>(), fixP, fixT, inv_scale);
     hipLaunchKernelGGL(k_tree_counters, dim3(1), dim3(1), 0, c.stream, n, T.base.as<int>(),
                        T.counters.as<int>());
+    if (clamp) return OVG_OK;
+    // reference tree: sums level by level in the reference's order
     int gn = (int)((cap_nodes + bs - 1) / bs);
     int *hist = T.level_hist.as<int>(), *list = T.level_list.as<int>();
     hipLaunchKernelGGL(k_level_count, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
@@ -770,26 +984,40 @@ int build_tree(Ctx &c, const double *const x[3])
     return OVG_OK;
 }
 
-// leaves (count 1, not the root) take their point's position and segment
-__global__ void k_leaf_sums(const int *__restrict__ counters, Node *__restrict__ nodes,
-                            const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
-                            const double *__restrict__ sy, const double *__restrict__ sz,
-                            const double *__restrict__ qx, const double *__restrict__ qy,
-                            const double *__restrict__ qz)
+// Refit (tuned trees): every node's sums from the prefix sums at the new
+// positions; leaves (count 1, not the root) take their point.
+__global__ void k_refit_nodes(const int *__restrict__ counters, Node *__restrict__ nodes,
+                              const NodeMeta *__restrict__ meta, const double *__restrict__ sx,
+                              const double *__restrict__ sy, const double *__restrict__ sz,
+                              const double *__restrict__ qx, const double *__restrict__ qy,
+                              const double *__restrict__ qz, const Fix7 *__restrict__ fixP,
+                              const Fix7 *__restrict__ fixT, const TangleParams *__restrict__ Pp,
+                              double inv_scale)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d == 0 || d >= counters[1] || nodes[d].count != 1) return;
-    const int j = meta[d].start;
+    if (d >= counters[1]) return;
     Node &nd = nodes[d];
-    nd.cx = sx[j]; nd.cy = sy[j]; nd.cz = sz[j];
-    nd.qx = qx[j]; nd.qy = qy[j]; nd.qz = qz[j];
+    const int j = meta[d].start, cnt = nd.count;
+    if (d != 0 && cnt == 1) {
+        nd.cx = sx[j]; nd.cy = sy[j]; nd.cz = sz[j];
+        nd.qx = qx[j]; nd.qy = qy[j]; nd.qz = qz[j];
+    } else {
+        const TangleParams &P = *Pp;
+        const double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
+        fix_node_sums(nd, fixP, fixT, j, j + cnt, lo, inv_scale);
+    }
 }
 
+// Same tree (keys, order, nodes, cell sizes) at new positions x: positions,
+// segment vectors and node sums recomputed.  Tuned trees only (every point
+// in the box, so the root's range is all sorted points).
 int refit_tree(Ctx &c, const double *const x[3])
 {
     TreeBufs &T = c.tree;
     const int n = c.n_live;
     if (n == 0) return OVG_OK;
+    if (c.params.traverse == OVG_TRAVERSE_POINT)
+        return set_error(OVG_EINVAL, "refit_tree: the reference tree is rebuilt, not refitted");
     const int bs = 256, gs = (n + bs - 1) / bs;
     const size_t cap_nodes = 2 * (size_t)n + 2;
     const int gn = (int)((cap_nodes + bs - 1) / bs);
@@ -799,20 +1027,13 @@ int refit_tree(Ctx &c, const double *const x[3])
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
                        T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
                        T.delta.as<int8_t>(), T.rank.as<int>());
-    hipLaunchKernelGGL(k_leaf_sums, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
+    double inv_scale = 0;
+    OVG_TRY(fix_prefix(c, n, inv_scale));
+    hipLaunchKernelGGL(k_refit_nodes, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
                        T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                        T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
-                       T.sqy.as<double>(), T.sqz.as<double>());
-    const int gl = std::min(gn, 1024);
-    int *hist = T.level_hist.as<int>(), *list = T.level_list.as<int>();
-    for (int level = kMaxLevel + 1; level >= 0; --level)
-        hipLaunchKernelGGL(k_sums_level, dim3(gl), dim3(bs), 0, c.stream, level, gn,
-                           T.counters.as<int>(), hist, list, T.nodes.as<Node>(),
-                           T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
-                           T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                           T.sqz.as<double>(), n);
-    hipLaunchKernelGGL(k_finish_means, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
-                       T.nodes.as<Node>());
+                       T.sqy.as<double>(), T.sqz.as<double>(), T.fix.as<Fix7>(),
+                       T.fix_tiles.as<Fix7>(), dparams(c), inv_scale);
     return OVG_OK;
 }
 
