@@ -136,19 +136,30 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v)  // bit b -> bit 3b
     return v;
 }
 
+// Tuned keys resolve kTunedLevels levels (cells of 2^-16 of the box, 3e-6
+// cm on a 0.2 cm box): the digits of deeper levels are zero, so the radix
+// sort skips their bits, and points sharing a deepest cell sit in it as
+// single-point leaves (that cell's Lm is its true size, so acceptance is
+// never coarser than with deeper cells).
+constexpr int kTunedLevels = 16;
+constexpr int kTunedLowBit = 63 - 3 * kTunedLevels;  // lowest key bit in use
+
 __device__ __forceinline__ uint64_t morton_key(const TangleParams &P, double x, double y, double z)
 {
     const double p3[3] = {x, y, z};
+    constexpr double cells = (double)(1 << kTunedLevels);
     uint64_t q[3];
     for (int a = 0; a < 3; ++a) {
-        double u = (p3[a] - P.lo[a]) / P.L[a] * 2097152.0;
-        u = fmin(fmax(u, 0.0), 2097151.0);
+        double u = (p3[a] - P.lo[a]) / P.L[a] * cells;
+        u = fmin(fmax(u, 0.0), cells - 1.0);
         q[a] = (uint64_t)u;
     }
-    q[0] = ~q[0] & 0x1fffff;
-    // level l's digit comes from coordinate bit b = 21 - l and sits at key
-    // bits 63 - 3l = 3b (x), 3b + 1 (y), 3b + 2 (z)
-    return spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+    q[0] = ~q[0] & ((1u << kTunedLevels) - 1);
+    // level l's digit comes from coordinate bit b = 21 - l (of a 21-bit
+    // coordinate, here q << 5) and sits at key bits 63 - 3l = 3b (x),
+    // 3b + 1 (y), 3b + 2 (z)
+    constexpr int sh = kMaxLevel - kTunedLevels;
+    return spread3(q[0] << sh) | spread3(q[1] << sh) << 1 | spread3(q[2] << sh) << 2;
 }
 
 // clamp != 0 (tuned modes): a point outside the root box is placed by its
@@ -364,47 +375,83 @@ __global__ void k_rs_hist(const uint64_t *__restrict__ keys, int n, int shift,
     hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// One tile of kRsTile keys per block, in input order (round, wave, lane):
+// ranks among equal digits from 64-lane ballots, per-(round, wave) digit
+// counts in LDS scanned per digit, then the tile is laid out by digit in LDS
+// and written with consecutive threads on consecutive output positions of
+// each digit's run.  Stable.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
     int *__restrict__ vout, int n, int shift, const int *__restrict__ offs, int nblocks)
 {
-    __shared__ int running[256];
-    __shared__ int wcnt[kRsThreads / 64][256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    running[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
-    for (int q = 0; q < kRsThreads / 64; ++q) wcnt[q][threadIdx.x] = 0;
+    constexpr int kSlots = kRsRounds * (kRsThreads / 64);
+    __shared__ unsigned short cnt[kSlots][256];  // counts, then exclusive prefix per digit
+    __shared__ int dstart[256];                  // tile-local start of each digit
+    __shared__ int gbase[256];                   // this block's output offset per digit
+    __shared__ int wsum[kRsThreads / 64];
+    __shared__ uint64_t skey[kRsTile];
+    __shared__ int sval[kRsTile];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int j = t; j < kSlots * 256; j += kRsThreads) (&cnt[0][0])[j] = 0;
+    gbase[t] = offs[t * nblocks + blockIdx.x];
     __syncthreads();
     const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-    int base = blockIdx.x * kRsTile;
+    const int base = blockIdx.x * kRsTile;
+    uint64_t key[kRsRounds];
+    int val[kRsRounds], dig[kRsRounds], rk[kRsRounds];
+#pragma unroll
     for (int r = 0; r < kRsRounds; ++r) {
-        int i = base + r * kRsThreads + threadIdx.x;
-        bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        int val = valid ? vin[i] : 0;
-        int digit = valid ? (int)((key >> shift) & 255) : 0;
+        const int i = base + r * kRsThreads + t;
+        const bool valid = i < n;
+        key[r] = valid ? kin[i] : 0;
+        val[r] = valid ? vin[i] : 0;
+        dig[r] = valid ? (int)((key[r] >> shift) & 255) : 0;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
-            uint64_t m = __ballot((digit >> b) & 1);
-            peers &= ((digit >> b) & 1) ? m : ~m;
+            const uint64_t m = __ballot((dig[r] >> b) & 1);
+            peers &= ((dig[r] >> b) & 1) ? m : ~m;
         }
-        int rank = __popcll(peers & below);
-        int leader = __ffsll((unsigned long long)peers) - 1;
-        if (valid && lane == leader) wcnt[w][digit] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = running[digit] + rank;
-            for (int q = 0; q < w; ++q) pos += wcnt[q][digit];
-            kout[pos] = key;
-            vout[pos] = val;
+        rk[r] = __popcll(peers & below);
+        const int leader = __ffsll((unsigned long long)peers) - 1;
+        if (valid && lane == leader) cnt[r * (kRsThreads / 64) + w][dig[r]] = __popcll(peers);
+    }
+    __syncthreads();
+    // thread t = digit t: exclusive prefix over the slots, then over digits
+    int tot = 0;
+    for (int q = 0; q < kSlots; ++q) {
+        const int c = cnt[q][t];
+        cnt[q][t] = (unsigned short)tot;
+        tot += c;
+    }
+    int incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int pre = 0;
+    for (int q = 0; q < w; ++q) pre += wsum[q];
+    dstart[t] = pre + incl - tot;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        const int i = base + r * kRsThreads + t;
+        if (i < n) {
+            const int d = dig[r];
+            const int lp = dstart[d] + cnt[r * (kRsThreads / 64) + w][d] + rk[r];
+            skey[lp] = key[r];
+            sval[lp] = val[r];
         }
-        __syncthreads();
-        int add = 0;
-        for (int q = 0; q < kRsThreads / 64; ++q) {
-            add += wcnt[q][threadIdx.x];
-            wcnt[q][threadIdx.x] = 0;
-        }
-        running[threadIdx.x] += add;
-        __syncthreads();
+    }
+    __syncthreads();
+    const int nvalid = min(kRsTile, n - base);
+    for (int j = t; j < nvalid; j += kRsThreads) {
+        const uint64_t k = skey[j];
+        const int d = (int)((k >> shift) & 255);
+        const int pos = gbase[d] + j - dstart[d];
+        kout[pos] = k;
+        vout[pos] = sval[j];
     }
 }
 
@@ -435,9 +482,11 @@ void radix_sort(Ctx &c, int n, int first_shift, int last_shift)
 }
 
 // ----------------------------------------------------------- topology ----
-__device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b)
+// Common levels of two keys; equal keys share every level the keys resolve
+// (max_lv: kMaxLevel for the reference tree, kTunedLevels for tuned trees).
+__device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b, int max_lv)
 {
-    if (a == b) return kMaxLevel;
+    if (a == b) return max_lv;
     return (__clzll((long long)(a ^ b)) - 1) / 3;
 }
 
@@ -451,7 +500,7 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
                                 double *__restrict__ sx, double *__restrict__ sy,
                                 double *__restrict__ sz, double *__restrict__ qx,
                                 double *__restrict__ qy, double *__restrict__ qz,
-                                int8_t *__restrict__ delta, int *__restrict__ rank)
+                                int8_t *__restrict__ delta, int *__restrict__ rank, int max_lv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -470,7 +519,7 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
         ax = bx - x; ay = by - y; az = bz - z;
     }
     qx[i] = ax; qy[i] = ay; qz[i] = az;
-    if (i + 1 < n_in) delta[i] = (int8_t)lcp_levels(keys[i], keys[i + 1]);
+    if (i + 1 < n_in) delta[i] = (int8_t)lcp_levels(keys[i], keys[i + 1], max_lv);
 }
 
 // last j in [i+1, n_in-1] with lcp(K[i], K[j]) >= v (at least i+1
@@ -478,13 +527,13 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
 // (steps 1, 2, 4, .. then bisection of the last step) takes about
 // 2 log2(node size) key loads: most nodes hold a few points.
 __device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int i, int n_in,
-                                         int v)
+                                         int v, int max_lv)
 {
     const uint64_t ki = keys[i];
     int lo = i + 1, hi = n_in - 1;
     for (int step = 1; lo < hi; step <<= 1) {
         const int j = lo + step < hi ? lo + step : hi;
-        if (lcp_levels(ki, keys[j]) >= v) {
+        if (lcp_levels(ki, keys[j], max_lv) >= v) {
             lo = j;
         } else {
             hi = j - 1;
@@ -493,7 +542,7 @@ __device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int 
     }
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (lcp_levels(ki, keys[mid]) >= v) lo = mid;
+        if (lcp_levels(ki, keys[mid], max_lv) >= v) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -690,7 +739,7 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const double *__restrict__ sy, const double *__restrict__ sz,
                         const double *__restrict__ qx, const double *__restrict__ qy,
                         const double *__restrict__ qz, const Fix7 *__restrict__ fixP,
-                        const Fix7 *__restrict__ fixT, double inv_scale)
+                        const Fix7 *__restrict__ fixT, double inv_scale, int max_lv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -775,8 +824,8 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
     int v = dl + 1;
     if (v < 1) v = 1;
     while (v <= dr) {
-        int e = range_end(keys, i, n_in, v);
-        int w = lcp_levels(ki, keys[e]);
+        int e = range_end(keys, i, n_in, v, max_lv);
+        int w = lcp_levels(ki, keys[e], max_lv);
         emit(w, e, e - i + 1);
         v = w + 1;
     }
@@ -931,7 +980,10 @@ int build_tree(Ctx &c, const double *const x[3])
     hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
                        x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
                        T.counters.as<int>(), clamp);
-    radix_sort(c, n, 0, 56);
+    // tuned keys: bits kTunedLowBit .. 62 in 8-bit digits (6 passes)
+    const int max_lv = clamp ? kTunedLevels : kMaxLevel;
+    if (clamp) radix_sort(c, n, kTunedLowBit, 55);
+    else radix_sort(c, n, 0, 56);
     hipLaunchKernelGGL(k_count_inbox, dim3((n + 1 + bs - 1) / bs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.counters.as<int>());
     hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
@@ -939,7 +991,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
                        T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
-                       T.delta.as<int8_t>(), T.rank.as<int>());
+                       T.delta.as<int8_t>(), T.rank.as<int>(), max_lv);
     // tuned trees (every point in the box): sums from prefix sums as the
     // nodes are written
     double inv_scale = 0;
@@ -951,7 +1003,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>(), fixP, fixT, inv_scale);
+                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv);
     // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
     launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n, T.base.as<int>() + n,
                           T.scan_tmp);
@@ -960,7 +1012,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>(), fixP, fixT, inv_scale);
+                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv);
     hipLaunchKernelGGL(k_tree_counters, dim3(1), dim3(1), 0, c.stream, n, T.base.as<int>(),
                        T.counters.as<int>());
     if (clamp) return OVG_OK;
@@ -1026,7 +1078,7 @@ int refit_tree(Ctx &c, const double *const x[3])
                        c.fwd.as<int>(), dparams(c), T.counters.as<int>(), T.sorted_pts.as<int>(),
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
                        T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
-                       T.delta.as<int8_t>(), T.rank.as<int>());
+                       T.delta.as<int8_t>(), T.rank.as<int>(), kTunedLevels);
     double inv_scale = 0;
     OVG_TRY(fix_prefix(c, n, inv_scale));
     hipLaunchKernelGGL(k_refit_nodes, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
