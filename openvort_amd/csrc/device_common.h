@@ -148,6 +148,34 @@ __device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float 
 // t the target's offset from a common centre) R x Q = a x Q - t x Q: the
 // entry carries A = a x Q, the pair adds m A to sa and m Q to sq, and the
 // caller forms sa - t x sq and applies f_bs once per batch of entries.
+// seg_field32_far without the guards, for pairs where none can trigger:
+// the target lies outside the segment's Thales sphere (centre a + Q/2,
+// radius |Q|/2) by a margin, so 1 + R.Rp/(|R||Rp|) stays away from 0 and
+// |R|, |Rp| from 0 (the caller checks each entry against the whole target
+// box).  A zero entry placed far away (a = (1e15, 0, 0), Q = A = 0) adds
+// exactly zero.
+__device__ __forceinline__ void seg_field32_far_clear(float Rx, float Ry, float Rz, float Qx,
+                                                      float Qy, float Qz, float Ax, float Ay,
+                                                      float Az, float &sax, float &say,
+                                                      float &saz, float &sqx, float &sqy,
+                                                      float &sqz)
+{
+    const float Px = Rx + Qx, Py = Ry + Qy, Pz = Rz + Qz;
+    const float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
+    const float p2 = fmaf(Pz, Pz, fmaf(Py, Py, Px * Px));
+    const float dot = fmaf(Rz, Pz, fmaf(Ry, Py, Rx * Px));
+    const float ia = __builtin_amdgcn_rsqf(r2);
+    const float ib = __builtin_amdgcn_rsqf(p2);
+    const float iab = ia * ib;
+    const float m = (ia + ib) * iab * __builtin_amdgcn_rcpf(fmaf(dot, iab, 1.0f));
+    sax = fmaf(Ax, m, sax);
+    say = fmaf(Ay, m, say);
+    saz = fmaf(Az, m, saz);
+    sqx = fmaf(Qx, m, sqx);
+    sqy = fmaf(Qy, m, sqy);
+    sqz = fmaf(Qz, m, sqz);
+}
+
 __device__ __forceinline__ void seg_field32_far(float Rx, float Ry, float Rz, float Qx, float Qy,
                                                 float Qz, float Ax, float Ay, float Az,
                                                 float &sax, float &say, float &saz, float &sqx,

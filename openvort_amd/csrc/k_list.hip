@@ -143,6 +143,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     unsigned long long *__restrict__ icount)
 {
     static_assert(64 % PPW == 0 && 4 * 64 / PPW <= kListFar, "slices of a full group");
+    static_assert(kListFar <= 128, "the far flush checks two entries per lane");
     __shared__ int2 stk[kListStack];
     __shared__ float4 farA[kListFar];  // offset a from the copy's group centre, qx
     __shared__ float4 farB[kListFar];  // qy, qz, (a x q)_x, (a x q)_y
@@ -188,6 +189,11 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                  cz = uniform(0.5 * (loz + hiz));
     const float tx = on ? (float)(rx - cx) : 0.0f, ty = on ? (float)(ry - cy) : 0.0f,
                 tz = on ? (float)(rz - cz) : 0.0f;
+    // half extents of the target box about (cx, cy, cz), FP32, with slack
+    // for the rounding of the target offsets
+    const float bhx = (float)(0.5 * (hix - lox)) * 1.000001f,
+                bhy = (float)(0.5 * (hiy - loy)) * 1.000001f,
+                bhz = (float)(0.5 * (hiz - loz)) * 1.000001f;
     const int leaf = P.leaf_size;
     const int leaf_bits = 32 - __clz(leaf);  // bits of counts 0..leaf
     const double cs = P.crit_scale;
@@ -197,25 +203,34 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     // a copy's sources are shifted by +L*shift (its targets are r - L*shift)
     double ax = 0, ay = 0, az = 0;
     unsigned long long c64 = 0, c32 = 0;
-    int far_n = 0, near_n = 0;
+    int far_n = 0, far_x = 0, near_n = 0;  // far: clear entries from the front, cross from the back
     const int n_copies = any ? 1 + P.n_images : 0;
     const bool recorded = g < rec_groups;  // record / replay slot of this group
     const size_t rec_base = (size_t)g * rec_cap;
     int rec_n = 0;
     bool rec_over = !recorded;
 
+    // a zero far entry placed far away: adds exactly zero without guards
+    auto pad_far = [&](int i) {
+        farA[i] = make_float4(1e15f, 0.f, 0.f, 0.f);
+        farB[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        farC[i] = 0.f;
+    };
     auto flush_far = [&]() {
-        // pad to whole rounds of 4 S entries with zero entries, which add
-        // exactly zero (a zero offset from the centre is rejected by the
-        // guard, any other gives a finite m times zero vectors)
-        const int fpad = (far_n + 4 * S - 1) / (4 * S) * (4 * S);
-        for (int i = far_n + lane; i < fpad; i += 64) {
-            farA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            farB[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            farC[i] = 0.f;
-        }
-        __syncthreads();
         float sax = 0.f, say = 0.f, saz = 0.f, sqx = 0.f, sqy = 0.f, sqz = 0.f;
+        // cross entries (from the back of the list) with the reference's
+        // guards, before the padding may overwrite them
+        for (int j = slice; j < far_x; j += S) {
+            const int i = kListFar - 1 - j;
+            const float4 a = farA[i];
+            const float4 b = farB[i];
+            seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i], sax,
+                            say, saz, sqx, sqy, sqz);
+        }
+        // clear entries padded to whole rounds of 4 S
+        const int fpad = (far_n + 4 * S - 1) / (4 * S) * (4 * S);
+        for (int i = far_n + lane; i < fpad; i += 64) pad_far(i);
+        __syncthreads();
         // slice s takes entries 4s .. 4s + 3 of each round of 4 S
         for (int i0 = 4 * slice; i0 < fpad; i0 += 4 * S) {
 #pragma unroll
@@ -223,20 +238,22 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                 const int i = i0 + u;
                 const float4 a = farA[i];
                 const float4 b = farB[i];
-                seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i],
-                                sax, say, saz, sqx, sqy, sqz);
+                seg_field32_far_clear(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w,
+                                      farC[i], sax, say, saz, sqx, sqy, sqz);
             }
         }
         // sum of m (a - t) x q = sum m a x q - t x sum m q
         ax += (double)(sax - fmaf(ty, sqz, -(tz * sqy))) * f64;
         ay += (double)(say - fmaf(tz, sqx, -(tx * sqz))) * f64;
         az += (double)(saz - fmaf(tx, sqy, -(ty * sqx))) * f64;
-        if (on) {  // real entries of this slice: blocks of 4 with index = slice mod S
+        if (on) {  // pairs of this slice: clear blocks of 4 with index = slice mod S, cross
             const int nf = far_n >> 2, rem = far_n & 3;
             c32 += 4 * (nf > slice ? (nf - slice + S - 1) / S : 0) + (nf % S == slice ? rem : 0);
+            c32 += far_x > slice ? (far_x - slice + S - 1) / S : 0;
         }
         __syncthreads();
         far_n = 0;
+        far_x = 0;
     };
     auto flush_near = [&]() {
         __syncthreads();
@@ -268,26 +285,51 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         near_n = 0;
     };
     // append one entry per predicated lane: near as a packed index, far as
-    // FP32 offsets from the group centre
+    // FP32 offsets from the group centre.  A far entry (offset a, vector q)
+    // is classified against the target box by its Thales sphere (centre
+    // a + q/2, radius h = |q|/2), inside which R.Rp < 0:
+    //   clear  - the box stays outside by a margin (distance^2 >= 1.04 h^2
+    //            + 2e-10 cm^2): the reference's guards (|R|, |Rp| >= 1e-8,
+    //            R.Rp/|R||Rp| >= 1e-8) hold for every target, so these pairs
+    //            run without them;
+    //   inside - the whole box lies inside (farthest corner^2 <= 0.96 h^2):
+    //            every pair fails the last guard and adds 0, as in the
+    //            reference; not stored;
+    //   cross  - the rest (about 2% on dense tangles: long monopoles),
+    //            evaluated with the guards.
     auto emit = [&](bool want, bool near, unsigned idx, float ex, float ey, float ez,
                     float gx, float gy, float gz) {
-        const unsigned long long mf = __ballot(want && !near);
+        bool clear = true, cross = false;
+        if (MIXED) {
+            const float mx = fabsf(fmaf(0.5f, gx, ex)), my = fabsf(fmaf(0.5f, gy, ey)),
+                        mz = fabsf(fmaf(0.5f, gz, ez));
+            const float dx = fmaxf(mx - bhx, 0.f), dy = fmaxf(my - bhy, 0.f),
+                        dz = fmaxf(mz - bhz, 0.f);
+            const float fx = mx + bhx, fy = my + bhy, fz = mz + bhz;
+            const float q2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+            clear = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) >= fmaf(0.26f, q2, 2e-10f);
+            cross = !clear && !(fmaf(fz, fz, fmaf(fy, fy, fx * fx)) <= 0.24f * q2);
+        }
+        const bool wf = want && !near;
+        const unsigned long long mf = __ballot(wf && clear);
+        const unsigned long long mx = __ballot(wf && cross);
         const unsigned long long mn = __ballot(want && near);
-        const int cf = __popcll(mf), cn = __popcll(mn);
-        if (cf && far_n + cf > far_cap) flush_far();
+        const int cf = __popcll(mf), cx = __popcll(mx), cn = __popcll(mn);
+        if ((cf | cx) && far_n + far_x + cf + cx > far_cap) flush_far();
         if (cn && near_n + cn > kListNear) flush_near();
-        if (want && !near) {
-            const int pos = far_n + prefix_below(mf);
+        if (wf && (clear || cross)) {
+            const int pos = clear ? far_n + prefix_below(mf) : kListFar - 1 - far_x - prefix_below(mx);
             farA[pos] = make_float4(ex, ey, ez, gx);
             farB[pos] = make_float4(gy, gz, fmaf(ey, gz, -(ez * gy)), fmaf(ez, gx, -(ex * gz)));
             farC[pos] = fmaf(ex, gy, -(ey * gx));
         }
         if (want && near) nearI[near_n + prefix_below(mn)] = idx;
         far_n += cf;
+        far_x += cx;
         near_n += cn;
-        if (MODE == kRecord) {
-            const unsigned long long mw = mf | mn;
-            const int cw = cf + cn;
+        if (MODE == kRecord) {  // every accepted entry, whatever its class
+            const unsigned long long mw = __ballot(want);
+            const int cw = __popcll(mw);
             if (rec_n + cw <= rec_cap) {
                 if (want) rec[rec_base + rec_n + prefix_below(mw)] = idx;
             } else {
@@ -428,7 +470,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     }
     }  // walk
     if (MODE == kRecord && recorded && lane == 0) rec_count[g] = rec_over ? -1 : rec_n;
-    if (far_n) flush_far();
+    if (far_n | far_x) flush_far();
     if (near_n) flush_near();
     // slice partials in slice order
     for (int s = 1; s < S; ++s) {
