@@ -29,7 +29,7 @@
 namespace ovg {
 
 constexpr int kSmBlock = 256;
-constexpr int kSmallLoop = 1024;     // DFT of the kept band, 256 threads per loop
+constexpr int kSmallLoop = 256;      // DFT of the kept band, 256 threads per loop
 
 // ---- shared pieces -----------------------------------------------------------
 
@@ -258,8 +258,9 @@ constexpr int kTile = 1 << kTileLog;
 constexpr int kChunk = 1024;       // points per block in the unwrap and the write-back
 constexpr int kPassBlock = 256;    // kTile/4 radix-4 butterflies per block, 4 per thread
 constexpr int kTileBlock = 512;
-// n > kSmallLoop gives 2N-1 > 2048, so M >= kTile: every signal is whole tiles
-static_assert(2 * (kSmallLoop + 1) - 1 > kTile / 2, "transforms shorter than a tile");
+// The convolution length is at least kTile (any M >= 2N-1 gives the same
+// cyclic convolution result), so every signal is whole tiles and a loop of up
+// to 1024 points is one LDS tile pass.
 
 struct BsLoop {
     int loop;         // index into the loop table
@@ -465,7 +466,7 @@ __global__ void k_bs_setup(int nb, const BsLoop *__restrict__ L, const double *_
     s.N = s.span ? 2 * n - 2 : n;
     s.kr = s.ki = 0;
     s.filt = kept_band(s.N, fcut, s.kr, s.ki);
-    int logM = 0;
+    int logM = kTileLog;
     while ((1 << logM) < 2 * s.N - 1) ++logM;
     s.logM = logM;
     S[b] = s;
@@ -730,7 +731,7 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
         const int lp = big[i];
         const int n = c.h_loop_off[lp + 1] - c.h_loop_off[lp];
         const long long Nmax = 2LL * n - 2;
-        int logM = 0;
+        int logM = kTileLog;
         while ((1LL << logM) < 2 * Nmax - 1) ++logM;
         if (logM > 30) return set_error(OVG_ELIMIT, "a loop of %d points is too long to smooth", n);
         BsLoop &B = L[i];

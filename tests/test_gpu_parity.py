@@ -366,7 +366,8 @@ def test_smooth_fft_path_and_spanning_loops():
     rng = np.random.default_rng(11)
     pieces = []
     # a 3000-point ring (FFT path), a 2000-point spanning line (FFT path,
-    # odd extension), a 300-point spanning line and a small ring (DFT path)
+    # odd extension), a 300-point spanning line (FFT path, one LDS tile) and
+    # a small ring (DFT path)
     R = 0.03
     ph = 2 * np.pi * np.arange(3000) / 3000
     ring = np.stack([0.05 + R * np.cos(ph), 0.05 + R * np.sin(ph), np.full(3000, 0.03)], 1)
@@ -532,13 +533,14 @@ def test_smooth_many_long_loops_scattered_slots():
     """Thirty loops longer than 1024 points in one batched transform: rings
     and box-spanning lines of assorted lengths (1025 up, transform lengths
     4096 to 65536), points scattered over the slots in random order with
-    empty slots between them, and a few short loops beside them."""
+    empty slots between them, and shorter loops beside them on both sides
+    of the 256-point split between the kept-band DFT and the transform."""
     from oracle import smooth as S
     D = 0.1
     rng = np.random.default_rng(21)
     sizes = [1025, 1026, 2047, 2049, 4000, 1500, 12000, 3333, 1100, 9000] + \
         list(rng.integers(1025, 14000, size=20))
-    sizes += [40, 200, 1024]
+    sizes += [40, 200, 255, 256, 257, 258, 600, 1024]
     pieces = []
     for i, m in enumerate(sizes):
         if i % 3 == 1:
