@@ -216,7 +216,12 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         farB[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         farC[i] = 0.f;
     };
-    auto flush_far = [&]() {
+    // flush_far(false) evaluates whole rounds of 4 S clear entries and keeps
+    // the remainder (< 4 S) at the front of the list for the next flush,
+    // when the remainder and one more round of emission fit (S <= 16); the
+    // group's last flush, flush_far(true), pads to whole rounds instead.
+    const bool keep_rem = far_cap - (4 * S - 1) >= 64;
+    auto flush_far = [&](bool last) {
         float sax = 0.f, say = 0.f, saz = 0.f, sqx = 0.f, sqy = 0.f, sqz = 0.f;
         // cross entries (from the back of the list) with the reference's
         // guards, before the padding may overwrite them
@@ -227,8 +232,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i], sax,
                             say, saz, sqx, sqy, sqz);
         }
-        // clear entries padded to whole rounds of 4 S
-        const int fpad = (far_n + 4 * S - 1) / (4 * S) * (4 * S);
+        // clear entries: whole rounds of 4 S, the last flush padded
+        const bool pad = last || !keep_rem;
+        const int fpad = pad ? (far_n + 4 * S - 1) / (4 * S) * (4 * S) : far_n / (4 * S) * (4 * S);
+        const int nreal = pad ? far_n : fpad;  // real entries evaluated now
         for (int i = far_n + lane; i < fpad; i += 64) pad_far(i);
         __syncthreads();
         // slice s takes entries 4s .. 4s + 3 of each round of 4 S
@@ -247,12 +254,26 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         ay += (double)(say - fmaf(tz, sqx, -(tx * sqz))) * f64;
         az += (double)(saz - fmaf(tx, sqy, -(ty * sqx))) * f64;
         if (on) {  // pairs of this slice: clear blocks of 4 with index = slice mod S, cross
-            const int nf = far_n >> 2, rem = far_n & 3;
+            const int nf = nreal >> 2, rem = nreal & 3;
             c32 += 4 * (nf > slice ? (nf - slice + S - 1) / S : 0) + (nf % S == slice ? rem : 0);
             c32 += far_x > slice ? (far_x - slice + S - 1) / S : 0;
         }
         __syncthreads();
-        far_n = 0;
+        // the remainder [fpad, far_n) moves to the front (disjoint ranges:
+        // fpad >= 4 S > remainder whenever fpad > 0)
+        const int keep = far_n - nreal;
+        if (keep > 0 && fpad > 0) {
+            if (lane < keep) {
+                const float4 a = farA[fpad + lane];
+                const float4 b = farB[fpad + lane];
+                const float cc = farC[fpad + lane];
+                farA[lane] = a;
+                farB[lane] = b;
+                farC[lane] = cc;
+            }
+            __syncthreads();
+        }
+        far_n = keep;
         far_x = 0;
     };
     auto flush_near = [&]() {
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         const unsigned long long mx = __ballot(wf && cross);
         const unsigned long long mn = __ballot(want && near);
         const int cf = __popcll(mf), cx = __popcll(mx), cn = __popcll(mn);
-        if ((cf | cx) && far_n + far_x + cf + cx > far_cap) flush_far();
+        if ((cf | cx) && far_n + far_x + cf + cx > far_cap) flush_far(false);
         if (cn && near_n + cn > kListNear) flush_near();
         if (wf && (clear || cross)) {
             const int pos = clear ? far_n + prefix_below(mf) : kListFar - 1 - far_x - prefix_below(mx);
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     }
     }  // walk
     if (MODE == kRecord && recorded && lane == 0) rec_count[g] = rec_over ? -1 : rec_n;
-    if (far_n | far_x) flush_far();
+    if (far_n | far_x) flush_far(true);
     if (near_n) flush_near();
     // slice partials in slice order
     for (int s = 1; s < S; ++s) {
