@@ -360,6 +360,22 @@ def main():
                    "seconds": r["seconds"]}
             log(f"cpu: {cpu['value']:.4g} point-steps/s on {r['points']} points")
 
+    # HBM bytes per launch of the traversal: measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated) for this
+    # workload and setting, committed beside the kernel-trace summary
+    traffic, traffic_src = None, None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "k_bh_list_mixed_c3_traffic.json")
+    cfg_now = {"workload": args.workload,
+               "points_per_wave": tune["points_per_wave"] if tune else args.ppw,
+               "leaf_size": tune["leaf_size"] if tune else args.leaf,
+               "group_level": tune["group_level"] if tune else args.group_level}
+    if (os.path.exists(tfile) and trav == ov.TRAVERSE_LIST and prec == ov.PREC_MIXED):
+        with open(tfile) as f:
+            tj = json.load(f)
+        if all(tj.get(k) == v for k, v in cfg_now.items()):
+            traffic, traffic_src = tj["traffic_bytes_per_launch"], tj["source"]
+
     out = {
         "metric": METRIC, "value": value, "unit": "point-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
@@ -382,7 +398,8 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "reuse_lists": reuse,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
                      "kernel": {ov.TRAVERSE_LIST: "k_bh_list", ov.TRAVERSE_GROUP: "k_bh_group",
                                 ov.TRAVERSE_POINT: "k_bh_point"}[trav]
                      + ("<mixed>" if prec == ov.PREC_MIXED else "<fp64>"),
