@@ -360,6 +360,35 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     std::vector<int> &conn = c.h_conn;
     std::vector<int> &flag = c.rc_flag;  // all 0 between passes
     if ((int)flag.size() < n) flag.assign((size_t)n, 0);
+    // The pass reaches each endpoint's links, flag and five-point stencil
+    // (two links either way) through chains of dependent loads into the
+    // position and link mirrors, tens of MB that are not in cache.  Touch
+    // them first in three rounds of independent prefetches so that the
+    // misses overlap; this changes no result.
+    {
+        const int *cn = conn.data();
+        auto pf_x = [&](int q) {
+            __builtin_prefetch(X + 3 * (size_t)q);
+            __builtin_prefetch(X + 3 * (size_t)q + 2);
+        };
+        for (int e : ends) {
+            __builtin_prefetch(cn + 2 * (size_t)e);
+            __builtin_prefetch(flag.data() + e);
+            pf_x(e);
+        }
+        for (int e : ends)
+            for (int q : {cn[2 * (size_t)e], cn[2 * (size_t)e + 1]})
+                if (q >= 0) {
+                    __builtin_prefetch(cn + 2 * (size_t)q);
+                    __builtin_prefetch(flag.data() + q);
+                    pf_x(q);
+                }
+        for (int e : ends) {
+            const int f = cn[2 * (size_t)e], r = cn[2 * (size_t)e + 1];
+            if (f >= 0 && cn[2 * (size_t)f] >= 0) pf_x(cn[2 * (size_t)f]);
+            if (r >= 0 && cn[2 * (size_t)r + 1] >= 0) pf_x(cn[2 * (size_t)r + 1]);
+        }
+    }
     std::vector<int> flagged;
     struct Change {
         int seg, old_f, new_f;
