@@ -43,13 +43,15 @@ def log(msg):
 
 
 def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
-    """Run n_steps device-resident steps in the order of src/main.c:127-138;
-    returns per-step seconds."""
-    lat = []
+    """Run n_steps device-resident steps in the order of src/main.c:127-138
+    without synchronising between them (the host queues a step while the GPU
+    finishes the previous one, as a driver loop would); returns per-step
+    seconds: device time between the completions of consecutive steps, from
+    stream markers (the first from a marker queued before it)."""
     tm = t0
+    e.marker_record(0)
     for s in range(n_steps):
         k = step0 + s
-        a = time.perf_counter()
         if reconnect:
             e.reconnect(tm)
             e.eliminate_small_loops(5)
@@ -62,9 +64,9 @@ def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
         e.update_tangle(tm)
         e.rk4_step(tm, dt)
         e.enforce_periodic()
-        e.synchronize()
-        lat.append(time.perf_counter() - a)
+        e.marker_record(s + 1)
         tm += dt
+    lat = [e.marker_elapsed(s, s + 1) * 1e-3 for s in range(n_steps)]
     return lat, tm
 
 

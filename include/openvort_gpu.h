@@ -311,6 +311,19 @@ int ovg_set_timing(ovg_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int ovg_synchronize(ovg_ctx *ctx);
 
+/* ---- stream markers (no counterpart in the reference) ----------------- */
+/*
+ * Time a caller-defined sequence of calls (a time step, say) without
+ * synchronising after each one: ovg_marker_record queues a marker with id
+ * `id` (0 <= id < OVG_MAX_MARKERS; recording an id again moves it) on the
+ * context's stream behind the work already queued; ovg_marker_elapsed waits
+ * for marker id1 and returns the device time in milliseconds between the
+ * two markers (both recorded, id0 before id1).
+ */
+#define OVG_MAX_MARKERS 65536
+int ovg_marker_record(ovg_ctx *ctx, int id);
+int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms);
+
 /* ---- autotuning (no counterpart in the reference) --------------------- */
 /*
  * Time the list traversal (OVG_TRAVERSE_LIST) on the context's current

@@ -128,6 +128,8 @@ _sig("ovg_line_length", C.c_int, C.c_void_p, _dp, _ip)
 _sig("ovg_get_stats", C.c_int, C.c_void_p, _P(Stats))
 _sig("ovg_set_timing", C.c_int, C.c_void_p, C.c_int)
 _sig("ovg_synchronize", C.c_int, C.c_void_p)
+_sig("ovg_marker_record", C.c_int, C.c_void_p, C.c_int)
+_sig("ovg_marker_elapsed", C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double))
 _sig("ovg_generate_rings", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double,
      C.c_double, C.c_int, _dp, _ip, _ip)
 _sig("ovg_rings_for_points", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
@@ -378,6 +380,17 @@ class Engine:
 
     def synchronize(self):
         _check(_lib.ovg_synchronize(self._h), "ovg_synchronize")
+
+    def marker_record(self, mid):
+        """Queue stream marker `mid` behind the work queued so far."""
+        _check(_lib.ovg_marker_record(self._h, int(mid)), "ovg_marker_record")
+
+    def marker_elapsed(self, mid0, mid1):
+        """Device milliseconds between two recorded markers (waits for mid1)."""
+        ms = C.c_double()
+        _check(_lib.ovg_marker_elapsed(self._h, int(mid0), int(mid1), C.byref(ms)),
+               "ovg_marker_elapsed")
+        return ms.value
 
 
 def write_tangle_file(path, t, vels=None, tangents=None, normals=None, digits=15):

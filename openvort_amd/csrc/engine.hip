@@ -377,6 +377,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
         if (e) hipEventDestroy(e);
     for (auto e : c.ev_pending) hipEventDestroy(e);
     for (auto e : c.ev_pool) hipEventDestroy(e);
+    for (auto e : c.markers)
+        if (e) hipEventDestroy(e);
     if (c.xpin_from) hipEventDestroy(c.xpin_from);
     if (c.xpin_done) hipEventDestroy(c.xpin_done);
     if (c.copy_stream) hipStreamDestroy(c.copy_stream);
@@ -843,6 +845,36 @@ extern "C" int ovg_synchronize(ovg_ctx *ctx)
     hipSetDevice(c.device);
     OVG_HIP(hipStreamSynchronize(c.stream));
     return check_flags(c);
+}
+
+extern "C" int ovg_marker_record(ovg_ctx *ctx, int id)
+{
+    clear_error();
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    if (id < 0 || id >= OVG_MAX_MARKERS)
+        return set_error(OVG_EINVAL, "marker id %d outside [0, %d)", id, OVG_MAX_MARKERS);
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if ((int)c.markers.size() <= id) c.markers.resize((size_t)id + 1, nullptr);
+    if (!c.markers[id]) OVG_HIP(hipEventCreate(&c.markers[id]));
+    OVG_HIP(hipEventRecord(c.markers[id], c.stream));
+    return OVG_OK;
+}
+
+extern "C" int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms)
+{
+    clear_error();
+    if (!ctx || !ms) return set_error(OVG_EINVAL, "context or output is NULL");
+    Ctx &c = ctx->c;
+    for (int id : {id0, id1})
+        if (id < 0 || id >= (int)c.markers.size() || !c.markers[id])
+            return set_error(OVG_EINVAL, "marker %d was never recorded", id);
+    hipSetDevice(c.device);
+    OVG_HIP(hipEventSynchronize(c.markers[id1]));
+    float f = 0.f;
+    OVG_HIP(hipEventElapsedTime(&f, c.markers[id0], c.markers[id1]));
+    *ms = f;
+    return OVG_OK;
 }
 
 extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pairs)

@@ -158,6 +158,7 @@ struct Ctx {
     hipEvent_t ev[8] = {};
     // per-evaluation timing events, resolved when statistics are read so
     // that timing adds no synchronisation to the timed work
+    std::vector<hipEvent_t> markers;     // ovg_marker_record, created on first use
     std::vector<hipEvent_t> ev_pool;
     std::vector<hipEvent_t> ev_pending;  // triples: start, tree done, BS done
 };

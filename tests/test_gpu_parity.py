@@ -336,6 +336,34 @@ def test_deterministic_repeat():
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
+def test_stream_markers_time_queued_steps():
+    """Markers queued between unsynchronised steps give the device time of
+    each step: positive, adding up to the bracketing interval, and within
+    the host's wall time around them; bad ids are refused."""
+    import time
+    t, p, dt = configs.make("c2", rings=40, traverse=ov.TRAVERSE_LIST, leaf_size=1,
+                            precision=ov.PREC_MIXED)
+    say("stream markers around three queued steps")
+    with engine(t, p) as e:
+        e.synchronize()
+        a = time.perf_counter()
+        e.marker_record(0)
+        for s in range(3):
+            e.update_tangle(s * dt)
+            e.rk4_step(s * dt, dt)
+            e.marker_record(s + 1)
+        per = [e.marker_elapsed(s, s + 1) for s in range(3)]
+        wall = (time.perf_counter() - a) * 1e3
+        total = e.marker_elapsed(0, 3)
+        assert all(x > 0 for x in per)
+        assert abs(sum(per) - total) < 1e-2 * total + 1e-3
+        assert total <= wall * 1.01 + 0.05
+        with pytest.raises(ov.OvgError):
+            e.marker_elapsed(0, 999)
+        with pytest.raises(ov.OvgError):
+            e.marker_record(-1)
+
+
 def test_all_empty_tangle():
     say("all-EMPTY tangle")
     t, p, _ = configs.make("c1", rings=1)
