@@ -25,7 +25,7 @@
 #include <vector>
 
 #include "engine_internal.h"
-#include "host_phys.h"
+#include "host_recon.h"
 
 namespace ovg {
 
@@ -242,41 +242,6 @@ int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs)
     return OVG_OK;
 }
 
-}  // namespace ovg
-
-using namespace ovg;
-
-// do_reconnection, src/vortex_dynamics.c:128-175, on the host link mirror.
-// Returns 0 (no rewiring) or 1/2 for the two rewiring cases.
-static int do_reconnection(const TangleParams &P, const double *X, int *conn, int k, int l)
-{
-    int kf = conn[2 * k], kr = conn[2 * k + 1], lf = conn[2 * l], lr = conn[2 * l + 1];
-    auto plen = [&](int a, int b) {
-        double w[3] = {X[3 * (size_t)b], X[3 * (size_t)b + 1], X[3 * (size_t)b + 2]};
-        host::pwrap(P, X + 3 * (size_t)a, w);
-        return host::dist3(X + 3 * (size_t)a, w);
-    };
-    double l_kl = plen(k, l), l_kflr = plen(kf, lr), l_kkf = plen(k, kf), l_llr = plen(l, lr);
-    double l_krlf = plen(kr, lf), l_kkr = plen(k, kr), l_llf = plen(l, lf);
-    double cf1 = l_kl + l_kflr - l_kkf - l_llr;
-    double cf2 = l_kl + l_krlf - l_kkr - l_llf;
-    if (cf1 > 0 && cf2 > 0) return 0;
-    if (cf1 < cf2) {
-        conn[2 * k] = l;
-        conn[2 * l + 1] = k;
-        conn[2 * kf + 1] = lr;
-        conn[2 * lr] = kf;
-        return 1;
-    }
-    conn[2 * l] = k;
-    conn[2 * k + 1] = l;
-    conn[2 * lf + 1] = kr;
-    conn[2 * kr] = lf;
-    return 2;
-}
-
-namespace ovg {
-
 __global__ void k_gather3(int m, const int *__restrict__ idx, const double *__restrict__ a,
                           const double *__restrict__ b, const double *__restrict__ c,
                           double *__restrict__ out)
@@ -357,179 +322,12 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     const double *X = nullptr;
     OVG_TRY(host_positions(c, &X));
     // greedy pass in (k, l) order
-    std::vector<int> &conn = c.h_conn;
-    std::vector<int> &flag = c.rc_flag;  // all 0 between passes
-    if ((int)flag.size() < n) flag.assign((size_t)n, 0);
-    // The pass reaches each endpoint's links, flag and five-point stencil
-    // (two links either way) through chains of dependent loads into the
-    // position and link mirrors, tens of MB that are not in cache.  Touch
-    // them first in three rounds of independent prefetches so that the
-    // misses overlap; this changes no result.
-    {
-        const int *cn = conn.data();
-        auto pf_x = [&](int q) {
-            __builtin_prefetch(X + 3 * (size_t)q);
-            __builtin_prefetch(X + 3 * (size_t)q + 2);
-        };
-        for (int e : ends) {
-            __builtin_prefetch(cn + 2 * (size_t)e);
-            __builtin_prefetch(flag.data() + e);
-            pf_x(e);
-        }
-        for (int e : ends)
-            for (int q : {cn[2 * (size_t)e], cn[2 * (size_t)e + 1]})
-                if (q >= 0) {
-                    __builtin_prefetch(cn + 2 * (size_t)q);
-                    __builtin_prefetch(flag.data() + q);
-                    pf_x(q);
-                }
-        for (int e : ends) {
-            const int f = cn[2 * (size_t)e], r = cn[2 * (size_t)e + 1];
-            if (f >= 0 && cn[2 * (size_t)f] >= 0) pf_x(cn[2 * (size_t)f]);
-            if (r >= 0 && cn[2 * (size_t)r + 1] >= 0) pf_x(cn[2 * (size_t)r + 1]);
-        }
-    }
-    std::vector<int> flagged;
-    struct Change {
-        int seg, old_f, new_f;
-    };
-    std::vector<Change> changes;
-    const double cutoff = std::cos(c.params.rec_angle);
-    // Direct mode applies the change of every rewired segment (exact); tree
-    // mode only those within 25 rec_dist of the point: a rewiring moves a
-    // segment pair of extent ~rec_dist, whose field beyond that radius is
-    // below 1e-4 of the local one and far below the tree's own error.
-    const bool exact = c.params.rec_velocity != OVG_RECVEL_TREE;
-    const double rc2 = (25 * c.params.rec_dist) * (25 * c.params.rec_dist);
-    auto near = [&](int a, int b) {
-        double d2 = 0;
-        for (int q = 0; q < 3; ++q) {
-            double d = X[3 * (size_t)a + q] - X[3 * (size_t)b + q];
-            if (P.periodic[q]) d -= P.L[q] * std::nearbyint(d / P.L[q]);
-            d2 += d * d;
-        }
-        return d2 < rc2;
-    };
-    // Tree mode: the rewired segments indexed on a periodic cell grid of
-    // cell side >= the cutoff, so a query visits its 27 neighbour cells;
-    // the matches are applied in the order of `changes`, as a full scan
-    // with the same predicate would.
-    int gn[3] = {0, 0, 0};
-    bool use_grid = !exact;
-    for (int q = 0; q < 3 && use_grid; ++q) {
-        const double h = 25 * c.params.rec_dist;
-        gn[q] = h > 0 ? (int)std::min(128.0, std::floor(P.L[q] / h)) : 0;
-        use_grid = gn[q] >= 3 && P.periodic[q];
-    }
-    std::vector<int> ghead, gnext, gtouched, found;
-    auto gcell = [&](int slot) {
-        int ci[3];
-        for (int q = 0; q < 3; ++q) {
-            const double u = (X[3 * (size_t)slot + q] - P.lo[q]) / P.L[q];
-            int v = (int)std::floor(u * gn[q]) % gn[q];
-            ci[q] = v < 0 ? v + gn[q] : v;
-        }
-        return (ci[2] * gn[1] + ci[1]) * gn[0] + ci[0];
-    };
-    if (use_grid) ghead.assign((size_t)gn[0] * gn[1] * gn[2], -1);
-    auto index_change = [&](int idx) {
-        if (!use_grid) return;
-        const int cell = gcell(changes[idx].seg);
-        if (ghead[cell] < 0) gtouched.push_back(cell);
-        gnext.push_back(ghead[cell]);
-        ghead[cell] = idx;
-    };
-    auto velocity = [&](int p, double *tn, double *nr, double *vels) {
-        int e = where[p];
-        double bs[3];
-        for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
-        auto apply = [&](const Change &ch) {
-            double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
-            host::segment_all_images(P, X, ch.seg, ch.old_f, p, vo);
-            host::segment_all_images(P, X, ch.seg, ch.new_f, p, vn);
-            for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
-        };
-        // change of the segments rewired so far
-        if (use_grid) {
-            found.clear();
-            const int pc = gcell(p);
-            const int px = pc % gn[0], py = (pc / gn[0]) % gn[1], pz = pc / (gn[0] * gn[1]);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int cx = (px + dx + gn[0]) % gn[0], cy = (py + dy + gn[1]) % gn[1],
-                                  cz = (pz + dz + gn[2]) % gn[2];
-                        for (int i = ghead[(cz * gn[1] + cy) * gn[0] + cx]; i >= 0; i = gnext[i])
-                            if (near(changes[i].seg, p)) found.push_back(i);
-                    }
-            std::sort(found.begin(), found.end());
-            for (int i : found) apply(changes[i]);
-        } else {
-            for (const Change &ch : changes)
-                if (exact || near(ch.seg, p)) apply(ch);
-        }
-        double vs[3];
-        host::compose(P, X, conn.data(), p, tn, nr, bs, vs, vels);
-    };
-    int nrec = 0;
-    int i = 0;
-    while (i < total) {
-        int k = ck[i];
-        int iend = i;
-        while (iend < total && ck[iend] == k) ++iend;
-        if (flag[k]) { i = iend; continue; }
-        // k's links, tangent and velocity stay fixed until a rewiring, which
-        // ends this loop
-        double tk[3], nk[3], vk[3];
-        host::tangent_normal(P, X, conn.data(), k, tk, nk);
-        bool have_vk = false;
-        for (int j = i; j < iend; ++j) {
-            int l = cl[j];
-            if (flag[l]) continue;
-            double w[3] = {X[3 * (size_t)l], X[3 * (size_t)l + 1], X[3 * (size_t)l + 2]};
-            host::pwrap(P, X + 3 * (size_t)k, w);
-            double tl[3], nl[3];
-            host::tangent_normal(P, X, conn.data(), l, tl, nl);
-            if (host::len3(tk) < 0.5 || host::len3(tl) < 0.5) continue;
-            double calpha = host::dot3(tk, tl) / host::len3(tk) / host::len3(tl);
-            if (calpha > cutoff) continue;
-            double vl[3];
-            if (!have_vk) velocity(k, tk, nk, vk);
-            have_vk = true;
-            velocity(l, tl, nl, vl);
-            double dx[3], dv[3];
-            for (int a = 0; a < 3; ++a) dx[a] = X[3 * (size_t)k + a] - w[a];
-            for (int a = 0; a < 3; ++a) dv[a] = vk[a] - vl[a];
-            if (host::dot3(dx, dv) > 0) continue;
-            int knext = conn[2 * k], kprev = conn[2 * k + 1];
-            int lnext = conn[2 * l], lprev = conn[2 * l + 1];
-            int kind = do_reconnection(P, X, conn.data(), k, l);
-            if (!kind) continue;
-            for (int sl : {k, l, knext, kprev, lnext, lprev}) mark_dirty(c, sl);
-            if (kind == 1) {
-                changes.push_back({k, knext, l});
-                index_change((int)changes.size() - 1);
-                changes.push_back({lprev, l, knext});
-                index_change((int)changes.size() - 1);
-            } else {
-                changes.push_back({l, lnext, k});
-                index_change((int)changes.size() - 1);
-                changes.push_back({kprev, k, lnext});
-                index_change((int)changes.size() - 1);
-            }
-            for (int sl : {k, conn[2 * k], conn[2 * k + 1], knext, kprev, l, conn[2 * l],
-                           conn[2 * l + 1], lnext, lprev}) {
-                flag[sl]++;
-                flagged.push_back(sl);
-            }
-            ++nrec;
-            break;
-        }
-        i = iend;
-    }
+    const int nrec = host::greedy_pass(
+        P, X, c.h_conn.data(), n, ck, cl, ends, where, bsp.data(), bsi.data(),
+        c.params.rec_angle, c.params.rec_dist, c.params.rec_velocity != OVG_RECVEL_TREE, c.rc_flag,
+        [&](int sl) { mark_dirty(c, sl); });
     *n_rec = nrec;
     for (int i = 0; i < m; ++i) where[ends[i]] = -1;
-    for (int sl : flagged) flag[sl] = 0;
     if (nrec) OVG_TRY(commit_topology(c));  // the rewired links only
     return OVG_OK;
 }
