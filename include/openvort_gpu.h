@@ -311,6 +311,36 @@ int ovg_set_timing(ovg_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int ovg_synchronize(ovg_ctx *ctx);
 
+/* ---- loop injection (src/vortex_injection.c) --------------------------- */
+/*
+ * ovg_loop_points: add_loop's geometry (src/vortex_injection.c:44-103): a
+ * circle of radius r about `center` in the plane normal to `dir`, with
+ * (int)(4 pi r / (dl_max + dl_min)) points, returned in *n_points; the first
+ * min(cap, *n_points) go to xyz (AoS).  Host only, bit-identical to the
+ * reference's points.
+ * ovg_inject_loop_draw: inject_loop's schedule (:711-745) with the caller
+ * holding its state: rng is the 48-bit drand48 state in erand48 form
+ * (srand48(s) is {0x330E, s & 0xffff, (s >> 16) & 0xffff}) and
+ * *last_injection the time of the last injection (the reference's static,
+ * 0 at start).  Every call draws the direction (three draws); when
+ * t - *last_injection >= 1/frequency it also draws the centre (on the top
+ * face of the box) and the radius (0.05 to 0.25 box widths), sets
+ * *last_injection = t and *due = 1, else *due = 0.  Host only.
+ * ovg_add_loop: add_loop on a resident tangle: the points fill the next
+ * empty slots in the reference's order (the slots double first while fewer
+ * are empty than needed), linked in order and closed.  *n_points = points
+ * added.  Loops with no point are refused (OVG_EINVAL).
+ * Injection of straight line pairs (inject_line_pairs) pins the lines to
+ * walls, which this engine does not support.
+ */
+int ovg_loop_points(const double center[3], const double dir[3], double r, double dl_min,
+                    double dl_max, double *xyz, int cap, int *n_points);
+int ovg_inject_loop_draw(double t, double frequency, const double box_lo[3],
+                         const double box_hi[3], unsigned short rng[3], double *last_injection,
+                         double center[3], double dir[3], double *r, int *due);
+int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double dir[3], double r,
+                 double dl_min, double dl_max, int *n_points);
+
 /* ---- stream markers (no counterpart in the reference) ----------------- */
 /*
  * Time a caller-defined sequence of calls (a time step, say) without

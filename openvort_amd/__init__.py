@@ -123,6 +123,10 @@ _sig("ovg_reconnect", C.c_int, C.c_void_p, C.c_double, _ip)
 _sig("ovg_smooth", C.c_int, C.c_void_p)
 _sig("ovg_eliminate_small_loops", C.c_int, C.c_void_p, C.c_int, _ip)
 _sig("ovg_remesh", C.c_int, C.c_void_p, C.c_double, C.c_double, _ip, _ip)
+_sig("ovg_loop_points", C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip)
+_sig("ovg_inject_loop_draw", C.c_int, C.c_double, C.c_double, _dp, _dp, _P(C.c_ushort), _dp, _dp,
+     _dp, _dp, _ip)
+_sig("ovg_add_loop", C.c_int, C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _ip)
 _sig("ovg_velocity_at", C.c_int, C.c_void_p, C.c_int, _ip, C.c_double, _dp, _dp)
 _sig("ovg_line_length", C.c_int, C.c_void_p, _dp, _ip)
 _sig("ovg_get_stats", C.c_int, C.c_void_p, _P(Stats))
@@ -333,6 +337,17 @@ class Engine:
     def smooth(self):
         _check(_lib.ovg_smooth(self._h), "ovg_smooth")
 
+    def add_loop(self, center, direction, r, dl_min, dl_max):
+        """add_loop (src/vortex_injection.c:44-103) on the resident tangle;
+        returns the number of points added."""
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        n = C.c_int32(0)
+        _check(_lib.ovg_add_loop(self._h, _d(c), _d(d), r, dl_min, dl_max, C.byref(n)),
+               "ovg_add_loop")
+        self.n = _lib.ovg_num_slots(self._h)
+        return n.value
+
     def velocity_at(self, slots, t=0.0):
         slots = np.ascontiguousarray(slots, dtype=np.int32)
         vs = np.zeros((len(slots), 3))
@@ -391,6 +406,39 @@ class Engine:
         _check(_lib.ovg_marker_elapsed(self._h, int(mid0), int(mid1), C.byref(ms)),
                "ovg_marker_elapsed")
         return ms.value
+
+
+def loop_points(center, direction, r, dl_min, dl_max):
+    """add_loop's points (src/vortex_injection.c:44-103), (n, 3)."""
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    n = C.c_int32(0)
+    _check(_lib.ovg_loop_points(_d(c), _d(d), r, dl_min, dl_max, None, 0, C.byref(n)),
+           "ovg_loop_points")
+    out = np.zeros((max(n.value, 0), 3))
+    _check(_lib.ovg_loop_points(_d(c), _d(d), r, dl_min, dl_max, _d(out), len(out), C.byref(n)),
+           "ovg_loop_points")
+    return out
+
+
+def rand48_state(seed):
+    """The drand48 state after srand48(seed), in erand48 form."""
+    return np.array([0x330E, seed & 0xFFFF, (seed >> 16) & 0xFFFF], dtype=np.uint16)
+
+
+def inject_loop_draw(t, frequency, box_lo, box_hi, rng, last_injection):
+    """inject_loop's schedule (src/vortex_injection.c:711-745) with explicit
+    state: rng (uint16[3], advanced in place) and the last injection time.
+    Returns (due, center, direction, r, last_injection)."""
+    lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+    hi = np.ascontiguousarray(box_hi, dtype=np.float64)
+    assert rng.dtype == np.uint16 and rng.shape == (3,) and rng.flags.c_contiguous
+    last = C.c_double(last_injection)
+    c, d, r, due = np.zeros(3), np.zeros(3), C.c_double(0), C.c_int32(0)
+    _check(_lib.ovg_inject_loop_draw(t, frequency, _d(lo), _d(hi),
+                                     rng.ctypes.data_as(_P(C.c_ushort)), C.byref(last), _d(c),
+                                     _d(d), C.byref(r), C.byref(due)), "ovg_inject_loop_draw")
+    return bool(due.value), c, d, r.value, last.value
 
 
 def write_tangle_file(path, t, vels=None, tangents=None, normals=None, digits=15):

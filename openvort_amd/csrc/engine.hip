@@ -1018,25 +1018,38 @@ struct HostTangle {
             }
         return -1;
     }
-    // get_next_empty_node :241-253 with expand_tangle :68-101
+    // expand_tangle(tangle, 2 N), src/tangle.c:68-101: the new slots are
+    // EMPTY with no links, then next_empty = get_next_empty_node
+    void expand()
+    {
+        int old = N;
+        N = 2 * old;
+        if (!grown) {
+            Xv.assign(X, X + 3 * (size_t)old);
+            grown = true;
+        }
+        Xv.resize(3 * (size_t)N, 0.0);
+        X = Xv.data();
+        conn.resize(2 * (size_t)N, -1);
+        st.resize(2 * (size_t)N, -1);
+        next_empty = search();
+    }
+    // get_next_empty_node :241-253
     int next_empty_node()
     {
         int idx = search();
         while (idx < 0 || st[2 * (size_t)idx] != OVG_EMPTY) {
-            int old = N;
-            N = 2 * old;
-            if (!grown) {
-                Xv.assign(X, X + 3 * (size_t)old);
-                grown = true;
-            }
-            Xv.resize(3 * (size_t)N, 0.0);
-            X = Xv.data();
-            conn.resize(2 * (size_t)N, -1);
-            st.resize(2 * (size_t)N, -1);
-            next_empty = search();
+            expand();
             idx = search();
         }
         return idx;
+    }
+    // number_of_empty_nodes :128-133
+    int empty_count() const
+    {
+        int e = 0;
+        for (int k = 0; k < N; ++k) e += st[2 * (size_t)k] == OVG_EMPTY;
+        return e;
     }
 };
 
@@ -1187,6 +1200,64 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
         launch_wrap(c);  // enforce_periodic_boundaries after insertions (:1057-1060)
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
+    return OVG_OK;
+}
+
+// add_loop, src/vortex_injection.c:44-103: the loop's points (the geometry
+// of ovg_loop_points) in the tangle's next empty slots, linked in order and
+// closed; the tangle doubles first while it has fewer empty slots than
+// points (:50).  The new links and points reach the device in one commit.
+extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double dir[3], double r,
+                            double dl_min, double dl_max, int *n_points)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (n_points) *n_points = 0;
+    int npts = 0;
+    if (ovg_loop_points(center, dir, r, dl_min, dl_max, nullptr, 0, &npts) != OVG_OK)
+        return set_error(OVG_EINVAL, "add_loop: bad arguments");
+    if (npts < 1)
+        return set_error(OVG_EINVAL, "add_loop: radius %g gives %d points with dl %g..%g", r, npts,
+                         dl_min, dl_max);
+    std::vector<double> pts(3 * (size_t)npts);
+    ovg_loop_points(center, dir, r, dl_min, dl_max, pts.data(), npts, &npts);
+    const double *Xp = nullptr;
+    OVG_TRY(host_positions(c, &Xp));
+    HostTangle h(c.n, Xp, c.h_conn, c.h_status, c.next_empty);
+    while (h.empty_count() < npts) h.expand();
+    std::vector<int> slots;
+    int first = -1, last = -1, cur = -1;
+    for (int k = 0; k < npts; ++k) {  // :73-97
+        cur = h.next_empty_node();
+        if (first < 0) first = cur;
+        h.st[2 * (size_t)cur] = OVG_FREE;
+        for (int q = 0; q < 3; ++q) h.X[3 * (size_t)cur + q] = pts[3 * (size_t)k + q];
+        if (cur != first) {
+            h.conn[2 * (size_t)cur + 1] = last;
+            h.conn[2 * (size_t)last] = cur;
+        }
+        last = cur;
+        slots.push_back(cur);
+    }
+    h.conn[2 * (size_t)cur] = first;  // :100-101
+    h.conn[2 * (size_t)first + 1] = cur;
+    c.next_empty = h.next_empty;
+    if (h.grown) {
+        OVG_TRY(alloc_state(c, h.N));
+        c.n = h.N;
+        OVG_TRY(upload_positions(c, h.X));
+        OVG_TRY(upload_topology(c, c.h_conn.data(), c.h_status.data()));
+    } else {
+        for (int sl : slots) {
+            mark_dirty(c, sl);
+            c.new_pos.push_back(sl);
+            for (int q = 0; q < 3; ++q) c.new_xyz.push_back(h.X[3 * (size_t)sl + q]);
+        }
+        OVG_TRY(commit_topology(c));
+    }
+    if (n_points) *n_points = npts;
     return OVG_OK;
 }
 

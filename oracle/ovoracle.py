@@ -193,6 +193,9 @@ class RefLib:
         L.ref_load_tangle.restype = vp
         L.ref_get_status.argtypes = [vp, _ip]
         L.ref_time_phases.argtypes = [vp, C.c_double, C.c_double, _dp]
+        L.ref_add_loop.argtypes = [vp, _dp, _dp, C.c_double, C.c_double, C.c_double]
+        L.ref_inject_loops.argtypes = [vp, C.c_long, _dp, C.c_int, C.c_double, C.c_double,
+                                       C.c_double]
 
     def image_table(self, n):
         out = np.zeros((26, 3), dtype=np.int32)
@@ -255,6 +258,19 @@ class RefTangle:
 
     def remesh(self, min_dist, max_dist):
         self.ref.lib.ref_remesh(self.h, min_dist, max_dist)
+        self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def add_loop(self, center, direction, r, dl_min, dl_max):
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        self.ref.lib.ref_add_loop(self.h, _d(c), _d(d), r, dl_min, dl_max)
+        self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def inject_loops(self, seed, times, frequency, dl_min, dl_max):
+        """srand48(seed), then inject_loop at each time (its static last
+        injection time persists for the life of the library)."""
+        ts = np.ascontiguousarray(times, dtype=np.float64)
+        self.ref.lib.ref_inject_loops(self.h, seed, _d(ts), len(ts), frequency, dl_min, dl_max)
         self.n = self.ref.lib.ref_num_slots(self.h)
 
     def save(self, path):

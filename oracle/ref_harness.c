@@ -23,6 +23,7 @@
 #include "vortex_constants.h"
 #include "vortex_dynamics.h"
 #include "external_velocity.h"
+#include "vortex_injection.h"
 #include "octree.h"
 #include "../include/openvort_gpu.h"
 
@@ -253,4 +254,28 @@ void ref_time_phases(void *h, double t, double dt, double *sec)
     sec[0] = t1 - t0;
     sec[1] = t2 - t1;
     sec[2] = t3 - t2;
+}
+
+/* add_loop (src/vortex_injection.c:44-103) with the driver's dl_min/dl_max
+ * globals set to the given values. */
+void ref_add_loop(void *h, const double *center, const double *dir, double r, double dl_min,
+                  double dl_max)
+{
+    global_dl_min = dl_min;
+    global_dl_max = dl_max;
+    struct vec3 c = vec3(center[0], center[1], center[2]);
+    struct vec3 d = vec3(dir[0], dir[1], dir[2]);
+    add_loop((struct tangle_state *)h, &c, &d, r);
+}
+
+/* srand48(seed), then inject_loop (src/vortex_injection.c:711-745) at each
+ * of the n times.  inject_loop keeps its last injection time in a static
+ * that starts at 0 when the library is loaded and persists across calls. */
+void ref_inject_loops(void *h, long seed, const double *times, int n, double frequency,
+                      double dl_min, double dl_max)
+{
+    global_dl_min = dl_min;
+    global_dl_max = dl_max;
+    srand48(seed);
+    for (int i = 0; i < n; ++i) inject_loop((struct tangle_state *)h, times[i], frequency);
 }

@@ -458,6 +458,46 @@ def test_remesh_golden():
     assert np.array_equal(o["vnodes"][live], g["x_after"][live])
 
 
+def test_add_loop_matches_reference(ref):
+    """ovg_add_loop on a resident tangle with holes, then a loop larger than
+    the empty slots left (the slots double), against the compiled reference's
+    add_loop on the same tangle: slots, links, status and positions bit for
+    bit; the grown tangle still evaluates."""
+    t, p, _ = configs.make("c1", rings=6)
+    # empty the third ring: its slots become holes for the first loop
+    st, conn = t.status.copy(), t.conn.copy()
+    k0 = int(t.live[len(t.live) // 2])
+    ring, k = [], k0
+    while True:
+        ring.append(k)
+        k = conn[k, 0]
+        if k == k0:
+            break
+    st[ring] = [ov.EMPTY, -1]
+    conn[ring] = -1
+    t = ov.Tangle(t.vnodes.copy(), conn, st)
+    dl_min, dl_max = 5e-4, 1.5e-3
+    loops = [((0.05, 0.05, 0.05), (0.2, 0.4, -0.9), 0.01),
+             ((0.03, 0.06, 0.02), (0.0, 1.0, 0.1), 0.2)]  # 1256 points: the slots double
+    say(f"add_loop into {len(ring)} holes of {t.n} slots, then a 1256-point loop")
+    rt = ref.tangle(t, p)
+    with engine(t, p) as e:
+        for c, d, r in loops:
+            got_n = e.add_loop(c, d, r, dl_min, dl_max)
+            rt.add_loop(c, d, r, dl_min, dl_max)
+            assert got_n == int(4 * np.pi * r / (dl_max + dl_min))
+            assert e.n == rt.n
+            gc, gs = e.download_connections()
+            gx = e.download()["vnodes"]
+            o, rs = rt.get(), rt.status()
+            assert np.array_equal(gs[:, 0], rs[:, 0]) and np.array_equal(gc, o["conn"])
+            live = rs[:, 0] != ov.EMPTY
+            assert np.array_equal(gx[live], o["vnodes"][live])
+        assert e.n == 2 * t.n
+        e.update_tangle(0.0)
+        assert np.isfinite(e.download()["vels"][gs[:, 0] != ov.EMPTY]).all()
+
+
 def test_topology_sequence_vs_oracle(oracle):
     """remesh / small-loop removal / reconnection in sequence on one engine
     (device candidates, host pass on the touched slots only, delta commits)

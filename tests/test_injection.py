@@ -1,0 +1,68 @@
+"""Loop injection (src/vortex_injection.c) against the compiled reference:
+add_loop's points and links bit for bit, including the fallback base when
+the direction is along x, and inject_loop's random schedule (directions,
+centres on the top face, radii, and the frequency gate) from the same
+srand48 seed.  CPU only."""
+import numpy as np
+import pytest
+
+import openvort_amd as ov
+from openvort_amd import configs
+
+
+def empty_tangle(n):
+    x = np.zeros((n, 3))
+    conn = np.full((n, 2), -1, dtype=np.int32)
+    st = np.full((n, 2), -1, dtype=np.int32)
+    return ov.Tangle(x, conn, st)
+
+
+def loop_in_order(vn, conn, first, n):
+    """Positions along forward links from slot `first`, n points."""
+    out, k = [], first
+    for _ in range(n):
+        out.append(vn[k])
+        k = conn[k, 0]
+    assert k == first  # closed
+    return np.array(out)
+
+
+@pytest.mark.parametrize("center,direction,r", [
+    ((0.05, 0.04, 0.1), (0.3, -0.5, -0.8), 0.012),
+    ((0.01, 0.09, 0.03), (1.0, 0.0, 0.0), 0.02),      # along x: the (0,1,0) branch
+    ((0.07, 0.02, 0.05), (0.0, 0.0, 2.0), 0.004),     # not unit length, as add_loop takes it
+])
+def test_loop_points_match_reference(ref, center, direction, r):
+    _, p, _ = configs.make("c1", rings=1)
+    dl_min, dl_max = 5e-4, 1.5e-3
+    rt = ref.tangle(empty_tangle(64), p)      # 64 slots: add_loop doubles them as needed
+    rt.add_loop(center, direction, r, dl_min, dl_max)
+    got = ov.loop_points(center, direction, r, dl_min, dl_max)
+    o = rt.get()
+    assert len(got) == int(4 * np.pi * r / (dl_max + dl_min)) and len(got) >= 3
+    want = loop_in_order(o["vnodes"], o["conn"], 0, len(got))
+    assert np.array_equal(got, want)
+    assert (rt.status()[:len(got), 0] == ov.FREE).all()
+
+
+def test_inject_schedule_matches_reference(ref):
+    _, p, _ = configs.make("c1", rings=1)
+    dl_min, dl_max, freq, seed = 5e-4, 1.5e-3, 2000.0, 12345
+    times = np.arange(0, 12) * 3e-4         # every 3e-4 s against a 5e-4 s period
+    rt = ref.tangle(empty_tangle(32), p)
+    rt.inject_loops(seed, times, freq, dl_min, dl_max)
+    rng, last, loops = ov.rand48_state(seed), 0.0, []
+    for t in times:
+        due, c, d, r, last = ov.inject_loop_draw(t, freq, p.box_lo[:], p.box_hi[:], rng, last)
+        if due:
+            assert c[2] == p.box_hi[2] and 0.05 * 0.1 <= r <= 0.25 * 0.1
+            assert abs(np.linalg.norm(d) - 1) < 1e-15
+            loops.append(ov.loop_points(c, d, r, dl_min, dl_max))
+    assert 4 <= len(loops) < len(times)
+    o = rt.get()
+    first = 0
+    for pts in loops:  # the reference filled the slots in order, loop after loop
+        want = loop_in_order(o["vnodes"], o["conn"], first, len(pts))
+        assert np.array_equal(pts, want)
+        first += len(pts)
+    assert (rt.status()[first:, 0] == ov.EMPTY).all()
