@@ -53,6 +53,25 @@ extern "C" {
 #define OVG_WALL_PERIODIC  1
 #define OVG_WALL_MIRROR    2
 
+/* boundary faces, include/boundaries.h:25-35 (bit f of image_reflect) */
+#define OVG_FACE_LEFT   0   /* -x */
+#define OVG_FACE_RIGHT  1   /* +x */
+#define OVG_FACE_BACK   2   /* -y */
+#define OVG_FACE_FRONT  3   /* +y */
+#define OVG_FACE_DOWN   4   /* -z */
+#define OVG_FACE_UP     5   /* +z */
+
+/* image tables of src/boundary_images.c:75-302, for
+ * ovg_params_set_wall_images */
+#define OVG_IMAGES_WALL_1_OPEN          1  /* wall_1_open: DOWN mirror only  */
+#define OVG_IMAGES_WALL_1_6             2  /* wall_1_6                       */
+#define OVG_IMAGES_WALL_1_18            3  /* wall_1_18                      */
+#define OVG_IMAGES_WALL_1_26            4  /* wall_1_26                      */
+#define OVG_IMAGES_WALL_2_4             5  /* wall_2_4: z walls, x/y periodic */
+#define OVG_IMAGES_WALL_2_2             6  /* wall_2_2: y, z walls           */
+#define OVG_IMAGES_WALL_PARALLEL_PLANES 7  /* wall_parallel_planes           */
+#define OVG_IMAGES_WALL_CANAL           8  /* wall_canal: x periodic         */
+
 /* external velocity kinds, include/external_velocity.h:27-31 */
 #define OVG_FLOW_NONE        0
 #define OVG_FLOW_SIMPLE      1
@@ -107,7 +126,7 @@ typedef struct ovg_params {
     double  box_lo[3];            /* domain_box.bottom_left_back              */
     double  box_hi[3];            /* domain_box.top_right_front               */
     int32_t wall[6];              /* LEFT RIGHT BACK FRONT DOWN UP, OVG_WALL_* */
-    int32_t n_images;             /* 0 (open) or 6/18/26 periodic shifts      */
+    int32_t n_images;             /* entries of images[] / image_reflect[]    */
     int32_t images[26][3];        /* integer box shifts, reference order      */
 
     int32_t method;               /* OVG_METHOD_*                             */
@@ -144,6 +163,17 @@ typedef struct ovg_params {
                                      the step's first stage); 0 = every stage
                                      walks its own tree (the reference's
                                      build_tree per velocity evaluation)     */
+    int32_t image_reflect[26];    /* per image, the faces it mirrors through
+                                     (bit OVG_FACE_*; image_tangle.reflect of
+                                     include/boundaries.h:66-70): the target
+                                     moves to 2 * wall - r on that axis
+                                     (src/tangle.c:380-410) and, in Barnes-Hut
+                                     mode, the image's velocity is reflected
+                                     (src/boundaries.c:201-211); the direct
+                                     mode adds it unreflected, as the
+                                     reference does (src/tangle.c:678-683).
+                                     0 = pure shift.  At most one face per
+                                     axis.  Not taken by OVG_TRAVERSE_LIST.  */
     int32_t pad_;
 } ovg_params;
 
@@ -155,6 +185,16 @@ void ovg_params_default(ovg_params *p, double D);
 /* Load the periodic image table with n = 6, 18 or 26 entries
  * (src/boundary_images.c:13-72); n = 0 clears it. */
 int ovg_params_set_periodic_images(ovg_params *p, int n);
+
+/* Load one of the reference's mirror-wall image tables (OVG_IMAGES_WALL_*,
+ * src/boundary_images.c:75-302) into images / image_reflect / n_images, in
+ * the reference's order and with its quirks (wall_1_18's {1,-1,0} entry
+ * mirrors through LEFT; wall_parallel_planes repeats two entries).
+ * wall_1_6 declares 6 entries but holds 5 (reading the sixth is out of
+ * bounds in the reference): this table has the 5.  The walls themselves
+ * (p->wall) are the caller's, as in the reference's set-up
+ * (src/initial_conditions.c:135-165). */
+int ovg_params_set_wall_images(ovg_params *p, int config);
 
 const char *ovg_last_error(void);
 const char *ovg_version(void);

@@ -27,6 +27,10 @@ _lib = C.CDLL(LIB_PATH)
 OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ELIMIT = 0, 1, 2, 3, 4, 5
 EMPTY, FREE, PINNED, PINNED_SLIP = -1, 0, 1, 2
 WALL_OPEN, WALL_PERIODIC, WALL_MIRROR = 0, 1, 2
+FACE_LEFT, FACE_RIGHT, FACE_BACK, FACE_FRONT, FACE_DOWN, FACE_UP = range(6)
+# mirror-wall image tables (include/openvort_gpu.h OVG_IMAGES_WALL_*)
+(IMAGES_WALL_1_OPEN, IMAGES_WALL_1_6, IMAGES_WALL_1_18, IMAGES_WALL_1_26, IMAGES_WALL_2_4,
+ IMAGES_WALL_2_2, IMAGES_WALL_PARALLEL_PLANES, IMAGES_WALL_CANAL) = range(1, 9)
 FLOW_NONE, FLOW_SIMPLE, FLOW_DECREASING = 0, 1, 2
 METHOD_DIRECT, METHOD_BH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -56,7 +60,8 @@ class Params(C.Structure):
         ("freq_to_cutoff", C.c_double), ("rk4_classic", C.c_int32),
         ("rec_velocity", C.c_int32), ("mixed_near", C.c_double),
         ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
-        ("reuse_lists", C.c_int32), ("pad_", C.c_int32),
+        ("reuse_lists", C.c_int32), ("image_reflect", C.c_int32 * 26),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -95,6 +100,7 @@ _sig("ovg_params_sizeof", C.c_size_t)
 _sig("ovg_stats_sizeof", C.c_size_t)
 _sig("ovg_params_default", None, _P(Params), C.c_double)
 _sig("ovg_params_set_periodic_images", C.c_int, _P(Params), C.c_int)
+_sig("ovg_params_set_wall_images", C.c_int, _P(Params), C.c_int)
 _sig("ovg_create", C.c_int, _P(C.c_void_p), C.c_int)
 _sig("ovg_destroy", C.c_int, C.c_void_p)
 _sig("ovg_set_params", C.c_int, C.c_void_p, _P(Params))
@@ -190,6 +196,11 @@ def default_params(D, **kw):
 
 def set_images(p, n):
     _check(_lib.ovg_params_set_periodic_images(C.byref(p), n), "set_images")
+
+
+def set_wall_images(p, config):
+    """Load one of the reference's mirror-wall image tables (IMAGES_WALL_*)."""
+    _check(_lib.ovg_params_set_wall_images(C.byref(p), config), "set_wall_images")
 
 
 class Tangle:

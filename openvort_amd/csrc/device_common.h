@@ -22,6 +22,8 @@ struct TangleParams {
     int n_images;
     int pad0;
     double Ls[kMaxImages][3];  // L * shift, the image offsets (src/tangle.c:378)
+    double W2[kMaxImages][3];  // 2 * wall coordinate on mirrored axes (src/tangle.c:385-400)
+    int refl[kMaxImages];      // per image: bit a set = mirrored on axis a
     double f_bs;        // KAPPA / 4 / M_PI               (src/tangle.c:545)
     double kappa, a0, sqrtE;
     double evs[3];      // external superfluid flow at this call's t
@@ -39,6 +41,36 @@ struct TangleParams {
 struct Vec3 {
     double x, y, z;
 };
+
+// shifted, src/tangle.c:370-414: r - L * shift, then 2 * wall - r (the
+// unshifted r) on each mirrored axis.
+__host__ __device__ __forceinline__ void image_point(const TangleParams &P, int im, double rx,
+                                                     double ry, double rz, double &qx,
+                                                     double &qy, double &qz)
+{
+    qx = rx - P.Ls[im][0];
+    qy = ry - P.Ls[im][1];
+    qz = rz - P.Ls[im][2];
+    const int m = P.refl[im];
+    if (m) {
+        if (m & 1) qx = P.W2[im][0] - rx;
+        if (m & 2) qy = P.W2[im][1] - ry;
+        if (m & 4) qz = P.W2[im][2] - rz;
+    }
+}
+
+// mirror_vector_reflect, src/boundaries.c:201-211, for each mirrored axis:
+// v + n (-2 v.n) with n = +-e_a changes only component a, to v_a + (-2 v_a).
+__host__ __device__ __forceinline__ void image_reflect(const TangleParams &P, int im, double &vx,
+                                                       double &vy, double &vz)
+{
+    const int m = P.refl[im];
+    if (m) {
+        if (m & 1) vx = vx + -2 * vx;
+        if (m & 2) vy = vy + -2 * vy;
+        if (m & 4) vz = vz + -2 * vz;
+    }
+}
 
 __device__ __forceinline__ double dot3(double ax, double ay, double az, double bx,
                                        double by, double bz)

@@ -148,8 +148,15 @@ TangleParams host_tparams(const ovg_params &p, double t)
     P.periodic[2] = p.wall[4] == OVG_WALL_PERIODIC;
     for (int f = 0; f < 6; ++f) P.wrap_face[f] = p.wall[f] == OVG_WALL_PERIODIC;
     P.n_images = p.n_images;
-    for (int j = 0; j < p.n_images; ++j)
+    for (int j = 0; j < p.n_images; ++j) {
         for (int k = 0; k < 3; ++k) P.Ls[j][k] = P.L[k] * p.images[j][k];
+        for (int f = 0; f < 6; ++f)
+            if (p.image_reflect[j] >> f & 1) {
+                const int a = f / 2;  // LEFT/RIGHT x, BACK/FRONT y, DOWN/UP z
+                P.refl[j] |= 1 << a;
+                P.W2[j][a] = 2 * (f % 2 ? p.box_hi[a] : p.box_lo[a]);
+            }
+    }
     P.f_bs = p.kappa / 4 / M_PI;
     P.kappa = p.kappa;
     P.a0 = p.vortex_width;
@@ -394,15 +401,27 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
     for (int k = 0; k < 3; ++k)
         if (!(p->box_hi[k] > p->box_lo[k]) || !std::isfinite(p->box_hi[k] - p->box_lo[k]))
             return set_error(OVG_EINVAL, "box axis %d is empty or not finite", k);
-    for (int f = 0; f < 6; ++f) {
-        if (p->wall[f] == OVG_WALL_MIRROR)
-            return set_error(OVG_EUNSUPPORTED,
-                             "mirror walls are outside the engine's scope (periodic or open only)");
-        if (p->wall[f] != OVG_WALL_OPEN && p->wall[f] != OVG_WALL_PERIODIC)
+    for (int f = 0; f < 6; ++f)
+        if (p->wall[f] != OVG_WALL_OPEN && p->wall[f] != OVG_WALL_PERIODIC &&
+            p->wall[f] != OVG_WALL_MIRROR)
             return set_error(OVG_EINVAL, "wall[%d] = %d is not a wall kind", f, p->wall[f]);
-    }
     if (p->n_images < 0 || p->n_images > kMaxImages)
         return set_error(OVG_EINVAL, "n_images = %d out of range", p->n_images);
+    bool mirrored = false;
+    for (int j = 0; j < p->n_images; ++j) {
+        const int m = p->image_reflect[j];
+        if (m & ~63)
+            return set_error(OVG_EINVAL, "image_reflect[%d] = %d is not a face mask", j, m);
+        for (int a = 0; a < 3; ++a)
+            if ((m >> (2 * a) & 3) == 3)
+                return set_error(OVG_EINVAL,
+                                 "image_reflect[%d] mirrors through both faces of axis %d", j, a);
+        mirrored |= m != 0;
+    }
+    if (mirrored && p->method == OVG_METHOD_BH && p->traverse == OVG_TRAVERSE_LIST)
+        return set_error(OVG_EUNSUPPORTED,
+                         "mirror images: the list traversal takes shift images only "
+                         "(use OVG_TRAVERSE_GROUP or OVG_TRAVERSE_POINT)");
     if (p->method != OVG_METHOD_DIRECT && p->method != OVG_METHOD_BH)
         return set_error(OVG_EINVAL, "unknown method %d", p->method);
     if (p->precision != OVG_PREC_FP64 && p->precision != OVG_PREC_MIXED)

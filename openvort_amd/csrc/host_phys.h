@@ -59,8 +59,10 @@ inline bool seg_field(const double *R, const double *Rp, double f_bs, double *v)
 
 // Direct Biot-Savart of segment m (forward f) at point p (slot pk), all
 // images; the primary box skips the two LIA segments (src/tangle.c:658).
+// reflect: mirror images reflected as in the Barnes-Hut path
+// (src/tangle.c:644-646); the direct path adds them unreflected (:678-683).
 inline void segment_all_images(const TangleParams &P, const double *X, int m, int f, int pk,
-                               double *acc)
+                               double *acc, bool reflect)
 {
     const double *xm = X + 3 * (size_t)m;
     double r2[3] = {X[3 * (size_t)f], X[3 * (size_t)f + 1], X[3 * (size_t)f + 2]};
@@ -69,13 +71,14 @@ inline void segment_all_images(const TangleParams &P, const double *X, int m, in
     for (int im = -1; im < P.n_images; ++im) {
         if (im < 0 && (m == pk || f == pk)) continue;
         double q[3] = {r[0], r[1], r[2]};
-        if (im >= 0)
-            for (int a = 0; a < 3; ++a) q[a] = r[a] - P.Ls[im][a];
+        if (im >= 0) image_point(P, im, r[0], r[1], r[2], q[0], q[1], q[2]);
         double R[3] = {xm[0] - q[0], xm[1] - q[1], xm[2] - q[2]};
         double Rp[3] = {r2[0] - q[0], r2[1] - q[1], r2[2] - q[2]};
         double v[3];
-        if (seg_field(R, Rp, P.f_bs, v))
+        if (seg_field(R, Rp, P.f_bs, v)) {
+            if (reflect && im >= 0) image_reflect(P, im, v[0], v[1], v[2]);
             for (int a = 0; a < 3; ++a) acc[a] += v[a];
+        }
     }
 }
 

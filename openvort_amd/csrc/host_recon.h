@@ -207,8 +207,10 @@ inline int greedy_pass(const TangleParams &P, const double *X, int *conn, int n,
         for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
         auto apply = [&](const Change &ch) {
             double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
-            segment_all_images(P, X, ch.seg, ch.old_f, p, vo);
-            segment_all_images(P, X, ch.seg, ch.new_f, p, vn);
+            // direct mode (exact) adds mirror images unreflected, the tree
+            // mode reflects them, as their base velocities do
+            segment_all_images(P, X, ch.seg, ch.old_f, p, vo, !exact);
+            segment_all_images(P, X, ch.seg, ch.new_f, p, vn, !exact);
             for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
         };
         // change of the segments rewired so far

@@ -63,6 +63,82 @@ extern "C" int ovg_params_set_periodic_images(ovg_params *p, int n)
                 ++j;
             }
     p->n_images = j;
+    for (int i = 0; i < 26; ++i) p->image_reflect[i] = 0;
+    return OVG_OK;
+}
+
+// Mirror-wall image tables of src/boundary_images.c:75-230, entry by entry
+// (the order is the order of the image sums): shift (sx, sy, sz) and the
+// mask of the faces mirrored through.  Kept as the reference has them,
+// quirks included: wall_1_18's {1,-1,0} entry has number_of_reflects 1 with
+// an empty (zero-filled) list, i.e. LEFT; wall_parallel_planes lists
+// {-1,1,-1} and {-1,1,1} twice and {1,-1,-1}, {1,-1,1} not at all.
+namespace {
+constexpr int L_ = 1 << OVG_FACE_LEFT, B_ = 1 << OVG_FACE_BACK, F_ = 1 << OVG_FACE_FRONT,
+              D_ = 1 << OVG_FACE_DOWN, U_ = 1 << OVG_FACE_UP;
+struct WallImage {
+    signed char s[3];
+    int mask;
+};
+const WallImage kWall1Open[] = {{{0, 0, -1}, D_}};
+const WallImage kWall16[] = {{{-1, 0, 0}, 0}, {{0, -1, 0}, 0}, {{0, 0, -1}, D_},
+                             {{0, 1, 0}, 0},  {{1, 0, 0}, 0}};
+const WallImage kWall118[] = {
+    {{-1, -1, 0}, 0}, {{-1, 0, -1}, D_}, {{-1, 0, 0}, 0},  {{-1, 0, 1}, 0},  {{-1, 1, 0}, 0},
+    {{0, -1, -1}, D_}, {{0, -1, 0}, 0},  {{0, -1, 1}, 0},  {{0, 0, -1}, D_}, {{0, 0, 1}, 0},
+    {{0, 1, -1}, D_},  {{0, 1, 0}, 0},   {{0, 1, 1}, 0},   {{1, -1, 0}, L_}, {{1, 0, -1}, D_},
+    {{1, 0, 0}, 0},    {{1, 0, 1}, 0},   {{1, 1, 0}, 0}};
+const WallImage kWall126[] = {
+    {{-1, -1, -1}, D_}, {{-1, -1, 0}, 0}, {{-1, -1, 1}, 0}, {{-1, 0, -1}, D_}, {{-1, 0, 0}, 0},
+    {{-1, 0, 1}, 0},    {{-1, 1, -1}, D_}, {{-1, 1, 0}, 0}, {{-1, 1, 1}, 0},   {{0, -1, -1}, D_},
+    {{0, -1, 0}, 0},    {{0, -1, 1}, 0},  {{0, 0, -1}, D_}, {{0, 0, 1}, 0},    {{0, 1, -1}, D_},
+    {{0, 1, 0}, 0},     {{0, 1, 1}, 0},   {{1, -1, -1}, D_}, {{1, -1, 0}, 0},  {{1, -1, 1}, 0},
+    {{1, 0, -1}, D_},   {{1, 0, 0}, 0},   {{1, 0, 1}, 0},   {{1, 1, -1}, D_},  {{1, 1, 0}, 0},
+    {{1, 1, 1}, 0}};
+const WallImage kWall24[] = {{{-1, 0, 0}, 0}, {{1, 0, 0}, 0},   {{0, -1, 0}, 0},
+                             {{0, 1, 0}, 0},  {{0, 0, -1}, D_}, {{0, 0, 1}, U_}};
+const WallImage kWall22[] = {{{-1, 0, 0}, 0}, {{1, 0, 0}, 0},   {{0, -1, 0}, B_},
+                             {{0, 1, 0}, F_}, {{0, 0, -1}, D_}, {{0, 0, 1}, U_}};
+const WallImage kWallPlanes[] = {
+    {{-1, 0, 0}, 0},  {{1, 0, 0}, 0},   {{0, -1, 0}, 0},  {{0, 1, 0}, 0},   {{0, 0, -1}, D_},
+    {{0, 0, 1}, U_},  {{-1, -1, 0}, 0}, {{1, -1, 0}, 0},  {{-1, 1, 0}, 0},  {{1, 1, 0}, 0},
+    {{1, 0, -1}, D_}, {{-1, 0, -1}, D_}, {{0, 1, -1}, D_}, {{0, -1, -1}, D_}, {{1, 0, 1}, U_},
+    {{-1, 0, 1}, U_}, {{0, 1, 1}, U_},  {{0, -1, 1}, U_}, {{1, 1, -1}, D_}, {{-1, 1, -1}, D_},
+    {{-1, 1, -1}, D_}, {{-1, -1, -1}, D_}, {{1, 1, 1}, U_}, {{-1, 1, 1}, U_}, {{-1, 1, 1}, U_},
+    {{-1, -1, 1}, U_}};
+const WallImage kWallCanal[] = {
+    {{-1, 0, 0}, 0},       {{1, 0, 0}, 0},        {{0, -1, 0}, B_},      {{0, 1, 0}, F_},
+    {{0, 0, -1}, D_},      {{0, 0, 1}, U_},       {{0, -1, -1}, B_ | D_}, {{0, -1, 1}, B_ | U_},
+    {{0, 1, -1}, F_ | D_}, {{0, 1, 1}, F_ | U_},  {{-1, -1, 0}, B_},     {{-1, 1, 0}, F_},
+    {{-1, 0, -1}, D_},     {{-1, 0, 1}, U_},      {{-1, -1, -1}, B_ | D_}, {{-1, -1, 1}, B_ | U_},
+    {{-1, 1, -1}, F_ | D_}, {{-1, 1, 1}, F_ | U_}, {{1, -1, 0}, B_},      {{1, 1, 0}, F_},
+    {{1, 0, -1}, D_},      {{1, 0, 1}, U_},       {{1, -1, -1}, B_ | D_}, {{1, -1, 1}, B_ | U_},
+    {{1, 1, -1}, F_ | D_}, {{1, 1, 1}, F_ | U_}};
+}  // namespace
+
+extern "C" int ovg_params_set_wall_images(ovg_params *p, int config)
+{
+    if (!p) return ovg::set_error(OVG_EINVAL, "params is NULL");
+    const WallImage *t = nullptr;
+    int n = 0;
+    switch (config) {
+#define OVG_TABLE_(id, arr) case id: t = arr; n = (int)(sizeof(arr) / sizeof(arr[0])); break;
+    OVG_TABLE_(OVG_IMAGES_WALL_1_OPEN, kWall1Open)
+    OVG_TABLE_(OVG_IMAGES_WALL_1_6, kWall16)
+    OVG_TABLE_(OVG_IMAGES_WALL_1_18, kWall118)
+    OVG_TABLE_(OVG_IMAGES_WALL_1_26, kWall126)
+    OVG_TABLE_(OVG_IMAGES_WALL_2_4, kWall24)
+    OVG_TABLE_(OVG_IMAGES_WALL_2_2, kWall22)
+    OVG_TABLE_(OVG_IMAGES_WALL_PARALLEL_PLANES, kWallPlanes)
+    OVG_TABLE_(OVG_IMAGES_WALL_CANAL, kWallCanal)
+#undef OVG_TABLE_
+    default: return ovg::set_error(OVG_EINVAL, "unknown wall image table %d", config);
+    }
+    for (int j = 0; j < 26; ++j) {
+        for (int k = 0; k < 3; ++k) p->images[j][k] = j < n ? t[j].s[k] : 0;
+        p->image_reflect[j] = j < n ? t[j].mask : 0;
+    }
+    p->n_images = n;
     return OVG_OK;
 }
 

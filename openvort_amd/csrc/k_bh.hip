@@ -48,11 +48,7 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_point(
         if (!finite) n_nodes = 0, tx = ty = tz = __builtin_nan("");
         for (int im = -1; im < P.n_images; ++im) {
             double qx = rx, qy = ry, qz = rz;
-            if (im >= 0) {
-                qx = rx - P.Ls[im][0];
-                qy = ry - P.Ls[im][1];
-                qz = rz - P.Ls[im][2];
-            }
+            if (im >= 0) image_point(P, im, rx, ry, rz, qx, qy, qz);
             double ax = finite ? 0 : __builtin_nan(""), ay = ax, az = ax;
             int d = 0;
             while (d < n_nodes) {
@@ -77,6 +73,7 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_point(
             if (im < 0) {
                 px[k] = ax; py[k] = ay; pz[k] = az;
             } else {
+                image_reflect(P, im, ax, ay, az);  // src/tangle.c:644-646
                 tx += ax; ty += ay; tz += az;
             }
         }
@@ -255,17 +252,14 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_group(
     double tx = nan0, ty = nan0, tz = nan0;
     for (int im = -1; im < P.n_images; ++im) {
         double qx = rx, qy = ry, qz = rz;
-        if (im >= 0) {
-            qx = rx - P.Ls[im][0];
-            qy = ry - P.Ls[im][1];
-            qz = rz - P.Ls[im][2];
-        }
+        if (im >= 0) image_point(P, im, rx, ry, rz, qx, qy, qz);
         double ax = nan0, ay = nan0, az = nan0;
         tree_walk<MIXED, LOCKSTEP>(T, walk, qx, qy, qz, ax, ay, az, c64, c32);
         if (valid) {
             if (im < 0) {
                 px[k] = ax; py[k] = ay; pz[k] = az;
             } else {
+                image_reflect(P, im, ax, ay, az);  // src/tangle.c:644-646
                 tx += ax; ty += ay; tz += az;
             }
         }
@@ -299,14 +293,11 @@ __global__ __launch_bounds__(kBhBlock) void k_bh_split(
     int k = valid ? tlist[t] : 0;
     double qx = valid ? X[k] : 0, qy = valid ? Y[k] : 0, qz = valid ? Z[k] : 0;
     const bool walk = valid && isfinite(qx) && isfinite(qy) && isfinite(qz);
-    if (c > 0) {
-        qx = qx - P.Ls[c - 1][0];
-        qy = qy - P.Ls[c - 1][1];
-        qz = qz - P.Ls[c - 1][2];
-    }
+    if (c > 0) image_point(P, c - 1, qx, qy, qz, qx, qy, qz);
     const double nan0 = walk ? 0.0 : __builtin_nan("");
     double ax = nan0, ay = nan0, az = nan0;
     tree_walk<MIXED, false>(T, walk, qx, qy, qz, ax, ay, az, c64, c32);
+    if (c > 0) image_reflect(P, c - 1, ax, ay, az);  // src/tangle.c:644-646
     if (valid) {
         size_t o = (size_t)3 * j;
         part[o] = ax; part[o + 1] = ay; part[o + 2] = az;

@@ -70,11 +70,11 @@ __global__ __launch_bounds__(kDirBlock) void k_direct(
                     }
                 }
             }
-            // image boxes: shifted(r) = r - L*shift (src/tangle.c:370-414)
+            // image boxes at shifted(r) (src/tangle.c:370-414); the direct
+            // path adds them unreflected (src/tangle.c:678-683)
             for (int im = 0; im < nimg; ++im) {
-                double qx = rx - P.Ls[im][0];
-                double qy = ry - P.Ls[im][1];
-                double qz = rz - P.Ls[im][2];
+                double qx, qy, qz;
+                image_point(P, im, rx, ry, rz, qx, qy, qz);
                 if (MIXED) {
                     float vx, vy, vz;
                     seg_field32((float)(ax - qx), (float)(ay - qy), (float)(az - qz),

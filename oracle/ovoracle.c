@@ -225,14 +225,37 @@ static void lia(const double *vnodes, const int *conn, const ovg_params *p,
     for (int k = 0; k < 3; ++k) out[k] = vv[k] * f;
 }
 
-/* shifted, src/tangle.c:370-414, periodic part */
-static void shifted(const double *r, const int *shift, const ovg_params *p,
-                    double *rs)
+/* shifted, src/tangle.c:370-414: the periodic shift, then each mirror
+ * face of image j (bit f of image_reflect[j], faces LEFT RIGHT BACK FRONT
+ * DOWN UP) sets its coordinate to 2 * wall - r (:385-400) */
+static void shifted(const double *r, int j, const ovg_params *p, double *rs)
 {
     for (int k = 0; k < 3; ++k) {
         double L = p->box_hi[k] - p->box_lo[k];
         rs[k] = r[k];
-        rs[k] -= L * shift[k];
+        rs[k] -= L * p->images[j][k];
+    }
+    for (int f = 0; f < 6; ++f) {
+        if (!(p->image_reflect[j] >> f & 1)) continue;
+        int a = f / 2;
+        double wall = f % 2 ? p->box_hi[a] : p->box_lo[a];
+        rs[a] = 2 * wall - r[a];
+    }
+}
+
+/* mirror_vector_reflect, src/boundaries.c:201-211, for every mirror face of
+ * image j, with boundary_normals of src/boundaries.c:11-18 */
+static void mirror_reflect(double *v, int j, const ovg_params *p)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (!(p->image_reflect[j] >> f & 1)) continue;
+        double n[3] = {0, 0, 0};
+        n[f / 2] = f % 2 ? -1 : 1;
+        double nc = 0;
+        for (int c = 0; c < 3; ++c) nc += v[c] * n[c];
+        double tmp[3];
+        for (int c = 0; c < 3; ++c) tmp[c] = n[c] * (-2 * nc);
+        for (int c = 0; c < 3; ++c) v[c] = v[c] + tmp[c];
     }
 }
 
@@ -498,8 +521,9 @@ static void velocity_one(int n, const double *vnodes, const int *conn,
         for (int c = 0; c < 3; ++c) vs[c] = vs[c] + vt[c];
         for (int j = 0; j < p->n_images; ++j) {
             double rs[3], vsh[3];
-            shifted(x, p->images[j], p, rs);
+            shifted(x, j, p, rs);
             rtree_vs(tree, 0, rs, p->theta, p->kappa, vsh, count);
+            mirror_reflect(vsh, j, p);  /* src/tangle.c:644-646 */
             for (int c = 0; c < 3; ++c) tot[c] = tot[c] + vsh[c];
         }
     } else {
@@ -512,7 +536,7 @@ static void velocity_one(int n, const double *vnodes, const int *conn,
         }
         for (int j = 0; j < p->n_images; ++j) {
             double rs[3];
-            shifted(x, p->images[j], p, rs);
+            shifted(x, j, p, rs);
             for (int m = 0; m < n; ++m) {
                 if (conn[2 * m] == -1) continue;
                 double sv[3];
@@ -562,7 +586,7 @@ void ovo_bs_direct_one(int n, const double *vnodes, const int *conn,
     }
     for (int j = 0; j < p->n_images; ++j) {
         double rs[3];
-        shifted(x, p->images[j], p, rs);
+        shifted(x, j, p, rs);
         for (int m = 0; m < n; ++m) {
             if (conn[2 * m] == -1) continue;
             double sv[3];

@@ -171,6 +171,8 @@ class RefLib:
         vp = C.c_void_p
         L.ref_setup.argtypes = [P, C.c_int]
         L.ref_image_table.argtypes = [C.c_int, _ip]
+        L.ref_wall_table.argtypes = [C.c_int, _ip, _ip]
+        L.ref_set_wall_images.argtypes = [C.c_void_p, C.c_int]
         L.ref_tangle_new.argtypes = [C.c_int, _dp, _ip, _ip, P]
         L.ref_tangle_new.restype = vp
         L.ref_tangle_free.argtypes = [vp]
@@ -202,11 +204,25 @@ class RefLib:
         m = self.lib.ref_image_table(n, _i(out))
         return out[:m]
 
-    def tangle(self, t: Tangle, p, threads=8):
+    def wall_table(self, config):
+        """(shifts, face masks) of the reference's mirror-wall table."""
+        sh = np.zeros((26, 3), dtype=np.int32)
+        mk = np.zeros(26, dtype=np.int32)
+        m = self.lib.ref_wall_table(config, _i(sh), _i(mk))
+        if m < 0:
+            raise ValueError(f"no wall table {config}")
+        return sh[:m], mk[:m]
+
+    def tangle(self, t: Tangle, p, threads=8, wall_images=None):
+        """wall_images: an OVG_IMAGES_WALL_* id, the reference's own table
+        (p's images must then be that table, as set_wall_images loads it)."""
         self.lib.ref_setup(C.byref(p), threads)
         h = self.lib.ref_tangle_new(t.n, _d(t.vnodes), _i(t.conn), _i(t.status), C.byref(p))
         if not h:
             raise ValueError("reference has no such image table")
+        if wall_images is not None and self.lib.ref_set_wall_images(h, wall_images) != 0:
+            self.lib.ref_tangle_free(h)
+            raise ValueError(f"no wall table {wall_images}")
         return RefTangle(self, h, t.n)
 
 

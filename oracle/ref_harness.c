@@ -72,6 +72,11 @@ static int set_box(struct tangle_state *t, const ovg_params *p)
         t->box.top_right_front.p[k] = p->box_hi[k];
     }
     for (int k = 0; k < 6; ++k) t->box.wall[k] = (wall_type)p->wall[k];
+    for (int j = 0; j < p->n_images && j < 26; ++j)
+        if (p->image_reflect[j]) {
+            t->bimg = open_boundaries;  /* mirror tables: ref_set_wall_images */
+            return 0;
+        }
     switch (p->n_images) {
     case 0: t->bimg = open_boundaries; break;
     case 6: t->bimg = periodic_6; break;
@@ -92,6 +97,58 @@ int ref_image_table(int n, int *out)
     for (int j = 0; j < b->n; ++j)
         for (int k = 0; k < 3; ++k) out[3 * j + k] = b->images[j].shift[k];
     return b->n;
+}
+
+/* The reference's mirror-wall image table for an OVG_IMAGES_WALL_* id
+ * (src/boundary_images.c:255-302); NULL for an unknown id. */
+static const struct boundary_images *wall_table(int config)
+{
+    switch (config) {
+    case OVG_IMAGES_WALL_1_OPEN: return &wall_1_open;
+    case OVG_IMAGES_WALL_1_6: return &wall_1_6;
+    case OVG_IMAGES_WALL_1_18: return &wall_1_18;
+    case OVG_IMAGES_WALL_1_26: return &wall_1_26;
+    case OVG_IMAGES_WALL_2_4: return &wall_2_4;
+    case OVG_IMAGES_WALL_2_2: return &wall_2_2;
+    case OVG_IMAGES_WALL_PARALLEL_PLANES: return &wall_parallel_planes;
+    case OVG_IMAGES_WALL_CANAL: return &wall_canal;
+    default: return NULL;
+    }
+}
+
+/* Entries actually stored in a wall table: wall_1_6 declares n = 6 over a
+ * 5-entry array (reading the sixth is out of bounds), so only 5 are read. */
+static int wall_table_len(int config, const struct boundary_images *b)
+{
+    return config == OVG_IMAGES_WALL_1_6 && b->n > 5 ? 5 : b->n;
+}
+
+/* Copy of a reference wall table: shifts and the face mask of
+ * reflect[0 .. number_of_reflects).  Returns the entry count or -1. */
+int ref_wall_table(int config, int *shift, int *mask)
+{
+    const struct boundary_images *b = wall_table(config);
+    if (!b) return -1;
+    int n = wall_table_len(config, b);
+    for (int j = 0; j < n; ++j) {
+        mask[j] = 0;
+        for (int k = 0; k < 3; ++k) shift[3 * j + k] = b->images[j].shift[k];
+        for (int r = 0; r < b->images[j].number_of_reflects; ++r)
+            mask[j] |= 1 << b->images[j].reflect[r];
+    }
+    return n;
+}
+
+/* Use a reference wall table for tangle h (its walls come from the params
+ * given to ref_tangle_new).  wall_1_6 is cut to its 5 stored entries. */
+int ref_set_wall_images(void *h, int config)
+{
+    struct tangle_state *t = (struct tangle_state *)h;
+    const struct boundary_images *b = wall_table(config);
+    if (!b) return -1;
+    t->bimg = *b;
+    t->bimg.n = wall_table_len(config, b);
+    return 0;
 }
 
 void *ref_tangle_new(int n, const double *vnodes, const int *conn,

@@ -1,0 +1,86 @@
+"""GPU parity with mirror walls: the engine's direct and Barnes-Hut paths
+(point and group traversals) against the CPU oracle on the same tangle,
+with the reference's wall image tables.  The oracle is pinned bit for bit
+to the compiled reference and to the reflected-copy identity in
+tests/test_walls.py.
+
+Tolerances: FP64 velocities within 1e-10 relative L2 (summation order),
+tangents and normals 1e-14, positions after one RK4 step 1e-14 cm; mixed
+precision within 1e-5 of the FP64 path."""
+import sys
+
+import numpy as np
+import pytest
+
+import openvort_amd as ov
+from conftest import rel_l2
+from test_walls import wall_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+CASES = [
+    # (table, method, traverse, precision)
+    (ov.IMAGES_WALL_CANAL, ov.METHOD_DIRECT, ov.TRAVERSE_POINT, ov.PREC_FP64),
+    (ov.IMAGES_WALL_CANAL, ov.METHOD_BH, ov.TRAVERSE_POINT, ov.PREC_FP64),
+    (ov.IMAGES_WALL_CANAL, ov.METHOD_BH, ov.TRAVERSE_GROUP, ov.PREC_FP64),
+    (ov.IMAGES_WALL_1_18, ov.METHOD_BH, ov.TRAVERSE_POINT, ov.PREC_FP64),
+    (ov.IMAGES_WALL_1_18, ov.METHOD_DIRECT, ov.TRAVERSE_POINT, ov.PREC_FP64),
+    (ov.IMAGES_WALL_PARALLEL_PLANES, ov.METHOD_BH, ov.TRAVERSE_GROUP, ov.PREC_FP64),
+    (ov.IMAGES_WALL_2_2, ov.METHOD_BH, ov.TRAVERSE_POINT, ov.PREC_FP64),
+    (ov.IMAGES_WALL_1_OPEN, ov.METHOD_BH, ov.TRAVERSE_GROUP, ov.PREC_FP64),
+]
+
+
+@pytest.mark.parametrize("config,method,traverse,precision", CASES)
+def test_wall_update_and_rk4_vs_oracle(oracle, config, method, traverse, precision):
+    t, p = wall_case(config, method)
+    p.traverse = traverse
+    p.precision = precision
+    live = t.live
+    say(f"walls table {config} method {method} traverse {traverse}: {len(live)} points")
+    want = oracle.update_tangle(t, p)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        e.update_tangle(0.0)
+        o = e.download()
+        assert rel_l2(o["tangents"][live], want["tangents"][live]) < 1e-14
+        assert rel_l2(o["normals"][live], want["normals"][live]) < 1e-14
+        assert rel_l2(o["vs"][live], want["vs"][live]) < 1e-10
+        assert rel_l2(o["vels"][live], want["vels"][live]) < 1e-10
+        dt = 1e-4
+        e.rk4_step(0.0, dt)
+        x = e.download()["vnodes"][live]
+    w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
+    oracle.rk4_step(w, p, 0.0, dt, want["vels"])
+    assert np.abs(x - w.vnodes[live]).max() < 1e-14
+
+
+def test_wall_group_mixed_precision(oracle):
+    t, p = wall_case(ov.IMAGES_WALL_CANAL, ov.METHOD_BH)
+    p.traverse = ov.TRAVERSE_GROUP
+    live = t.live
+    say(f"walls canal, group traversal FP64 and mixed: {len(live)} points")
+    out = {}
+    for prec in (ov.PREC_FP64, ov.PREC_MIXED):
+        p.precision = prec
+        with ov.Engine(p) as e:
+            e.upload(t)
+            e.update_tangle(0.0)
+            out[prec] = e.download()
+    assert rel_l2(out[ov.PREC_MIXED]["vs"][live], out[ov.PREC_FP64]["vs"][live]) < 1e-5
+
+
+def test_wall_list_traversal_is_rejected():
+    t, p = wall_case(ov.IMAGES_WALL_CANAL, ov.METHOD_BH)
+    p.traverse = ov.TRAVERSE_LIST
+    say("walls with the list traversal: EUNSUPPORTED")
+    with pytest.raises(ov.OvgError) as ei:
+        with ov.Engine(p):
+            pass
+    assert ei.value.code == ov.EUNSUPPORTED

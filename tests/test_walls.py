@@ -1,0 +1,132 @@
+"""Mirror walls (images with reflections): the engine's tables against the
+compiled reference's, the oracle against the compiled reference bit for
+bit, and the image construction against what the mathematics fixes (a
+mirror image is the reflected tangle with reversed circulation, so with
+theta -> 0 it equals an explicit reflected and reversed copy)."""
+import numpy as np
+import pytest
+
+import openvort_amd as ov
+from openvort_amd import configs
+from conftest import rel_l2
+
+P_, M_, O_ = ov.WALL_PERIODIC, ov.WALL_MIRROR, ov.WALL_OPEN
+# walls of each table's geometry (src/initial_conditions.c:135-165 and the
+# table comments of src/boundary_images.c:75-230), faces LEFT..UP
+WALLS = {
+    ov.IMAGES_WALL_1_OPEN: (O_, O_, O_, O_, M_, O_),
+    ov.IMAGES_WALL_1_6: (P_, P_, P_, P_, M_, P_),
+    ov.IMAGES_WALL_1_18: (P_, P_, P_, P_, M_, P_),
+    ov.IMAGES_WALL_1_26: (P_, P_, P_, P_, M_, P_),
+    ov.IMAGES_WALL_2_4: (P_, P_, P_, P_, M_, M_),
+    ov.IMAGES_WALL_2_2: (P_, P_, M_, M_, M_, M_),
+    ov.IMAGES_WALL_PARALLEL_PLANES: (P_, P_, P_, P_, M_, M_),
+    ov.IMAGES_WALL_CANAL: (P_, P_, M_, M_, M_, M_),
+}
+
+
+def rings_inside(seed, n, D, R, dxi):
+    """Rings wholly inside the box [0, D]^3."""
+    rng = np.random.default_rng(seed)
+    xs, conns = [], []
+    off = 0
+    for _ in range(n):
+        c = rng.uniform(R * 1.1, D - R * 1.1, 3)
+        nrm = rng.normal(size=3)
+        nrm /= np.linalg.norm(nrm)
+        u = np.cross(nrm, [1.0, 0, 0] if abs(nrm[0]) < 0.9 else [0, 1.0, 0])
+        u /= np.linalg.norm(u)
+        v = np.cross(nrm, u)
+        m = int(2 * np.pi * R / dxi)
+        ph = 2 * np.pi * np.arange(m) / m
+        xs.append(c + R * (np.cos(ph)[:, None] * u + np.sin(ph)[:, None] * v))
+        conns.append(np.stack([off + (np.arange(m) + 1) % m, off + (np.arange(m) - 1) % m], 1))
+        off += m
+    x = np.concatenate(xs)
+    st = np.zeros((off, 2), dtype=np.int32)
+    st[:, 1] = -1
+    return ov.Tangle(x, np.concatenate(conns).astype(np.int32), st)
+
+
+def wall_case(config, method, seed=7, n=6, theta=0.4):
+    D, dxi = 0.1, 2e-3
+    t = rings_inside(seed, n, D, 0.012, dxi)
+    p = configs.params(D, dxi, configs.T16, 0.5, method=method, theta=theta)
+    for f, w in enumerate(WALLS[config]):
+        p.wall[f] = w
+    ov.set_wall_images(p, config)
+    return t, p
+
+
+@pytest.mark.parametrize("config", sorted(WALLS))
+def test_wall_tables_are_the_reference_tables(ref, config):
+    p = ov.Params()
+    ov.set_wall_images(p, config)
+    shifts, masks = ref.wall_table(config)
+    n = p.n_images
+    assert n == len(shifts)
+    got = np.array([[p.images[j][k] for k in range(3)] for j in range(n)])
+    assert np.array_equal(got, shifts)
+    assert np.array_equal(np.array(p.image_reflect[:n]), masks)
+    assert all(p.image_reflect[j] == 0 for j in range(n, 26))
+
+
+def test_periodic_table_clears_reflections():
+    p = ov.Params()
+    ov.set_wall_images(p, ov.IMAGES_WALL_CANAL)
+    ov.set_images(p, 26)
+    assert all(v == 0 for v in p.image_reflect)
+
+
+@pytest.mark.parametrize("config", [ov.IMAGES_WALL_CANAL, ov.IMAGES_WALL_1_18,
+                                    ov.IMAGES_WALL_PARALLEL_PLANES, ov.IMAGES_WALL_1_OPEN,
+                                    ov.IMAGES_WALL_2_2])
+@pytest.mark.parametrize("method", [ov.METHOD_DIRECT, ov.METHOD_BH])
+def test_wall_velocities_bitexact_vs_reference(oracle, ref, config, method):
+    """update_tangle with mirror images: the BH path reflects each image's
+    velocity (src/tangle.c:644-646), the direct path adds it unreflected
+    (src/tangle.c:678-683, kept as the reference has it)."""
+    t, p = wall_case(config, method)
+    want = oracle.update_tangle(t, p)
+    r = ref.tangle(t, p, wall_images=config)
+    r.update_tangle(0.0)
+    got = r.get()
+    live = t.live
+    for key in ("tangents", "normals", "vs", "vels"):
+        assert np.array_equal(got[key][live], want[key][live]), key
+
+
+def test_mirror_image_is_the_reflected_reversed_copy(oracle):
+    """One mirror wall at z = 0 (wall_1_open), theta -> 0: the field of the
+    image equals that of the tangle reflected through the wall with its
+    orientation reversed, placed explicitly in an open box.  A dropped
+    reflection of the velocity, a wrong wall coordinate or an unreversed
+    copy each fail this by O(1)."""
+    D = 0.1
+    t, p = wall_case(ov.IMAGES_WALL_1_OPEN, ov.METHOD_BH, theta=1e-6)
+    base = oracle.update_tangle(t, p)
+    live = t.live
+    nl = len(live)
+    x = t.vnodes[live]
+    c = t.conn[live]
+    mx = x.copy()
+    mx[:, 2] = -mx[:, 2]                 # 2 * 0 - z
+    mc = c[:, ::-1] + nl                 # forward and reverse swapped
+    v = np.concatenate([x, mx])
+    cc = np.concatenate([c, mc]).astype(np.int32)
+    st = np.zeros((2 * nl, 2), dtype=np.int32)
+    st[:, 1] = -1
+    big = ov.Tangle(v, cc, st)
+    q = ov.Params.from_buffer_copy(p)
+    q.n_images = 0
+    for f in range(6):
+        q.wall[f] = ov.WALL_OPEN
+    q.box_lo[2] = -D
+    o = oracle.update_tangle(big, q)
+    assert rel_l2(o["vs"][:nl], base["vs"][live]) < 1e-12
+    assert rel_l2(o["vels"][:nl], base["vels"][live]) < 1e-12
+    # the image matters: without it the velocities differ at O(1e-2)
+    q2 = ov.Params.from_buffer_copy(p)
+    q2.n_images = 0
+    alone = oracle.update_tangle(t, q2)
+    assert rel_l2(alone["vs"][live], base["vs"][live]) > 1e-3
