@@ -40,7 +40,7 @@ extern "C" {
 #define OVG_EINVAL       1   /* bad argument, shape or state */
 #define OVG_ENOMEM       2   /* device or host allocation failed / cap exceeded */
 #define OVG_EDEVICE      3   /* HIP runtime error, no device, missing code object */
-#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. mirror walls) */
+#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. pinned nodes) */
 #define OVG_ELIMIT       5   /* an internal fixed-size limit was exceeded */
 
 /* ---- node status and wall kinds (reference enums) -------------------- */
@@ -371,7 +371,7 @@ int ovg_synchronize(ovg_ctx *ctx);
  * are empty than needed), linked in order and closed.  *n_points = points
  * added.  Loops with no point are refused (OVG_EINVAL).
  * Injection of straight line pairs (inject_line_pairs) pins the lines to
- * walls, which this engine does not support.
+ * walls; pinned nodes are not supported by this engine.
  */
 int ovg_loop_points(const double center[3], const double dir[3], double r, double dl_min,
                     double dl_max, double *xyz, int cap, int *n_points);
