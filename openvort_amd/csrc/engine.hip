@@ -357,7 +357,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
     if (c.copy_stream) hipStreamSynchronize(c.copy_stream);
-    DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
+    DBuf *all[] = {&c.fwd, &c.rev, &c.status, &c.pinw, &c.live, &c.src, &c.partial, &c.dpart, &c.seg_ids,
                    &c.tparams, &c.flags, &c.counter64, &c.rc_cell, &c.rc_sorted, &c.rc_start,
                    &c.rc_pairs, &c.rc_count, &c.rc_tmp, &c.sm_buf, &c.split_part,
                    &c.loop_off, &c.loop_slots, &c.lt_buf, &c.sm_fft, &c.sm_big, &c.sm_tw,
@@ -463,25 +463,28 @@ extern "C" int ovg_get_params(const ovg_ctx *ctx, ovg_params *p)
     return OVG_OK;
 }
 
-// Validate topology: FREE slots must sit on closed loops with consistent
-// links; pinned nodes need mirror walls, which are out of scope.
+// Validate topology: FREE slots have both links; PINNED / PINNED_SLIP slots
+// (on the wall pin_wall, src/tangle.c:170-185) may have an open end; every
+// link that exists is consistent.
 static int validate_topology(int n, const int *conn, const int *status)
 {
     for (int k = 0; k < n; ++k) {
         int s = status[2 * k];
         if (s == OVG_EMPTY) continue;
-        if (s == OVG_PINNED || s == OVG_PINNED_SLIP)
-            return set_error(OVG_EUNSUPPORTED, "slot %d is pinned; wall pinning is out of scope", k);
-        if (s != OVG_FREE) return set_error(OVG_EINVAL, "slot %d has status %d", k, s);
+        const bool pinned = s == OVG_PINNED || s == OVG_PINNED_SLIP;
+        if (!pinned && s != OVG_FREE) return set_error(OVG_EINVAL, "slot %d has status %d", k, s);
+        if (pinned && (status[2 * k + 1] < 0 || status[2 * k + 1] > 5))
+            return set_error(OVG_EINVAL, "pinned slot %d has pin_wall %d", k, status[2 * k + 1]);
         int f = conn[2 * k], r = conn[2 * k + 1];
-        if (f < 0 || f >= n || r < 0 || r >= n)
+        const bool f_open = pinned && f == -1, r_open = pinned && r == -1;
+        if ((!f_open && (f < 0 || f >= n)) || (!r_open && (r < 0 || r >= n)))
             return set_error(OVG_EUNSUPPORTED,
-                             "slot %d has an open end (forward %d, reverse %d): only closed "
-                             "loops are supported",
+                             "slot %d has an open end (forward %d, reverse %d) and is not "
+                             "pinned: only closed loops and pinned lines are supported",
                              k, f, r);
-        if (status[2 * f] == OVG_EMPTY || status[2 * r] == OVG_EMPTY)
+        if ((!f_open && status[2 * f] == OVG_EMPTY) || (!r_open && status[2 * r] == OVG_EMPTY))
             return set_error(OVG_EINVAL, "slot %d links to an EMPTY slot", k);
-        if (conn[2 * f + 1] != k || conn[2 * r] != k)
+        if ((!f_open && conn[2 * f + 1] != k) || (!r_open && conn[2 * r] != k))
             return set_error(OVG_EINVAL, "slot %d: forward/reverse links are inconsistent", k);
     }
     return OVG_OK;
@@ -555,6 +558,18 @@ static int upload_topology(Ctx &c, const int *conn, const int *status)
     std::vector<int> st(n);
     for (int k = 0; k < n; ++k) st[k] = status[2 * k];
     OVG_HIP(hipMemcpy(c.status.p, st.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    // pin walls, kept only while the tangle has pinned nodes (the topology
+    // passes, which would have to maintain them, refuse such tangles)
+    c.has_pinned = false;
+    for (int k = 0; k < n; ++k) {
+        const bool pinned = st[k] == OVG_PINNED || st[k] == OVG_PINNED_SLIP;
+        c.has_pinned |= pinned;
+        st[k] = pinned ? status[2 * k + 1] : -1;
+    }
+    if (c.has_pinned) {
+        OVG_TRY(c.pinw.ensure(sizeof(int) * (size_t)n));
+        OVG_HIP(hipMemcpy(c.pinw.p, st.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    }
     if (c.n_live) {
         OVG_HIP(hipMemcpy(c.live.p, live.data(), sizeof(int) * (size_t)c.n_live,
                           hipMemcpyHostToDevice));
@@ -775,6 +790,9 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !slots)) return set_error(OVG_EINVAL, "bad slot list");
     for (int i = 0; i < m; ++i)
@@ -900,6 +918,9 @@ extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs,
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     if (max_pairs < 0 || (max_pairs > 0 && !pairs))
         return set_error(OVG_EINVAL, "bad pair buffer");
     Ctx &c = ctx->c;
@@ -911,6 +932,9 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int nrec = 0;
@@ -923,6 +947,9 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return smooth_loops(c);
@@ -939,6 +966,9 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     int n = c.n;
     std::vector<int> &conn = c.h_conn, &st = c.h_status;
@@ -1128,6 +1158,9 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_added) *n_added = 0;
@@ -1231,6 +1264,9 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    if (ctx->c.has_pinned)
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
+                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_points) *n_points = 0;

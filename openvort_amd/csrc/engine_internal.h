@@ -98,6 +98,8 @@ struct Ctx {
     // state, SoA over slots
     DBuf x[3], vels[3], vs[3], tan[3], nor[3];
     DBuf fwd, rev, status;
+    DBuf pinw;               // pin_wall per slot (-1 when not pinned); set while has_pinned
+    bool has_pinned = false; // the resident tangle has PINNED / PINNED_SLIP nodes
     DBuf live, src;                 // slot lists
     // RK4 scratch
     DBuf xs[2][3], move[3], k[3], stan[3], snor[3];

@@ -24,18 +24,62 @@ __device__ __forceinline__ double dist3(double ax, double ay, double az, double 
     return sqrt(dot3(dx, dy, dz, dx, dy, dz));
 }
 
-// update_tangent_normal, src/tangle.c:423-502
+// step_node, src/tangle.c:155-194, as a position: `where` links from k.  A
+// pinned node at its open end walks the other way instead and the result
+// is mirrored through its wall (mirror_shift, src/boundaries.c:159-193);
+// the mirrors of nested ends apply innermost first.  A pinned node with no
+// link at all has no stencil: NaN.
+__device__ void step_pos(const TangleParams &P, const double *__restrict__ X,
+                         const double *__restrict__ Y, const double *__restrict__ Z,
+                         const int *__restrict__ fwd, const int *__restrict__ rev,
+                         const int *__restrict__ st, const int *__restrict__ pinw, int k,
+                         int where, double &x, double &y, double &z)
+{
+    int walls[4], nw = 0, i = k;
+    for (int guard = 0; where != 0 && guard < 8; ++guard) {
+        const int s = st[i];
+        const int nx = where > 0 ? fwd[i] : rev[i];
+        if ((s == OVG_PINNED || s == OVG_PINNED_SLIP) && nx < 0) {
+            if (nw < 4) walls[nw++] = pinw[i];
+            where = -where;
+            continue;
+        }
+        i = nx;
+        where += where > 0 ? -1 : 1;
+    }
+    x = X[i]; y = Y[i]; z = Z[i];
+    if (where != 0) x = y = z = __builtin_nan("");
+    for (int j = nw - 1; j >= 0; --j) {
+        const int w = walls[j], a = w >> 1;
+        const double wall = (w & 1) ? P.hi[a] : P.lo[a];
+        if (a == 0) x = 2 * wall - x;
+        else if (a == 1) y = 2 * wall - y;
+        else z = 2 * wall - z;
+    }
+}
+
+// update_tangent_normal, src/tangle.c:423-502.  pinw == nullptr: no pinned
+// nodes, every neighbour is a link away.
 __device__ void tangent_normal(const TangleParams &P, const double *__restrict__ X,
                                const double *__restrict__ Y, const double *__restrict__ Z,
                                const int *__restrict__ fwd, const int *__restrict__ rev,
+                               const int *__restrict__ st, const int *__restrict__ pinw,
                                int k, double t[3], double nrm[3])
 {
-    int i1 = fwd[k], i2 = fwd[i1], im1 = rev[k], im2 = rev[im1];
     double s0x = X[k], s0y = Y[k], s0z = Z[k];
-    double s1x = X[i1], s1y = Y[i1], s1z = Z[i1];
-    double s2x = X[i2], s2y = Y[i2], s2z = Z[i2];
-    double m1x = X[im1], m1y = Y[im1], m1z = Z[im1];
-    double m2x = X[im2], m2y = Y[im2], m2z = Z[im2];
+    double s1x, s1y, s1z, s2x, s2y, s2z, m1x, m1y, m1z, m2x, m2y, m2z;
+    if (pinw) {
+        step_pos(P, X, Y, Z, fwd, rev, st, pinw, k, 1, s1x, s1y, s1z);
+        step_pos(P, X, Y, Z, fwd, rev, st, pinw, k, 2, s2x, s2y, s2z);
+        step_pos(P, X, Y, Z, fwd, rev, st, pinw, k, -1, m1x, m1y, m1z);
+        step_pos(P, X, Y, Z, fwd, rev, st, pinw, k, -2, m2x, m2y, m2z);
+    } else {
+        int i1 = fwd[k], i2 = fwd[i1], im1 = rev[k], im2 = rev[im1];
+        s1x = X[i1]; s1y = Y[i1]; s1z = Z[i1];
+        s2x = X[i2]; s2y = Y[i2]; s2z = Z[i2];
+        m1x = X[im1]; m1y = Y[im1]; m1z = Z[im1];
+        m2x = X[im2]; m2y = Y[im2]; m2z = Z[im2];
+    }
     double w[4][3];
     w[0][0] = s2x; w[0][1] = s2y; w[0][2] = s2z;
     pwrap(P, s0x, s0y, s0z, w[0][0], w[0][1], w[0][2]);
@@ -86,6 +130,7 @@ __device__ void tangent_normal(const TangleParams &P, const double *__restrict__
 __global__ void k_stencil(int n_live, const int *__restrict__ live, const double *__restrict__ X,
                           const double *__restrict__ Y, const double *__restrict__ Z,
                           const int *__restrict__ fwd, const int *__restrict__ rev,
+                          const int *__restrict__ st, const int *__restrict__ pinw,
                           const TangleParams *__restrict__ Pp, double *__restrict__ tx,
                           double *__restrict__ ty, double *__restrict__ tz,
                           double *__restrict__ nx, double *__restrict__ ny,
@@ -96,7 +141,7 @@ __global__ void k_stencil(int n_live, const int *__restrict__ live, const double
     const TangleParams &P = *Pp;
     int k = live[i];
     double t[3], n[3];
-    tangent_normal(P, X, Y, Z, fwd, rev, k, t, n);
+    tangent_normal(P, X, Y, Z, fwd, rev, st, pinw, k, t, n);
     tx[k] = t[0]; ty[k] = t[1]; tz[k] = t[2];
     nx[k] = n[0]; ny[k] = n[1]; nz[k] = n[2];
 }
@@ -108,6 +153,7 @@ void launch_stencil(Ctx &c, const double *const x[3], double *const tan[3],
     int bs = 256, gs = (c.n_live + bs - 1) / bs;
     hipLaunchKernelGGL(k_stencil, dim3(gs), dim3(bs), 0, c.stream, c.n_live,
                        c.live.as<int>(), x[0], x[1], x[2], c.fwd.as<int>(), c.rev.as<int>(),
+                       c.status.as<int>(), c.has_pinned ? c.pinw.as<int>() : nullptr,
                        dparams(c), tan[0], tan[1], tan[2], nor[0], nor[1], nor[2]);
 }
 
@@ -128,8 +174,13 @@ __global__ void k_prep_segments(int n_src, const int *__restrict__ src,
     int m = src[j];
     int f = fwd[m];
     double ax = X[m], ay = Y[m], az = Z[m];
-    double bx = X[f], by = Y[f], bz = Z[f];
-    pwrap(P, ax, ay, az, bx, by, bz);
+    // an open end (a pinned node with no forward link) has no segment
+    // (src/tangle.c:658): a zero-length one adds exactly zero
+    double bx = ax, by = ay, bz = az;
+    if (f >= 0) {
+        bx = X[f]; by = Y[f]; bz = Z[f];
+        pwrap(P, ax, ay, az, bx, by, bz);
+    }
     s1x[j] = ax; s1y[j] = ay; s1z[j] = az;
     s2x[j] = bx; s2y[j] = by; s2z[j] = bz;
     ids[j] = make_int2(m, f);
@@ -151,6 +202,7 @@ struct ComposeArgs {
     const int *live;
     const double *X, *Y, *Z;          // evaluation positions
     const int *fwd, *rev;
+    const int *st, *pinw;             // status, pin walls (pinw null: no pinned nodes)
     const double *tin[3], *nin[3];    // tangents/normals when not computed here
     double *tout[3], *nout[3];        // where computed tangents go (may be null)
     const double *bsp[3], *bsi[3];    // Biot-Savart: primary and images
@@ -183,7 +235,7 @@ __global__ void k_compose(ComposeArgs a)
     int k = a.live[i];
     double t[3], nr[3];
     if (OWN_TAN) {
-        tangent_normal(P, a.X, a.Y, a.Z, a.fwd, a.rev, k, t, nr);
+        tangent_normal(P, a.X, a.Y, a.Z, a.fwd, a.rev, a.st, a.pinw, k, t, nr);
         if (a.tout[0]) {
             for (int c = 0; c < 3; ++c) {
                 a.tout[c][k] = t[c];
@@ -198,9 +250,16 @@ __global__ void k_compose(ComposeArgs a)
     }
     // lia_velocity
     double px = a.X[k], py = a.Y[k], pz = a.Z[k];
-    int inx = a.fwd[k], ipr = a.rev[k];
-    double fx = a.X[inx], fy = a.Y[inx], fz = a.Z[inx];
-    double qx = a.X[ipr], qy = a.Y[ipr], qz = a.Z[ipr];
+    double fx, fy, fz, qx, qy, qz;
+    const int sk = a.pinw ? a.st[k] : OVG_FREE;
+    if (a.pinw) {
+        step_pos(P, a.X, a.Y, a.Z, a.fwd, a.rev, a.st, a.pinw, k, 1, fx, fy, fz);
+        step_pos(P, a.X, a.Y, a.Z, a.fwd, a.rev, a.st, a.pinw, k, -1, qx, qy, qz);
+    } else {
+        int inx = a.fwd[k], ipr = a.rev[k];
+        fx = a.X[inx]; fy = a.Y[inx]; fz = a.Z[inx];
+        qx = a.X[ipr]; qy = a.Y[ipr]; qz = a.Z[ipr];
+    }
     pwrap(P, px, py, pz, fx, fy, fz);           // seg_pwrap(p, next)
     double wx = px, wy = py, wz = pz;
     pwrap(P, qx, qy, qz, wx, wy, wz);           // seg_pwrap(prev, p)
@@ -231,6 +290,21 @@ __global__ void k_compose(ComposeArgs a)
         ve[0] = ve[0] + n0 * ap;
         ve[1] = ve[1] + n1 * ap;
         ve[2] = ve[2] + n2 * ap;
+    }
+    if (sk == OVG_PINNED) {
+        // src/tangle.c:608-615: no self-induced motion, the boundary's velocity
+        for (int c = 0; c < 3; ++c) {
+            v[c] = 0;
+            ve[c] = P.evb[c];
+        }
+    } else if (sk == OVG_PINNED_SLIP) {
+        // src/tangle.c:718-725: the velocity through the wall removed;
+        // boundary_normals (src/boundaries.c:11-18) are +-e_axis
+        const int w = a.pinw[k], ax = w >> 1;
+        double n3[3] = {0, 0, 0};
+        n3[ax] = (w & 1) ? -1.0 : 1.0;
+        double nv = dot3(n3[0], n3[1], n3[2], ve[0], ve[1], ve[2]);
+        for (int c = 0; c < 3; ++c) ve[c] = ve[c] - n3[c] * nv;
     }
     if (a.vs[0])
         for (int c = 0; c < 3; ++c) a.vs[c][k] = v[c];
@@ -267,6 +341,8 @@ void launch_compose(Ctx &c, const double *const x[3], bool own_tangents, int sta
     a.X = x[0]; a.Y = x[1]; a.Z = x[2];
     a.fwd = c.fwd.as<int>();
     a.rev = c.rev.as<int>();
+    a.st = c.status.as<int>();
+    a.pinw = c.has_pinned ? c.pinw.as<int>() : nullptr;
     a.P = dparams(c);
     for (int q = 0; q < 3; ++q) {
         a.tin[q] = c.tan[q].as<double>();

@@ -118,26 +118,53 @@ static void segment_field(const double *vnodes, const int *conn,
     ovo_segment_field(vnodes + 3 * i, r2, r, p->kappa, out);
 }
 
-/* step_node, src/tangle.c:155-194, FREE nodes only (the pinned branch
- * needs mirror walls, which the engine does not support). */
-static const double *step_node(const double *vnodes, const int *conn, int i,
-                               int where)
+/* mirror_shift, src/boundaries.c:159-193: the coordinate of the wall's
+ * axis becomes 2 * wall - v */
+static void mirror_shift(const double *v, int wall, const ovg_params *p, double *out)
 {
-    while (where > 0) { i = conn[2 * i]; --where; }
-    while (where < 0) { i = conn[2 * i + 1]; ++where; }
-    return vnodes + 3 * i;
+    for (int c = 0; c < 3; ++c) out[c] = v[c];
+    int a = wall / 2;
+    double wall_pos = wall % 2 ? p->box_hi[a] : p->box_lo[a];
+    out[a] = 2 * wall_pos - out[a];
+}
+
+/* step_node, src/tangle.c:155-194, into out: FREE nodes walk the links; a
+ * pinned node (status NULL = all FREE) at its open end walks the other way
+ * and mirrors the result through its wall. */
+static void step_node(const double *vnodes, const int *conn, const int *status,
+                      const ovg_params *p, int i, int where, double *out)
+{
+    if (where == 0) {
+        for (int c = 0; c < 3; ++c) out[c] = vnodes[3 * i + c];
+        return;
+    }
+    int st = status ? status[2 * i] : OVG_FREE;
+    if (st == OVG_PINNED || st == OVG_PINNED_SLIP) {
+        int next = where > 0 ? conn[2 * i] : conn[2 * i + 1];
+        if (next >= 0) {
+            step_node(vnodes, conn, status, p, next, where > 0 ? where - 1 : where + 1, out);
+            return;
+        }
+        double o[3];
+        step_node(vnodes, conn, status, p, i, -where, o);
+        mirror_shift(o, status[2 * i + 1], p, out);
+        return;
+    }
+    step_node(vnodes, conn, status, p, where > 0 ? conn[2 * i] : conn[2 * i + 1],
+              where > 0 ? where - 1 : where + 1, out);
 }
 
 /* update_tangent_normal, src/tangle.c:423-502 */
-static void tangent_normal(const double *vnodes, const int *conn,
+static void tangent_normal(const double *vnodes, const int *conn, const int *status,
                            const ovg_params *p, int k, double *tan,
                            double *nor)
 {
     const double *s0 = vnodes + 3 * k;
-    const double *s1 = step_node(vnodes, conn, k, 1);
-    const double *s2 = step_node(vnodes, conn, k, 2);
-    const double *sm1 = step_node(vnodes, conn, k, -1);
-    const double *sm2 = step_node(vnodes, conn, k, -2);
+    double s1[3], s2[3], sm1[3], sm2[3];
+    step_node(vnodes, conn, status, p, k, 1, s1);
+    step_node(vnodes, conn, status, p, k, 2, s2);
+    step_node(vnodes, conn, status, p, k, -1, sm1);
+    step_node(vnodes, conn, status, p, k, -2, sm2);
     double w[4][3], w12[3], wm12[3];
     pwrap(s0, s2, p, w[0]);
     pwrap(s0, s1, p, w[1]);
@@ -179,7 +206,7 @@ void ovo_tangents_normals(int n, const double *vnodes, const int *conn,
 {
     for (int k = 0; k < n; ++k) {
         if (status[2 * k] == OVG_EMPTY) continue;
-        tangent_normal(vnodes, conn, p, k, tangents + 3 * k, normals + 3 * k);
+        tangent_normal(vnodes, conn, status, p, k, tangents + 3 * k, normals + 3 * k);
     }
 }
 
@@ -206,12 +233,14 @@ static void ext_flow(const ovg_flow *f, double t, double *res)
 }
 
 /* lia_velocity, src/tangle.c:568-592 */
-static void lia(const double *vnodes, const int *conn, const ovg_params *p,
-                int i, const double *tan, const double *nor, double *out)
+static void lia(const double *vnodes, const int *conn, const int *status,
+                const ovg_params *p, int i, const double *tan, const double *nor,
+                double *out)
 {
     const double *x = vnodes + 3 * i;
-    const double *next = step_node(vnodes, conn, i, 1);
-    const double *prev = step_node(vnodes, conn, i, -1);
+    double next[3], prev[3];
+    step_node(vnodes, conn, status, p, i, 1, next);
+    step_node(vnodes, conn, status, p, i, -1, prev);
     double wf[3], wr[3];
     pwrap(x, next, p, wf);
     pwrap(prev, x, p, wr);
@@ -510,7 +539,7 @@ static void velocity_one(int n, const double *vnodes, const int *conn,
         return;
     }
     const double *x = vnodes + 3 * k;
-    lia(vnodes, conn, p, k, tangents + 3 * k, normals + 3 * k, vs);
+    lia(vnodes, conn, status, p, k, tangents + 3 * k, normals + 3 * k, vs);
     double evs[3];
     ext_flow(&p->vs, t, evs);
     for (int c = 0; c < 3; ++c) vs[c] = vs[c] + evs[c];
@@ -560,6 +589,15 @@ static void velocity_one(int n, const double *vnodes, const int *conn,
         cross3(tmp, tn, tmp);
         for (int c = 0; c < 3; ++c) tmp[c] = tmp[c] * -p->alpha_p;
         for (int c = 0; c < 3; ++c) vels[c] = vels[c] + tmp[c];
+    }
+    if (status[2 * k] == OVG_PINNED_SLIP) {  /* src/tangle.c:718-725 */
+        int w = status[2 * k + 1];
+        double nrm[3] = {0, 0, 0};
+        nrm[w / 2] = w % 2 ? -1 : 1;  /* boundary_normals, src/boundaries.c:11-18 */
+        double nv = 0;
+        for (int c = 0; c < 3; ++c) nv += nrm[c] * vels[c];
+        for (int c = 0; c < 3; ++c) nrm[c] = nrm[c] * nv;
+        for (int c = 0; c < 3; ++c) vels[c] = vels[c] - nrm[c];
     }
 }
 
@@ -786,8 +824,8 @@ int ovo_reconnect(int n, const double *vnodes, int *conn, const int *status,
             if (flag[l]) continue;
             double w[3];
             pwrap(vnodes + 3 * k, vnodes + 3 * l, p, w);
-            tangent_normal(vnodes, conn, p, k, tn + 3 * k, nm + 3 * k);
-            tangent_normal(vnodes, conn, p, l, tn + 3 * l, nm + 3 * l);
+            tangent_normal(vnodes, conn, status, p, k, tn + 3 * k, nm + 3 * k);
+            tangent_normal(vnodes, conn, status, p, l, tn + 3 * l, nm + 3 * l);
             const double *d1 = tn + 3 * k, *d2 = tn + 3 * l;
             if (len3(d1) < 0.5 || len3(d2) < 0.5) continue;
             double calpha = ndot3(d1, d2);
@@ -931,8 +969,8 @@ static int add_point(int *N, int cap, double *vnodes, int *conn, int *status,
     status[2 * nw + 1] = -1;
     int next = conn[2 * pt];
     double t0[3], n0[3], t1[3], n1[3];
-    tangent_normal(vnodes, conn, p, pt, t0, n0);
-    tangent_normal(vnodes, conn, p, next, t1, n1);
+    tangent_normal(vnodes, conn, status, p, pt, t0, n0);
+    tangent_normal(vnodes, conn, status, p, next, t1, n1);
     double s0[3], s1[3];
     memcpy(s0, vnodes + 3 * pt, 24);
     pwrap(s0, vnodes + 3 * next, p, s1);

@@ -84,3 +84,54 @@ def test_wall_list_traversal_is_rejected():
         with ov.Engine(p):
             pass
     assert ei.value.code == ov.EUNSUPPORTED
+
+
+PINNED_CASES = [
+    # (method, traverse, slip)
+    (ov.METHOD_DIRECT, ov.TRAVERSE_POINT, False),
+    (ov.METHOD_DIRECT, ov.TRAVERSE_POINT, True),
+    (ov.METHOD_BH, ov.TRAVERSE_POINT, False),
+    (ov.METHOD_BH, ov.TRAVERSE_GROUP, True),
+]
+
+
+@pytest.mark.parametrize("method,traverse,slip", PINNED_CASES)
+def test_pinned_lines_vs_oracle(oracle, method, traverse, slip):
+    from test_walls import pinned_case
+    t, p = pinned_case(method, slip)
+    p.traverse = traverse
+    live = t.live
+    say(f"pinned lines (slip {slip}) method {method} traverse {traverse}: {len(live)} points")
+    want = oracle.update_tangle(t, p)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        e.update_tangle(0.0)
+        o = e.download()
+        assert rel_l2(o["tangents"][live], want["tangents"][live]) < 1e-14
+        assert rel_l2(o["normals"][live], want["normals"][live]) < 1e-14
+        assert rel_l2(o["vs"][live], want["vs"][live]) < 1e-10
+        assert rel_l2(o["vels"][live], want["vels"][live]) < 1e-10
+        ends = np.where(t.status[:, 0] != ov.FREE)[0]
+        if slip:
+            assert np.all(o["vels"][ends, 2] == 0)
+        else:
+            assert np.all(o["vels"][ends] == 0)
+        dt = 1e-4
+        e.rk4_step(0.0, dt)
+        x = e.download()["vnodes"][live]
+    w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
+    oracle.rk4_step(w, p, 0.0, dt, want["vels"])
+    assert np.abs(x - w.vnodes[live]).max() < 1e-14
+
+
+def test_pinned_lines_refuse_topology_passes():
+    from test_walls import pinned_case
+    t, p = pinned_case(ov.METHOD_BH, False)
+    p.traverse = ov.TRAVERSE_GROUP
+    say("pinned lines: reconnect / remesh / smooth refused")
+    with ov.Engine(p) as e:
+        e.upload(t)
+        for call in (lambda: e.reconnect(0.0), lambda: e.remesh(1e-3, 3e-3), e.smooth):
+            with pytest.raises(ov.OvgError) as ei:
+                call()
+            assert ei.value.code == ov.EUNSUPPORTED

@@ -130,3 +130,86 @@ def test_mirror_image_is_the_reflected_reversed_copy(oracle):
     q2.n_images = 0
     alone = oracle.update_tangle(t, q2)
     assert rel_l2(alone["vs"][live], base["vs"][live]) > 1e-3
+
+
+def pinned_case(method, slip=False, seed=11, lines=3, rings=3):
+    """Wavy lines from the DOWN wall (z = 0) to the UP wall (z = D), their
+    end points pinned there, plus rings; parallel-planes geometry."""
+    D, dxi = 0.1, 2e-3
+    t = rings_inside(seed, rings, D, 0.012, dxi)
+    rng = np.random.default_rng(seed + 1)
+    xs, conns, sts = [t.vnodes], [t.conn], [t.status]
+    off = t.n
+    # 52 points: spacing D/51, so no point but the pinned ends lies on a
+    # dyadic plane of the box, where the tuned trees break ties upward
+    # and the reference tree downward
+    m = 52
+    pin = ov.PINNED_SLIP if slip else ov.PINNED
+    for _ in range(lines):
+        x0, y0 = rng.uniform(0.02, 0.08, 2)
+        amp, ph = rng.uniform(0.002, 0.006), rng.uniform(0, 2 * np.pi)
+        z = np.linspace(0.0, D, m)
+        pts = np.stack([x0 + amp * np.sin(2 * np.pi * z / D + ph),
+                        y0 + amp * np.cos(3 * np.pi * z / D + ph), z], 1)
+        j = np.arange(m)
+        c = np.stack([np.where(j < m - 1, off + j + 1, -1), np.where(j > 0, off + j - 1, -1)], 1)
+        st = np.zeros((m, 2), dtype=np.int32)
+        st[:, 1] = -1
+        st[0] = (pin, ov.FACE_DOWN)
+        st[-1] = (pin, ov.FACE_UP)
+        xs.append(pts)
+        conns.append(c)
+        sts.append(st)
+        off += m
+    tt = ov.Tangle(np.concatenate(xs), np.concatenate(conns).astype(np.int32),
+                   np.concatenate(sts).astype(np.int32))
+    p = configs.params(D, dxi, configs.T16, 0.5, method=method, theta=0.4)
+    for f, w in enumerate(WALLS[ov.IMAGES_WALL_PARALLEL_PLANES]):
+        p.wall[f] = w
+    ov.set_wall_images(p, ov.IMAGES_WALL_PARALLEL_PLANES)
+    return tt, p
+
+
+@pytest.mark.parametrize("slip", [False, True])
+@pytest.mark.parametrize("method", [ov.METHOD_DIRECT, ov.METHOD_BH])
+def test_pinned_lines_bitexact_vs_reference(oracle, ref, method, slip):
+    """Lines pinned to mirror walls: the stencil walks past an open end into
+    the mirrored ghost (step_node, src/tangle.c:170-185), PINNED nodes take
+    the boundary velocity (:608-615), PINNED_SLIP ones lose their normal
+    velocity (:718-725)."""
+    t, p = pinned_case(method, slip)
+    want = oracle.update_tangle(t, p)
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.update_tangle(0.0)
+    got = r.get()
+    live = t.live
+    for key in ("tangents", "normals", "vs", "vels"):
+        assert np.array_equal(got[key][live], want[key][live]), key
+    ends = np.where(t.status[:, 0] != ov.FREE)[0]
+    if slip:  # no velocity through the wall
+        assert np.all(want["vels"][ends, 2] == 0)
+    else:
+        assert np.all(want["vels"][ends] == 0) and np.all(want["vs"][ends] == 0)
+
+
+def test_pinned_ghost_is_the_mirror_image(oracle):
+    """A straight line normal to the wall: its ghost continues it, so the
+    pinned end's tangent is the line direction and its normal vanishes."""
+    D, dxi = 0.1, 2e-3
+    m = 21
+    z = np.linspace(0.0, D, m)
+    x = np.stack([np.full(m, 0.05), np.full(m, 0.04), z], 1)
+    j = np.arange(m)
+    c = np.stack([np.where(j < m - 1, j + 1, -1), np.where(j > 0, j - 1, -1)], 1)
+    st = np.zeros((m, 2), dtype=np.int32)
+    st[:, 1] = -1
+    st[0] = (ov.PINNED, ov.FACE_DOWN)
+    st[-1] = (ov.PINNED, ov.FACE_UP)
+    t = ov.Tangle(x, c.astype(np.int32), st)
+    p = configs.params(D, dxi, configs.T16, 0.5, method=ov.METHOD_DIRECT)
+    for f, w in enumerate(WALLS[ov.IMAGES_WALL_PARALLEL_PLANES]):
+        p.wall[f] = w
+    ov.set_wall_images(p, ov.IMAGES_WALL_PARALLEL_PLANES)
+    tn, nm = oracle.tangents_normals(t, p)
+    assert np.allclose(tn[[0, -1]], [[0, 0, 1], [0, 0, 1]], atol=1e-12)
+    assert np.abs(nm[[0, -1]]).max() < 1e-9
