@@ -40,7 +40,7 @@ extern "C" {
 #define OVG_EINVAL       1   /* bad argument, shape or state */
 #define OVG_ENOMEM       2   /* device or host allocation failed / cap exceeded */
 #define OVG_EDEVICE      3   /* HIP runtime error, no device, missing code object */
-#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. pinned nodes) */
+#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. topology passes on pinned lines) */
 #define OVG_ELIMIT       5   /* an internal fixed-size limit was exceeded */
 
 /* ---- node status and wall kinds (reference enums) -------------------- */
