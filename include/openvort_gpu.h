@@ -218,7 +218,10 @@ int ovg_get_params(const ovg_ctx *ctx, ovg_params *p);
  * Upload a tangle of n slots.  Replaces positions, connections and status;
  * velocities, tangents and normals become zero.  Re-upload whenever the
  * host changed capacity or topology (the reference's expand_tangle
- * reallocates, src/tangle.c:68-101).
+ * reallocates, src/tangle.c:68-101).  FREE slots need both links; PINNED
+ * and PINNED_SLIP slots (pin_wall 0..5, OVG_FACE_*) may end a line
+ * (link -1).  A tangle with pinned slots is evaluated and stepped; its
+ * topology passes return OVG_EUNSUPPORTED.
  */
 int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes,
                const int *connections, const int *status);
@@ -371,7 +374,8 @@ int ovg_synchronize(ovg_ctx *ctx);
  * are empty than needed), linked in order and closed.  *n_points = points
  * added.  Loops with no point are refused (OVG_EINVAL).
  * Injection of straight line pairs (inject_line_pairs) pins the lines to
- * walls; pinned nodes are not supported by this engine.
+ * walls: pinned lines are evaluated and stepped by this engine, but the
+ * topology passes refuse them, so the injection is the reference's.
  */
 int ovg_loop_points(const double center[3], const double dir[3], double r, double dl_min,
                     double dl_max, double *xyz, int cap, int *n_points);
