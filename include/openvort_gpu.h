@@ -173,7 +173,8 @@ typedef struct ovg_params {
                                      mode adds it unreflected, as the
                                      reference does (src/tangle.c:678-683).
                                      0 = pure shift.  At most one face per
-                                     axis.  Not taken by OVG_TRAVERSE_LIST.  */
+                                     axis.  reuse_lists takes shift images
+                                     only.                                      */
     int32_t pad_;
 } ovg_params;
 

@@ -106,7 +106,6 @@ static ovg_ctx *prepare(const struct tangle_state *t)
     for (int f = 0; f < 6; ++f) p.wall[f] = (int)t->box.wall[f];
     if (t->bimg.n > 26) die("image table larger than 26", OVG_EINVAL);
     p.n_images = t->bimg.n;
-    int mirrored = 0;
     for (int j = 0; j < t->bimg.n; ++j) {
         const struct image_tangle *im = &t->bimg.images[j];
         for (int k = 0; k < 3; ++k) p.images[j][k] = im->shift[k];
@@ -114,15 +113,13 @@ static ovg_ctx *prepare(const struct tangle_state *t)
         p.image_reflect[j] = 0;
         for (int r = 0; r < im->number_of_reflects && r < 3; ++r)
             if (im->reflect[r] >= 0 && im->reflect[r] < 6) p.image_reflect[j] |= 1 << im->reflect[r];
-        mirrored |= p.image_reflect[j] != 0;
     }
     p.method = use_BH ? OVG_METHOD_BH : OVG_METHOD_DIRECT;
     p.theta = BH_resolution;
     p.precision = g_precision;
     /* mixed precision: pairs within two of the largest point spacings stay FP64 */
     p.mixed_near = 2 * global_dl_max;
-    /* the list traversal takes shift images only */
-    p.traverse = mirrored && g_traverse == OVG_TRAVERSE_LIST ? OVG_TRAVERSE_GROUP : g_traverse;
+    p.traverse = g_traverse;
     p.leaf_size = g_leaf;
     p.rec_dist = rec_dist;
     p.rec_angle = reconnection_angle_cutoff;

@@ -418,10 +418,10 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
                                  "image_reflect[%d] mirrors through both faces of axis %d", j, a);
         mirrored |= m != 0;
     }
-    if (mirrored && p->method == OVG_METHOD_BH && p->traverse == OVG_TRAVERSE_LIST)
+    if (mirrored && p->reuse_lists)
         return set_error(OVG_EUNSUPPORTED,
-                         "mirror images: the list traversal takes shift images only "
-                         "(use OVG_TRAVERSE_GROUP or OVG_TRAVERSE_POINT)");
+                         "mirror images: reuse_lists (recorded interaction lists) takes shift "
+                         "images only");
     if (p->method != OVG_METHOD_DIRECT && p->method != OVG_METHOD_BH)
         return set_error(OVG_EINVAL, "unknown method %d", p->method);
     if (p->precision != OVG_PREC_FP64 && p->precision != OVG_PREC_MIXED)

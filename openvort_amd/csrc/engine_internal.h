@@ -80,8 +80,9 @@ struct TreeBufs {
 // counters[kGroupCounter]: number of target groups of the list traversal
 constexpr int kGroupCounter = 40;
 // list traversal launch modes: walk; walk and record each group's accepted
-// entries; replay the recorded entries at new positions (same tree)
-constexpr int kWalk = 0, kRecord = 1, kReplay = 2;
+// entries; replay the recorded entries at new positions (same tree); walk
+// with mirror images (kMirror, chosen by the launch when an image reflects)
+constexpr int kWalk = 0, kRecord = 1, kReplay = 2, kMirror = 3;
 // evaluation phases (eval_bs)
 constexpr int kEvalPlain = 0, kEvalStep = 1, kEvalStage = 2;
 

@@ -123,6 +123,21 @@ __device__ __forceinline__ int xcd_group(int b, int nb)
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// A mirrored copy's source (kMirror), one axis: segment (x, x + q)
+// reflected on the mirrored axes (r), with sh = 2 * wall there and L * shift
+// on the others; c is the reflected x.  With M the reflection, the
+// reference's M v(M r + c) equals det(M) times the field of the reflected
+// segments at r, so a copy mirrored on an odd number of axes also reverses
+// its segments (rev): start x~ + q~, vector -q~.
+__device__ __forceinline__ void mirror_axis(bool r, bool rev, double sh, double &x, double &q,
+                                            double &c)
+{
+    const double qt = r ? -q : q;
+    c = r ? sh - x : x + sh;
+    x = rev ? c + qt : c;
+    q = rev ? -qt : qt;
+}
+
 // MODE: kWalk, kRecord (walk and keep every group's accepted entries, in
 // emission order, in rec[g * rec_cap ..], their count in rec_count[g], -1
 // when they do not fit), kReplay (a group with a record evaluates those
@@ -143,6 +158,13 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     unsigned long long *__restrict__ icount)
 {
     static_assert(64 % PPW == 0 && 4 * 64 / PPW <= kListFar, "slices of a full group");
+    // kMirror: copies may mirror.  A mirrored copy of segment (x, q) is the
+    // segment reflected through the walls, reversed when the copy mirrors an
+    // odd number of axes (the reference's reflected target and reflected
+    // velocity, src/tangle.c:380-410,644-646, moved onto the source; see
+    // mirror_axis); acceptance takes the reflected centre (2 w - x on a
+    // mirrored axis, x + L s on the others).
+    constexpr bool MIR = MODE == kMirror;
     static_assert(kListFar <= 128, "the far flush checks two entries per lane");
     __shared__ int2 stk[kListStack];
     __shared__ float4 farA[kListFar];  // offset a from the copy's group centre, qx
@@ -150,14 +172,18 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     __shared__ float farC[kListFar];   // (a x q)_z
     __shared__ unsigned nearI[kListNear];
     __shared__ int bEnd[64], bStart[64], bCopy[64];  // bucket cells of one round
-    __shared__ double shL[27][3];                    // copy shifts
+    __shared__ double shL[27][3];                    // copy shifts (2 * wall on mirrored axes)
+    __shared__ int shM[27];                          // mirrored axes of each copy (kMirror)
 
     const TangleParams &P = *Pp;
     const int lane = lane_id();
     const int n_groups = *n_groups_p;
     if (lane < 27) {
         const bool img = lane >= 1 && lane <= P.n_images;
-        for (int q = 0; q < 3; ++q) shL[lane][q] = img ? P.Ls[lane - 1][q] : 0.0;
+        const int m = MIR && img ? P.refl[lane - 1] : 0;
+        for (int q = 0; q < 3; ++q)
+            shL[lane][q] = !img ? 0.0 : (m >> q & 1) ? P.W2[lane - 1][q] : P.Ls[lane - 1][q];
+        if (MIR) shM[lane] = m;
     }
     __syncthreads();
     for (int g = xcd_group(blockIdx.x, gridDim.x); g < n_groups; g += gridDim.x) {
@@ -290,6 +316,18 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                 Rx = sx[idx]; Ry = sy[idx]; Rz = sz[idx];
                 Qx = sqx[idx]; Qy = sqy[idx]; Qz = sqz[idx];
             }
+            if (MIR && shM[cp]) {
+                const int m = shM[cp];
+                double c0, c1, c2;
+                const bool rv = __popc(m) & 1;
+                mirror_axis(m & 1, rv, shL[cp][0], Rx, Qx, c0);
+                mirror_axis(m & 2, rv, shL[cp][1], Ry, Qy, c1);
+                mirror_axis(m & 4, rv, shL[cp][2], Rz, Qz, c2);
+                Rx = Rx - rx;
+                Ry = Ry - ry;
+                Rz = Rz - rz;
+                return;
+            }
             Rx = (Rx + shL[cp][0]) - rx;
             Ry = (Ry + shL[cp][1]) - ry;
             Rz = (Rz + shL[cp][2]) - rz;
@@ -411,8 +449,16 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             nd = nodes[d];
             skp = min(max(nd.skip, d + 1), end);
             if (nd.count > 0) {
-                dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix, hiy,
-                                hiz);
+                if (MIR && shM[cp]) {
+                    const int m = shM[cp];
+                    dd2 = box_dist2((m & 1) ? shx - nd.cx : nd.cx + shx,
+                                    (m & 2) ? shy - nd.cy : nd.cy + shy,
+                                    (m & 4) ? shz - nd.cz : nd.cz + shz, lox, loy, loz, hix,
+                                    hiy, hiz);
+                } else {
+                    dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix,
+                                    hiy, hiz);
+                }
                 if (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs)
                     kind = 1;
                 else if (nd.count <= leaf || skp == d + 1)
@@ -432,10 +478,23 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (opn) stk[base + (int)sib] = make_int2(d + 1, skp | (cp << kCopyShift));
         sp += __popcll(ms) + __popcll(mo);
         // monopoles and single segments
+        // (emit is wave-wide: the entry is formed per lane first)
+        float mex = (float)((nd.cx + shx) - cx), mey = (float)((nd.cy + shy) - cy),
+              mez = (float)((nd.cz + shz) - cz);
+        float mgx = (float)nd.qx, mgy = (float)nd.qy, mgz = (float)nd.qz;
+        if (MIR && shM[cp]) {
+            const int m = shM[cp];
+            double x0 = nd.cx, y0 = nd.cy, z0 = nd.cz, q0 = nd.qx, q1 = nd.qy, q2 = nd.qz;
+            double c0, c1, c2;
+            const bool rv = __popc(m) & 1;
+            mirror_axis(m & 1, rv, shx, x0, q0, c0);
+            mirror_axis(m & 2, rv, shy, y0, q1, c1);
+            mirror_axis(m & 4, rv, shz, z0, q2, c2);
+            mex = (float)(x0 - cx); mey = (float)(y0 - cy); mez = (float)(z0 - cz);
+            mgx = (float)q0; mgy = (float)q1; mgz = (float)q2;
+        }
         emit(kind == 1, !MIXED || dd2 < near2,
-             (unsigned)d | ((unsigned)cp << kCopyShift) | kNodeBit,
-             (float)((nd.cx + shx) - cx), (float)((nd.cy + shy) - cy),
-             (float)((nd.cz + shz) - cz), (float)nd.qx, (float)nd.qy, (float)nd.qz);
+             (unsigned)d | ((unsigned)cp << kCopyShift) | kNodeBit, mex, mey, mez, mgx, mgy, mgz);
         // buckets: the round's bucket cells laid end to end, expanded 64
         // segments per pass (one per lane, its cell found by binary search
         // over the cells' end offsets)
@@ -477,11 +536,24 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                     }
                     sg = bStart[lo] + j - (lo ? bEnd[lo - 1] : 0);
                     bcp = bCopy[lo];
+                    if (MIR && shM[bcp]) {
+                        const int m = shM[bcp];
+                        double x0 = sx[sg], y0 = sy[sg], z0 = sz[sg];
+                        double q0 = sqx[sg], q1 = sqy[sg], q2 = sqz[sg], c0, c1, c2;
+                        const bool rv = __popc(m) & 1;
+                        mirror_axis(m & 1, rv, shL[bcp][0], x0, q0, c0);
+                        mirror_axis(m & 2, rv, shL[bcp][1], y0, q1, c1);
+                        mirror_axis(m & 4, rv, shL[bcp][2], z0, q2, c2);
+                        gx = (float)q0; gy = (float)q1; gz = (float)q2;
+                        ex = (float)(x0 - cx); ey = (float)(y0 - cy); ez = (float)(z0 - cz);
+                        b2 = box_dist2(c0, c1, c2, lox, loy, loz, hix, hiy, hiz);
+                    } else {
                     const double ux = sx[sg] + shL[bcp][0], uy = sy[sg] + shL[bcp][1],
                                  uz = sz[sg] + shL[bcp][2];
                     gx = (float)sqx[sg]; gy = (float)sqy[sg]; gz = (float)sqz[sg];
                     ex = (float)(ux - cx); ey = (float)(uy - cy); ez = (float)(uz - cz);
                     b2 = box_dist2(ux, uy, uz, lox, loy, loz, hix, hiy, hiz);
+                    }
                 }
                 emit(want, !MIXED || b2 < near2, (unsigned)sg | ((unsigned)bcp << kCopyShift),
                      ex, ey, ez, gx, gy, gz);
@@ -618,7 +690,10 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                          c.n_live);
     TreeBufs &T = c.tree;
     const int ppw = list_points_per_wave(c.params);
-    const int mode = c.list_mode;
+    int mode = c.list_mode;
+    bool mirrored = false;
+    for (int j = 0; j < c.params.n_images; ++j) mirrored |= c.params.image_reflect[j] != 0;
+    if (mirrored) mode = kMirror;  // reuse_lists is refused with mirror images
     // a replay keeps the recorded run's groups (same tree, same order)
     if (mode != kReplay) {
         switch (ppw) {
@@ -660,13 +735,15 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
-    auto by_mode = [&](auto kw, auto kr, auto kp) {
+    auto by_mode = [&](auto kw, auto kr, auto kp, auto km) {
         if (mode == kRecord) go(kr);
         else if (mode == kReplay) go(kp);
+        else if (mode == kMirror) go(km);
         else go(kw);
     };
-#define OVG_LIST_MODES(M, W) \
-    by_mode(k_bh_list<M, W, kWalk>, k_bh_list<M, W, kRecord>, k_bh_list<M, W, kReplay>)
+#define OVG_LIST_MODES(M, W)                                                                \
+    by_mode(k_bh_list<M, W, kWalk>, k_bh_list<M, W, kRecord>, k_bh_list<M, W, kReplay>, \
+            k_bh_list<M, W, kMirror>)
     switch (ppw) {
     case 8: mixed ? OVG_LIST_MODES(true, 8) : OVG_LIST_MODES(false, 8); break;
     case 16: mixed ? OVG_LIST_MODES(true, 16) : OVG_LIST_MODES(false, 16); break;
@@ -753,12 +830,17 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
+    bool mirrored = false;
+    for (int j = 0; j < c.params.n_images; ++j) mirrored |= c.params.image_reflect[j] != 0;
+#define OVG_LIST_WALK(M, W) \
+    (mirrored ? go(k_bh_list<M, W, kMirror>) : go(k_bh_list<M, W, kWalk>))
     switch (ppw) {
-    case 8: mixed ? go(k_bh_list<true, 8, kWalk>) : go(k_bh_list<false, 8, kWalk>); break;
-    case 16: mixed ? go(k_bh_list<true, 16, kWalk>) : go(k_bh_list<false, 16, kWalk>); break;
-    case 32: mixed ? go(k_bh_list<true, 32, kWalk>) : go(k_bh_list<false, 32, kWalk>); break;
-    default: mixed ? go(k_bh_list<true, 64, kWalk>) : go(k_bh_list<false, 64, kWalk>); break;
+    case 8: mixed ? OVG_LIST_WALK(true, 8) : OVG_LIST_WALK(false, 8); break;
+    case 16: mixed ? OVG_LIST_WALK(true, 16) : OVG_LIST_WALK(false, 16); break;
+    case 32: mixed ? OVG_LIST_WALK(true, 32) : OVG_LIST_WALK(false, 32); break;
+    default: mixed ? OVG_LIST_WALK(true, 64) : OVG_LIST_WALK(false, 64); break;
     }
+#undef OVG_LIST_WALK
     // the host vectors must outlive the copies above
     OVG_HIP(hipStreamSynchronize(c.stream));
     return OVG_OK;

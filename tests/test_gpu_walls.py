@@ -76,10 +76,65 @@ def test_wall_group_mixed_precision(oracle):
     assert rel_l2(out[ov.PREC_MIXED]["vs"][live], out[ov.PREC_FP64]["vs"][live]) < 1e-5
 
 
-def test_wall_list_traversal_is_rejected():
+@pytest.mark.parametrize("config", [ov.IMAGES_WALL_CANAL, ov.IMAGES_WALL_1_18,
+                                    ov.IMAGES_WALL_PARALLEL_PLANES])
+def test_wall_list_small_theta_matches_point(config):
+    """theta -> 0: the list traversal (mirrored copies as reflected, reversed
+    sources) and the point traversal (reflected targets, reflected
+    velocities) reduce to the same segment sums; summation order and the
+    frame of the rounding alone separate them."""
+    t, p = wall_case(config, ov.METHOD_BH, theta=1e-6)
+    live = t.live
+    say(f"walls table {config}: list vs point traversal at theta 1e-6, FP64")
+    out = {}
+    for trav in (ov.TRAVERSE_LIST, ov.TRAVERSE_POINT):
+        p.traverse = trav
+        with ov.Engine(p) as e:
+            e.upload(t)
+            e.update_tangle(0.0)
+            out[trav] = e.download()
+    for key in ("vs", "vels"):
+        assert rel_l2(out[ov.TRAVERSE_LIST][key][live], out[ov.TRAVERSE_POINT][key][live]) < 1e-11
+
+
+def test_pinned_list_small_theta_matches_point():
+    from test_walls import pinned_case
+    t, p = pinned_case(ov.METHOD_BH, True)
+    p.theta = 1e-6
+    live = t.live
+    say("pinned lines: list vs point traversal at theta 1e-6, FP64")
+    out = {}
+    for trav in (ov.TRAVERSE_LIST, ov.TRAVERSE_POINT):
+        p.traverse = trav
+        with ov.Engine(p) as e:
+            e.upload(t)
+            e.update_tangle(0.0)
+            out[trav] = e.download()
+    assert rel_l2(out[ov.TRAVERSE_LIST]["vels"][live], out[ov.TRAVERSE_POINT]["vels"][live]) < 1e-11
+
+
+@pytest.mark.parametrize("ppw", [16, 32])
+def test_wall_list_mixed_vs_fp64(ppw):
     t, p = wall_case(ov.IMAGES_WALL_CANAL, ov.METHOD_BH)
     p.traverse = ov.TRAVERSE_LIST
-    say("walls with the list traversal: EUNSUPPORTED")
+    p.points_per_wave = ppw
+    live = t.live
+    say(f"walls canal, list traversal {ppw} points per wave: FP64 and mixed")
+    out = {}
+    for prec in (ov.PREC_FP64, ov.PREC_MIXED):
+        p.precision = prec
+        with ov.Engine(p) as e:
+            e.upload(t)
+            e.update_tangle(0.0)
+            out[prec] = e.download()
+    assert rel_l2(out[ov.PREC_MIXED]["vs"][live], out[ov.PREC_FP64]["vs"][live]) < 1e-6
+
+
+def test_wall_reuse_lists_is_rejected():
+    t, p = wall_case(ov.IMAGES_WALL_CANAL, ov.METHOD_BH)
+    p.traverse = ov.TRAVERSE_LIST
+    p.reuse_lists = 1
+    say("walls with reuse_lists: EUNSUPPORTED")
     with pytest.raises(ov.OvgError) as ei:
         with ov.Engine(p):
             pass
