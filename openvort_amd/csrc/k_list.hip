@@ -445,8 +445,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         int skp = 0;   // subtree end, kept inside the popped range
         double dd2 = 0;
         int st = 0;  // first sorted segment of the cell (buckets)
+        double ncx = 0, ncy = 0, ncz = 0;  // the node's centre in this copy
         if (act) {
             nd = nodes[d];
+            ncx = nd.cx + shx; ncy = nd.cy + shy; ncz = nd.cz + shz;
             skp = min(max(nd.skip, d + 1), end);
             if (nd.count > 0) {
                 if (MIR && shM[cp]) {
@@ -456,8 +458,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
                                     (m & 4) ? shz - nd.cz : nd.cz + shz, lox, loy, loz, hix,
                                     hiy, hiz);
                 } else {
-                    dd2 = box_dist2(nd.cx + shx, nd.cy + shy, nd.cz + shz, lox, loy, loz, hix,
-                                    hiy, hiz);
+                    dd2 = box_dist2(ncx, ncy, ncz, lox, loy, loz, hix, hiy, hiz);
                 }
                 if (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs)
                     kind = 1;
@@ -479,8 +480,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         sp += __popcll(ms) + __popcll(mo);
         // monopoles and single segments
         // (emit is wave-wide: the entry is formed per lane first)
-        float mex = (float)((nd.cx + shx) - cx), mey = (float)((nd.cy + shy) - cy),
-              mez = (float)((nd.cz + shz) - cz);
+        float mex = (float)(ncx - cx), mey = (float)(ncy - cy), mez = (float)(ncz - cz);
         float mgx = (float)nd.qx, mgy = (float)nd.qy, mgz = (float)nd.qz;
         if (MIR && shM[cp]) {
             const int m = shM[cp];
