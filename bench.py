@@ -227,7 +227,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
-              f"(asked for --gpus {args.gpus}); see DESIGN.md for how it would shard.",
+              f"(asked for --gpus {args.gpus}); the traversal would shard by target "
+              f"groups with the tree replicated on every GPU.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -3,8 +3,8 @@
  * prototypes, for OpenVort's hot-path functions, implemented on the GPU
  * through the C ABI of openvort_gpu.h (library: libopenvort_dropin.so).
  *
- * A reference build links these instead of its own definitions (see
- * INTEGRATION.md).  They read the reference's globals
+ * A reference build links these instead of its own definitions, ahead
+ * of the reference's objects on the link line.  They read the reference's globals
  * (include/vortex_constants.h:11-60, include/external_velocity.h:29-33),
  * which the reference's src/initial_conditions.c defines, and the
  * struct tangle_state of the caller, whose layout is restated below
