@@ -790,9 +790,6 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !slots)) return set_error(OVG_EINVAL, "bad slot list");
     for (int i = 0; i < m; ++i)

@@ -190,3 +190,23 @@ def test_pinned_lines_refuse_topology_passes():
             with pytest.raises(ov.OvgError) as ei:
                 call()
             assert ei.value.code == ov.EUNSUPPORTED
+
+
+def test_pinned_velocity_at_matches_oracle(oracle):
+    """ovg_velocity_at (the reference's update_velocity with tree == NULL,
+    which wall clipping calls) on a tangle with pinned lines: slots on the
+    lines, at and next to the pinned ends, and on the rings."""
+    from test_walls import pinned_case
+    t, p = pinned_case(ov.METHOD_DIRECT, True)
+    ends = np.where(t.status[:, 0] != ov.FREE)[0]
+    slots = np.unique(np.concatenate([ends, t.conn[ends[::2], 0], t.live[::29]]))
+    say(f"velocity_at on pinned lines: {len(slots)} slots")
+    tn, nm = oracle.tangents_normals(t, p)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        e.update_tangents_normals()
+        vs, vels = e.velocity_at(slots, 0.0)
+    for i, k in enumerate(slots):
+        ws, wv = oracle.velocity_direct_one(t, p, int(k), 0.0, tn, nm)
+        assert np.abs(vs[i] - ws).max() <= 1e-12 * max(1.0, np.abs(ws).max()), k
+        assert np.abs(vels[i] - wv).max() <= 1e-12 * max(1.0, np.abs(wv).max()), k
