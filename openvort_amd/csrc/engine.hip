@@ -963,9 +963,6 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
     Ctx &c = ctx->c;
     int n = c.n;
     std::vector<int> &conn = c.h_conn, &st = c.h_status;
@@ -983,7 +980,39 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
             if (next == start || next < 0) break;
         }
     };
-    if (c.loops_full_scan || cutoff > c.small_cutoff_seen) {
+    if (c.has_pinned) {
+        // open lines ending on pinned nodes: the reference's full pass
+        // (src/tangle.c:1063-1128).  The forward walk that reaches a pinned
+        // end restarts at k and adds the steps back to the other end; the
+        // erase then runs forward from that end.  Every pass is a full one
+        // while pinned lines exist.
+        std::vector<char> visited(n, 0);
+        for (int k = 0; k < n; ++k) {
+            if (st[2 * (size_t)k] == OVG_EMPTY || visited[k]) continue;
+            visited[k] = 1;
+            int loop = 0, here = k, next = conn[2 * (size_t)k];
+            while (next != k) {
+                const int sh = st[2 * (size_t)here];
+                if ((sh == OVG_PINNED || sh == OVG_PINNED_SLIP) && next < 0) {
+                    here = k;
+                    next = conn[2 * (size_t)here + 1];
+                    while (next >= 0) {
+                        visited[here] = 1;
+                        here = next;
+                        next = conn[2 * (size_t)here + 1];
+                        loop++;
+                    }
+                    break;
+                }
+                visited[here] = 1;
+                here = next;
+                next = conn[2 * (size_t)here];
+                loop++;
+            }
+            if (loop < cutoff) erase_loop(here);
+        }
+        c.loops_full_scan = true;
+    } else if (c.loops_full_scan || cutoff > c.small_cutoff_seen) {
         std::vector<char> visited(n, 0);
         for (int k = 0; k < n; ++k) {
             if (st[2 * (size_t)k] == OVG_EMPTY || visited[k]) continue;

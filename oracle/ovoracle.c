@@ -894,6 +894,20 @@ int ovo_eliminate_small_loops(int n, int *conn, int *status, int cutoff)
         visited[k] = 1;
         int loop = 0, here = k, next = conn[2 * k];
         while (next != k) {
+            int sh = status[2 * here];
+            if ((sh == OVG_PINNED || sh == OVG_PINNED_SLIP) && next < 0) {
+                /* an open line: count back from k to its other end, on top
+                 * of the forward steps (src/tangle.c:1081-1093) */
+                here = k;
+                next = conn[2 * here + 1];
+                while (next >= 0) {
+                    visited[here] = 1;
+                    here = next;
+                    next = conn[2 * here + 1];
+                    loop++;
+                }
+                break;
+            }
             visited[here] = 1;
             here = next;
             next = conn[2 * here];

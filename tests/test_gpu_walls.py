@@ -210,3 +210,24 @@ def test_pinned_velocity_at_matches_oracle(oracle):
         ws, wv = oracle.velocity_direct_one(t, p, int(k), 0.0, tn, nm)
         assert np.abs(vs[i] - ws).max() <= 1e-12 * max(1.0, np.abs(ws).max()), k
         assert np.abs(vels[i] - wv).max() <= 1e-12 * max(1.0, np.abs(wv).max()), k
+
+
+def test_pinned_small_lines_vs_oracle(oracle):
+    from test_walls import lines_and_rings
+    t, p = lines_and_rings([2, 3, 5, 6, 7, 40, 4])
+    say(f"small-loop removal with pinned lines: {len(t.live)} points")
+    w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
+    want = oracle.eliminate_small_loops(w, 6)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        got = e.eliminate_small_loops(6)
+        conn, st = e.download_connections()
+        # the evaluation still runs on what is left
+        e.update_tangle(0.0)
+        o = e.download()
+    assert got == want
+    assert np.array_equal(conn, w.conn)
+    assert np.array_equal(st[:, 0], w.status[:, 0])
+    live = w.live
+    ref = oracle.update_tangle(w, p)
+    assert rel_l2(o["vels"][live], ref["vels"][live]) < 1e-10

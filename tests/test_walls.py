@@ -245,3 +245,48 @@ def test_mirror_image_orientation_follows_reflection_parity(oracle, faces, rever
         q.box_lo[a] = -D
     o = oracle.update_tangle(big, q)
     assert rel_l2(o["vs"][:nl], base["vs"][live]) < 1e-12
+
+
+def lines_and_rings(lengths, seed=5, rings=3):
+    """Rings plus lines of the given point counts from the DOWN wall to the
+    UP wall, pinned at both ends (parallel-planes geometry)."""
+    D, dxi = 0.1, 2e-3
+    t = rings_inside(seed, rings, D, 0.012, dxi)
+    rng = np.random.default_rng(seed)
+    xs, conns, sts = [t.vnodes], [t.conn], [t.status]
+    off = t.n
+    for m in lengths:
+        x0, y0 = rng.uniform(0.02, 0.08, 2)
+        z = np.linspace(0.0, D, m)
+        xs.append(np.stack([np.full(m, x0), np.full(m, y0), z], 1))
+        j = np.arange(m)
+        conns.append(np.stack([np.where(j < m - 1, off + j + 1, -1),
+                               np.where(j > 0, off + j - 1, -1)], 1))
+        st = np.zeros((m, 2), dtype=np.int32)
+        st[:, 1] = -1
+        st[0] = (ov.PINNED, ov.FACE_DOWN)
+        st[-1] = (ov.PINNED_SLIP, ov.FACE_UP)
+        sts.append(st)
+        off += m
+    t = ov.Tangle(np.concatenate(xs), np.concatenate(conns).astype(np.int32),
+                  np.concatenate(sts).astype(np.int32))
+    p = configs.params(D, dxi, configs.T16, 0.5, method=ov.METHOD_DIRECT)
+    for f, w in enumerate(WALLS[ov.IMAGES_WALL_PARALLEL_PLANES]):
+        p.wall[f] = w
+    ov.set_wall_images(p, ov.IMAGES_WALL_PARALLEL_PLANES)
+    return t, p
+
+
+def test_small_open_lines_removed_as_reference(oracle, ref):
+    """eliminate_small_loops with lines pinned to walls: an open line counts
+    its forward steps to the pinned end plus the steps back from the start
+    point (src/tangle.c:1081-1093) and is erased forward from its far end."""
+    t, p = lines_and_rings([2, 3, 5, 6, 7, 40, 4])
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.eliminate_small_loops(6)
+    want_conn, want_st = r.get()["conn"], r.status()
+    w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
+    removed = oracle.eliminate_small_loops(w, 6)
+    assert np.array_equal(w.conn, want_conn)
+    assert np.array_equal(w.status[:, 0], want_st[:, 0])
+    assert removed == 2 + 3 + 5 + 6 + 4  # lines of at most 6 points
