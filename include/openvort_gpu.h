@@ -200,6 +200,16 @@ int ovg_params_set_wall_images(ovg_params *p, int config);
 const char *ovg_last_error(void);
 const char *ovg_version(void);
 
+/* Cap on the device memory, in bytes, that the buffers of every context of
+ * this process may hold together; 0 (the default) is no cap.  A buffer that
+ * would exceed it is not allocated and the call that needed it fails with
+ * OVG_ENOMEM, as it would on a failed hipMalloc.  Returns the previous cap.
+ * (No counterpart in the reference: a bound for processes that share the
+ * GPU's memory, and a way to exercise the out-of-memory paths.) */
+size_t ovg_set_device_memory_limit(size_t bytes);
+/* Device bytes held by the buffers of every context of this process. */
+size_t ovg_device_memory_in_use(void);
+
 /* sizeof(ovg_params) and sizeof(ovg_stats) as compiled into the library,
  * for bindings that restate the structs (ctypes, FFI). */
 size_t ovg_params_sizeof(void);

@@ -144,6 +144,8 @@ _sig("ovg_generate_rings", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
      C.c_double, C.c_int, _dp, _ip, _ip)
 _sig("ovg_rings_for_points", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
      C.c_double, C.c_double)
+_sig("ovg_set_device_memory_limit", C.c_size_t, C.c_size_t)
+_sig("ovg_device_memory_in_use", C.c_size_t)
 
 if _lib.ovg_params_sizeof() != C.sizeof(Params) or _lib.ovg_stats_sizeof() != C.sizeof(Stats):
     raise ImportError("ctypes struct layout does not match libopenvort_gpu.so")
@@ -172,6 +174,16 @@ def _i(a):
 def lib():
     """The loaded ctypes CDLL of libopenvort_gpu.so."""
     return _lib
+
+
+def set_device_memory_limit(nbytes):
+    """Cap the device bytes all contexts of this process may hold (0 = no
+    cap); returns the previous cap."""
+    return _lib.ovg_set_device_memory_limit(int(nbytes))
+
+
+def device_memory_in_use():
+    return _lib.ovg_device_memory_in_use()
 
 
 def default_params(D, **kw):
@@ -277,6 +289,12 @@ class Engine:
     @property
     def handle(self):
         return self._h
+
+    def get_params(self):
+        """The context's parameters (after ovg_autotune, with its pick)."""
+        p = Params()
+        _check(_lib.ovg_get_params(self._h, C.byref(p)), "ovg_get_params")
+        return p
 
     def set_params(self, p):
         self.params = p

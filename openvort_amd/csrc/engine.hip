@@ -1,6 +1,7 @@
 // Host side of the C ABI: context, state upload/download, the parameter
 // block and the sequencing of the hot path (update_tangle, rk4_step,
 // enforce_periodic_boundaries).  Kernels live in k_*.hip.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,17 +17,31 @@ struct ovg_ctx {
 
 namespace ovg {
 
+// Device bytes held by every DBuf of the process, and the cap set by
+// ovg_set_device_memory_limit (0 = none).
+static std::atomic<size_t> g_dev_in_use{0};
+static std::atomic<size_t> g_dev_limit{0};
+
 int DBuf::ensure(size_t n)
 {
     if (n <= bytes) return OVG_OK;
     if (p) {
         hipFree(p);
+        g_dev_in_use -= bytes;
         p = nullptr;
         bytes = 0;
     }
     size_t want = n < 256 ? 256 : n;
+    const size_t cap = g_dev_limit.load();
+    const size_t held = g_dev_in_use.fetch_add(want) + want;
+    if (cap && held > cap) {
+        g_dev_in_use -= want;
+        return set_error(OVG_ENOMEM, "device buffer of %zu bytes exceeds the memory limit "
+                         "(%zu of %zu bytes held)", want, held - want, cap);
+    }
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
+        g_dev_in_use -= want;
         p = nullptr;
         return set_error(OVG_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
@@ -36,7 +51,10 @@ int DBuf::ensure(size_t n)
 
 void DBuf::release()
 {
-    if (p) hipFree(p);
+    if (p) {
+        hipFree(p);
+        g_dev_in_use -= bytes;
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -267,7 +285,7 @@ int eval_bs(Ctx &c, const double *const x[3], int phase)
         launch_prep_segments(c, x);
         mark(c, &e1);
         Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
-        launch_direct(c, x, c.n_live, c.live.as<int>(), bp.v, bi.v, 0);
+        OVG_TRY(launch_direct(c, x, c.n_live, c.live.as<int>(), bp.v, bi.v, 0));
         // every (target, segment, image) pair except the two LIA segments
         double pairs = (double)c.n_live * c.n_src * (1 + c.params.n_images) - 2.0 * c.n_live;
         c.stats.interactions += pairs;
@@ -298,7 +316,7 @@ static int fetch_stats(Ctx &c, ovg_stats *out)
     out->tree_nodes = tc[1];
     out->n_points = c.n_live;
     std::memset(&c.stats, 0, sizeof c.stats);
-    hipMemsetAsync(c.counter64.p, 0, 16, c.stream);
+    OVG_HIP(hipMemsetAsync(c.counter64.p, 0, 16, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
     return OVG_OK;
 }
@@ -306,6 +324,10 @@ static int fetch_stats(Ctx &c, ovg_stats *out)
 }  // namespace ovg
 
 using namespace ovg;
+
+extern "C" size_t ovg_set_device_memory_limit(size_t bytes) { return g_dev_limit.exchange(bytes); }
+
+extern "C" size_t ovg_device_memory_in_use(void) { return g_dev_in_use.load(); }
 
 extern "C" int ovg_create(ovg_ctx **out, int device)
 {
@@ -344,10 +366,14 @@ extern "C" int ovg_create(ovg_ctx **out, int device)
         ovg_destroy(ctx);
         return r;
     }
-    hipMemsetAsync(c.flags.p, 0, 64, c.stream);
-    hipMemsetAsync(c.counter64.p, 0, 64, c.stream);
+    if (hipMemsetAsync(c.flags.p, 0, 64, c.stream) != hipSuccess ||
+        hipMemsetAsync(c.counter64.p, 0, 64, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+        ovg_destroy(ctx);
+        return set_error(OVG_EDEVICE, "clearing the context's device flags failed");
+    }
     *out = ctx;
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_destroy(ovg_ctx *ctx)
@@ -453,7 +479,7 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
     c.rec_ok = false;  // recorded interaction lists belong to the old settings
     c.have_params = true;
     push_params(c, 0.0);
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_get_params(const ovg_ctx *ctx, ovg_params *p)
@@ -605,7 +631,7 @@ extern "C" int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes, const int *
                                n, b->as<double>());
     }
     OVG_HIP(hipStreamSynchronize(c.stream));
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes)
@@ -614,7 +640,7 @@ extern "C" int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes)
     OVG_TRY(check_ready(ctx));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return upload_positions(c, vnodes);
+    return launch_status(upload_positions(c, vnodes));
 }
 
 extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, const int *status)
@@ -624,7 +650,7 @@ extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, cons
     Ctx &c = ctx->c;
     OVG_TRY(validate_topology(c.n, connections, status));
     hipSetDevice(c.device);
-    return upload_topology(c, connections, status);
+    return launch_status(upload_topology(c, connections, status));
 }
 
 extern "C" int ovg_upload_field(ovg_ctx *ctx, int field, const double *values)
@@ -649,7 +675,7 @@ extern "C" int ovg_upload_field(ovg_ctx *ctx, int field, const double *values)
     hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
                        c.partial.as<double>(), dp(dst[0]), dp(dst[1]), dp(dst[2]));
     OVG_HIP(hipStreamSynchronize(c.stream));
-    return OVG_OK;
+    return launch_status();
 }
 
 static int check_flags(Ctx &c)
@@ -658,7 +684,7 @@ static int check_flags(Ctx &c)
     OVG_HIP(hipMemcpyAsync(&f, c.flags.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
     if (f) {
-        hipMemsetAsync(c.flags.p, 0, sizeof(int), c.stream);
+        OVG_HIP(hipMemsetAsync(c.flags.p, 0, sizeof(int), c.stream));
         if (f & 2) return set_error(OVG_EINVAL, "non-finite positions in the tangle state");
         return set_error(OVG_EINVAL, "a point left the box by more than 256 periods");
     }
@@ -684,7 +710,7 @@ extern "C" int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *
                                hipMemcpyDeviceToHost, c.stream));
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
-    return check_flags(c);
+    return launch_status(check_flags(c));
 }
 
 extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *status)
@@ -703,7 +729,7 @@ extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *sta
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
     if (status) std::memcpy(status, c.h_status.data(), sizeof(int) * 2 * (size_t)n);
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_num_slots(const ovg_ctx *ctx) { return ctx ? ctx->c.n : -1; }
@@ -718,7 +744,7 @@ extern "C" int ovg_update_tangents_normals(ovg_ctx *ctx)
     hipSetDevice(c.device);
     Triple x = tri(c.x), t = tri(c.tan), nrm = tri(c.nor);
     launch_stencil(c, x.v, t.v, nrm.v);
-    return OVG_OK;
+    return launch_status();
 }
 
 static int velocities(Ctx &c, double t, bool own_tangents)
@@ -736,7 +762,7 @@ extern "C" int ovg_update_velocities(ovg_ctx *ctx, double t)
     OVG_TRY(check_ready(ctx));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return velocities(c, t, false);
+    return launch_status(velocities(c, t, false));
 }
 
 extern "C" int ovg_update_tangle(ovg_ctx *ctx, double t)
@@ -745,7 +771,7 @@ extern "C" int ovg_update_tangle(ovg_ctx *ctx, double t)
     OVG_TRY(check_ready(ctx));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return velocities(c, t, true);
+    return launch_status(velocities(c, t, true));
 }
 
 extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
@@ -772,7 +798,7 @@ extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
         launch_compose(c, xe, true, s, dt);
     }
     c.rec_ok = false;  // recorded lists serve one step
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_enforce_periodic(ovg_ctx *ctx)
@@ -782,7 +808,7 @@ extern "C" int ovg_enforce_periodic(ovg_ctx *ctx)
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     launch_wrap(c);
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, double *vs,
@@ -806,8 +832,9 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
     Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
     ovg_params saved = c.params;
     c.params.precision = OVG_PREC_FP64;
-    launch_direct(c, x.v, m, c.rc_tmp.as<int>(), bp.v, bi.v, 0);
+    const int rd = launch_direct(c, x.v, m, c.rc_tmp.as<int>(), bp.v, bi.v, 0);
     c.params = saved;
+    OVG_TRY(rd);
     // compose over the selected slots only
     int saved_live = c.n_live;
     DBuf saved_list = c.live;
@@ -830,7 +857,7 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
         for (int i = 0; i < m; ++i)
             for (int q = 0; q < 3; ++q) out[3 * i + q] = hv[3 * (size_t)slots[i] + q];
     }
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_line_length(ovg_ctx *ctx, double *length, int *n_points)
@@ -852,7 +879,7 @@ extern "C" int ovg_line_length(ovg_ctx *ctx, double *length, int *n_points)
     for (int b = 0; b < gs; ++b) L += part[b];
     if (length) *length = L;
     if (n_points) *n_points = c.n_live;
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_get_stats(ovg_ctx *ctx, ovg_stats *s)
@@ -861,14 +888,14 @@ extern "C" int ovg_get_stats(ovg_ctx *ctx, ovg_stats *s)
     if (!ctx || !s) return set_error(OVG_EINVAL, "NULL argument");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return fetch_stats(c, s);
+    return launch_status(fetch_stats(c, s));
 }
 
 extern "C" int ovg_set_timing(ovg_ctx *ctx, int on)
 {
     if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
     ctx->c.timing = on;
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_synchronize(ovg_ctx *ctx)
@@ -892,7 +919,7 @@ extern "C" int ovg_marker_record(ovg_ctx *ctx, int id)
     if ((int)c.markers.size() <= id) c.markers.resize((size_t)id + 1, nullptr);
     if (!c.markers[id]) OVG_HIP(hipEventCreate(&c.markers[id]));
     OVG_HIP(hipEventRecord(c.markers[id], c.stream));
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms)
@@ -908,7 +935,7 @@ extern "C" int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms)
     float f = 0.f;
     OVG_HIP(hipEventElapsedTime(&f, c.markers[id0], c.markers[id1]));
     *ms = f;
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pairs)
@@ -916,13 +943,13 @@ extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs,
     clear_error();
     OVG_TRY(check_ready(ctx));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
+                                           "add_loop on tangles with pinned lines are not built");
     if (max_pairs < 0 || (max_pairs > 0 && !pairs))
         return set_error(OVG_EINVAL, "bad pair buffer");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return recon_candidates(c, max_pairs, pairs, n_pairs);
+    return launch_status(recon_candidates(c, max_pairs, pairs, n_pairs));
 }
 
 extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
@@ -930,14 +957,14 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
     clear_error();
     OVG_TRY(check_ready(ctx));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
+                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int nrec = 0;
     OVG_TRY(reconnect_pass(c, t, &nrec));
     if (n_reconnections) *n_reconnections = nrec;
-    return OVG_OK;
+    return launch_status();
 }
 
 extern "C" int ovg_smooth(ovg_ctx *ctx)
@@ -945,11 +972,11 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
     clear_error();
     OVG_TRY(check_ready(ctx));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
+                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
-    return smooth_loops(c);
+    return launch_status(smooth_loops(c));
 }
 
 // eliminate_small_loops, src/tangle.c:1063-1128 (closed loops).  Loops are
@@ -1012,6 +1039,17 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
             if (loop < cutoff) erase_loop(here);
         }
         c.loops_full_scan = true;
+        // when the pass erased every pinned line the tangle is closed loops
+        // again: the topology passes take it and pin walls are dropped
+        bool pinned_left = false;
+        for (int k = 0; k < n && !pinned_left; ++k) {
+            const int s = st[2 * (size_t)k];
+            pinned_left = s == OVG_PINNED || s == OVG_PINNED_SLIP;
+        }
+        if (!pinned_left) {
+            c.has_pinned = false;
+            c.pinw.release();
+        }
     } else if (c.loops_full_scan || cutoff > c.small_cutoff_seen) {
         std::vector<char> visited(n, 0);
         for (int k = 0; k < n; ++k) {
@@ -1053,7 +1091,7 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
             if ((size_t)k < c.loop_dirty_mark.size()) c.loop_dirty_mark[k] = 0;
         c.loop_dirty.clear();
     }
-    return OVG_OK;
+    return launch_status();
 }
 
 // ---- remesh (src/tangle.c:1018-1061) on the host mirrors ----------------
@@ -1185,8 +1223,8 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     clear_error();
     OVG_TRY(check_ready(ctx));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
+                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_added) *n_added = 0;
@@ -1278,7 +1316,7 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
         launch_wrap(c);  // enforce_periodic_boundaries after insertions (:1057-1060)
         OVG_HIP(hipStreamSynchronize(c.stream));
     }
-    return OVG_OK;
+    return launch_status();
 }
 
 // add_loop, src/vortex_injection.c:44-103: the loop's points (the geometry
@@ -1291,8 +1329,8 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
     clear_error();
     OVG_TRY(check_ready(ctx));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: topology passes (wall clipping, "
-                                           "reconnection, remesh, loops, smoothing) are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
+                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_points) *n_points = 0;
@@ -1339,7 +1377,7 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
         OVG_TRY(commit_topology(c));
     }
     if (n_points) *n_points = npts;
-    return OVG_OK;
+    return launch_status();
 }
 
 // ---- autotuning of the list traversal --------------------------------------
@@ -1409,5 +1447,5 @@ extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
     push_params(c, 0.0);
     OVG_HIP(hipMemcpyAsync(c.counter64.p, keep, 16, hipMemcpyHostToDevice, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
-    return OVG_OK;
+    return launch_status();
 }

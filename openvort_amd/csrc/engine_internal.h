@@ -13,7 +13,11 @@
 namespace ovg {
 
 int set_error(int code, const char *fmt, ...);
+// Clears the calling thread's error text and any pending HIP launch error.
 void clear_error();
+// r when r != OVG_OK; otherwise OVG_EDEVICE when a kernel launch since the
+// last clear_error() failed (hipGetLastError), else OVG_OK.
+int launch_status(int r = OVG_OK);
 
 #define OVG_HIP(call)                                                                  \
     do {                                                                               \
@@ -182,8 +186,8 @@ void launch_line_length(Ctx &c, double *d_out);
 void launch_check_finite(Ctx &c, const double *const x[3]);
 
 // k_direct.hip
-void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
-                   double *const out_prim[3], double *const out_img[3], int chunks);
+int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
+                  double *const out_prim[3], double *const out_img[3], int chunks);
 
 // k_tree.hip
 int build_tree(Ctx &c, const double *const x[3]);
@@ -243,12 +247,12 @@ int reconnect_pass(Ctx &c, double t, int *n_rec);
 
 // k_tree.hip: stable LSD radix sort of c.tree.keys[0]/vals[0] over the
 // 8-bit digits at first_shift..last_shift (an even number of passes)
-void radix_sort(Ctx &c, int n, int first_shift, int last_shift);
+int radix_sort(Ctx &c, int n, int first_shift, int last_shift);
 
 // scan helper (k_tree.hip): exclusive scan of n int32 values, in -> out,
 // writes the total to *d_total when non-null.
-void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
-                           DBuf &tmp);
+int launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
+                          DBuf &tmp);
 
 inline const TangleParams *dparams(const Ctx &c) { return c.tparams.as<TangleParams>(); }
 

@@ -24,7 +24,20 @@ int set_error(int code, const char *fmt, ...)
     return code;
 }
 
-void clear_error() { g_last_error.clear(); }
+void clear_error()
+{
+    g_last_error.clear();
+    (void)hipGetLastError();
+}
+
+int launch_status(int r)
+{
+    if (r != OVG_OK) return r;
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(OVG_EDEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+    return OVG_OK;
+}
 
 }  // namespace ovg
 

@@ -57,11 +57,8 @@ __global__ __launch_bounds__(kDirBlock) void k_direct(
             // primary box: skip the two segments handled by LIA (:658)
             if (!(id.x == k || id.y == k)) {
                 if (MIXED) {
-                    float vx, vy, vz;
-                    seg_field32((float)(ax - rx), (float)(ay - ry), (float)(az - rz),
-                                (float)(bx - rx), (float)(by - ry), (float)(bz - rz),
-                                (float)f, vx, vy, vz);
-                    apx += vx; apy += vy; apz += vz;
+                    seg_field_mixed(ax - rx, ay - ry, az - rz, bx - rx, by - ry, bz - rz, f,
+                                    (float)f, apx, apy, apz);
                 } else {
                     double vx, vy, vz;
                     if (seg_field64(ax - rx, ay - ry, az - rz, bx - rx, by - ry, bz - rz, f,
@@ -76,11 +73,8 @@ __global__ __launch_bounds__(kDirBlock) void k_direct(
                 double qx, qy, qz;
                 image_point(P, im, rx, ry, rz, qx, qy, qz);
                 if (MIXED) {
-                    float vx, vy, vz;
-                    seg_field32((float)(ax - qx), (float)(ay - qy), (float)(az - qz),
-                                (float)(bx - qx), (float)(by - qy), (float)(bz - qz),
-                                (float)f, vx, vy, vz);
-                    aix += vx; aiy += vy; aiz += vz;
+                    seg_field_mixed(ax - qx, ay - qy, az - qz, bx - qx, by - qy, bz - qz, f,
+                                    (float)f, aix, aiy, aiz);
                 } else {
                     double vx, vy, vz;
                     if (seg_field64(ax - qx, ay - qy, az - qz, bx - qx, by - qy, bz - qz, f,
@@ -119,10 +113,10 @@ __global__ void k_direct_reduce(int n_tgt, int chunks, const int *__restrict__ t
     ix[k] = s[3]; iy[k] = s[4]; iz[k] = s[5];
 }
 
-void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
-                   double *const out_prim[3], double *const out_img[3], int chunks)
+int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
+                  double *const out_prim[3], double *const out_img[3], int chunks)
 {
-    if (n_tgt == 0) return;
+    if (n_tgt == 0) return OVG_OK;
     int tiles = (n_tgt + kDirBlock - 1) / kDirBlock;
     if (chunks <= 0) {
         // aim for >= 4 blocks per CU on 256 CUs, chunks of >= 1 source tile
@@ -136,7 +130,7 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
     if (chunk_len == 0) chunk_len = kDirBlock;
     chunks = (c.n_src + chunk_len - 1) / chunk_len;
     if (chunks < 1) chunks = 1;
-    if (c.dpart.ensure(sizeof(double) * 6 * (size_t)n_tgt * chunks) != OVG_OK) return;
+    OVG_TRY(c.dpart.ensure(sizeof(double) * 6 * (size_t)n_tgt * chunks));
     dim3 grid(tiles, chunks);
     bool mixed = c.params.precision == OVG_PREC_MIXED;
     if (mixed)
@@ -155,6 +149,7 @@ void launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt
     hipLaunchKernelGGL(k_direct_reduce, dim3(gs), dim3(bs), 0, c.stream, n_tgt, chunks,
                        d_tgt_slots, c.dpart.as<double>(), out_prim[0], out_prim[1],
                        out_prim[2], out_img[0], out_img[1], out_img[2]);
+    return launch_status();
 }
 
 }  // namespace ovg

@@ -183,7 +183,7 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
     hipLaunchKernelGGL(k_cell_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), X, Y,
                        Z, dparams(c), g, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
     // sort by cell id (< 2^24): four 8-bit passes (an even count)
-    radix_sort(c, n, 0, 24);
+    OVG_TRY(radix_sort(c, n, 0, 24));
     int *start = c.rc_start.as<int>(), *end = start + ncell;
     hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, ncell,
                        start, INT_MAX);
@@ -303,8 +303,9 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
             launch_prep_segments(c, xv);
             ovg_params saved = c.params;
             c.params.precision = OVG_PREC_FP64;
-            launch_direct(c, xv, m, c.rc_tmp.as<int>(), op, oi, 0);
+            const int rd = launch_direct(c, xv, m, c.rc_tmp.as<int>(), op, oi, 0);
             c.params = saved;
+            OVG_TRY(rd);
         }
         OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)m));
         hipLaunchKernelGGL(k_gather3, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,

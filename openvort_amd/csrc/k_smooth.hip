@@ -965,8 +965,8 @@ static int build_loops(Ctx &c)
     int *L = d2;  // the free one of lab2 / len
     hipLaunchKernelGGL(k_lt_heads, dim3(g), dim3(256), 0, c.stream, n, c.fwd.as<int>(), lab, d,
                        flag, L);
-    launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp);
-    launch_exclusive_scan(c, L, hoff, n, hoff + n, c.tree.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp));
+    OVG_TRY(launch_exclusive_scan(c, L, hoff, n, hoff + n, c.tree.scan_tmp));
     int hv[3] = {0, 0, 0};
     OVG_HIP(hipMemcpyAsync(&hv[0], flag + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     OVG_HIP(hipMemcpyAsync(&hv[1], hoff + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -195,7 +195,7 @@ int commit_topology(Ctx &c)
     int *flag = c.rc_tmp.as<int>();
     hipLaunchKernelGGL(k_live_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
                        c.status.as<int>(), flag);
-    launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp));
     hipLaunchKernelGGL(k_live_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
                        c.status.as<int>(), flag, c.live.as<int>(), c.src.as<int>());
     OVG_HIP(hipMemcpyAsync(&live, flag + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -270,7 +270,7 @@ int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int>
                        c.status.as<int>(), c.fwd.as<int>(), c.rev.as<int>(), c.x[0].as<double>(),
                        c.x[1].as<double>(), c.x[2].as<double>(), dparams(c), min_dist, max_dist,
                        flag);
-    launch_exclusive_scan(c, flag, pos, n, pos + n, c.tree.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, flag, pos, n, pos + n, c.tree.scan_tmp));
     hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, flag, pos,
                        lst);
     int cnt = 0;

@@ -346,15 +346,19 @@ static void scan_level(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
     }
 }
 
-void launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
-                           DBuf &tmp)
+__global__ void k_zero_ints(int *a, int n);
+
+int launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
+                          DBuf &tmp)
 {
     if (n <= 0) {
-        if (d_total) hipMemsetAsync(d_total, 0, sizeof(int), c.stream);
-        return;
+        // a kernel, not a memset: callers may be captured into a HIP graph
+        if (d_total) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.stream, d_total, 1);
+        return OVG_OK;
     }
-    tmp.ensure(sizeof(int) * scan_tmp_ints(n));
+    OVG_TRY(tmp.ensure(sizeof(int) * scan_tmp_ints(n)));
     scan_level(c, d_in, d_out, n, d_total, tmp.as<int>());
+    return OVG_OK;
 }
 
 // ---------------------------------------------------------- radix sort ----
@@ -456,17 +460,17 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
 }
 
 // Sorts c.tree.keys[0]/vals[0] (n entries) in place (result back in [0]).
-void radix_sort(Ctx &c, int n, int first_shift, int last_shift)
+int radix_sort(Ctx &c, int n, int first_shift, int last_shift)
 {
     TreeBufs &T = c.tree;
     int nblocks = (n + kRsTile - 1) / kRsTile;
-    T.hist.ensure(sizeof(int) * 256 * (size_t)nblocks);
+    OVG_TRY(T.hist.ensure(sizeof(int) * 256 * (size_t)nblocks));
     int src = 0;
     for (int shift = first_shift; shift <= last_shift; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c.stream,
                            T.keys[src].as<uint64_t>(), n, shift, T.hist.as<int>(), nblocks);
-        launch_exclusive_scan(c, T.hist.as<int>(), T.hist.as<int>(), 256 * nblocks, nullptr,
-                              T.scan_tmp);
+        OVG_TRY(launch_exclusive_scan(c, T.hist.as<int>(), T.hist.as<int>(), 256 * nblocks,
+                                      nullptr, T.scan_tmp));
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c.stream,
                            T.keys[src].as<uint64_t>(), T.vals[src].as<int>(),
                            T.keys[1 - src].as<uint64_t>(), T.vals[1 - src].as<int>(), n, shift,
@@ -474,11 +478,12 @@ void radix_sort(Ctx &c, int n, int first_shift, int last_shift)
         src = 1 - src;
     }
     if (src != 0) {
-        hipMemcpyAsync(T.keys[0].p, T.keys[1].p, sizeof(uint64_t) * n,
-                       hipMemcpyDeviceToDevice, c.stream);
-        hipMemcpyAsync(T.vals[0].p, T.vals[1].p, sizeof(int) * n, hipMemcpyDeviceToDevice,
-                       c.stream);
+        OVG_HIP(hipMemcpyAsync(T.keys[0].p, T.keys[1].p, sizeof(uint64_t) * n,
+                               hipMemcpyDeviceToDevice, c.stream));
+        OVG_HIP(hipMemcpyAsync(T.vals[0].p, T.vals[1].p, sizeof(int) * n,
+                               hipMemcpyDeviceToDevice, c.stream));
     }
+    return OVG_OK;
 }
 
 // ----------------------------------------------------------- topology ----
@@ -973,8 +978,8 @@ int build_tree(Ctx &c, const double *const x[3])
     OVG_TRY(T.rank.ensure(sizeof(int) * ((size_t)c.n + 1)));
     DBuf *sv[6] = {&T.sx, &T.sy, &T.sz, &T.sqx, &T.sqy, &T.sqz};
     for (auto *b : sv) OVG_TRY(b->ensure(sizeof(double) * (n + 1)));
-    hipMemsetAsync(T.counters.p, 0, sizeof(int) * 64, c.stream);
-    if (n == 0) return OVG_OK;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.stream, T.counters.as<int>(), 64);
+    if (n == 0) return launch_status();
     int bs = 256, gs = (n + bs - 1) / bs;
     int clamp = c.params.traverse != OVG_TRAVERSE_POINT;
     hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
@@ -982,8 +987,8 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counters.as<int>(), clamp);
     // tuned keys: bits kTunedLowBit .. 62 in 8-bit digits (6 passes)
     const int max_lv = clamp ? kTunedLevels : kMaxLevel;
-    if (clamp) radix_sort(c, n, kTunedLowBit, 55);
-    else radix_sort(c, n, 0, 56);
+    if (clamp) OVG_TRY(radix_sort(c, n, kTunedLowBit, 55));
+    else OVG_TRY(radix_sort(c, n, 0, 56));
     hipLaunchKernelGGL(k_count_inbox, dim3((n + 1 + bs - 1) / bs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.counters.as<int>());
     hipLaunchKernelGGL(k_sorted_gather, dim3(gs), dim3(bs), 0, c.stream, n,
@@ -1005,8 +1010,8 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
                        T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv);
     // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
-    launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n, T.base.as<int>() + n,
-                          T.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n,
+                                  T.base.as<int>() + n, T.scan_tmp));
     hipLaunchKernelGGL(k_nodes<true>, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.delta.as<int8_t>(), T.counters.as<int>(),
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
@@ -1021,7 +1026,7 @@ int build_tree(Ctx &c, const double *const x[3])
     int *hist = T.level_hist.as<int>(), *list = T.level_list.as<int>();
     hipLaunchKernelGGL(k_level_count, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
                        T.nodes.as<Node>(), T.meta.as<NodeMeta>(), hist);
-    launch_exclusive_scan(c, hist, hist, kLevels * gn, hist + kLevels * gn, T.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, hist, hist, kLevels * gn, hist + kLevels * gn, T.scan_tmp));
     hipLaunchKernelGGL(k_level_scatter, dim3(gn), dim3(bs), 0, c.stream, T.counters.as<int>(),
                        T.nodes.as<Node>(), T.meta.as<NodeMeta>(), hist, list);
     const int gl = std::min(gn, 1024);
