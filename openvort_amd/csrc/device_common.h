@@ -142,50 +142,59 @@ __device__ __forceinline__ bool seg_field64(double Rx, double Ry, double Rz, dou
     return true;
 }
 
-// FP32 pair formula for the far field: R and Rp arrive already rounded to
-// FP32 (from FP64 differences, or as offsets within one target group).
-// Same guards as the reference (thresholds in FP32), native reciprocal
-// square roots, one reciprocal, fused multiply-adds (this path makes no
-// claim to the reference's rounding; the library is otherwise built with
-// contraction off).
-__device__ __forceinline__ void seg_field32(float Rx, float Ry, float Rz, float Px, float Py,
-                                            float Pz, float f_bs, float &vx, float &vy,
-                                            float &vz)
+// A pair of the mixed-precision paths from FP64 R and Rp, in FP32 (native
+// reciprocal square roots, one reciprocal, fused multiply-adds: this path
+// makes no claim to the reference's rounding; the library is otherwise
+// built with contraction off, and (|R|+|Rp|)/(|R||Rp|(|R||Rp| + R.Rp)) is
+// evaluated as (ia + ib) ia ib / (1 + R.Rp ia ib) with ia = 1/|R|,
+// ib = 1/|Rp|), except where FP32 cannot take the reference's guard
+// decisions -- the normalised dot within 1e-5 of its threshold 1e-8 (FP32
+// resolves it to about 1e-6), or |R|^2 or |Rp|^2 within a factor 2 of
+// 1e-16 -- where the pair is evaluated by seg_field64, so every guard
+// decides as in the reference (a flipped guard adds or drops the whole
+// contribution).  Adds into (ax, ay, az); returns true when FP32 was used.
+__device__ __forceinline__ bool seg_field_mixed(double Rx, double Ry, double Rz, double Px,
+                                                double Py, double Pz, double f64, float f32,
+                                                double &ax, double &ay, double &az)
 {
-    float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
-    float p2 = fmaf(Pz, Pz, fmaf(Py, Py, Px * Px));
-    float dot = fmaf(Rz, Pz, fmaf(Ry, Py, Rx * Px));
-    float ia = __builtin_amdgcn_rsqf(r2);
-    float ib = __builtin_amdgcn_rsqf(p2);
-    float iab = ia * ib;
-    float nd = dot * iab;
-    bool ok = (r2 >= 1e-16f) && (p2 >= 1e-16f) && (nd >= 1e-8f);
-    // (|R|+|Rp|)/(|R||Rp|(|R||Rp| + dot)) = (ia + ib) / (1 + dot*iab) * iab
-    float m = f_bs * (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
-    m = ok ? m : 0.0f;
-    float cx = fmaf(Ry, Pz, -(Rz * Py));
-    float cy = fmaf(Rz, Px, -(Rx * Pz));
-    float cz = fmaf(Rx, Py, -(Ry * Px));
-    vx = cx * m;
-    vy = cy * m;
-    vz = cz * m;
+    const float rx = (float)Rx, ry = (float)Ry, rz = (float)Rz;
+    const float px = (float)Px, py = (float)Py, pz = (float)Pz;
+    const float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+    const float p2 = fmaf(pz, pz, fmaf(py, py, px * px));
+    const float dot = fmaf(rz, pz, fmaf(ry, py, rx * px));
+    const float ia = __builtin_amdgcn_rsqf(r2);
+    const float ib = __builtin_amdgcn_rsqf(p2);
+    const float iab = ia * ib;
+    const float nd = dot * iab;
+    if (fabsf(nd - 1e-8f) < 1e-5f || r2 < 2e-16f || p2 < 2e-16f) {
+        double vx, vy, vz;
+        if (seg_field64(Rx, Ry, Rz, Px, Py, Pz, f64, vx, vy, vz)) {
+            ax += vx; ay += vy; az += vz;
+        }
+        return false;
+    }
+    if (nd < 1e-8f) return true;  // inside the Thales sphere: the reference adds 0
+    const float m = f32 * (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
+    ax += (double)(fmaf(ry, pz, -(rz * py)) * m);
+    ay += (double)(fmaf(rz, px, -(rx * pz)) * m);
+    az += (double)(fmaf(rx, py, -(ry * px)) * m);
+    return true;
 }
 
 // seg_field32 for far-field entries given R and the segment vector
-// Q = Rp - R: Rp = R + Q is formed explicitly (a monopole's Q, the sum of
+// Q = Rp - R, without the guards, for pairs where none can trigger: the
+// target lies outside the segment's Thales sphere (centre a + Q/2, radius
+// |Q|/2) by a margin, so 1 + R.Rp/(|R||Rp|) stays away from 0 and |R|,
+// |Rp| from 0 (the caller checks each entry against the whole target box
+// and sends the others to the FP64 path, where the guards decide as in the
+// reference).  Rp = R + Q is formed explicitly (a monopole's Q, the sum of
 // its cell's segments, can reach past the target, where |Rp|^2 expanded as
-// r2 + 2 R.Q + |Q|^2 would cancel).  The three guards are folded into one
-// comparison of the smallest of r2, |Rp|^2 and 1e-8 times the normalised
-// dot against 1e-16.  R x Rp = R x Q, and with R = a - t (a the entry's and
-// t the target's offset from a common centre) R x Q = a x Q - t x Q: the
-// entry carries A = a x Q, the pair adds m A to sa and m Q to sq, and the
-// caller forms sa - t x sq and applies f_bs once per batch of entries.
-// seg_field32_far without the guards, for pairs where none can trigger:
-// the target lies outside the segment's Thales sphere (centre a + Q/2,
-// radius |Q|/2) by a margin, so 1 + R.Rp/(|R||Rp|) stays away from 0 and
-// |R|, |Rp| from 0 (the caller checks each entry against the whole target
-// box).  A zero entry placed far away (a = (1e15, 0, 0), Q = A = 0) adds
-// exactly zero.
+// r2 + 2 R.Q + |Q|^2 would cancel).  R x Rp = R x Q, and with R = a - t (a
+// the entry's and t the target's offset from a common centre)
+// R x Q = a x Q - t x Q: the entry carries A = a x Q, the pair adds m A to
+// sa and m Q to sq, and the caller forms sa - t x sq and applies f_bs once
+// per batch of entries.  A zero entry placed far away (a = (1e15, 0, 0),
+// Q = A = 0) adds exactly zero.
 __device__ __forceinline__ void seg_field32_far_clear(float Rx, float Ry, float Rz, float Qx,
                                                       float Qy, float Qz, float Ax, float Ay,
                                                       float Az, float &sax, float &say,
@@ -200,30 +209,6 @@ __device__ __forceinline__ void seg_field32_far_clear(float Rx, float Ry, float 
     const float ib = __builtin_amdgcn_rsqf(p2);
     const float iab = ia * ib;
     const float m = (ia + ib) * iab * __builtin_amdgcn_rcpf(fmaf(dot, iab, 1.0f));
-    sax = fmaf(Ax, m, sax);
-    say = fmaf(Ay, m, say);
-    saz = fmaf(Az, m, saz);
-    sqx = fmaf(Qx, m, sqx);
-    sqy = fmaf(Qy, m, sqy);
-    sqz = fmaf(Qz, m, sqz);
-}
-
-__device__ __forceinline__ void seg_field32_far(float Rx, float Ry, float Rz, float Qx, float Qy,
-                                                float Qz, float Ax, float Ay, float Az,
-                                                float &sax, float &say, float &saz, float &sqx,
-                                                float &sqy, float &sqz)
-{
-    const float Px = Rx + Qx, Py = Ry + Qy, Pz = Rz + Qz;
-    const float r2 = fmaf(Rz, Rz, fmaf(Ry, Ry, Rx * Rx));
-    const float p2 = fmaf(Pz, Pz, fmaf(Py, Py, Px * Px));
-    const float dot = fmaf(Rz, Pz, fmaf(Ry, Py, Rx * Px));
-    const float ia = __builtin_amdgcn_rsqf(r2);
-    const float ib = __builtin_amdgcn_rsqf(p2);
-    const float iab = ia * ib;
-    const float nd = dot * iab;
-    const bool ok = fminf(fminf(r2, p2), nd * 1e-8f) >= 1e-16f;
-    float m = (ia + ib) * iab * __builtin_amdgcn_rcpf(1.0f + nd);
-    m = ok ? m : 0.0f;
     sax = fmaf(Ax, m, sax);
     say = fmaf(Ay, m, say);
     saz = fmaf(Az, m, saz);

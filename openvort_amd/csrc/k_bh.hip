@@ -136,11 +136,11 @@ __device__ __forceinline__ void tree_walk(const TreeView &T, bool on, double qx,
         if (leafnode || d2 > nd.Lm * nd.Lm * T.cs) {
             // monopole / single segment: R = com - r, Rp = R + sum ds
             if (MIXED && !(leafnode && d2 < T.near2)) {
-                float vx, vy, vz;
-                seg_field32((float)ex, (float)ey, (float)ez, (float)(ex + nd.qx),
-                            (float)(ey + nd.qy), (float)(ez + nd.qz), T.f32, vx, vy, vz);
-                ax += vx; ay += vy; az += vz;
-                ++c32;
+                if (seg_field_mixed(ex, ey, ez, ex + nd.qx, ey + nd.qy, ez + nd.qz, T.f64,
+                                    T.f32, ax, ay, az))
+                    ++c32;
+                else
+                    ++c64;
             } else {
                 double vx, vy, vz;
                 if (seg_field64(ex, ey, ez, ex + nd.qx, ey + nd.qy, ez + nd.qz, T.f64, vx, vy,
@@ -158,12 +158,11 @@ __device__ __forceinline__ void tree_walk(const TreeView &T, bool on, double qx,
                 double b2 = bx * bx + by * by + bz * bz;
                 if (b2 < 1e-16) continue;
                 if (MIXED && !(b2 < T.near2)) {
-                    float vx, vy, vz;
-                    seg_field32((float)bx, (float)by, (float)bz, (float)(bx + T.sqx[s]),
-                                (float)(by + T.sqy[s]), (float)(bz + T.sqz[s]), T.f32, vx, vy,
-                                vz);
-                    ax += vx; ay += vy; az += vz;
-                    ++c32;
+                    if (seg_field_mixed(bx, by, bz, bx + T.sqx[s], by + T.sqy[s],
+                                        bz + T.sqz[s], T.f64, T.f32, ax, ay, az))
+                        ++c32;
+                    else
+                        ++c64;
                 } else {
                     double vx, vy, vz;
                     if (seg_field64(bx, by, bz, bx + T.sqx[s], by + T.sqy[s], bz + T.sqz[s],

@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     // a copy's sources are shifted by +L*shift (its targets are r - L*shift)
     double ax = 0, ay = 0, az = 0;
     unsigned long long c64 = 0, c32 = 0;
-    int far_n = 0, far_x = 0, near_n = 0;  // far: clear entries from the front, cross from the back
+    int far_n = 0, near_n = 0;  // far: clear FP32 entries; near: FP64 entries
     const int n_copies = any ? 1 + P.n_images : 0;
     const bool recorded = g < rec_groups;  // record / replay slot of this group
     const size_t rec_base = (size_t)g * rec_cap;
@@ -249,15 +249,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const bool keep_rem = far_cap - (4 * S - 1) >= 64;
     auto flush_far = [&](bool last) {
         float sax = 0.f, say = 0.f, saz = 0.f, sqx = 0.f, sqy = 0.f, sqz = 0.f;
-        // cross entries (from the back of the list) with the reference's
-        // guards, before the padding may overwrite them
-        for (int j = slice; j < far_x; j += S) {
-            const int i = kListFar - 1 - j;
-            const float4 a = farA[i];
-            const float4 b = farB[i];
-            seg_field32_far(a.x - tx, a.y - ty, a.z - tz, a.w, b.x, b.y, b.z, b.w, farC[i], sax,
-                            say, saz, sqx, sqy, sqz);
-        }
         // clear entries: whole rounds of 4 S, the last flush padded
         const bool pad = last || !keep_rem;
         const int fpad = pad ? (far_n + 4 * S - 1) / (4 * S) * (4 * S) : far_n / (4 * S) * (4 * S);
@@ -279,10 +270,9 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         ax += (double)(sax - fmaf(ty, sqz, -(tz * sqy))) * f64;
         ay += (double)(say - fmaf(tz, sqx, -(tx * sqz))) * f64;
         az += (double)(saz - fmaf(tx, sqy, -(ty * sqx))) * f64;
-        if (on) {  // pairs of this slice: clear blocks of 4 with index = slice mod S, cross
+        if (on) {  // pairs of this slice: blocks of 4 with index = slice mod S
             const int nf = nreal >> 2, rem = nreal & 3;
             c32 += 4 * (nf > slice ? (nf - slice + S - 1) / S : 0) + (nf % S == slice ? rem : 0);
-            c32 += far_x > slice ? (far_x - slice + S - 1) / S : 0;
         }
         __syncthreads();
         // the remainder [fpad, far_n) moves to the front (disjoint ranges:
@@ -300,7 +290,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             __syncthreads();
         }
         far_n = keep;
-        far_x = 0;
     };
     auto flush_near = [&]() {
         __syncthreads();
@@ -354,11 +343,15 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     //   inside - the whole box lies inside (farthest corner^2 <= 0.96 h^2):
     //            every pair fails the last guard and adds 0, as in the
     //            reference; not stored;
-    //   cross  - the rest (about 2% on dense tangles: long monopoles),
-    //            evaluated with the guards.
+    //   cross  - the rest (about 2% on dense tangles: long monopoles): near
+    //            entries, FP64 with the reference's guards.  The last guard
+    //            is a sign test on a cosine; evaluated in FP32, a pair within
+    //            ~1e-7 of the threshold could flip and add (or drop) its whole
+    //            contribution, which on the 1.4M-point tangle put a few
+    //            points' velocities off by percents.
     auto emit = [&](bool want, bool near, unsigned idx, float ex, float ey, float ez,
                     float gx, float gy, float gz) {
-        bool clear = true, cross = false;
+        bool clear = true, inside = false;
         if (MIXED) {
             const float mx = fabsf(fmaf(0.5f, gx, ex)), my = fabsf(fmaf(0.5f, gy, ey)),
                         mz = fabsf(fmaf(0.5f, gz, ez));
@@ -367,24 +360,23 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             const float fx = mx + bhx, fy = my + bhy, fz = mz + bhz;
             const float q2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
             clear = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) >= fmaf(0.26f, q2, 2e-10f);
-            cross = !clear && !(fmaf(fz, fz, fmaf(fy, fy, fx * fx)) <= 0.24f * q2);
+            inside = !clear && fmaf(fz, fz, fmaf(fy, fy, fx * fx)) <= 0.24f * q2;
         }
-        const bool wf = want && !near;
-        const unsigned long long mf = __ballot(wf && clear);
-        const unsigned long long mx = __ballot(wf && cross);
-        const unsigned long long mn = __ballot(want && near);
-        const int cf = __popcll(mf), cx = __popcll(mx), cn = __popcll(mn);
-        if ((cf | cx) && far_n + far_x + cf + cx > far_cap) flush_far(false);
+        const bool nr = near || !clear;  // FP64 entries: near field and cross
+        const bool keep = want && !(MIXED && !near && inside);
+        const unsigned long long mf = __ballot(keep && !nr);
+        const unsigned long long mn = __ballot(keep && nr);
+        const int cf = __popcll(mf), cn = __popcll(mn);
+        if (cf && far_n + cf > far_cap) flush_far(false);
         if (cn && near_n + cn > kListNear) flush_near();
-        if (wf && (clear || cross)) {
-            const int pos = clear ? far_n + prefix_below(mf) : kListFar - 1 - far_x - prefix_below(mx);
+        if (keep && !nr) {
+            const int pos = far_n + prefix_below(mf);
             farA[pos] = make_float4(ex, ey, ez, gx);
             farB[pos] = make_float4(gy, gz, fmaf(ey, gz, -(ez * gy)), fmaf(ez, gx, -(ex * gz)));
             farC[pos] = fmaf(ex, gy, -(ey * gx));
         }
-        if (want && near) nearI[near_n + prefix_below(mn)] = idx;
+        if (keep && nr) nearI[near_n + prefix_below(mn)] = idx;
         far_n += cf;
-        far_x += cx;
         near_n += cn;
         if (MODE == kRecord) {  // every accepted entry, whatever its class
             const unsigned long long mw = __ballot(want);
@@ -563,7 +555,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     }
     }  // walk
     if (MODE == kRecord && recorded && lane == 0) rec_count[g] = rec_over ? -1 : rec_n;
-    if (far_n | far_x) flush_far(true);
+    if (far_n) flush_far(true);
     if (near_n) flush_near();
     // slice partials in slice order
     for (int s = 1; s < S; ++s) {
@@ -668,10 +660,10 @@ static int build_groups(Ctx &c)
     const int8_t *delta = T.delta.as<int8_t>();
     const int gs = (n + 255) / 256;
     hipLaunchKernelGGL(k_grp_cut, dim3(gs), dim3(256), 0, c.stream, n, delta, L0, a);
-    launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp));
     hipLaunchKernelGGL(k_grp_cells, dim3(gs), dim3(256), 0, c.stream, n, delta, L0, a, b, cc);
     hipLaunchKernelGGL(k_grp_flag<PPW>, dim3(gs), dim3(256), 0, c.stream, n, b, cc, a);
-    launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp);
+    OVG_TRY(launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp));
     hipLaunchKernelGGL(k_grp_emit<PPW>, dim3(gs), dim3(256), 0, c.stream, n, a, b, cc,
                        T.groups.as<int2>(), T.counters.as<int>());
     return OVG_OK;
