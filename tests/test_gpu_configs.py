@@ -1,0 +1,255 @@
+"""GPU tests at the sizes of the large workloads, with the settings the
+benchmark times.
+
+- Large single tangle (4000 rings, D = 0.2 cm, dxi = 5e-4 cm, 1,386,017
+  points): the list traversal at leaf 1, 16 points per wavefront and the
+  automatic group level, and at the autotuner's pick.  Mixed precision
+  against FP64 on the same tree: within 1e-6 relative L2 over every point
+  (the FP32 far field's rounding only).  FP64 list traversal against the
+  FP64 direct sum (ovg_velocity_at, the reference's update_velocity with
+  tree == NULL) on a seeded 2048-point sample: no larger than the error of
+  the reference-acceptance point traversal on the same sample, since the
+  group test never accepts a node that the per-point test opens.
+- Batched ensemble (2k-100k point tangles, D = 0.1 cm, dxi = 8e-4 cm): every
+  job's final state bit-identical alone and in the concurrent mix, and every
+  job's mixed-precision velocities within 1e-6 of FP64 on the same tree.
+- Serving stream sizes (5k-500k points, D up to 0.2 cm, dxi down to
+  5e-4 cm): the same two checks.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import openvort_amd as ov
+from openvort_amd import configs
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c3():
+    t, p, dt = configs.make("c3", precision=ov.PREC_MIXED, traverse=ov.TRAVERSE_LIST,
+                            leaf_size=1)
+    p.points_per_wave = 16
+    p.group_level = 0
+    rng = np.random.default_rng(7)
+    sub = np.sort(rng.choice(t.live, size=2048, replace=False))
+    say(f"c3 tangle: {len(t.live)} points, {t.n} slots")
+    with ov.Engine(p) as e:
+        e.upload(t)
+        e.update_tangents_normals()
+        direct_vs, _ = e.velocity_at(sub, 0.0)
+    return t, p, sub, direct_vs
+
+
+def evaluate(t, p, **kw):
+    q = ov.Params.from_buffer_copy(p)
+    for k, v in kw.items():
+        setattr(q, k, v)
+    with ov.Engine(q) as e:
+        e.upload(t)
+        e.update_tangle(0.0)
+        o = e.download()
+        s = e.stats()
+    return o, s
+
+
+def test_c3_headline_setting_precision_and_truncation(c3):
+    t, p, sub, direct_vs = c3
+    live = t.live
+    assert len(live) == 1386017
+    say("c3 list traversal leaf 1, 16 points per wave, auto group level: mixed and FP64")
+    o32, s32 = evaluate(t, p)
+    o64, s64 = evaluate(t, p, precision=ov.PREC_FP64)
+    prec = rel_l2(o32["vs"][live], o64["vs"][live])
+    say("c3 point traversal (reference acceptance, FP64)")
+    opt, _ = evaluate(t, p, precision=ov.PREC_FP64, traverse=ov.TRAVERSE_POINT)
+    e_list = rel_l2(o64["vs"][sub], direct_vs)
+    e_mixed = rel_l2(o32["vs"][sub], direct_vs)
+    e_point = rel_l2(opt["vs"][sub], direct_vs)
+    say(f"c3 precision (mixed vs FP64, same tree) {prec:.3e}; vs direct on 2048 points: "
+        f"list FP64 {e_list:.3e}, list mixed {e_mixed:.3e}, point {e_point:.3e}; "
+        f"pairs/pt {s64.interactions / len(live):.0f}")
+    assert prec < 1e-6
+    assert s32.interactions_fp32 > 0.9 * s32.interactions
+    assert s64.interactions_fp32 == 0
+    assert e_list <= e_point
+    assert e_mixed <= e_point
+    assert np.isfinite(o32["vels"][live]).all()
+
+
+def test_c3_autotuned_setting_precision(c3):
+    t, p, sub, direct_vs = c3
+    live = t.live
+    with ov.Engine(p) as e:
+        e.upload(t)
+        rows, best = e.autotune(repeats=1)
+        q = e.get_params()
+    say(f"c3 autotune pick: {rows[best]}")
+    o32, _ = evaluate(t, q)
+    o64, _ = evaluate(t, q, precision=ov.PREC_FP64)
+    prec = rel_l2(o32["vs"][live], o64["vs"][live])
+    e_list = rel_l2(o64["vs"][sub], direct_vs)
+    opt, _ = evaluate(t, p, precision=ov.PREC_FP64, traverse=ov.TRAVERSE_POINT)
+    e_point = rel_l2(opt["vs"][sub], direct_vs)
+    say(f"c3 tuned: precision {prec:.3e}, list vs direct {e_list:.3e}, point {e_point:.3e}")
+    assert prec < 1e-6
+    assert e_list <= e_point
+
+
+def job_tangle(job, precision, traverse=ov.TRAVERSE_LIST):
+    """The tangle and parameters the request loop builds for a job
+    (ovs_run's job set-up)."""
+    t = ov.generate_rings(job.seed, job.n_rings, job.D, job.r_min, job.r_max, job.dxi)
+    n_cap = 512
+    while n_cap < len(t.live):
+        n_cap *= 2
+    t = ov.generate_rings(job.seed, job.n_rings, job.D, job.r_min, job.r_max, job.dxi,
+                          n_cap=n_cap)
+    p = configs.params(job.D, job.dxi, (job.alpha, job.alpha_p), job.v_ns,
+                       precision=precision, theta=job.theta, leaf_size=1, traverse=traverse)
+    p.points_per_wave = job.points_per_wave
+    return t, p
+
+
+def mix_equals_alone(jobs, workers):
+    from openvort_amd import serve
+    alone = []
+    for j in jobs:
+        rep, res = serve.run([j], workers=1, max_active=1, realtime=False)
+        assert rep["jobs_done"] == 1 and res[0].failed == 0
+        alone.append(res[0])
+    rep, res = serve.run(jobs, workers=workers, max_active=len(jobs), realtime=False)
+    assert rep["jobs_done"] == len(jobs) and rep["jobs_failed"] == 0
+    assert rep["peak_active"] > 1
+    for j, a, b in zip(jobs, alone, res):
+        assert b.steps_done == a.steps_done == j.steps
+        assert b.checksum == a.checksum, j.seed
+        assert b.points_final == a.points_final and b.line_length == a.line_length
+    return rep
+
+
+def per_job_precision(jobs):
+    worst = 0.0
+    for j in jobs:
+        t, p = job_tangle(j, ov.PREC_MIXED)
+        live = t.live
+        o32, _ = evaluate(t, p)
+        o64, _ = evaluate(t, p, precision=ov.PREC_FP64)
+        prec = rel_l2(o32["vs"][live], o64["vs"][live])
+        worst = max(worst, prec)
+        assert prec < 1e-6, (j.seed, len(live), prec)
+    return worst
+
+
+ENSEMBLE_SIZES = (2000, 4000, 8000, 15000, 30000, 50000, 75000, 100000)
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    from openvort_amd import serve
+    jobs = serve.c4_jobs(n=len(ENSEMBLE_SIZES), steps=3)
+    for j, pts in zip(jobs, ENSEMBLE_SIZES):
+        j.n_rings = ov.rings_for_points(j.seed, pts, j.D, j.r_min, j.r_max, j.dxi)
+        j.smooth_every = 2
+    return jobs
+
+
+def test_ensemble_mix_equals_alone(ensemble):
+    say(f"ensemble: {len(ensemble)} jobs of 2k-100k points alone, then together on 4 workers")
+    rep = mix_equals_alone(ensemble, 4)
+    say(f"ensemble mix: {rep['point_steps_per_s']:.3g} point-steps/s")
+
+
+def test_ensemble_precision(ensemble):
+    say("ensemble: mixed vs FP64 on the same tree, every job")
+    say(f"ensemble worst precision error {per_job_precision(ensemble):.3e}")
+
+
+# (target points, D, dxi): the smallest and largest job sizes, the largest
+# box and the finest spacing of the serving stream
+SERVING_CASES = ((5000, 0.1, 1e-3), (60000, 0.15, 7e-4), (250000, 0.2, 5e-4),
+                 (500000, 0.2, 5e-4))
+
+
+@pytest.fixture(scope="module")
+def serving():
+    from openvort_amd import serve
+    jobs = serve.c5_jobs(n=len(SERVING_CASES), steps_scale=0.0)
+    for j, (pts, D, dxi) in zip(jobs, SERVING_CASES):
+        j.D, j.dxi = D, dxi
+        j.n_rings = ov.rings_for_points(j.seed, pts, D, j.r_min, j.r_max, dxi)
+        j.steps = 2
+        j.smooth_every = 2
+        j.arrival_s = 0.0
+    return jobs
+
+
+def test_serving_sizes_mix_equals_alone(serving):
+    say("serving sizes: 4 jobs of 5k-500k points alone, then together on 4 workers")
+    rep = mix_equals_alone(serving, 4)
+    say(f"serving mix: {rep['point_steps_per_s']:.3g} point-steps/s")
+
+
+def test_serving_sizes_precision(serving):
+    say("serving sizes: mixed vs FP64 on the same tree, every job")
+    say(f"serving worst precision error {per_job_precision(serving):.3e}")
+
+
+def test_reconnect_tree_vs_direct_decisions_c2():
+    """The reconnection pass with the step's Barnes-Hut velocities (the
+    benchmark's setting) against the reference's direct-sum velocities
+    (src/vortex_dynamics.c:250-251), each pass from the same state, over 10
+    steps of the full 200-ring tangle advanced with direct-sum decisions.
+    The decisions differ only where the approach test dx.dv <= 0 changes
+    sign within the tree's velocity error."""
+    t, p, dt = configs.make("c2", precision=ov.PREC_MIXED, traverse=ov.TRAVERSE_LIST,
+                            leaf_size=1)
+    p.points_per_wave = 16
+    pd = ov.Params.from_buffer_copy(p)
+    pd.rec_velocity = ov.RECVEL_DIRECT
+    pt = ov.Params.from_buffer_copy(p)
+    pt.rec_velocity = ov.RECVEL_TREE
+    rb = configs.remesh_bounds("c2")
+    say(f"c2 reconnection decisions, tree vs direct velocities: {len(t.live)} points")
+    rows = []
+    with ov.Engine(pd) as e, ov.Engine(pt) as f:
+        e.upload(t)
+        tm = 0.0
+        for s in range(10):
+            x = e.download()["vnodes"]
+            conn, st = e.download_connections()
+            f.upload(ov.Tangle(x, conn, st))
+            cand = len(e.reconnect_candidates())
+            nd = e.reconnect(tm)
+            nt = f.reconnect(tm)
+            cd, _ = e.download_connections()
+            ct, _ = f.download_connections()
+            diff = int(np.count_nonzero(np.any(cd != ct, axis=1)))
+            rows.append((cand, nd, nt, diff))
+            e.eliminate_small_loops(5)
+            if s % 10 == 0:
+                e.smooth()
+            e.enforce_periodic()
+            e.remesh(*rb)
+            e.eliminate_small_loops(5)
+            e.update_tangle(tm)
+            e.rk4_step(tm, dt)
+            e.enforce_periodic()
+            tm += dt
+    for s, (cand, nd, nt, diff) in enumerate(rows):
+        say(f"step {s}: {cand} candidate pairs, reconnections direct {nd} tree {nt}, "
+            f"slots with different links {diff}")
+    tot_d = sum(r[1] for r in rows)
+    tot_t = sum(r[2] for r in rows)
+    same = sum(1 for r in rows if r[3] == 0)
+    say(f"total reconnections direct {tot_d}, tree {tot_t}; identical links in {same} of 10")
+    assert tot_d > 0
+    assert abs(tot_t - tot_d) <= max(2, 0.1 * tot_d)

@@ -148,11 +148,12 @@ def test_c2_mixed_precision_error(c2):
     assert trunc < trunc_ref
 
 
-@pytest.mark.parametrize("ppw", [8, 16, 32, 64])
-def test_c2_list_traversal(c2, ppw):
+@pytest.mark.parametrize("ppw,leaf", [(8, 16), (16, 16), (32, 16), (64, 16), (16, 1), (8, 1),
+                                      (32, 4)])
+def test_c2_list_traversal(c2, ppw, leaf):
     g, t, p = c2
-    say(f"c2 BH list traversal, {ppw} points per wave, leaf 16: FP64 and mixed")
-    kw = dict(traverse=ov.TRAVERSE_LIST, leaf_size=16, points_per_wave=ppw)
+    say(f"c2 BH list traversal, {ppw} points per wave, leaf {leaf}: FP64 and mixed")
+    kw = dict(traverse=ov.TRAVERSE_LIST, leaf_size=leaf, points_per_wave=ppw)
     o64, s64 = run_c2(t, p, **kw)
     o32, s32 = run_c2(t, p, precision=ov.PREC_MIXED, **kw)
     live, sub = t.live, g["sub"]

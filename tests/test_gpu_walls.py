@@ -231,3 +231,49 @@ def test_pinned_small_lines_vs_oracle(oracle):
     live = w.live
     ref = oracle.update_tangle(w, p)
     assert rel_l2(o["vels"][live], ref["vels"][live]) < 1e-10
+
+
+@pytest.mark.parametrize("config,leaf", [(ov.IMAGES_WALL_CANAL, 1), (ov.IMAGES_WALL_CANAL, 16),
+                                         (ov.IMAGES_WALL_1_18, 1),
+                                         (ov.IMAGES_WALL_PARALLEL_PLANES, 1),
+                                         (ov.IMAGES_WALL_PARALLEL_PLANES, 8), ("pinned", 1)])
+def test_wall_list_truncation_vs_exact_sum(config, leaf):
+    """theta 0.4 with mirror images: the list traversal's acceptance of a
+    mirrored copy (its reflected centre against the group box) and its
+    reflected multi-segment monopoles, against the exact reflected sum.  The
+    exact sum is the reference-acceptance point traversal at theta 1e-6
+    (every source a single segment; equal to the oracle's reflected copies,
+    tests/test_walls.py).  The direct path is no reference here: it adds
+    mirror images unreflected, as the reference's direct sum does.  The list
+    error must not exceed the point traversal's at theta 0.4: its group test
+    never accepts a cell that the per-point test opens, so a wrong
+    reflection or a misplaced mirrored centre shows as a larger error."""
+    from test_walls import pinned_case
+    if config == "pinned":
+        t, p = pinned_case(ov.METHOD_BH, True)
+    else:
+        t, p = wall_case(config, ov.METHOD_BH, n=24)
+    live = t.live
+    say(f"walls {config}, leaf {leaf}: list and point at theta 0.4 vs the exact sum, "
+        f"{len(live)} points")
+
+    def run(trav, theta, leaf_size=1, prec=ov.PREC_FP64):
+        q = ov.Params.from_buffer_copy(p)
+        q.traverse, q.theta, q.leaf_size, q.precision = trav, theta, leaf_size, prec
+        with ov.Engine(q) as e:
+            e.upload(t)
+            e.update_tangle(0.0)
+            s = e.stats()
+            return e.download()["vs"][live], s
+
+    exact, _ = run(ov.TRAVERSE_POINT, 1e-6)
+    point, sp = run(ov.TRAVERSE_POINT, 0.4)
+    lst, sl = run(ov.TRAVERSE_LIST, 0.4, leaf)
+    mixed, _ = run(ov.TRAVERSE_LIST, 0.4, leaf, ov.PREC_MIXED)
+    e_point, e_list, e_mixed = rel_l2(point, exact), rel_l2(lst, exact), rel_l2(mixed, exact)
+    say(f"vs exact: point {e_point:.3e} ({sp.interactions:.0f} pairs), list {e_list:.3e} "
+        f"({sl.interactions:.0f} pairs), list mixed {e_mixed:.3e}")
+    # the tangle is large enough that theta 0.4 accepts cells
+    assert e_point > 1e-6
+    assert e_list <= e_point
+    assert e_mixed <= e_point
