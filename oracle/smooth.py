@@ -11,11 +11,16 @@ numpy.fft.rfft: r_k = Re X_k, i_k = Im X_k with X_k = sum_j x_j e^{-2 pi i jk/N}
 the band of entries the reference zeroes is removed and the halfcomplex
 inverse (HC2R) is taken, then divided by N as the reference does.
 
-The forward plan receives FFTW_FORWARD (-1) as its r2r kind (:142); this
-restatement takes it as R2HC, the layout the reference's own comment at
-:148 describes.  That reading cannot be checked without FFTW: FFTW-level
-parity of the smoothing is unpinned.  The restatement itself is pinned by
-closed forms in tests/test_smooth_oracle.py.
+The forward plan receives FFTW_FORWARD (-1) as its r2r kind (:142) and the
+backward plan FFTW_BACKWARD (+1, :163).  In FFTW 3's fftw_r2r_kind
+enumeration +1 is FFTW_HC2R (R2HC = 0, HC2R = 1, DHT = 2, REDFT00 = 3, ...),
+so the backward plan is the halfcomplex inverse; -1 is no r2r kind at all.
+This restatement takes the forward plan as R2HC, the transform whose
+inverse HC2R is and the layout the reference's own comment at :148
+describes.  That reading cannot be checked without FFTW: FFTW-level parity
+of the smoothing is unpinned.  The restatement itself (the filter, the
+detrend and odd extension of box-spanning loops) is pinned by closed forms
+in tests/test_smooth_oracle.py.
 """
 import numpy as np
 
@@ -53,6 +58,34 @@ def low_pass(x, freq_to_cutoff):
     # HC2R of the kept entries; irfft normalises by 1/N already, which is the
     # reference's division by N after the unnormalised backward transform
     return np.fft.irfft(Y, n=N)
+
+
+def spans_box(xs, L):
+    """The reference's test for a loop that wraps the box: its unwrapped
+    start and end points more than half a box width apart along x's box
+    side (src/vortex_smooth.c:246-247)."""
+    return np.sqrt(np.sum((xs[0] - xs[-1]) ** 2)) > L[0] / 2
+
+
+def detrend_odd(a):
+    """One coordinate of a box-spanning loop (src/vortex_smooth.c:248-285):
+    the straight line from the first to the last point is subtracted (both
+    ends become 0) and the result is continued to length 2m - 2 by its odd
+    reflection about the last point, ext[m + j] = -a[m - 2 - j].  Returns
+    (ext, coef, x0) with the line x0 + coef * j."""
+    a = np.array(a, dtype=np.float64)
+    m = len(a)
+    coef = (a[m - 1] - a[0]) / (m - 1)
+    x0 = a[0]
+    a[1:m - 1] = a[1:m - 1] - coef * np.arange(1, m - 1) - x0
+    a[0] = a[m - 1] = 0.0
+    return np.concatenate([a, -a[m - 2:0:-1]]), coef, x0
+
+
+def retrend(f, m, coef, x0):
+    """The first m filtered values with the line added back
+    (src/vortex_smooth.c:287-292)."""
+    return f[:m] + coef * np.arange(m) + x0
 
 
 def pwrap_next(prev, cur, L, periodic):
@@ -93,19 +126,13 @@ def smooth_tangle(vnodes, conn, status, box_lo, box_hi, periodic, freq_to_cutoff
         xs[0] = vnodes[slots[0]]
         for j in range(1, m):
             xs[j] = pwrap_next(xs[j - 1], vnodes[slots[j]], L, periodic)
-        span = np.sqrt(np.sum((xs[0] - xs[m - 1]) ** 2)) > L[0] / 2
+        span = spans_box(xs, L)
         ys = np.zeros_like(xs)
         for c in range(3):
-            a = xs[:, c].copy()
             if span:
-                coef = (a[m - 1] - a[0]) / (m - 1)
-                x0 = a[0]
-                a[1:m - 1] = a[1:m - 1] - coef * np.arange(1, m - 1) - x0
-                a[0] = a[m - 1] = 0.0
-                ext = np.concatenate([a, -a[m - 2:0:-1]])
-                f = low_pass(ext, freq_to_cutoff)[:m]
-                ys[:, c] = f + coef * np.arange(m) + x0
+                ext, coef, x0 = detrend_odd(xs[:, c])
+                ys[:, c] = retrend(low_pass(ext, freq_to_cutoff), m, coef, x0)
             else:
-                ys[:, c] = low_pass(a, freq_to_cutoff)
+                ys[:, c] = low_pass(xs[:, c], freq_to_cutoff)
         out[slots] = ys
     return out
