@@ -71,28 +71,34 @@ def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
 
 
 def accuracy(ov, t, p_fast, p_plain, n_sample, seed=7):
-    """Velocity error of both paths on the initial tangle against the FP64
+    """Velocity error on the initial tangle: both paths against the FP64
     direct sum (ovg_velocity_at, the reference's update_velocity with
-    tree == NULL) at a seeded sample of points: relative L2 over the sample."""
+    tree == NULL) at a seeded sample of points (relative L2 over the
+    sample), and the fast path's precision error alone: mixed against FP64
+    with the same tree and acceptance, over every point."""
     live = t.live
     rng = np.random.default_rng(seed)
     sub = np.sort(rng.choice(live, size=min(n_sample, len(live)), replace=False))
+    p64 = ov.Params.from_buffer_copy(p_fast)
+    p64.precision = ov.PREC_FP64
     out = {}
     direct = None
-    for name, q in (("fast", p_fast), ("plain_fp64", p_plain)):
+    for name, q in (("fast", p_fast), ("fast_fp64", p64), ("plain_fp64", p_plain)):
         e = ov.Engine(q)
         e.upload(t)
         e.update_tangle(0.0)
-        v = e.download()["vels"][sub]
+        out[name] = e.download()["vs"]
         if direct is None:
-            direct = e.velocity_at(sub, 0.0)[1]
+            direct = e.velocity_at(sub, 0.0)[0]
         e.close()
-        out[name] = v
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    return {"sample_points": int(len(sub)), "field": "vels",
-            "fast_vs_direct": rel(out["fast"], direct),
-            "plain_fp64_vs_direct": rel(out["plain_fp64"], direct),
-            "fast_vs_plain_fp64": rel(out["fast"], out["plain_fp64"])}
+    return {"sample_points": int(len(sub)), "field": "vs (Biot-Savart + LIA + imposed v_s)",
+            "fast_vs_direct": rel(out["fast"][sub], direct),
+            "fast_fp64_vs_direct": rel(out["fast_fp64"][sub], direct),
+            "plain_fp64_vs_direct": rel(out["plain_fp64"][sub], direct),
+            "fast_vs_plain_fp64": rel(out["fast"][sub], out["plain_fp64"][sub]),
+            "mixed_vs_fp64_same_tree": rel(out["fast"][live], out["fast_fp64"][live]),
+            "mixed_vs_fp64_points": int(len(live))}
 
 
 def serve_bench(args):
@@ -161,14 +167,13 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v) * 1e3, q))
 
 
-def cpu_baseline(workload, rings, threads, steps):
-    """The reference (oracle/_ref/ovref_bench, built with the reference
-    Makefile's flags) on a bounded sample of the same workload."""
+def ref_bench(t, p, dt, threads, steps=1, reconnect=0, mode="steps", timeout=1200):
+    """One run of the reference timer (oracle/_ref/ovref_bench, the
+    reference built with its Makefile's flags) on tangle t; its JSON line,
+    or None."""
     from oracle.ovoracle import REF_BENCH
-    from openvort_amd import configs
     if not os.path.exists(REF_BENCH):
         return None
-    t, p, dt = configs.make(workload, rings=rings)
     with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
         f.write(np.int32(t.n).tobytes())
         f.write(bytes(p))
@@ -178,15 +183,121 @@ def cpu_baseline(workload, rings, threads, steps):
         path = f.name
     env = dict(os.environ, LSAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS=str(threads))
     try:
-        out = subprocess.run([REF_BENCH, path, str(threads), str(steps), repr(dt), "1"],
-                             capture_output=True, text=True, env=env, timeout=600)
+        out = subprocess.run([REF_BENCH, path, str(threads), str(steps), repr(dt),
+                              str(reconnect), mode],
+                             capture_output=True, text=True, env=env, timeout=timeout)
     finally:
         os.unlink(path)
     if out.returncode != 0:
-        log(f"cpu baseline failed: {out.stderr[-400:]}")
+        log(f"cpu baseline ({mode}) failed: {out.stderr[-400:]}")
         return None
-    r = json.loads(out.stdout.strip().splitlines()[-1])
-    return r
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def sub_box(ov, t, side):
+    """The points of t inside the corner cube [lo, lo + side)^3 of its box,
+    as closed loops of the same slots' positions: each maximal run of
+    consecutive points inside the cube is closed on itself (runs of fewer
+    than 8 points are dropped), followed by empty slots in t's ratio of
+    slots to points.  Line density and spacing are the full tangle's, so
+    the candidate pairs per point of the reconnection scan are too, away
+    from the cube's faces."""
+    inside = np.all(t.vnodes < side, axis=1) & (t.status[:, 0] != ov.EMPTY)
+    fwd, rev = t.conn[:, 0], t.conn[:, 1]
+    seen = np.zeros(t.n, dtype=bool)
+    runs = []
+    for k in np.nonzero(inside)[0]:
+        if seen[k]:
+            continue
+        a = k  # walk back to the run's first point (a loop wholly inside stops at k)
+        while inside[rev[a]] and rev[a] != k:
+            a = rev[a]
+        run, b = [], a
+        while inside[b] and not seen[b]:
+            seen[b] = True
+            run.append(b)
+            b = fwd[b]
+        if len(run) >= 8:
+            runs.append(run)
+    m = sum(len(r) for r in runs)
+    # the full tangle's ratio of slots to points: the reference scans every
+    # slot, empty ones included
+    n_cap = max(m, int(round(m * t.n / len(t.live))))
+    v = np.zeros((n_cap, 3))
+    c = np.full((n_cap, 2), -1, dtype=np.int32)
+    s = np.zeros((n_cap, 2), dtype=np.int32)
+    s[:, 0] = ov.EMPTY
+    o = 0
+    for run in runs:
+        L = len(run)
+        v[o:o + L] = t.vnodes[run]
+        j = np.arange(L)
+        c[o:o + L, 0] = o + (j + 1) % L
+        c[o:o + L, 1] = o + (j - 1) % L
+        s[o:o + L] = (ov.FREE, -1)
+        o += L
+    return ov.Tangle(v, c, s)
+
+
+def cpu_baseline(workload, threads, sub_points, sample_rings, sample_steps, full_step):
+    """The reference on the host cores, per phase, on the benchmark tangle:
+    one octree build and one update_tangle (tangents, tree, the velocity of
+    every point) at full size; rk4_step is three more such evaluations, so
+    it is taken as 3 x update_tangle unless full_step asks to time it too
+    (it takes that long); one reconnect on a corner sub-box of sub_points
+    points at the same line density, scaled by (N / N_sub)^2 (the
+    reference's reconnect is a serial all-pairs scan whose candidates per
+    point are fixed by the density, each costing two direct sums over every
+    segment: both parts grow as N^2).  A step is reconnect + update_tangle +
+    rk4_step + wrap (FFTW, the reference's smoothing library, is absent, so
+    smoothing is not timed).  Beside it, the bounded sample of earlier
+    versions: `sample_steps` full steps of the first `sample_rings` rings."""
+    import openvort_amd as ov
+    from openvort_amd import configs
+    t, p, dt = configs.make(workload)
+    n = len(t.live)
+    ph = ref_bench(t, p, dt, threads, steps=1 if full_step else 0, mode="phases",
+                   timeout=3000)
+    if not ph:
+        return None
+    rk4_measured = ph["rk4_step_s"] >= 0
+    if not rk4_measured:
+        ph["rk4_step_s"] = 3 * ph["update_tangle_s"]
+    D = p.box_hi[0] - p.box_lo[0]
+    st = sub_box(ov, t, D * (sub_points / n) ** (1.0 / 3.0))
+    rc = ref_bench(st, p, dt, threads, reconnect=1, mode="reconnect")
+    if not rc:
+        return None
+    scale = (n / rc["points"]) ** 2
+    rec_s = rc["reconnect_s"] * scale
+    step_s = rec_s + ph["update_tangle_s"] + ph["rk4_step_s"] + ph["wrap_s"]
+    out = {"value": n / step_s, "unit": "point-steps/s", "cores": threads, "kind": "reference",
+           "sample": (f"{workload} at full size ({n} points): one octree_build and one "
+                      f"update_tangle timed, rk4_step "
+                      + ("timed" if rk4_measured else "taken as 3 x update_tangle (its three "
+                         "evaluations)")
+                      + f"; reconnect timed on a corner "
+                      f"sub-box of {rc['points']} points at the same line density and scaled "
+                      f"by (N/N_sub)^2 = {scale:.4g}; step = reconnect + update_tangle + "
+                      f"rk4_step + wrap; smoothing not timed (FFTW absent); reference "
+                      f"src/ built with its Makefile flags (-march=x86-64-v3)"),
+           "step_s": step_s,
+           "phases_s": {"tree_build": ph["tree_build_s"], "update_tangle": ph["update_tangle_s"],
+                        "rk4_step": ph["rk4_step_s"], "rk4_step_measured": rk4_measured,
+                        "wrap": ph["wrap_s"], "reconnect_extrapolated": rec_s},
+           "reconnect_sub_box": {"points": rc["points"], "seconds": rc["reconnect_s"],
+                                 "reconnections": rc["reconnections"]},
+           "nproc": os.cpu_count()}
+    if sample_rings > 0 and sample_steps > 0:
+        ts, ps, dts = configs.make(workload, rings=sample_rings)
+        r = ref_bench(ts, ps, dts, threads, steps=sample_steps, reconnect=1, mode="steps")
+        if r:
+            out["bounded_sample"] = {
+                "value": r["point_steps_per_s"], "points": r["points"], "steps": r["steps"],
+                "seconds": r["seconds"], "reconnect_s": r["reconnect_s"],
+                "sample": (f"first {sample_rings} rings of the {workload} stream, "
+                           f"{r['steps']} full steps incl. the O(N^2) reconnect")}
+    return out
 
 
 def main():
@@ -206,9 +317,19 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--reuse-steps", type=int, default=5,
                     help="also time this many steps with reuse_lists = 1 (0 = skip)")
-    ap.add_argument("--cpu-rings", type=int, default=62)
+    ap.add_argument("--rec-direct-steps", type=int, default=3,
+                    help="also time this many steps with the reference's direct-sum "
+                         "reconnection velocities (0 = skip)")
+    ap.add_argument("--cpu-rings", type=int, default=62,
+                    help="CPU baseline: rings of the secondary bounded sample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-full-step", action="store_true",
+                    help="CPU baseline: also time rk4_step at full size (minutes)")
+    ap.add_argument("--cpu-sub-points", type=int, default=25000,
+                    help="CPU baseline: points of the sub-box the reconnect is timed on")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = the host cores this process may use, "
+                         "at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep --ppw/--leaf/--group-level instead of tuning them first")
@@ -265,12 +386,6 @@ def main():
         ov.METHOD_BH, ov.PREC_FP64, ov.TRAVERSE_POINT, 1)
     q_plain.rec_velocity = recv
 
-    acc = None
-    if args.accuracy_sample > 0:
-        acc = accuracy(ov, t, p, q_plain, args.accuracy_sample)
-        log(f"accuracy vs direct: fast {acc['fast_vs_direct']:.3e}, "
-            f"plain {acc['plain_fp64_vs_direct']:.3e}")
-
     # ---- fast path ------------------------------------------------------
     e = ov.Engine(p)
     e.upload(t)
@@ -281,6 +396,14 @@ def main():
         rows, best = e.autotune(repeats=2)
         tune = dict(rows[best], seconds=time.perf_counter() - a, candidates=len(rows))
         log(f"autotune: {tune}")
+    p_run = e.get_params()  # the tuned setting that the timed steps use
+
+    acc = None
+    if args.accuracy_sample > 0:
+        acc = accuracy(ov, t, p_run, q_plain, args.accuracy_sample)
+        log(f"accuracy vs direct: fast {acc['fast_vs_direct']:.3e}, "
+            f"plain {acc['plain_fp64_vs_direct']:.3e}; mixed vs FP64 same tree "
+            f"{acc['mixed_vs_fp64_same_tree']:.3e}")
     rb = configs.remesh_bounds(args.workload)
     lat_w, tm = run_steps(e, args.warmup, 0.0, dt, 0, args.smooth_every, True, rb)
     e.stats()  # reset counters
@@ -306,10 +429,7 @@ def main():
     # ---- the same path with the step's first lists reused ----------------
     reuse = None
     if trav == ov.TRAVERSE_LIST and args.reuse_steps > 0:
-        qr = ov.Params.from_buffer_copy(p)
-        if tune:
-            qr.points_per_wave, qr.leaf_size, qr.group_level = (
-                tune["points_per_wave"], tune["leaf_size"], tune["group_level"])
+        qr = ov.Params.from_buffer_copy(p_run)
         qr.reuse_lists = 1
         e3 = ov.Engine(qr)
         e3.upload(t)
@@ -329,6 +449,43 @@ def main():
                          "(acceptance decided at the step's first stage; not the "
                          "headline value)"}
         log(f"reuse_lists: {reuse['ms_per_step']:.1f} ms/step")
+
+    # ---- the same path with the reference's reconnection velocities ------
+    rec_direct = None
+    if args.rec_direct_steps > 0 and p_run.rec_velocity != ov.RECVEL_DIRECT:
+        qd = ov.Params.from_buffer_copy(p_run)
+        qd.rec_velocity = ov.RECVEL_DIRECT
+        e4 = ov.Engine(qd)
+        e4.upload(t)
+        e4.synchronize()
+        _, tm4 = run_steps(e4, 1, 0.0, dt, 0, args.smooth_every, True, rb)
+        e4.synchronize()
+        a = time.perf_counter()
+        lat4, _ = run_steps(e4, args.rec_direct_steps, tm4, dt, 1, args.smooth_every, True, rb)
+        e4.synchronize()
+        w4 = time.perf_counter() - a
+        # one pass alone: its endpoints and wall time
+        pairs = e4.reconnect_candidates()
+        ends = len(np.unique(pairs)) if len(pairs) else 0
+        e4.synchronize()
+        a = time.perf_counter()
+        e4.reconnect(tm4)
+        e4.synchronize()
+        w_pass = time.perf_counter() - a
+        e4.close()
+        rec_direct = {"value": n_pts * args.rec_direct_steps / w4,
+                      "steps": args.rec_direct_steps,
+                      "ms_per_step": w4 / args.rec_direct_steps * 1e3,
+                      "p50_step_ms": pct(lat4, 50), "p99_step_ms": pct(lat4, 99),
+                      "reconnect_pass_ms": w_pass * 1e3, "candidate_pairs": int(len(pairs)),
+                      "candidate_endpoints": int(ends),
+                      "direct_pairs_per_pass": float(ends) * n_pts * (1 + p_run.n_images),
+                      "path": "as the headline path, with the reference's reconnection "
+                              "velocities: the FP64 direct sum at every candidate endpoint "
+                              "(update_velocity with tree == NULL, "
+                              "src/vortex_dynamics.c:250-251)"}
+        log(f"rec_velocity direct: {rec_direct['ms_per_step']:.1f} ms/step, pass "
+            f"{w_pass * 1e3:.1f} ms over {ends} endpoints")
 
     # ---- plain FP64 path ------------------------------------------------
     plain = None
@@ -350,18 +507,12 @@ def main():
     # ---- CPU baseline ---------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        r = cpu_baseline(args.workload, args.cpu_rings, args.cpu_threads, args.cpu_steps)
-        if r:
-            cpu = {"value": r["point_steps_per_s"], "unit": "point-steps/s",
-                   "cores": r["threads"], "kind": "reference",
-                   "sample": (f"first {args.cpu_rings} rings of the {args.workload} stream "
-                              f"({r['points']} points, same box and parameters), "
-                              f"{r['steps']} full steps incl. the O(N^2) reconnect "
-                              f"({r['reconnect_s']:.1f} s of {r['seconds']:.1f} s); "
-                              f"reference src/ built with its Makefile flags "
-                              f"(-march=x86-64-v3)"),
-                   "seconds": r["seconds"]}
-            log(f"cpu: {cpu['value']:.4g} point-steps/s on {r['points']} points")
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline(args.workload, threads, args.cpu_sub_points, args.cpu_rings,
+                           args.cpu_steps, args.cpu_full_step)
+        if cpu:
+            log(f"cpu: {cpu['value']:.4g} point-steps/s ({cpu['step_s']:.0f} s per step, "
+                f"phases {cpu['phases_s']})")
 
     # HBM bytes per launch of the traversal: measured with rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (copy-kernel calibrated) for this
@@ -369,10 +520,8 @@ def main():
     traffic, traffic_src = None, None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "k_bh_list_mixed_c3_traffic.json")
-    cfg_now = {"workload": args.workload,
-               "points_per_wave": tune["points_per_wave"] if tune else args.ppw,
-               "leaf_size": tune["leaf_size"] if tune else args.leaf,
-               "group_level": tune["group_level"] if tune else args.group_level}
+    cfg_now = {"workload": args.workload, "points_per_wave": p_run.points_per_wave,
+               "leaf_size": p_run.leaf_size, "group_level": p_run.group_level}
     if (os.path.exists(tfile) and trav == ov.TRAVERSE_LIST and prec == ov.PREC_MIXED):
         with open(tfile) as f:
             tj = json.load(f)
@@ -389,9 +538,8 @@ def main():
                                 f"dt {dt}, reconnect every step, smooth every "
                                 f"{args.smooth_every} steps"),
                    "points": n_pts, "precision": args.precision, "traverse": args.traverse,
-                   "leaf_size": tune["leaf_size"] if tune else args.leaf,
-                   "points_per_wave": tune["points_per_wave"] if tune else args.ppw,
-                   "group_level": tune["group_level"] if tune else args.group_level,
+                   "leaf_size": p_run.leaf_size, "points_per_wave": p_run.points_per_wave,
+                   "group_level": p_run.group_level,
                    "rec_velocity": args.rec_velocity, "autotune": tune},
         "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
         "interactions_per_s": interactions_per_s,
@@ -400,6 +548,7 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "reuse_lists": reuse,
+        "rec_velocity_direct": rec_direct,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
