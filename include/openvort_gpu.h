@@ -333,6 +333,26 @@ int ovg_smooth(ovg_ctx *ctx);
 int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t,
                     double *vs, double *vels);
 
+/*
+ * ovg_velocity_at without allocation (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * direct sum, compose and results as ovg_velocity_at (bit for bit), but
+ * d_slots (int[m]) and d_vs / d_vels (double[3*m], AoS; either may be NULL)
+ * are DEVICE pointers, the work is queued on the context's stream and the
+ * call returns without waiting (ovg_synchronize waits).  The chunk
+ * partials and a checked copy of d_slots live in a 32 MiB device array of
+ * the library, one per GPU, held until the process ends; nothing is
+ * allocated or freed.  m <= OVG_NOALLOC_MAX_TARGETS, else OVG_ELIMIT.  A slot
+ * out of range or EMPTY yields NaN outputs and makes the next
+ * ovg_synchronize or ovg_download fail with OVG_EINVAL.  Every call on a
+ * GPU shares its array: calls that overlap on one GPU (contexts driven from
+ * several threads, say) race; serialise them.
+ */
+#define OVG_NOALLOC_MAX_TARGETS 131072
+int ovg_velocity_at_noalloc(ovg_ctx *ctx, int m, const int *d_slots, double t,
+                            double *d_vs, double *d_vels);
+
 /* ---- diagnostics ----------------------------------------------------- */
 
 /* Line length sum |seg_pwrap(x_k, x_fwd(k))| over FREE slots with a

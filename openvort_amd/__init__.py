@@ -26,6 +26,7 @@ _lib = C.CDLL(LIB_PATH)
 # status codes (include/openvort_gpu.h)
 OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ELIMIT = 0, 1, 2, 3, 4, 5
 EMPTY, FREE, PINNED, PINNED_SLIP = -1, 0, 1, 2
+NOALLOC_MAX_TARGETS = 131072  # OVG_NOALLOC_MAX_TARGETS
 WALL_OPEN, WALL_PERIODIC, WALL_MIRROR = 0, 1, 2
 FACE_LEFT, FACE_RIGHT, FACE_BACK, FACE_FRONT, FACE_DOWN, FACE_UP = range(6)
 # mirror-wall image tables (include/openvort_gpu.h OVG_IMAGES_WALL_*)
@@ -134,6 +135,8 @@ _sig("ovg_inject_loop_draw", C.c_int, C.c_double, C.c_double, _dp, _dp, _P(C.c_u
      _dp, _dp, _ip)
 _sig("ovg_add_loop", C.c_int, C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _ip)
 _sig("ovg_velocity_at", C.c_int, C.c_void_p, C.c_int, _ip, C.c_double, _dp, _dp)
+_sig("ovg_velocity_at_noalloc", C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double,
+     C.c_void_p, C.c_void_p)
 _sig("ovg_line_length", C.c_int, C.c_void_p, _dp, _ip)
 _sig("ovg_get_stats", C.c_int, C.c_void_p, _P(Stats))
 _sig("ovg_set_timing", C.c_int, C.c_void_p, C.c_int)
@@ -384,6 +387,15 @@ class Engine:
         _check(_lib.ovg_velocity_at(self._h, len(slots), _i(slots), t, _d(vs), _d(vels)),
                "ovg_velocity_at")
         return vs, vels
+
+    def velocity_at_noalloc(self, m, d_slots, t=0.0, d_vs=None, d_vels=None):
+        """ovg_velocity_at_noalloc on device addresses (ints): d_slots
+        int32[m], d_vs / d_vels float64[m, 3] (or None); queued on the
+        context's stream, nothing allocated."""
+        _check(_lib.ovg_velocity_at_noalloc(self._h, int(m), C.c_void_p(d_slots), t,
+                                            C.c_void_p(d_vs) if d_vs else None,
+                                            C.c_void_p(d_vels) if d_vels else None),
+               "ovg_velocity_at_noalloc")
 
     def save(self, path, digits=15):
         """The tangle to a reference tangle file (src/utils.c:132-184)."""

@@ -372,6 +372,11 @@ extern "C" int ovg_create(ovg_ctx **out, int device)
         ovg_destroy(ctx);
         return set_error(OVG_EDEVICE, "clearing the context's device flags failed");
     }
+    // the allocation-free entry point's scratch, once per device
+    if ((r = direct_scratch_init(device)) != OVG_OK) {
+        ovg_destroy(ctx);
+        return r;
+    }
     *out = ctx;
     return launch_status();
 }
@@ -685,6 +690,9 @@ static int check_flags(Ctx &c)
     OVG_HIP(hipStreamSynchronize(c.stream));
     if (f) {
         OVG_HIP(hipMemsetAsync(c.flags.p, 0, sizeof(int), c.stream));
+        if (f & 4)
+            return set_error(OVG_EINVAL, "a target slot given to ovg_velocity_at_noalloc is out "
+                                         "of range or EMPTY (its outputs are NaN)");
         if (f & 2) return set_error(OVG_EINVAL, "non-finite positions in the tangle state");
         return set_error(OVG_EINVAL, "a point left the box by more than 256 periods");
     }
@@ -857,6 +865,64 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
         for (int i = 0; i < m; ++i)
             for (int q = 0; q < 3; ++q) out[3 * i + q] = hv[3 * (size_t)slots[i] + q];
     }
+    return launch_status();
+}
+
+// out[3 i + q] = v_q[slot_i] (AoS), NaN for a slot of -1
+__global__ void k_gather_aos(int m, const int *__restrict__ slots, const double *__restrict__ vx,
+                             const double *__restrict__ vy, const double *__restrict__ vz,
+                             double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int k = slots[i];
+    const double nan = __builtin_nan("");
+    out[3 * i] = k < 0 ? nan : vx[k];
+    out[3 * i + 1] = k < 0 ? nan : vy[k];
+    out[3 * i + 2] = k < 0 ? nan : vz[k];
+}
+
+extern "C" int ovg_velocity_at_noalloc(ovg_ctx *ctx, int m, const int *d_slots, double t,
+                                       double *d_vs, double *d_vels)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    if (m < 0 || (m > 0 && !d_slots)) return set_error(OVG_EINVAL, "bad slot list");
+    if (m > OVG_NOALLOC_MAX_TARGETS)
+        return set_error(OVG_ELIMIT,
+                         "ovg_velocity_at_noalloc: %d targets exceed its maximum of %d "
+                         "(OVG_NOALLOC_MAX_TARGETS); ovg_velocity_at takes any number",
+                         m, OVG_NOALLOC_MAX_TARGETS);
+    if (m == 0) return OVG_OK;
+    hipSetDevice(c.device);
+    push_params(c, t);
+    Triple x = tri(c.x);
+    launch_prep_segments(c, x.v);
+    Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
+    ovg_params saved = c.params;
+    c.params.precision = OVG_PREC_FP64;
+    const int *checked = nullptr;
+    const int rd = launch_direct_noalloc(c, x.v, m, d_slots, bp.v, bi.v, &checked);
+    c.params = saved;
+    OVG_TRY(rd);
+    // compose over the checked slots only (a view of the scratch list: the
+    // context's own live list is restored, nothing is allocated or freed)
+    const int saved_live = c.n_live;
+    const DBuf saved_list = c.live;
+    c.live.p = const_cast<int *>(checked);
+    c.live.bytes = sizeof(int) * (size_t)m;
+    c.n_live = m;
+    launch_compose(c, x.v, false, 0, 0.0);
+    c.live = saved_list;
+    c.n_live = saved_live;
+    const int gs = grid_for(m);
+    if (d_vs)
+        hipLaunchKernelGGL(k_gather_aos, dim3(gs), dim3(256), 0, c.stream, m, checked,
+                           dp(c.vs[0]), dp(c.vs[1]), dp(c.vs[2]), d_vs);
+    if (d_vels)
+        hipLaunchKernelGGL(k_gather_aos, dim3(gs), dim3(256), 0, c.stream, m, checked,
+                           dp(c.vels[0]), dp(c.vels[1]), dp(c.vels[2]), d_vels);
     return launch_status();
 }
 

@@ -188,6 +188,15 @@ void launch_check_finite(Ctx &c, const double *const x[3]);
 // k_direct.hip
 int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
                   double *const out_prim[3], double *const out_img[3], int chunks);
+// The same sum with its chunk partials in the device's static scratch array
+// (no allocation): d_slots is checked into scratch first (a slot out of
+// range or EMPTY becomes -1 and sets bit 4 of the device flags); *checked
+// receives that list.  At most OVG_NOALLOC_MAX_TARGETS targets.
+int launch_direct_noalloc(Ctx &c, const double *const x[3], int n_tgt, const int *d_slots,
+                          double *const out_prim[3], double *const out_img[3],
+                          const int **checked);
+// Resolve the device address of that scratch array for `device` (current).
+int direct_scratch_init(int device);
 
 // k_tree.hip
 int build_tree(Ctx &c, const double *const x[3]);

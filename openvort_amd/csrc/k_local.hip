@@ -233,6 +233,7 @@ __global__ void k_compose(ComposeArgs a)
     if (i >= a.n_live) return;
     const TangleParams &P = *a.P;
     int k = a.live[i];
+    if (k < 0) return;  // a target slot the caller's list had wrong (checked copy)
     double t[3], nr[3];
     if (OWN_TAN) {
         tangent_normal(P, a.X, a.Y, a.Z, a.fwd, a.rev, a.st, a.pinw, k, t, nr);
