@@ -237,6 +237,55 @@ int ovg_get_params(const ovg_ctx *ctx, ovg_params *p);
 int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes,
                const int *connections, const int *status);
 
+/*
+ * A forest: n_tangles independent tangles in one context, evaluated and
+ * stepped together (no counterpart in the reference, which holds one
+ * tangle per process; a batch of update_tangle / rk4_step calls of
+ * src/tangle.c:733-749 and src/vortex_dynamics.c:50-119).  Tangle j takes
+ * slots slot_offsets[j] .. slot_offsets[j+1]-1 of vnodes / connections /
+ * status (each tangle in the reference layout with its OWN slot numbering:
+ * its links are 0-based within its range), and params[j] (its box, images,
+ * flows, friction, theta, mixed_near ...).  Every tangle takes the
+ * Barnes-Hut list traversal, and precision, leaf_size, points_per_wave,
+ * group_level and rk4_classic must be tangle 0's (one launch runs them
+ * all); flows must not depend on time.  One tree build and one traversal
+ * launch per evaluation cover every tangle (tangle index above the Morton
+ * key, groups that never cross tangles, per-tangle parameter table), and
+ * each tangle's results are bit-identical to the same tangle uploaded
+ * alone.  ovg_update_tangle, ovg_update_velocities, ovg_update_tangents_normals,
+ * ovg_rk4_step, ovg_enforce_periodic, ovg_download*, ovg_upload_positions,
+ * ovg_upload_field, ovg_autotune and the graph calls take a forest (global
+ * slot indices in downloads); the topology passes, ovg_velocity_at*,
+ * ovg_set_params and ovg_save_tangle return OVG_EUNSUPPORTED.  ovg_upload
+ * makes the context a single tangle again.
+ */
+#define OVG_FOREST_MAX_TANGLES 32767
+int ovg_upload_forest(ovg_ctx *ctx, int n_tangles, const ovg_params *params,
+                      const int *slot_offsets, const double *vnodes,
+                      const int *connections, const int *status);
+/* Tangles in the context (1 unless a forest was uploaded). */
+int ovg_forest_size(const ovg_ctx *ctx);
+
+/*
+ * HIP graphs of call sequences (no counterpart in the reference).
+ * ovg_graph_begin starts capturing the work that the following calls queue
+ * on the context's stream; ovg_graph_end instantiates it (*n_nodes: its
+ * nodes) and ovg_graph_launch replays it `repeats` times, queued without
+ * waiting.  Only device work with fixed shapes can be captured: a step's
+ * ovg_update_tangle + ovg_rk4_step + ovg_enforce_periodic, say.  Run the
+ * sequence once before capturing it (buffers must not grow during a
+ * capture: OVG_EINVAL), keep timing off, and use flows that do not depend
+ * on time (the captured parameters are replayed as they are).  A call that
+ * waits for the device or copies to the host ends the capture with an
+ * error at ovg_graph_end.  The graph replays the same kernels on the same
+ * buffers: upload nothing that changes the slot count or the topology
+ * before replaying it.  ovg_graph_release frees it (ovg_destroy does too).
+ */
+int ovg_graph_begin(ovg_ctx *ctx);
+int ovg_graph_end(ovg_ctx *ctx, int *n_nodes);
+int ovg_graph_launch(ovg_ctx *ctx, int repeats);
+int ovg_graph_release(ovg_ctx *ctx);
+
 /* Upload only new positions (same n and topology). */
 int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes);
 

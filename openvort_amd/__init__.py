@@ -147,6 +147,12 @@ _sig("ovg_generate_rings", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
      C.c_double, C.c_int, _dp, _ip, _ip)
 _sig("ovg_rings_for_points", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
      C.c_double, C.c_double)
+_sig("ovg_upload_forest", C.c_int, C.c_void_p, C.c_int, _P(Params), _ip, _dp, _ip, _ip)
+_sig("ovg_forest_size", C.c_int, C.c_void_p)
+_sig("ovg_graph_begin", C.c_int, C.c_void_p)
+_sig("ovg_graph_end", C.c_int, C.c_void_p, _ip)
+_sig("ovg_graph_launch", C.c_int, C.c_void_p, C.c_int)
+_sig("ovg_graph_release", C.c_int, C.c_void_p)
 _sig("ovg_set_device_memory_limit", C.c_size_t, C.c_size_t)
 _sig("ovg_device_memory_in_use", C.c_size_t)
 
@@ -387,6 +393,42 @@ class Engine:
         _check(_lib.ovg_velocity_at(self._h, len(slots), _i(slots), t, _d(vs), _d(vels)),
                "ovg_velocity_at")
         return vs, vels
+
+    def upload_forest(self, tangles, params):
+        """Many tangles in this context (ovg_upload_forest): tangles[j] in
+        its own slot numbering with params[j].  Returns the slot offsets;
+        downloads hold tangle j at slots offsets[j] .. offsets[j + 1] - 1."""
+        assert len(tangles) == len(params) and tangles
+        off = np.zeros(len(tangles) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([t.n for t in tangles])
+        v = np.ascontiguousarray(np.concatenate([t.vnodes for t in tangles]))
+        c = np.ascontiguousarray(np.concatenate([t.conn for t in tangles]), dtype=np.int32)
+        s = np.ascontiguousarray(np.concatenate([t.status for t in tangles]), dtype=np.int32)
+        arr = (Params * len(params))(*params)
+        _check(_lib.ovg_upload_forest(self._h, len(tangles), arr, _i(off), _d(v), _i(c), _i(s)),
+               "ovg_upload_forest")
+        self.n = int(off[-1])
+        self.offsets = off
+        return off
+
+    def forest_size(self):
+        return _lib.ovg_forest_size(self._h)
+
+    def graph_begin(self):
+        """Capture the device work of the following calls (ovg_graph_begin)."""
+        _check(_lib.ovg_graph_begin(self._h), "ovg_graph_begin")
+
+    def graph_end(self):
+        """Instantiate the captured graph; returns its node count."""
+        n = C.c_int32(0)
+        _check(_lib.ovg_graph_end(self._h, C.byref(n)), "ovg_graph_end")
+        return n.value
+
+    def graph_launch(self, repeats=1):
+        _check(_lib.ovg_graph_launch(self._h, int(repeats)), "ovg_graph_launch")
+
+    def graph_release(self):
+        _check(_lib.ovg_graph_release(self._h), "ovg_graph_release")
 
     def velocity_at_noalloc(self, m, d_slots, t=0.0, d_vs=None, d_vels=None):
         """ovg_velocity_at_noalloc on device addresses (ints): d_slots

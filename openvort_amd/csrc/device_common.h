@@ -35,8 +35,32 @@ struct TangleParams {
     double theta;
     double crit_scale;  // 1/theta^2 for the squared acceptance test
     int leaf_size;
+    int group_level;    // list traversal: octree level that bounds a target group
+    double near2;       // mixed precision: FP64 zone, squared (cm^2)
+    // tuned trees: fixed-point units 2^-shift cm for this tangle's size
+    double fix_scale, fix_inv, fix_lim;
+    // Forests (many tangles in one context): entry 0 of the parameter table
+    // holds the tangle count and the per-slot tangle index; every entry
+    // holds its own tangle's constants.  n_tangles == 1: one tangle.
+    int n_tangles;
     int pad2;
+    const int *tid;
 };
+
+// Constants of the tangle that slot k belongs to.
+__device__ __forceinline__ const TangleParams &tparams_of(const TangleParams *Pp, int k)
+{
+    return Pp[0].n_tangles > 1 ? Pp[Pp[0].tid[k]] : Pp[0];
+}
+
+// Tuned-tree keys (list and group traversals): the tangle index above 48
+// Morton bits at key bits 1..48 (16 levels of 3-bit digits, level 1 most
+// significant); bit 0 is zero.
+constexpr int kKeyTangleShift = 49;
+__device__ __host__ __forceinline__ int key_tangle(uint64_t key)
+{
+    return (int)(key >> kKeyTangleShift);
+}
 
 struct Vec3 {
     double x, y, z;

@@ -21,10 +21,20 @@ namespace ovg {
 // ovg_set_device_memory_limit (0 = none).
 static std::atomic<size_t> g_dev_in_use{0};
 static std::atomic<size_t> g_dev_limit{0};
+// set while this thread captures a context's stream into a graph: buffers
+// must not move under the captured work (a grown buffer is freed and
+// reallocated)
+static thread_local bool t_capturing = false;
+static thread_local bool t_capture_failed = false;  // a call in the capture failed
 
 int DBuf::ensure(size_t n)
 {
     if (n <= bytes) return OVG_OK;
+    if (t_capturing) {
+        t_capture_failed = true;
+        return set_error(OVG_EINVAL, "a device buffer would grow during graph capture: run the "
+                         "captured calls once before capturing them");
+    }
     if (p) {
         hipFree(p);
         g_dev_in_use -= bytes;
@@ -188,14 +198,41 @@ TangleParams host_tparams(const ovg_params &p, double t)
     P.theta = p.theta;
     P.crit_scale = 1.0 / (p.theta * p.theta);
     P.leaf_size = p.leaf_size;
+    P.group_level = list_group_level(p);
+    P.near2 = p.mixed_near * p.mixed_near;
+    P.n_tangles = 1;
+    P.tid = nullptr;
     return P;
 }
 
+static bool flow_depends_on_time(const ovg_params &p)
+{
+    return p.vn.type == OVG_FLOW_DECREASING || p.vs.type == OVG_FLOW_DECREASING ||
+           p.vb.type == OVG_FLOW_DECREASING;
+}
+
+// A forest's parameter table is written once, when it is uploaded (its
+// flows do not depend on time); a single tangle's entry at every call.
 void push_params(Ctx &c, double t)
 {
+    if (c.n_tangles > 1) {
+        forest_table(c, false);  // an error shows in the caller's launch_status
+        return;
+    }
     TangleParams P = host_tparams(c.params, t);
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, c.stream, P,
                        c.tparams.as<TangleParams>());
+}
+
+// Calls that take one tangle only: OVG_EUNSUPPORTED on a forest.
+static int single_tangle(const Ctx &c, const char *what)
+{
+    if (c.n_tangles > 1)
+        return set_error(OVG_EUNSUPPORTED,
+                         "%s: this context holds a forest of %d tangles (ovg_upload_forest); "
+                         "it runs the evaluation and the RK4 step only",
+                         what, c.n_tangles);
+    return OVG_OK;
 }
 
 static int check_ready(const ovg_ctx *ctx, bool need_state = true)
@@ -407,8 +444,11 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
                   &T.scan_tmp, &T.counters, &T.sorted_pts, &T.sx, &T.sy, &T.sz, &T.sqx,
                   &T.sqy, &T.sqz, &T.grp_a, &T.grp_b, &T.grp_c, &T.groups,
                   &T.rank, &T.sub_tgt, &T.sub_groups, &T.sub_count, &T.rec,
-                  &T.rec_count, &T.fix, &T.fix_tiles};
+                  &T.rec_count, &T.fix, &T.fix_tiles, &T.roots};
     for (auto *b : tb) b->release();
+    c.tid.release();
+    if (c.graph_exec) hipGraphExecDestroy(c.graph_exec);
+    c.graph_exec = nullptr;
     if (c.h_xpin) hipHostFree(c.h_xpin);
     c.h_xpin = nullptr;
     for (auto &e : c.ev)
@@ -425,10 +465,8 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     return OVG_OK;
 }
 
-extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
+static int validate_params(const ovg_params *p)
 {
-    clear_error();
-    if (!ctx || !p) return set_error(OVG_EINVAL, "NULL argument");
     for (int k = 0; k < 3; ++k)
         if (!(p->box_hi[k] > p->box_lo[k]) || !std::isfinite(p->box_hi[k] - p->box_lo[k]))
             return set_error(OVG_EINVAL, "box axis %d is empty or not finite", k);
@@ -478,7 +516,16 @@ extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
                          "leaf_size 1 and FP64 only");
     if (!(p->kappa > 0) || !(p->vortex_width > 0))
         return set_error(OVG_EINVAL, "kappa and vortex_width must be > 0");
+    return OVG_OK;
+}
+
+extern "C" int ovg_set_params(ovg_ctx *ctx, const ovg_params *p)
+{
+    clear_error();
+    if (!ctx || !p) return set_error(OVG_EINVAL, "NULL argument");
+    OVG_TRY(validate_params(p));
     Ctx &c = ctx->c;
+    OVG_TRY(single_tangle(c, "ovg_set_params"));
     hipSetDevice(c.device);
     c.params = *p;
     c.rec_ok = false;  // recorded interaction lists belong to the old settings
@@ -625,6 +672,13 @@ extern "C" int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes, const int *
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     OVG_TRY(alloc_state(c, n));
+    if (c.n_tangles > 1) {  // a forest before: one tangle again
+        c.n_tangles = 1;
+        c.forest_params.clear();
+        c.forest_off.clear();
+        c.tid.release();
+        push_params(c, 0.0);
+    }
     c.n = n;
     c.next_empty = 0;
     OVG_TRY(upload_positions(c, vnodes));
@@ -637,6 +691,214 @@ extern "C" int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes, const int *
     }
     OVG_HIP(hipStreamSynchronize(c.stream));
     return launch_status();
+}
+
+// The forest's parameter table: every tangle's constants with the context's
+// launch settings (leaf size, group level: the autotuner changes them) and
+// its fixed-point units; written again when those settings change.
+int ovg::forest_table(Ctx &c, bool force)
+{
+    const int lf = c.params.leaf_size, gl = c.params.group_level;
+    if (!force && lf == c.forest_leaf && gl == c.forest_glevel) return OVG_OK;
+    const int K = c.n_tangles;
+    std::vector<TangleParams> tab((size_t)K);
+    for (int j = 0; j < K; ++j) {
+        ovg_params q = c.forest_params[j];
+        q.leaf_size = lf;
+        q.group_level = gl;
+        tab[j] = host_tparams(q, 0.0);
+        double Lmax = 0;
+        for (int a = 0; a < 3; ++a) Lmax = std::max(Lmax, q.box_hi[a] - q.box_lo[a]);
+        fix_units(c.forest_live[j], Lmax, tab[j].fix_scale, tab[j].fix_inv, tab[j].fix_lim);
+        tab[j].n_tangles = K;
+        tab[j].tid = c.tid.as<int>();
+    }
+    OVG_TRY(c.tparams.ensure(sizeof(TangleParams) * (size_t)K));
+    OVG_HIP(hipMemcpyAsync(c.tparams.p, tab.data(), sizeof(TangleParams) * (size_t)K,
+                           hipMemcpyHostToDevice, c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));  // tab is released on return
+    c.forest_leaf = lf;
+    c.forest_glevel = gl;
+    return OVG_OK;
+}
+
+extern "C" int ovg_upload_forest(ovg_ctx *ctx, int n_tangles, const ovg_params *params,
+                                 const int *slot_offsets, const double *vnodes,
+                                 const int *connections, const int *status)
+{
+    clear_error();
+    if (!ctx || n_tangles < 1 || !params || !slot_offsets)
+        return set_error(OVG_EINVAL, "bad arguments to ovg_upload_forest");
+    if (n_tangles > OVG_FOREST_MAX_TANGLES)
+        return set_error(OVG_ELIMIT, "%d tangles exceed OVG_FOREST_MAX_TANGLES (%d)", n_tangles,
+                         OVG_FOREST_MAX_TANGLES);
+    if (slot_offsets[0] != 0) return set_error(OVG_EINVAL, "slot_offsets[0] must be 0");
+    for (int j = 0; j < n_tangles; ++j)
+        if (slot_offsets[j + 1] < slot_offsets[j])
+            return set_error(OVG_EINVAL, "slot_offsets must not decrease (tangle %d)", j);
+    const int n = slot_offsets[n_tangles];
+    if (n > 0 && (!vnodes || !connections || !status))
+        return set_error(OVG_EINVAL, "bad arrays for ovg_upload_forest");
+    const ovg_params &p0 = params[0];
+    for (int j = 0; j < n_tangles; ++j) {
+        const ovg_params &q = params[j];
+        const int r = validate_params(&q);
+        if (r != OVG_OK) return set_error(r, "tangle %d: %s", j, ovg_last_error());
+        if (q.method != OVG_METHOD_BH || q.traverse != OVG_TRAVERSE_LIST || q.reuse_lists)
+            return set_error(OVG_EUNSUPPORTED, "tangle %d: a forest runs the Barnes-Hut list "
+                             "traversal (method BH, traverse LIST, no reuse_lists)", j);
+        if (q.precision != p0.precision || q.leaf_size != p0.leaf_size ||
+            q.points_per_wave != p0.points_per_wave || q.group_level != p0.group_level ||
+            q.rk4_classic != p0.rk4_classic)
+            return set_error(OVG_EINVAL, "tangle %d: precision, leaf_size, points_per_wave, "
+                             "group_level and rk4_classic are one launch's and must match "
+                             "tangle 0's", j);
+        if (flow_depends_on_time(q))
+            return set_error(OVG_EUNSUPPORTED, "tangle %d: a forest's parameter table is "
+                             "written once, so its flows must not depend on time", j);
+    }
+    // per tangle: the reference layout with its own slot numbering
+    std::vector<int> conn(2 * (size_t)n), st(2 * (size_t)n), tids((size_t)(n > 0 ? n : 1));
+    std::vector<int> live_count((size_t)n_tangles, 0);
+    for (int j = 0; j < n_tangles; ++j) {
+        const int o = slot_offsets[j], m = slot_offsets[j + 1] - o;
+        const int r = validate_topology(m, connections + 2 * (size_t)o, status + 2 * (size_t)o);
+        if (r != OVG_OK) return set_error(r, "tangle %d: %s", j, ovg_last_error());
+        for (int k = 0; k < m; ++k) {
+            const size_t g = (size_t)o + k;
+            for (int e = 0; e < 2; ++e) {
+                const int l = connections[2 * g + e];
+                conn[2 * g + e] = l < 0 ? -1 : l + o;
+            }
+            st[2 * g] = status[2 * g];
+            st[2 * g + 1] = status[2 * g + 1];
+            tids[g] = j;
+            if (status[2 * g] != OVG_EMPTY) {
+                ++live_count[j];
+                for (int q = 0; q < 3; ++q)
+                    if (!std::isfinite(vnodes[3 * g + q]))
+                        return set_error(OVG_EINVAL, "tangle %d slot %d has a non-finite "
+                                         "position", j, k);
+            }
+        }
+    }
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    OVG_TRY(alloc_state(c, n));
+    c.params = p0;
+    c.have_params = true;
+    c.rec_ok = false;
+    c.n = n;
+    c.next_empty = 0;
+    c.n_tangles = n_tangles;
+    c.forest_params.assign(params, params + n_tangles);
+    c.forest_off.assign(slot_offsets, slot_offsets + n_tangles + 1);
+    c.forest_live = live_count;
+    OVG_TRY(c.tid.ensure(sizeof(int) * tids.size()));
+    OVG_HIP(hipMemcpy(c.tid.p, tids.data(), sizeof(int) * tids.size(), hipMemcpyHostToDevice));
+    OVG_TRY(forest_table(c, true));
+    OVG_TRY(upload_positions(c, vnodes));
+    OVG_TRY(upload_topology(c, conn.data(), st.data()));
+    for (int q = 0; q < 3; ++q) {
+        DBuf *bufs[] = {&c.vels[q], &c.vs[q], &c.tan[q], &c.nor[q]};
+        for (auto *b : bufs)
+            hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n > 0 ? n : 1)), dim3(256), 0, c.stream,
+                               n, b->as<double>());
+    }
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return launch_status();
+}
+
+extern "C" int ovg_forest_size(const ovg_ctx *ctx) { return ctx ? ctx->c.n_tangles : 0; }
+
+// ---- HIP graphs of call sequences -------------------------------------------
+extern "C" int ovg_graph_begin(ovg_ctx *ctx)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    if (c.capturing) return set_error(OVG_EINVAL, "a capture is already open on this context");
+    if (c.timing)
+        return set_error(OVG_EINVAL, "timing events cannot be captured: ovg_set_timing(ctx, 0)");
+    bool timed = flow_depends_on_time(c.params);
+    for (const ovg_params &q : c.forest_params) timed |= flow_depends_on_time(q);
+    if (timed)
+        return set_error(OVG_EUNSUPPORTED, "a graph replays its parameters as captured, so the "
+                         "flows must not depend on time");
+    hipSetDevice(c.device);
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    OVG_TRY(positions_change(c));  // the captured work moves points when it runs
+    OVG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.capturing = true;
+    t_capturing = true;
+    t_capture_failed = false;
+    return OVG_OK;
+}
+
+extern "C" int ovg_graph_end(ovg_ctx *ctx, int *n_nodes)
+{
+    clear_error();
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    Ctx &c = ctx->c;
+    if (!c.capturing) return set_error(OVG_EINVAL, "no capture is open on this context");
+    hipSetDevice(c.device);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    c.capturing = false;
+    t_capturing = false;
+    (void)hipGetLastError();
+    if (e != hipSuccess || !g)
+        return set_error(OVG_EDEVICE, "the capture failed (%s): a call in it synchronised or "
+                         "copied to the host", hipGetErrorString(e));
+    if (t_capture_failed) {
+        hipGraphDestroy(g);
+        t_capture_failed = false;
+        return set_error(OVG_EINVAL, "a call during the capture failed (a buffer had to grow): "
+                         "nothing was instantiated");
+    }
+    if (c.graph_exec) {
+        hipGraphExecDestroy(c.graph_exec);
+        c.graph_exec = nullptr;
+    }
+    size_t nodes = 0;
+    hipGraphGetNodes(g, nullptr, &nodes);
+    const hipError_t ei = hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c.graph_exec = nullptr;
+        return set_error(OVG_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    if (n_nodes) *n_nodes = (int)nodes;
+    c.rec_ok = false;
+    return launch_status();
+}
+
+extern "C" int ovg_graph_launch(ovg_ctx *ctx, int repeats)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    Ctx &c = ctx->c;
+    if (!c.graph_exec) return set_error(OVG_EINVAL, "no graph has been captured on this context");
+    if (repeats < 0) return set_error(OVG_EINVAL, "repeats must be >= 0");
+    hipSetDevice(c.device);
+    OVG_TRY(positions_change(c));
+    for (int r = 0; r < repeats; ++r) OVG_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+    c.rec_ok = false;
+    return launch_status();
+}
+
+extern "C" int ovg_graph_release(ovg_ctx *ctx)
+{
+    clear_error();
+    if (!ctx) return set_error(OVG_EINVAL, "context is NULL");
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (c.graph_exec) {
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        hipGraphExecDestroy(c.graph_exec);
+        c.graph_exec = nullptr;
+    }
+    return OVG_OK;
 }
 
 extern "C" int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes)
@@ -652,6 +914,7 @@ extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, cons
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     OVG_TRY(validate_topology(c.n, connections, status));
     hipSetDevice(c.device);
@@ -824,6 +1087,7 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !slots)) return set_error(OVG_EINVAL, "bad slot list");
     for (int i = 0; i < m; ++i)
@@ -887,6 +1151,7 @@ extern "C" int ovg_velocity_at_noalloc(ovg_ctx *ctx, int m, const int *d_slots, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !d_slots)) return set_error(OVG_EINVAL, "bad slot list");
     if (m > OVG_NOALLOC_MAX_TARGETS)
@@ -1008,6 +1273,7 @@ extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs,
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
         return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
                                            "add_loop on tangles with pinned lines are not built");
@@ -1022,6 +1288,7 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
         return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
                                            "add_loop on tangles with pinned lines are not built");
@@ -1037,6 +1304,7 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
         return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
                                            "add_loop on tangles with pinned lines are not built");
@@ -1056,6 +1324,7 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     int n = c.n;
     std::vector<int> &conn = c.h_conn, &st = c.h_status;
@@ -1288,6 +1557,7 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
         return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
                                            "add_loop on tangles with pinned lines are not built");
@@ -1394,6 +1664,7 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
         return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
                                            "add_loop on tangles with pinned lines are not built");

@@ -80,6 +80,7 @@ struct TreeBufs {
     DBuf sub_tgt, sub_groups, sub_count;  // list traversal of a target subset
     DBuf rec, rec_count;     // recorded interaction lists (reuse_lists)
     DBuf fix, fix_tiles;     // tuned trees: fixed-point prefix sums of the sorted points
+    DBuf roots;              // DFS index of each tangle's root node
 };
 // counters[kGroupCounter]: number of target groups of the list traversal
 constexpr int kGroupCounter = 40;
@@ -99,6 +100,18 @@ struct Ctx {
     int n_live = 0;       // non-EMPTY slots
     int n_src = 0;        // non-EMPTY slots with a forward neighbour
     int timing = 0;
+    // Forest: n_tangles > 1 tangles in consecutive slot ranges, their
+    // parameters in a device table (tparams[t]) and a per-slot tangle index
+    // (tid); only the evaluation and the RK4 step run on a forest.
+    int n_tangles = 1;
+    std::vector<ovg_params> forest_params;
+    std::vector<int> forest_off;     // slot offsets, n_tangles + 1
+    std::vector<int> forest_live;    // live points per tangle
+    int forest_leaf = -1, forest_glevel = -1;  // launch settings the table holds
+    DBuf tid;
+    // HIP graph of a captured call sequence (ovg_graph_*)
+    bool capturing = false;
+    hipGraphExec_t graph_exec = nullptr;
 
     // state, SoA over slots
     DBuf x[3], vels[3], vs[3], tan[3], nor[3];
@@ -197,6 +210,12 @@ int launch_direct_noalloc(Ctx &c, const double *const x[3], int n_tgt, const int
                           const int **checked);
 // Resolve the device address of that scratch array for `device` (current).
 int direct_scratch_init(int device);
+
+// engine.hip: write a forest's parameter table (when the launch settings
+// changed since the last write, or always with force)
+int forest_table(Ctx &c, bool force);
+// k_tree.hip: fixed-point units of a tangle of n points, largest side Lmax
+void fix_units(int n, double Lmax, double &scale, double &inv, double &lim);
 
 // k_tree.hip
 int build_tree(Ctx &c, const double *const x[3]);

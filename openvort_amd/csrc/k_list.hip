@@ -152,8 +152,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const double *__restrict__ sy, const double *__restrict__ sz,
     const double *__restrict__ sqx, const double *__restrict__ sqy,
     const double *__restrict__ sqz, const int *__restrict__ counters,
-    const int *__restrict__ n_groups_p, const TangleParams *__restrict__ Pp, double near2,
-    double *__restrict__ px, double *__restrict__ py, double *__restrict__ pz,
+    const int *__restrict__ n_groups_p, const TangleParams *__restrict__ Pp,
+    const int *__restrict__ roots, double *__restrict__ px, double *__restrict__ py, double *__restrict__ pz,
     double *__restrict__ ix, double *__restrict__ iy, double *__restrict__ iz,
     unsigned long long *__restrict__ icount)
 {
@@ -175,19 +175,29 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     __shared__ double shL[27][3];                    // copy shifts (2 * wall on mirrored axes)
     __shared__ int shM[27];                          // mirrored axes of each copy (kMirror)
 
-    const TangleParams &P = *Pp;
     const int lane = lane_id();
     const int n_groups = *n_groups_p;
-    if (lane < 27) {
-        const bool img = lane >= 1 && lane <= P.n_images;
-        const int m = MIR && img ? P.refl[lane - 1] : 0;
-        for (int q = 0; q < 3; ++q)
-            shL[lane][q] = !img ? 0.0 : (m >> q & 1) ? P.W2[lane - 1][q] : P.Ls[lane - 1][q];
-        if (MIR) shM[lane] = m;
-    }
-    __syncthreads();
+    // a forest's groups each lie in one tangle: its parameters, copy shifts
+    // and root node (loaded again when the tangle changes)
+    const bool forest = Pp[0].n_tangles > 1;
+    int cur_tangle = -1;
     for (int g = xcd_group(blockIdx.x, gridDim.x); g < n_groups; g += gridDim.x) {
     const int2 grp = groups[g];
+    const int tg = forest ? __builtin_amdgcn_readfirstlane(Pp[0].tid[spts[grp.x]]) : 0;
+    const TangleParams &P = Pp[tg];
+    if (tg != cur_tangle) {
+        __syncthreads();
+        if (lane < 27) {
+            const bool img = lane >= 1 && lane <= P.n_images;
+            const int m = MIR && img ? P.refl[lane - 1] : 0;
+            for (int q = 0; q < 3; ++q)
+                shL[lane][q] = !img ? 0.0 : (m >> q & 1) ? P.W2[lane - 1][q] : P.Ls[lane - 1][q];
+            if (MIR) shM[lane] = m;
+        }
+        __syncthreads();
+        cur_tangle = tg;
+    }
+    const double near2 = P.near2;
     // lane = (slice, target) over the group's ny targets: S = 64 / ny slices
     // (at most 32, so a flush round of 4 S entries fits the far list), the
     // lanes from S * ny on idle.  A far flush holds whole rounds of 4 S.
@@ -417,8 +427,10 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             emit(want, !MIXED || e2 < near2, e, ex, ey, ez, gx, gy, gz);
         }
     } else {
-    if (lane < n_copies) stk[lane] = make_int2(0, n_nodes | (lane << kCopyShift));
-    int sp = n_nodes > 0 ? n_copies : 0;
+    const int root = forest ? roots[tg] : 0;
+    const int rend = forest ? nodes[root].skip : n_nodes;
+    if (lane < n_copies) stk[lane] = make_int2(root, rend | (lane << kCopyShift));
+    int sp = rend > root ? n_copies : 0;
     __syncthreads();
     while (sp > 0) {
         // each popped range pushes at most two: keep kStackMargin free
@@ -590,25 +602,35 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
 // the boundary of an octree cell of level L0, so each group's bounding box
 // is at most one such cell.  cut(i): point i starts a level-L0 cell
 // (common levels of sorted keys i-1 and i below L0).
-__device__ __forceinline__ int cell_cut(const int8_t *__restrict__ delta, int i, int L0)
+// In a forest each tangle has its own level (from its key); the boundary
+// between two tangles is always a cut (their keys share no level).
+__device__ __forceinline__ int cell_cut(const int8_t *__restrict__ delta,
+                                        const uint64_t *__restrict__ keys,
+                                        const TangleParams *__restrict__ Pp, int i, int L0)
 {
-    return i == 0 || delta[i - 1] < L0;
+    if (i == 0) return 1;
+    if (Pp[0].n_tangles > 1) L0 = Pp[key_tangle(keys[i])].group_level;
+    return delta[i - 1] < L0;
 }
 
-__global__ void k_grp_cut(int n, const int8_t *__restrict__ delta, int L0, int *__restrict__ a)
+__global__ void k_grp_cut(int n, const int8_t *__restrict__ delta,
+                          const uint64_t *__restrict__ keys, const TangleParams *__restrict__ Pp,
+                          int L0, int *__restrict__ a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = cell_cut(delta, i, L0);
+    if (i < n) a[i] = cell_cut(delta, keys, Pp, i, L0);
 }
 
 // a = exclusive scan of cuts: cell index b[i]; c[cell] = first point,
 // c[n_cells] = n.
-__global__ void k_grp_cells(int n, const int8_t *__restrict__ delta, int L0,
+__global__ void k_grp_cells(int n, const int8_t *__restrict__ delta,
+                            const uint64_t *__restrict__ keys,
+                            const TangleParams *__restrict__ Pp, int L0,
                             const int *__restrict__ a, int *__restrict__ b, int *__restrict__ c)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int cut = cell_cut(delta, i, L0);
+    int cut = cell_cut(delta, keys, Pp, i, L0);
     int cell = a[i] + cut - 1;
     b[i] = cell;
     if (cut) c[cell] = i;
@@ -659,9 +681,12 @@ static int build_groups(Ctx &c)
     int *a = T.grp_a.as<int>(), *b = T.grp_b.as<int>(), *cc = T.grp_c.as<int>();
     const int8_t *delta = T.delta.as<int8_t>();
     const int gs = (n + 255) / 256;
-    hipLaunchKernelGGL(k_grp_cut, dim3(gs), dim3(256), 0, c.stream, n, delta, L0, a);
+    const uint64_t *keys = T.keys[0].as<uint64_t>();
+    hipLaunchKernelGGL(k_grp_cut, dim3(gs), dim3(256), 0, c.stream, n, delta, keys, dparams(c),
+                       L0, a);
     OVG_TRY(launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp));
-    hipLaunchKernelGGL(k_grp_cells, dim3(gs), dim3(256), 0, c.stream, n, delta, L0, a, b, cc);
+    hipLaunchKernelGGL(k_grp_cells, dim3(gs), dim3(256), 0, c.stream, n, delta, keys,
+                       dparams(c), L0, a, b, cc);
     hipLaunchKernelGGL(k_grp_flag<PPW>, dim3(gs), dim3(256), 0, c.stream, n, b, cc, a);
     OVG_TRY(launch_exclusive_scan(c, a, a, n, a + n, T.scan_tmp));
     hipLaunchKernelGGL(k_grp_emit<PPW>, dim3(gs), dim3(256), 0, c.stream, n, a, b, cc,
@@ -685,6 +710,8 @@ int launch_bh_list(Ctx &c, const double *const x[3])
     int mode = c.list_mode;
     bool mirrored = false;
     for (int j = 0; j < c.params.n_images; ++j) mirrored |= c.params.image_reflect[j] != 0;
+    for (const ovg_params &q : c.forest_params)
+        for (int j = 0; j < q.n_images; ++j) mirrored |= q.image_reflect[j] != 0;
     if (mirrored) mode = kMirror;  // reuse_lists is refused with mirror images
     // a replay keeps the recorded run's groups (same tree, same order)
     if (mode != kReplay) {
@@ -712,7 +739,6 @@ int launch_bh_list(Ctx &c, const double *const x[3])
         rec_groups = c.rec_groups;
         rec_cap = kRecCap;
     }
-    const double near2 = c.params.mixed_near * c.params.mixed_near;
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(64), 0, c.stream, T.rec.as<unsigned>(),
@@ -721,8 +747,8 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                            T.meta.as<NodeMeta>(), T.sx.as<double>(), T.sy.as<double>(),
                            T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
                            T.sqz.as<double>(), T.counters.as<int>(),
-                           T.counters.as<int>() + kGroupCounter, dparams(c), near2,
-                           c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
+                           T.counters.as<int>() + kGroupCounter, dparams(c),
+                           T.roots.as<int>(), c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
     };
@@ -758,11 +784,12 @@ __global__ void k_rank_key(int m, const int *__restrict__ slots, const int *__re
     out_key[i] = keys[r];
 }
 
-// Same level of 3-bit digits as k_tree.hip's LCP levels.
+// Same levels of 3-bit digits as k_tree.hip's LCP levels of tuned keys
+// (one tangle: the Morton digits of 16 levels at key bits 1..48).
 static int host_lcp_levels(uint64_t a, uint64_t b)
 {
-    if (a == b) return kMaxLevel;
-    return (__builtin_clzll(a ^ b) - 1) / 3;
+    if (a == b) return 16;
+    return (__builtin_clzll(a ^ b) - 15) / 3;
 }
 
 int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d_slots)
@@ -808,7 +835,6 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     OVG_HIP(hipMemcpyAsync(d_tgt, tgt.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice,
                            c.stream));
     OVG_HIP(hipMemcpyAsync(T.sub_count.p, &ng, sizeof(int), hipMemcpyHostToDevice, c.stream));
-    const double near2 = c.params.mixed_near * c.params.mixed_near;
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(ng), dim3(64), 0, c.stream, nullptr, nullptr, 0, 0, d_grp,
@@ -816,7 +842,7 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
                            x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                            T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
                            T.sqy.as<double>(), T.sqz.as<double>(), T.counters.as<int>(),
-                           T.sub_count.as<int>(), dparams(c), near2,
+                           T.sub_count.as<int>(), dparams(c), T.roots.as<int>(),
                            c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);

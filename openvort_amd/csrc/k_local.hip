@@ -138,8 +138,8 @@ __global__ void k_stencil(int n_live, const int *__restrict__ live, const double
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
-    const TangleParams &P = *Pp;
     int k = live[i];
+    const TangleParams &P = tparams_of(Pp, k);
     double t[3], n[3];
     tangent_normal(P, X, Y, Z, fwd, rev, st, pinw, k, t, n);
     tx[k] = t[0]; ty[k] = t[1]; tz[k] = t[2];
@@ -231,9 +231,9 @@ __global__ void k_compose(ComposeArgs a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n_live) return;
-    const TangleParams &P = *a.P;
     int k = a.live[i];
     if (k < 0) return;  // a target slot the caller's list had wrong (checked copy)
+    const TangleParams &P = tparams_of(a.P, k);
     double t[3], nr[3];
     if (OWN_TAN) {
         tangent_normal(P, a.X, a.Y, a.Z, a.fwd, a.rev, a.st, a.pinw, k, t, nr);
@@ -417,8 +417,8 @@ __global__ void k_wrap(int n_live, const int *__restrict__ live, double *__restr
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
-    const TangleParams &P = *Pp;
     int k = live[i];
+    const TangleParams &P = tparams_of(Pp, k);
     double p[3] = {X[k], Y[k], Z[k]};
     int moved = 0;
     for (int it = 0; it < 256; ++it) {
@@ -476,10 +476,10 @@ __global__ void k_line_length(int n_live, const int *__restrict__ live,
 {
     __shared__ double red[256];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const TangleParams &P = *Pp;
     double l = 0;
     if (i < n_live) {
         int k = live[i];
+        const TangleParams &P = tparams_of(Pp, k);
         int f = fwd[k];
         if (f >= 0) {
             double bx = X[f], by = Y[f], bz = Z[f];

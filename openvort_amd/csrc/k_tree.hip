@@ -142,8 +142,12 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v)  // bit b -> bit 3b
 // single-point leaves (that cell's Lm is its true size, so acceptance is
 // never coarser than with deeper cells).
 constexpr int kTunedLevels = 16;
-constexpr int kTunedLowBit = 63 - 3 * kTunedLevels;  // lowest key bit in use
+// The 48 Morton bits sit at key bits 1..48 (level l's digit at 49 - 3l ..
+// 51 - 3l), the tangle index above them (key_tangle).
+constexpr int kTunedLowBit = 63 - 3 * kTunedLevels;  // where morton_key puts them
+constexpr int kTunedDown = kTunedLowBit - 1;         // shift down to bit 1
 
+// tangle << 49 | Morton bits of the clamped position in the tangle's box
 __device__ __forceinline__ uint64_t morton_key(const TangleParams &P, double x, double y, double z)
 {
     const double p3[3] = {x, y, z};
@@ -159,7 +163,8 @@ __device__ __forceinline__ uint64_t morton_key(const TangleParams &P, double x, 
     // coordinate, here q << 5) and sits at key bits 63 - 3l = 3b (x),
     // 3b + 1 (y), 3b + 2 (z)
     constexpr int sh = kMaxLevel - kTunedLevels;
-    return spread3(q[0] << sh) | spread3(q[1] << sh) << 1 | spread3(q[2] << sh) << 2;
+    const uint64_t m = spread3(q[0] << sh) | spread3(q[1] << sh) << 1 | spread3(q[2] << sh) << 2;
+    return m >> kTunedDown;
 }
 
 // clamp != 0 (tuned modes): a point outside the root box is placed by its
@@ -173,8 +178,8 @@ __global__ void k_keys(int n_live, const int *__restrict__ live, const double *_
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
-    const TangleParams &P = *Pp;
     int k = live[i];
+    const TangleParams &P = tparams_of(Pp, k);
     double x = X[k], y = Y[k], z = Z[k];
     if (clamp) {
         x = fmin(fmax(x, P.lo[0]), P.hi[0]);
@@ -184,7 +189,8 @@ __global__ void k_keys(int n_live, const int *__restrict__ live, const double *_
     double lo[3] = {P.lo[0], P.lo[1], P.lo[2]}, hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
     uint64_t key = kDropKey;
     if (clamp) {
-        key = morton_key(P, x, y, z);
+        const uint64_t t = Pp[0].n_tangles > 1 ? (uint64_t)Pp[0].tid[k] : 0;
+        key = t << kKeyTangleShift | morton_key(P, x, y, z);
     } else if (!in_closed(lo, hi, x, y, z)) {
         // outside the root box: in no child (src/octree.c:116-128).  Bit 63
         // puts it after every in-box point; the remaining bits are a Morton
@@ -489,9 +495,14 @@ int radix_sort(Ctx &c, int n, int first_shift, int last_shift)
 // ----------------------------------------------------------- topology ----
 // Common levels of two keys; equal keys share every level the keys resolve
 // (max_lv: kMaxLevel for the reference tree, kTunedLevels for tuned trees).
+// Tuned keys of two tangles share no level, not even the root: -1.
 __device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b, int max_lv)
 {
     if (a == b) return max_lv;
+    if (max_lv == kTunedLevels) {
+        if ((a ^ b) >> kKeyTangleShift) return -1;
+        return (__clzll((long long)(a ^ b)) - (63 - 3 * kTunedLevels)) / 3;
+    }
     return (__clzll((long long)(a ^ b)) - 1) / 3;
 }
 
@@ -509,9 +520,9 @@ __global__ void k_sorted_gather(int n_live, const uint64_t *__restrict__ keys,
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
-    const TangleParams &P = *Pp;
     int n_in = counters[0];
     int k = vals[i];
+    const TangleParams &P = tparams_of(Pp, k);
     spts[i] = k;
     rank[k] = i;
     double x = X[k], y = Y[k], z = Z[k];
@@ -561,9 +572,13 @@ __device__ __forceinline__ int range_end(const uint64_t *__restrict__ keys, int 
 // summation order.  Component 6 counts points whose values are not finite or
 // lie beyond the fixed-point range (more than 4 box lengths out); a cell
 // holding one gets NaN sums, as a floating-point sum would.
+// In a forest every tangle has its own units (its own size and point
+// count), so prefix sums over several tangles can pass 2^63: they are kept
+// modulo 2^64 (unsigned), and a cell's sums, differences within one tangle,
+// are exact.
 constexpr int kFix = 7;
 struct Fix7 {
-    long long v[kFix];
+    unsigned long long v[kFix];
 };
 constexpr int kFixThreads = 256, kFixItems = 4, kFixTile = kFixThreads * kFixItems;
 constexpr int kFixScanThreads = 1024;
@@ -576,66 +591,76 @@ static int fix_shift(int n, double Lmax)
     return std::max(-60, std::min(60, k));
 }
 
-__device__ __forceinline__ long long to_fix(double v, double scale, double lim, long long &bad)
+__device__ __forceinline__ unsigned long long to_fix(double v, double scale, double lim,
+                                                     unsigned long long &bad)
 {
     const double s = v * scale;
     if (!(fabs(s) <= lim)) {
         bad = 1;
         return 0;
     }
-    return __double2ll_rn(s);
+    return (unsigned long long)__double2ll_rn(s);
 }
 
-// Wave-inclusive scan of kFix 64-bit values.
-__device__ __forceinline__ void wave_scan_fix(long long s[kFix], int lane)
+// Wave-inclusive scan of kFix 64-bit values (modulo 2^64).
+__device__ __forceinline__ void wave_scan_fix(unsigned long long s[kFix], int lane)
 {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1)
         for (int f = 0; f < kFix; ++f) {
-            const long long u = __shfl_up(s[f], off, 64);
+            const unsigned long long u = __shfl_up(s[f], off, 64);
             if (lane >= off) s[f] += u;
         }
 }
 
 // P[i] (i <= n) = fixed-point sums of sorted points [tile start, i); T[t] =
-// the tile's total.  Tiles of kFixTile entries.
+// the tile's total.  Tiles of kFixTile entries.  A point's units are its
+// tangle's (tangle from its key in a forest, else the arguments).
 __global__ __launch_bounds__(kFixThreads) void k_fix_tile(
     int n, const double *__restrict__ sx, const double *__restrict__ sy,
     const double *__restrict__ sz, const double *__restrict__ qx, const double *__restrict__ qy,
-    const double *__restrict__ qz, double lo0, double lo1, double lo2, double scale, double lim,
-    Fix7 *__restrict__ P, Fix7 *__restrict__ T)
+    const double *__restrict__ qz, const uint64_t *__restrict__ keys,
+    const TangleParams *__restrict__ Pp, double lo0, double lo1, double lo2, double scale,
+    double lim, Fix7 *__restrict__ P, Fix7 *__restrict__ T)
 {
-    __shared__ long long wsum[kFixThreads / 64][kFix];
+    __shared__ unsigned long long wsum[kFixThreads / 64][kFix];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int base = blockIdx.x * kFixTile + threadIdx.x * kFixItems;
-    long long v[kFixItems][kFix], s[kFix];
+    const bool forest = Pp[0].n_tangles > 1;
+    unsigned long long v[kFixItems][kFix], s[kFix];
     for (int f = 0; f < kFix; ++f) s[f] = 0;
 #pragma unroll
     for (int j = 0; j < kFixItems; ++j) {
         const int i = base + j;
-        long long bad = 0;
+        unsigned long long bad = 0;
         if (i < n) {
-            v[j][0] = to_fix(sx[i] - lo0, scale, lim, bad);
-            v[j][1] = to_fix(sy[i] - lo1, scale, lim, bad);
-            v[j][2] = to_fix(sz[i] - lo2, scale, lim, bad);
-            v[j][3] = to_fix(qx[i], scale, lim, bad);
-            v[j][4] = to_fix(qy[i], scale, lim, bad);
-            v[j][5] = to_fix(qz[i], scale, lim, bad);
+            double l0 = lo0, l1 = lo1, l2 = lo2, sc = scale, li = lim;
+            if (forest) {
+                const TangleParams &Q = Pp[key_tangle(keys[i])];
+                l0 = Q.lo[0]; l1 = Q.lo[1]; l2 = Q.lo[2];
+                sc = Q.fix_scale; li = Q.fix_lim;
+            }
+            v[j][0] = to_fix(sx[i] - l0, sc, li, bad);
+            v[j][1] = to_fix(sy[i] - l1, sc, li, bad);
+            v[j][2] = to_fix(sz[i] - l2, sc, li, bad);
+            v[j][3] = to_fix(qx[i], sc, li, bad);
+            v[j][4] = to_fix(qy[i], sc, li, bad);
+            v[j][5] = to_fix(qz[i], sc, li, bad);
         } else {
             for (int f = 0; f < 6; ++f) v[j][f] = 0;
         }
         v[j][6] = bad;
         for (int f = 0; f < kFix; ++f) s[f] += v[j][f];
     }
-    long long incl[kFix];
+    unsigned long long incl[kFix];
     for (int f = 0; f < kFix; ++f) incl[f] = s[f];
     wave_scan_fix(incl, lane);
     if (lane == 63)
         for (int f = 0; f < kFix; ++f) wsum[w][f] = incl[f];
     __syncthreads();
-    long long ex[kFix], tot[kFix];
+    unsigned long long ex[kFix], tot[kFix];
     for (int f = 0; f < kFix; ++f) {
-        long long pre = 0, all = 0;
+        unsigned long long pre = 0, all = 0;
         for (int q = 0; q < kFixThreads / 64; ++q) {
             if (q < w) pre += wsum[q][f];
             all += wsum[q][f];
@@ -658,35 +683,36 @@ __global__ __launch_bounds__(kFixThreads) void k_fix_tile(
 // run of consecutive tiles.
 __global__ __launch_bounds__(kFixScanThreads) void k_fix_tiles(int tiles, Fix7 *__restrict__ T)
 {
-    __shared__ long long wsum[kFixScanThreads / 64][kFix];
+    __shared__ unsigned long long wsum[kFixScanThreads / 64][kFix];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int per = (tiles + kFixScanThreads - 1) / kFixScanThreads;
     const int t0 = threadIdx.x * per, t1 = min(t0 + per, tiles);
-    long long s[kFix];
+    unsigned long long s[kFix];
     for (int f = 0; f < kFix; ++f) s[f] = 0;
     for (int t = t0; t < t1; ++t)
         for (int f = 0; f < kFix; ++f) s[f] += T[t].v[f];
-    long long incl[kFix];
+    unsigned long long incl[kFix];
     for (int f = 0; f < kFix; ++f) incl[f] = s[f];
     wave_scan_fix(incl, lane);
     if (lane == 63)
         for (int f = 0; f < kFix; ++f) wsum[w][f] = incl[f];
     __syncthreads();
-    long long ex[kFix];
+    unsigned long long ex[kFix];
     for (int f = 0; f < kFix; ++f) {
-        long long pre = 0;
+        unsigned long long pre = 0;
         for (int q = 0; q < w; ++q) pre += wsum[q][f];
         ex[f] = pre + incl[f] - s[f];
     }
     for (int t = t0; t < t1; ++t)
         for (int f = 0; f < kFix; ++f) {
-            const long long u = T[t].v[f];
+            const unsigned long long u = T[t].v[f];
             T[t].v[f] = ex[f];
             ex[f] += u;
         }
 }
 
-// Sums of sorted points [a, b) into a node: centre of mass and circulation.
+// Sums of sorted points [a, b) (one tangle) into a node: centre of mass and
+// circulation, from the prefix sums in units of 1/inv_scale cm.
 __device__ __forceinline__ void fix_node_sums(Node &nd, const Fix7 *__restrict__ P,
                                               const Fix7 *__restrict__ T, int a, int b,
                                               const double lo[3], double inv_scale)
@@ -694,7 +720,8 @@ __device__ __forceinline__ void fix_node_sums(Node &nd, const Fix7 *__restrict__
     const Fix7 &pa = P[a], &pb = P[b];
     const Fix7 &ta = T[a / kFixTile], &tb = T[b / kFixTile];
     long long s[kFix];
-    for (int f = 0; f < kFix; ++f) s[f] = (pb.v[f] + tb.v[f]) - (pa.v[f] + ta.v[f]);
+    for (int f = 0; f < kFix; ++f)
+        s[f] = (long long)((pb.v[f] + tb.v[f]) - (pa.v[f] + ta.v[f]));
     const double nan = __builtin_nan("");
     const double inv = 1.0 / (double)(b - a);
     const bool bad = s[6] != 0;
@@ -706,25 +733,35 @@ __device__ __forceinline__ void fix_node_sums(Node &nd, const Fix7 *__restrict__
     nd.qz = bad ? nan : (double)s[5] * inv_scale;
 }
 
-// Fixed-point prefix sums of the sorted points (c.tree.sx .. sqz).
+// Fixed-point units of a tangle of n points in a box of largest side Lmax:
+// 2^shift per cm with n values of up to 4 Lmax summing below 2^62.
+void fix_units(int n, double Lmax, double &scale, double &inv, double &lim)
+{
+    const int shift = fix_shift(n, Lmax);
+    scale = std::ldexp(1.0, shift);
+    inv = std::ldexp(1.0, -shift);
+    lim = std::ldexp(1.0, 62) / (double)std::max(n, 1);
+}
+
+// Fixed-point prefix sums of the sorted points (c.tree.sx .. sqz).  One
+// tangle: units from its live points and box (inv_scale returned); a
+// forest: each tangle's units from its parameter entry.
 static int fix_prefix(Ctx &c, int n, double &inv_scale)
 {
     TreeBufs &T = c.tree;
     const ovg_params &p = c.params;
     double Lmax = 0;
     for (int k = 0; k < 3; ++k) Lmax = std::max(Lmax, p.box_hi[k] - p.box_lo[k]);
-    const int shift = fix_shift(n, Lmax);
-    const double scale = std::ldexp(1.0, shift);
-    const double lim = std::ldexp(1.0, 62) / (double)std::max(n, 1);
-    inv_scale = std::ldexp(1.0, -shift);
+    double scale, lim;
+    fix_units(n, Lmax, scale, inv_scale, lim);
     const int tiles = (n + 1 + kFixTile - 1) / kFixTile;
     OVG_TRY(T.fix.ensure(sizeof(Fix7) * ((size_t)n + 1)));
     OVG_TRY(T.fix_tiles.ensure(sizeof(Fix7) * (size_t)tiles));
     hipLaunchKernelGGL(k_fix_tile, dim3(tiles), dim3(kFixThreads), 0, c.stream, n,
                        T.sx.as<double>(), T.sy.as<double>(), T.sz.as<double>(),
-                       T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(), p.box_lo[0],
-                       p.box_lo[1], p.box_lo[2], scale, lim, T.fix.as<Fix7>(),
-                       T.fix_tiles.as<Fix7>());
+                       T.sqx.as<double>(), T.sqy.as<double>(), T.sqz.as<double>(),
+                       T.keys[0].as<uint64_t>(), dparams(c), p.box_lo[0], p.box_lo[1],
+                       p.box_lo[2], scale, lim, T.fix.as<Fix7>(), T.fix_tiles.as<Fix7>());
     hipLaunchKernelGGL(k_fix_tiles, dim3(1), dim3(kFixScanThreads), 0, c.stream, tiles,
                        T.fix_tiles.as<Fix7>());
     return OVG_OK;
@@ -734,7 +771,9 @@ static int fix_prefix(Ctx &c, int n, double &inv_scale)
 // fixP != nullptr (tuned trees): a cell's Lm is the root's largest side
 // times 2^-level and its sums come from the prefix sums; otherwise Lm
 // replays the reference's child boxes and the sums run level by level
-// afterwards.  Leaves take their point either way.
+// afterwards.  Leaves take their point either way.  Tuned trees of a
+// forest: every tangle's first sorted point starts that tangle's root (its
+// box, its points), whose DFS index goes to roots[tangle].
 template <bool EMIT>
 __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const int8_t *__restrict__ delta, const int *__restrict__ counters,
@@ -744,7 +783,8 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                         const double *__restrict__ sy, const double *__restrict__ sz,
                         const double *__restrict__ qx, const double *__restrict__ qy,
                         const double *__restrict__ qz, const Fix7 *__restrict__ fixP,
-                        const Fix7 *__restrict__ fixT, double inv_scale, int max_lv)
+                        const Fix7 *__restrict__ fixT, double inv_scale, int max_lv,
+                        int *__restrict__ roots)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -763,6 +803,7 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
             NodeMeta m{};
             m.start = 0; m.level = 0;
             meta[0] = m;
+            if (roots) roots[0] = 0;
         }
         return;
     }
@@ -771,8 +812,11 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
     int cnt = 0;
     int d = EMIT ? base[i] : 0;
     int total = EMIT ? base[n_live] : 0;
-    const TangleParams &P = *Pp;
     uint64_t ki = keys[i];
+    const bool forest = fixP && Pp[0].n_tangles > 1;
+    const int tg = forest ? key_tangle(ki) : 0;
+    const TangleParams &P = Pp[tg];
+    const double fixinv = forest ? P.fix_inv : inv_scale;
     const double Lroot = fmax(fmax(P.hi[0] - P.lo[0], P.hi[1] - P.lo[1]), P.hi[2] - P.lo[2]);
     const double plo[3] = {P.lo[0], P.lo[1], P.lo[2]};
     auto emit = [&](int level, int end, int count) {
@@ -785,7 +829,7 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
                 int lv = level < kMaxLevel ? level : kMaxLevel;
                 if (fixP) {
                     nd.Lm = ldexp(Lroot, -lv);
-                    fix_node_sums(nd, fixP, fixT, i, i + count, plo, inv_scale);
+                    fix_node_sums(nd, fixP, fixT, i, i + count, plo, fixinv);
                 } else {
                     // replay the reference's child boxes down to `level`
                     double lo[3] = {P.lo[0], P.lo[1], P.lo[2]};
@@ -809,19 +853,27 @@ __global__ void k_nodes(int n_live, const uint64_t *__restrict__ keys,
         }
         ++cnt;
     };
-    if (i == 0) {
-        // explicit root: the domain box with every live point (src/octree.c:141-150)
+    if (dl < 0) {
+        // explicit root: the domain box with every live point of the tangle
+        // (src/octree.c:141-150); one tangle: every live point, the dropped
+        // ones included
+        const int end = !forest ? n_live - 1 : dr < 0 ? i : range_end(keys, i, n_in, 0, max_lv);
         if (EMIT) {
             Node r{};
-            r.count = n_live;
-            r.skip = total;
+            r.count = end - i + 1;
+            r.skip = end + 1 < n_in ? base[end + 1] : total;
             double Lx = P.hi[0] - P.lo[0], Ly = P.hi[1] - P.lo[1], Lz = P.hi[2] - P.lo[2];
             r.Lm = Lx > Ly ? (Lx > Lz ? Lx : Lz) : (Ly > Lz ? Ly : Lz);
-            if (fixP) fix_node_sums(r, fixP, fixT, 0, n_live, plo, inv_scale);
+            if (fixP) fix_node_sums(r, fixP, fixT, i, end + 1, plo, fixinv);
+            if (r.count == 1) {
+                r.cx = sx[i]; r.cy = sy[i]; r.cz = sz[i];
+                r.qx = qx[i]; r.qy = qy[i]; r.qz = qz[i];
+            }
             nodes[d] = r;
             NodeMeta m{};
-            m.start = 0; m.level = 0;
+            m.start = i; m.level = 0;
             meta[d] = m;
+            if (roots) roots[tg] = d;
             ++d;
         }
         ++cnt;
@@ -974,6 +1026,7 @@ int build_tree(Ctx &c, const double *const x[3])
     OVG_TRY(T.level_hist.ensure(sizeof(int) * ((size_t)kLevels * ((cap_nodes + 255) / 256) + 1)));
     OVG_TRY(T.level_list.ensure(sizeof(int) * cap_nodes));
     OVG_TRY(T.counters.ensure(sizeof(int) * 64));
+    OVG_TRY(T.roots.ensure(sizeof(int) * (size_t)std::max(c.n_tangles, 1)));
     OVG_TRY(T.sorted_pts.ensure(sizeof(int) * (n + 1)));
     OVG_TRY(T.rank.ensure(sizeof(int) * ((size_t)c.n + 1)));
     DBuf *sv[6] = {&T.sx, &T.sy, &T.sz, &T.sqx, &T.sqy, &T.sqz};
@@ -985,9 +1038,11 @@ int build_tree(Ctx &c, const double *const x[3])
     hipLaunchKernelGGL(k_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), x[0], x[1],
                        x[2], dparams(c), T.keys[0].as<uint64_t>(), T.vals[0].as<int>(),
                        T.counters.as<int>(), clamp);
-    // tuned keys: bits kTunedLowBit .. 62 in 8-bit digits (6 passes)
+    // tuned keys: Morton bits 1 .. 48 in 8-bit digits (6 passes), and the
+    // tangle index above them in a forest (1 or 2 more)
     const int max_lv = clamp ? kTunedLevels : kMaxLevel;
-    if (clamp) OVG_TRY(radix_sort(c, n, kTunedLowBit, 55));
+    const int last = c.n_tangles > 256 ? 57 : c.n_tangles > 1 ? 49 : 41;
+    if (clamp) OVG_TRY(radix_sort(c, n, 1, last));
     else OVG_TRY(radix_sort(c, n, 0, 56));
     hipLaunchKernelGGL(k_count_inbox, dim3((n + 1 + bs - 1) / bs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), T.counters.as<int>());
@@ -1008,7 +1063,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv);
+                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv, T.roots.as<int>());
     // base[i] for i <= n; counts[n] is not written -> scan n entries, total at base[n]
     OVG_TRY(launch_exclusive_scan(c, T.counts.as<int>(), T.base.as<int>(), n,
                                   T.base.as<int>() + n, T.scan_tmp));
@@ -1017,7 +1072,7 @@ int build_tree(Ctx &c, const double *const x[3])
                        T.counts.as<int>(), T.base.as<int>(), T.nodes.as<Node>(),
                        T.meta.as<NodeMeta>(), dparams(c), T.sx.as<double>(), T.sy.as<double>(),
                        T.sz.as<double>(), T.sqx.as<double>(), T.sqy.as<double>(),
-                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv);
+                       T.sqz.as<double>(), fixP, fixT, inv_scale, max_lv, T.roots.as<int>());
     hipLaunchKernelGGL(k_tree_counters, dim3(1), dim3(1), 0, c.stream, n, T.base.as<int>(),
                        T.counters.as<int>());
     if (clamp) return OVG_OK;
@@ -1055,7 +1110,7 @@ __global__ void k_refit_nodes(const int *__restrict__ counters, Node *__restrict
     if (d >= counters[1]) return;
     Node &nd = nodes[d];
     const int j = meta[d].start, cnt = nd.count;
-    if (d != 0 && cnt == 1) {
+    if (cnt == 1) {  // a leaf (or a one-point root) takes its point
         nd.cx = sx[j]; nd.cy = sy[j]; nd.cz = sz[j];
         nd.qx = qx[j]; nd.qy = qy[j]; nd.qz = qz[j];
     } else {

@@ -192,6 +192,9 @@ extern "C" int ovg_save_tangle(ovg_ctx *ctx, const char *path, int digits)
 {
     ovg::clear_error();
     if (!ctx || !path) return ovg::set_error(OVG_EINVAL, "bad arguments to ovg_save_tangle");
+    if (ovg_forest_size(ctx) > 1)
+        return ovg::set_error(OVG_EUNSUPPORTED, "ovg_save_tangle: the context holds a forest; "
+                              "a tangle file holds one tangle");
     const int n = ovg_num_slots(ctx);
     if (n < 0) return ovg::set_error(OVG_EINVAL, "no tangle");
     std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), t(3 * (size_t)n), nr(3 * (size_t)n);
