@@ -157,6 +157,138 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def forest_members(n, seed=0xC0FFEE04):
+    """The batched ensemble's tangles and parameters (openvort_amd.serve
+    c4_jobs: sizes log-uniform 2k-100k, D = 0.1 cm, dxi = 8e-4 cm, v_ns
+    U[0.5, 2], 1.6 K or 1.9 K), mixed precision list traversal."""
+    import openvort_amd as ov
+    from openvort_amd import configs, serve
+    out = []
+    for j in serve.c4_jobs(seed=seed, n=n, steps=1):
+        t = ov.generate_rings(j.seed, j.n_rings, j.D, j.r_min, j.r_max, j.dxi)
+        p = configs.params(j.D, j.dxi, (j.alpha, j.alpha_p), j.v_ns, precision=ov.PREC_MIXED,
+                           leaf_size=1, traverse=ov.TRAVERSE_LIST)
+        p.points_per_wave = 16
+        out.append((t, p))
+    return out
+
+
+def forest_bench(args):
+    """The batched ensemble as one forest: every step's update_tangle +
+    rk4_step + wrap of all tangles in one captured HIP graph (one tree build
+    and one list launch per evaluation for all of them), replayed; against
+    the plain FP64 path (reference tree, per-point traversal) one tangle at
+    a time, and the same mixed path one tangle at a time, on the same
+    steps (no topology passes: the ensemble's step is the evaluation and
+    the RK4 update)."""
+    import openvort_amd as ov
+    n_jobs = args.jobs if args.jobs != 24 else 64
+    ms = forest_members(n_jobs)
+    n_pts = sum(len(t.live) for t, _ in ms)
+    dt = 5e-5
+    log(f"c4 forest: {len(ms)} tangles, {n_pts} points")
+
+    def step(e, tm):
+        e.update_tangle(tm)
+        e.rk4_step(tm, dt)
+        e.enforce_periodic()
+
+    e = ov.Engine(ms[0][1])
+    e.upload_forest([t for t, _ in ms], [p for _, p in ms])
+    tune = None
+    if not args.no_autotune:
+        rows, best = e.autotune(repeats=2)
+        tune = rows[best]
+        log(f"forest autotune: {tune}")
+    step(e, 0.0)
+    # the traversal's launch time and pairs: two eager steps with timing
+    e.stats()
+    e.set_timing(True)
+    for s in range(2):
+        step(e, (1 + s) * dt)
+    e.synchronize()
+    st = e.stats()
+    e.set_timing(False)
+    launches = max(st.bs_launches, 1)
+    bs_ms = st.ms_bs / launches
+    pairs = st.interactions / launches
+    # eager fused steps
+    e.synchronize()
+    a = time.perf_counter()
+    for s in range(args.steps):
+        step(e, (3 + s) * dt)
+    e.synchronize()
+    eager_ms = (time.perf_counter() - a) / args.steps * 1e3
+    # one step captured, replayed
+    e.graph_begin()
+    step(e, 0.0)
+    nodes = e.graph_end()
+    e.graph_launch(args.warmup)
+    e.synchronize()
+    e.marker_record(0)
+    a = time.perf_counter()
+    for s in range(args.steps):
+        e.graph_launch(1)
+        e.marker_record(s + 1)
+    e.synchronize()
+    wall = time.perf_counter() - a
+    lat = [e.marker_elapsed(s, s + 1) * 1e-3 for s in range(args.steps)]
+    e.close()
+    value = n_pts * args.steps / wall
+    log(f"forest graph: {wall / args.steps * 1e3:.1f} ms/step ({nodes} nodes), eager "
+        f"{eager_ms:.1f} ms/step, {value:.4g} point-steps/s")
+
+    def one_at_a_time(fp64, steps):
+        total, pts_steps = 0.0, 0.0
+        for t, p in ms:
+            q = ov.Params.from_buffer_copy(p)
+            if fp64:
+                q.precision, q.traverse, q.leaf_size = ov.PREC_FP64, ov.TRAVERSE_POINT, 1
+            f = ov.Engine(q)
+            f.upload(t)
+            step(f, 0.0)
+            f.synchronize()
+            a = time.perf_counter()
+            for s in range(steps):
+                step(f, (1 + s) * dt)
+            f.synchronize()
+            total += time.perf_counter() - a
+            pts_steps += len(t.live) * steps
+            f.close()
+        return pts_steps / total, total
+
+    plain_v, plain_s = one_at_a_time(True, args.plain_steps)
+    log(f"plain FP64 one at a time: {plain_v:.4g} point-steps/s")
+    sep_v, sep_s = one_at_a_time(False, args.plain_steps)
+    log(f"mixed one at a time: {sep_v:.4g} point-steps/s")
+    achieved = pairs * FLOPS_PER_PAIR / (bs_ms * 1e-3) if bs_ms > 0 else 0.0
+    out = {
+        "metric": "RK4 point-steps/s, c4 batched ensemble as one forest (one tree and one "
+                  "list launch per evaluation, HIP-graph step) vs plain FP64 one tangle at a time",
+        "value": value, "unit": "point-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": (f"c4: {len(ms)} seeded tangles, {n_pts} points, sizes "
+                                f"log-uniform 2k-100k, BH theta 0.4, 26 periodic images, "
+                                f"dt {dt}; step = update_tangle + rk4_step + wrap"),
+                   "tangles": len(ms), "points": n_pts, "autotune": tune, "graph_nodes": nodes},
+        "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
+        "eager_ms_per_step": eager_ms,
+        "plain_fp64": {"value": plain_v, "seconds": plain_s, "steps": args.plain_steps,
+                       "path": "FP64, reference split-until-one tree, per-point traversal, "
+                               "one tangle at a time"},
+        "mixed_one_at_a_time": {"value": sep_v, "seconds": sep_s, "steps": args.plain_steps},
+        "speedup_vs_plain": value / plain_v,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_bh_list<mixed> (forest)", "avg_launch_ms": bs_ms,
+                     "pairs_per_launch": pairs},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def ov_points(job):
     import openvort_amd as ov
     return ov.lib().ovg_generate_rings(job.seed, job.n_rings, job.D, job.r_min, job.r_max,
@@ -335,7 +467,11 @@ def main():
                     help="keep --ppw/--leaf/--group-level instead of tuning them first")
     ap.add_argument("--accuracy-sample", type=int, default=2048,
                     help="points of the direct-sum error check (0 = skip)")
-    ap.add_argument("--jobs", type=int, default=24, help="c4/c5: number of jobs")
+    ap.add_argument("--jobs", type=int, default=24,
+                    help="c5 (and c4 with --serve): number of jobs; c4 forest: 64 unless set")
+    ap.add_argument("--serve", action="store_true",
+                    help="c4: run the ensemble through the request loop (full steps with "
+                         "topology, concurrent contexts) instead of as one forest")
     ap.add_argument("--steps-scale", type=float, default=0.05,
                     help="c4/c5: fraction of each job's requested steps")
     ap.add_argument("--workers", type=int, default=8, help="c4/c5: stepping threads")
@@ -363,6 +499,9 @@ def main():
         log(f"copy probe: 5 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
         return
 
+    if args.workload == "c4" and not args.serve:
+        forest_bench(args)
+        return
     if args.workload in ("c4", "c5"):
         serve_bench(args)
         return

@@ -166,6 +166,60 @@ __device__ __forceinline__ bool seg_field64(double Rx, double Ry, double Rz, dou
     return true;
 }
 
+// 1/sqrt(x) and 1/x for normal x > 0 to double precision: the hardware
+// estimates (about 2^-23 relative) refined by two Newton steps each
+// (quadratic convergence: 2^-46, then below an ulp).
+__device__ __forceinline__ double rsq64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+
+__device__ __forceinline__ double rcp64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * fma(-x, y, 2.0);
+    y = y * fma(-x, y, 2.0);
+    return y;
+}
+
+// seg_field64 for the Barnes-Hut list traversal, where the pairs need the
+// reference's guard decisions but not its rounding: 1/|R| and 1/|Rp| from
+// rsq64 and the scalar from one rcp64 instead of two correctly rounded
+// square roots and a division.  Every guard is decided as in seg_field64
+// (|R|, |Rp| >= 1e-8 and (R.Rp/|R|)/|Rp| >= 1e-8): outside a 2^-40 band
+// around each threshold the cheap values decide, inside it the exact
+// expressions do.
+__device__ __forceinline__ bool seg_field64_fast(double Rx, double Ry, double Rz, double Px,
+                                                 double Py, double Pz, double f_bs,
+                                                 double &vx, double &vy, double &vz)
+{
+    const double r2 = fma(Rz, Rz, fma(Ry, Ry, Rx * Rx));
+    const double p2 = fma(Pz, Pz, fma(Py, Py, Px * Px));
+    const double dot = fma(Rz, Pz, fma(Ry, Py, Rx * Px));
+    const double t2 = 1e-16, lo = 1.0 - 0x1.0p-40, hi = 1.0 + 0x1.0p-40;
+    if (r2 < t2 * lo || p2 < t2 * lo) return false;
+    if (!(r2 > t2 * hi) && sqrt(dot3(Rx, Ry, Rz, Rx, Ry, Rz)) < 1e-8) return false;
+    if (!(p2 > t2 * hi) && sqrt(dot3(Px, Py, Pz, Px, Py, Pz)) < 1e-8) return false;
+    const double ia = rsq64(r2), ib = rsq64(p2);
+    const double iab = ia * ib;
+    const double nd = dot * iab;
+    if (nd < 1e-8 * lo) return false;
+    if (!(nd > 1e-8 * hi) &&
+        dot3(Rx, Ry, Rz, Px, Py, Pz) / sqrt(dot3(Rx, Ry, Rz, Rx, Ry, Rz)) /
+                sqrt(dot3(Px, Py, Pz, Px, Py, Pz)) <
+            1e-8)
+        return false;
+    const double m = f_bs * (ia + ib) * iab * rcp64(1.0 + nd);
+    vx = (Ry * Pz - Rz * Py) * m;
+    vy = (Rz * Px - Rx * Pz) * m;
+    vz = (Rx * Py - Ry * Px) * m;
+    return true;
+}
+
 // A pair of the mixed-precision paths from FP64 R and Rp, in FP32 (native
 // reciprocal square roots, one reciprocal, fused multiply-adds: this path
 // makes no claim to the reference's rounding; the library is otherwise

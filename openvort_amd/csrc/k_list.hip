@@ -334,7 +334,12 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         for (int i = slice; i < near_n; i += S) {
             double Rx, Ry, Rz, Qx, Qy, Qz, vx, vy, vz;
             load_near(nearI[i], Rx, Ry, Rz, Qx, Qy, Qz);
-            if (seg_field64(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz)) {
+            // FP64 mode: every pair, so the cheaper inverse square roots pay;
+            // mixed mode keeps the few near pairs' shorter code (registers)
+            const bool hit =
+                MIXED ? seg_field64(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz)
+                      : seg_field64_fast(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz);
+            if (hit) {
                 ax += vx; ay += vy; az += vz;
             }
         }
@@ -437,42 +442,34 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         const int room = kListStack - kStackMargin - sp;
         const int kp = min(min(sp, 64), max(1, room));
         const bool act = lane < kp;
-        int2 rg = make_int2(0, 0);
-        if (act) rg = stk[sp - kp + lane];
+        // every lane reads a range and a node: an idle lane takes the stack's
+        // first entry and the root of this tangle (valid addresses; its kind
+        // is forced to nothing below), so the round has no per-lane branches
+        const int2 rg = stk[act ? sp - kp + lane : 0];
         __syncthreads();
         sp -= kp;
-        const int d = rg.x;
-        const int cp = rg.y >> kCopyShift, end = rg.y & (int)kIdxMask;
+        const int d = act ? rg.x : root;
+        const int cp = act ? rg.y >> kCopyShift : 0, end = rg.y & (int)kIdxMask;
         const double shx = shL[cp][0], shy = shL[cp][1], shz = shL[cp][2];
-        Node nd{};
-        int kind = 0;  // 0 nothing, 1 entry, 2 bucket, 3 open
-        int skp = 0;   // subtree end, kept inside the popped range
-        double dd2 = 0;
-        int st = 0;  // first sorted segment of the cell (buckets)
-        double ncx = 0, ncy = 0, ncz = 0;  // the node's centre in this copy
-        if (act) {
-            nd = nodes[d];
-            ncx = nd.cx + shx; ncy = nd.cy + shy; ncz = nd.cz + shz;
-            skp = min(max(nd.skip, d + 1), end);
-            if (nd.count > 0) {
-                if (MIR && shM[cp]) {
-                    const int m = shM[cp];
-                    dd2 = box_dist2((m & 1) ? shx - nd.cx : nd.cx + shx,
-                                    (m & 2) ? shy - nd.cy : nd.cy + shy,
-                                    (m & 4) ? shz - nd.cz : nd.cz + shz, lox, loy, loz, hix,
-                                    hiy, hiz);
-                } else {
-                    dd2 = box_dist2(ncx, ncy, ncz, lox, loy, loz, hix, hiy, hiz);
-                }
-                if (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs)
-                    kind = 1;
-                else if (nd.count <= leaf || skp == d + 1)
-                    kind = 2;
-                else
-                    kind = 3;
-            }
-            if (kind == 2) st = meta[d].start;  // only buckets read their points
+        const Node nd = nodes[d];
+        // the node's centre in this copy
+        const double ncx = nd.cx + shx, ncy = nd.cy + shy, ncz = nd.cz + shz;
+        const int skp = min(max(nd.skip, d + 1), end);  // subtree end, inside the range
+        double dd2;
+        if (MIR && shM[cp]) {
+            const int m = shM[cp];
+            dd2 = box_dist2((m & 1) ? shx - nd.cx : ncx, (m & 2) ? shy - nd.cy : ncy,
+                            (m & 4) ? shz - nd.cz : ncz, lox, loy, loz, hix, hiy, hiz);
+        } else {
+            dd2 = box_dist2(ncx, ncy, ncz, lox, loy, loz, hix, hiy, hiz);
         }
+        // 0 nothing, 1 entry, 2 bucket, 3 open
+        const int kind = !act || nd.count <= 0                       ? 0
+                         : (nd.count == 1 || dd2 > nd.Lm * nd.Lm * cs) ? 1
+                         : (nd.count <= leaf || skp == d + 1)          ? 2
+                                                                       : 3;
+        int st = 0;  // first sorted segment of the cell (buckets)
+        if (kind == 2) st = meta[d].start;  // only buckets read their points
         // pushes (both ranges nonempty, first index beyond d, end <= the
         // popped range's end): sibling continuation, then the children
         const bool sib = act && skp < end;
