@@ -337,8 +337,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             // FP64 mode: every pair, so the cheaper inverse square roots pay;
             // mixed mode keeps the few near pairs' shorter code (registers)
             const bool hit =
-                MIXED ? seg_field64(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz)
-                      : seg_field64_fast(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz);
+                seg_field64_fast(Rx, Ry, Rz, Rx + Qx, Ry + Qy, Rz + Qz, f64, vx, vy, vz);
             if (hit) {
                 ax += vx; ay += vy; az += vz;
             }
@@ -382,8 +381,6 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         const unsigned long long mf = __ballot(keep && !nr);
         const unsigned long long mn = __ballot(keep && nr);
         const int cf = __popcll(mf), cn = __popcll(mn);
-        if (cf && far_n + cf > far_cap) flush_far(false);
-        if (cn && near_n + cn > kListNear) flush_near();
         if (keep && !nr) {
             const int pos = far_n + prefix_below(mf);
             farA[pos] = make_float4(ex, ey, ez, gx);
@@ -409,6 +406,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     if (MODE == kReplay && rc >= 0) {
         // the recorded entries at the current positions, 64 per round
         for (int b0 = 0; b0 < rc; b0 += 64) {
+            if (far_n + 64 > far_cap) flush_far(false);
+            if (near_n + 64 > kListNear) flush_near();
             const int j = b0 + lane;
             const bool want = j < rc;
             const unsigned e = want ? rec[rec_base + j] : 0u;
@@ -438,6 +437,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     int sp = rend > root ? n_copies : 0;
     __syncthreads();
     while (sp > 0) {
+        if (far_n + 64 > far_cap) flush_far(false);
+        if (near_n + 64 > kListNear) flush_near();
         // each popped range pushes at most two: keep kStackMargin free
         const int room = kListStack - kStackMargin - sp;
         const int kp = min(min(sp, 64), max(1, room));
@@ -523,6 +524,8 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
             }
             __syncthreads();
             for (int b0 = 0; b0 < btotal; b0 += 64) {
+            if (far_n + 64 > far_cap) flush_far(false);
+            if (near_n + 64 > kListNear) flush_near();
                 const int j = b0 + lane;
                 const bool want = j < btotal;
                 float ex = 0, ey = 0, ez = 0, gx = 0, gy = 0, gz = 0;
