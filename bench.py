@@ -198,7 +198,9 @@ def forest_bench(args):
     tune = None
     if not args.no_autotune:
         rows, best = e.autotune(repeats=2)
-        tune = rows[best]
+        tune = dict(rows[best], candidates=len(rows),
+                    shapes=[[r["list_stack"], r["list_tile"], r["list_blocks"], round(r["ms"], 3)]
+                            for r in rows[48:]])
         log(f"forest autotune: {tune}")
     step(e, 0.0)
     # the traversal's launch time and pairs: two eager steps with timing
@@ -444,6 +446,12 @@ def main():
     ap.add_argument("--ppw", type=int, default=16, help="list traversal: points per wavefront")
     ap.add_argument("--group-level", type=int, default=0,
                     help="list traversal: octree level that bounds a group (0 = auto)")
+    ap.add_argument("--stack", type=int, default=0,
+                    help="list traversal stack depth (0 = 384; the autotuner may change it)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="list traversal far-field flush tile, 96 or 128 (0 = 128)")
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="list traversal persistent grid, wavefronts per CU (0 = one per group)")
     ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])
     ap.add_argument("--smooth-every", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=3)
@@ -517,6 +525,7 @@ def main():
     p.rec_velocity = recv
     p.points_per_wave = args.ppw
     p.group_level = args.group_level
+    p.list_stack, p.list_tile, p.list_blocks = args.stack, args.tile, args.blocks
     n_pts = len(t.live)
     log(f"workload {args.workload}: {n_pts} points, {t.n} slots")
 
@@ -533,7 +542,9 @@ def main():
     if trav == ov.TRAVERSE_LIST and not args.no_autotune:
         a = time.perf_counter()
         rows, best = e.autotune(repeats=2)
-        tune = dict(rows[best], seconds=time.perf_counter() - a, candidates=len(rows))
+        tune = dict(rows[best], seconds=time.perf_counter() - a, candidates=len(rows),
+                    shapes=[[r["list_stack"], r["list_tile"], r["list_blocks"], round(r["ms"], 3)]
+                            for r in rows[48:]])
         log(f"autotune: {tune}")
     p_run = e.get_params()  # the tuned setting that the timed steps use
 
@@ -660,7 +671,9 @@ def main():
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "k_bh_list_mixed_c3_traffic.json")
     cfg_now = {"workload": args.workload, "points_per_wave": p_run.points_per_wave,
-               "leaf_size": p_run.leaf_size, "group_level": p_run.group_level}
+               "leaf_size": p_run.leaf_size, "group_level": p_run.group_level,
+               "list_stack": p_run.list_stack or 384, "list_tile": p_run.list_tile or 128,
+               "list_blocks": p_run.list_blocks}
     if (os.path.exists(tfile) and trav == ov.TRAVERSE_LIST and prec == ov.PREC_MIXED):
         with open(tfile) as f:
             tj = json.load(f)
@@ -678,7 +691,8 @@ def main():
                                 f"{args.smooth_every} steps"),
                    "points": n_pts, "precision": args.precision, "traverse": args.traverse,
                    "leaf_size": p_run.leaf_size, "points_per_wave": p_run.points_per_wave,
-                   "group_level": p_run.group_level,
+                   "group_level": p_run.group_level, "list_stack": p_run.list_stack or 384,
+                   "list_tile": p_run.list_tile or 128, "list_blocks": p_run.list_blocks,
                    "rec_velocity": args.rec_velocity, "autotune": tune},
         "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
         "interactions_per_s": interactions_per_s,

@@ -175,7 +175,16 @@ typedef struct ovg_params {
                                      0 = pure shift.  At most one face per
                                      axis.  reuse_lists takes shift images
                                      only.                                      */
-    int32_t pad_;
+    /* OVG_TRAVERSE_LIST launch shape (no counterpart in the reference; the
+     * autotuner sets them).  Results differ between shapes only in the
+     * FP32 summation order, except list_blocks, which changes nothing. */
+    int32_t list_stack;           /* traversal stack depth in ranges, 128..384
+                                     in steps of 64 (0 = 384)                   */
+    int32_t list_tile;            /* far-field flush tile in entries: 96 or
+                                     128 (0 = 128)                              */
+    int32_t list_blocks;          /* 0: one wavefront per target group; n > 0:
+                                     a persistent grid of n wavefronts per CU
+                                     (1..64) striding over the groups           */
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a
@@ -481,23 +490,30 @@ int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms);
 /* ---- autotuning (no counterpart in the reference) --------------------- */
 /*
  * Time the list traversal (OVG_TRAVERSE_LIST) on the context's current
- * tangle for every combination of points_per_wave {8, 16, 32, 64}, leaf_size
- * {1, 2, 4, 8} and group_level {L-1, L, L+1} (L: the automatic level),
- * one warm-up and `repeats` timed launches each on a tree built once, and
- * keep the fastest in the context's parameters.  Results differ between
- * configurations only within the traversal's truncation (each is never
- * coarser than the reference's per-point acceptance).
+ * tangle (or forest), one warm-up and `repeats` timed launches each on a
+ * tree built once, and keep the fastest in the context's parameters.  Stage
+ * one: every combination of points_per_wave {8, 16, 32, 64}, leaf_size
+ * {1, 2, 4, 8} and group_level {L-1, L, L+1} (L: the automatic level) at
+ * the context's launch shape.  Stage two, at the best of stage one: the
+ * launch shape, list_stack {128, 256, 384} x list_tile {96, 128} x
+ * list_blocks {0, 8, 16} (traversal stack depth, far-field flush tile,
+ * persistent grid).  Results differ between configurations only within the
+ * traversal's truncation (each is never coarser than the reference's
+ * per-point acceptance) and the FP32 summation order.
  */
 typedef struct ovg_tune_row {
-    int32_t points_per_wave, leaf_size, group_level, pad_;
+    int32_t points_per_wave, leaf_size, group_level;
+    int32_t list_stack, list_tile, list_blocks;  /* the launch shape (ovg_params) */
     double  ms;             /* average launch, HIP events               */
     double  pairs;          /* pair interactions per launch            */
 } ovg_tune_row;
+#define OVG_TUNE_MAX_ROWS 96
 typedef struct ovg_tune {
     int32_t n_rows, best;
-    ovg_tune_row rows[48];
+    ovg_tune_row rows[OVG_TUNE_MAX_ROWS];
 } ovg_tune;
 int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out);
+size_t ovg_tune_sizeof(void);
 
 /* ---- tangle files (src/utils.c:68-199) ------------------------------- */
 /*

@@ -62,18 +62,19 @@ class Params(C.Structure):
         ("rec_velocity", C.c_int32), ("mixed_near", C.c_double),
         ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
         ("reuse_lists", C.c_int32), ("image_reflect", C.c_int32 * 26),
-        ("pad_", C.c_int32),
+        ("list_stack", C.c_int32), ("list_tile", C.c_int32), ("list_blocks", C.c_int32),
     ]
 
 
 class TuneRow(C.Structure):
     _fields_ = [("points_per_wave", C.c_int32), ("leaf_size", C.c_int32),
-                ("group_level", C.c_int32), ("pad_", C.c_int32), ("ms", C.c_double),
+                ("group_level", C.c_int32), ("list_stack", C.c_int32),
+                ("list_tile", C.c_int32), ("list_blocks", C.c_int32), ("ms", C.c_double),
                 ("pairs", C.c_double)]
 
 
 class Tune(C.Structure):
-    _fields_ = [("n_rows", C.c_int32), ("best", C.c_int32), ("rows", TuneRow * 48)]
+    _fields_ = [("n_rows", C.c_int32), ("best", C.c_int32), ("rows", TuneRow * 96)]
 
 
 class Stats(C.Structure):
@@ -99,6 +100,7 @@ _sig("ovg_last_error", C.c_char_p)
 _sig("ovg_version", C.c_char_p)
 _sig("ovg_params_sizeof", C.c_size_t)
 _sig("ovg_stats_sizeof", C.c_size_t)
+_sig("ovg_tune_sizeof", C.c_size_t)
 _sig("ovg_params_default", None, _P(Params), C.c_double)
 _sig("ovg_params_set_periodic_images", C.c_int, _P(Params), C.c_int)
 _sig("ovg_params_set_wall_images", C.c_int, _P(Params), C.c_int)
@@ -156,7 +158,8 @@ _sig("ovg_graph_release", C.c_int, C.c_void_p)
 _sig("ovg_set_device_memory_limit", C.c_size_t, C.c_size_t)
 _sig("ovg_device_memory_in_use", C.c_size_t)
 
-if _lib.ovg_params_sizeof() != C.sizeof(Params) or _lib.ovg_stats_sizeof() != C.sizeof(Stats):
+if (_lib.ovg_params_sizeof() != C.sizeof(Params) or _lib.ovg_stats_sizeof() != C.sizeof(Stats)
+        or _lib.ovg_tune_sizeof() != C.sizeof(Tune)):
     raise ImportError("ctypes struct layout does not match libopenvort_gpu.so")
 
 
@@ -450,12 +453,15 @@ class Engine:
 
     def autotune(self, repeats=2):
         """Time the list traversal over points per wave, leaf size and group
-        level on the current tangle and keep the fastest; returns (rows,
-        best index), rows as dicts."""
+        level, then over the launch shape (stack depth, far-field flush
+        tile, persistent grid), on the current tangle or forest and keep the
+        fastest; returns (rows, best index), rows as dicts."""
         t = Tune()
         _check(_lib.ovg_autotune(self._h, repeats, C.byref(t)), "ovg_autotune")
         rows = [{"points_per_wave": r.points_per_wave, "leaf_size": r.leaf_size,
-                 "group_level": r.group_level, "ms": r.ms, "pairs": r.pairs}
+                 "group_level": r.group_level, "list_stack": r.list_stack,
+                 "list_tile": r.list_tile, "list_blocks": r.list_blocks, "ms": r.ms,
+                 "pairs": r.pairs}
                 for r in t.rows[:t.n_rows]]
         return rows, t.best
 

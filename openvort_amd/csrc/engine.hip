@@ -385,6 +385,7 @@ extern "C" int ovg_create(ovg_ctx **out, int device)
     ovg_ctx *ctx = new ovg_ctx();
     Ctx &c = ctx->c;
     c.device = device;
+    c.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return set_error(OVG_EDEVICE, "hipStreamCreate failed");
@@ -509,6 +510,12 @@ static int validate_params(const ovg_params *p)
         return set_error(OVG_EINVAL, "group_level must be in [0, 21]");
     if (p->reuse_lists != 0 && p->reuse_lists != 1)
         return set_error(OVG_EINVAL, "reuse_lists must be 0 or 1");
+    if (p->list_stack != 0 && (p->list_stack < 128 || p->list_stack > 384 || p->list_stack % 64))
+        return set_error(OVG_EINVAL, "list_stack must be 0 or 128..384 in steps of 64");
+    if (p->list_tile != 0 && p->list_tile != 96 && p->list_tile != 128)
+        return set_error(OVG_EINVAL, "list_tile must be 0, 96 or 128");
+    if (p->list_blocks < 0 || p->list_blocks > 64)
+        return set_error(OVG_EINVAL, "list_blocks must be in [0, 64]");
     if (p->traverse == OVG_TRAVERSE_POINT &&
         (p->leaf_size != 1 || p->precision != OVG_PREC_FP64) && p->method == OVG_METHOD_BH)
         return set_error(OVG_EINVAL,
@@ -749,10 +756,11 @@ extern "C" int ovg_upload_forest(ovg_ctx *ctx, int n_tangles, const ovg_params *
                              "traversal (method BH, traverse LIST, no reuse_lists)", j);
         if (q.precision != p0.precision || q.leaf_size != p0.leaf_size ||
             q.points_per_wave != p0.points_per_wave || q.group_level != p0.group_level ||
-            q.rk4_classic != p0.rk4_classic)
+            q.rk4_classic != p0.rk4_classic || q.list_stack != p0.list_stack ||
+            q.list_tile != p0.list_tile || q.list_blocks != p0.list_blocks)
             return set_error(OVG_EINVAL, "tangle %d: precision, leaf_size, points_per_wave, "
-                             "group_level and rk4_classic are one launch's and must match "
-                             "tangle 0's", j);
+                             "group_level, rk4_classic and the list launch shape are one "
+                             "launch's and must match tangle 0's", j);
         if (flow_depends_on_time(q))
             return set_error(OVG_EUNSUPPORTED, "tangle %d: a forest's parameter table is "
                              "written once, so its flows must not depend on time", j);
@@ -1740,47 +1748,77 @@ extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
     Triple x = tri(c.x);
     OVG_TRY(build_tree(c, x.v));
     hipEvent_t e0 = take_event(c), e1 = take_event(c);
-    const int ppws[4] = {8, 16, 32, 64}, leaves[4] = {1, 2, 4, 8};
     double best = 1e300;
     int n = 0;
+    // one setting: a warm-up launch, then `repeats` timed ones
+    auto time_row = [&](const ovg_tune_row &set) -> int {
+        c.params.points_per_wave = set.points_per_wave;
+        c.params.leaf_size = set.leaf_size;
+        c.params.group_level = set.group_level;
+        c.params.list_stack = set.list_stack;
+        c.params.list_tile = set.list_tile;
+        c.params.list_blocks = set.list_blocks;
+        push_params(c, 0.0);
+        OVG_TRY(launch_bh_list(c, x.v));
+        unsigned long long cnt0[2] = {0, 0}, cnt1[2] = {0, 0};
+        OVG_HIP(hipMemcpyAsync(cnt0, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < repeats; ++r) OVG_TRY(launch_bh_list(c, x.v));
+        OVG_HIP(hipEventRecord(e1, c.stream));
+        OVG_HIP(hipMemcpyAsync(cnt1, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        ovg_tune_row &row = out->rows[n];
+        row = set;
+        row.ms = ms / repeats;
+        row.pairs = ((double)(cnt1[0] - cnt0[0]) + (double)(cnt1[1] - cnt0[1])) / repeats;
+        if (row.ms < best) {
+            best = row.ms;
+            out->best = n;
+        }
+        ++n;
+        return OVG_OK;
+    };
+    // stage one: group size, bucket size and group level at the given shape
+    const int ppws[4] = {8, 16, 32, 64}, leaves[4] = {1, 2, 4, 8};
     for (int pw : ppws)
         for (int lf : leaves)
             for (int dl = -1; dl <= 1; ++dl) {
                 const int lvl = L + dl;
                 if (lvl < 1 || lvl > 21) continue;
-                c.params.points_per_wave = pw;
-                c.params.leaf_size = lf;
-                c.params.group_level = lvl;
-                push_params(c, 0.0);
-                OVG_TRY(launch_bh_list(c, x.v));  // warm-up
-                unsigned long long cnt0[2] = {0, 0}, cnt1[2] = {0, 0};
-                OVG_HIP(hipMemcpyAsync(cnt0, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
-                OVG_HIP(hipEventRecord(e0, c.stream));
-                for (int r = 0; r < repeats; ++r) OVG_TRY(launch_bh_list(c, x.v));
-                OVG_HIP(hipEventRecord(e1, c.stream));
-                OVG_HIP(hipMemcpyAsync(cnt1, c.counter64.p, 16, hipMemcpyDeviceToHost, c.stream));
-                OVG_HIP(hipStreamSynchronize(c.stream));
-                float ms = 0;
-                hipEventElapsedTime(&ms, e0, e1);
-                ovg_tune_row &row = out->rows[n];
-                row.points_per_wave = pw;
-                row.leaf_size = lf;
-                row.group_level = lvl;
-                row.ms = ms / repeats;
-                row.pairs = ((double)(cnt1[0] - cnt0[0]) + (double)(cnt1[1] - cnt0[1])) / repeats;
-                if (row.ms < best) {
-                    best = row.ms;
-                    out->best = n;
-                }
-                ++n;
+                ovg_tune_row set{};
+                set.points_per_wave = pw;
+                set.leaf_size = lf;
+                set.group_level = lvl;
+                set.list_stack = saved.list_stack;
+                set.list_tile = saved.list_tile;
+                set.list_blocks = saved.list_blocks;
+                OVG_TRY(time_row(set));
+            }
+    // stage two: the launch shape at the best of stage one
+    const ovg_tune_row first = out->rows[out->best];
+    const int stacks[3] = {128, 256, 384}, tiles[2] = {96, 128}, grids[3] = {0, 8, 16};
+    for (int st : stacks)
+        for (int tl : tiles)
+            for (int bl : grids) {
+                ovg_tune_row set = first;
+                set.list_stack = st;
+                set.list_tile = tl;
+                set.list_blocks = bl;
+                if (n < OVG_TUNE_MAX_ROWS) OVG_TRY(time_row(set));
             }
     out->n_rows = n;
     c.ev_pool.push_back(e0);
     c.ev_pool.push_back(e1);
     c.params = saved;
-    c.params.points_per_wave = out->rows[out->best].points_per_wave;
-    c.params.leaf_size = out->rows[out->best].leaf_size;
-    c.params.group_level = out->rows[out->best].group_level;
+    const ovg_tune_row &win = out->rows[out->best];
+    c.params.points_per_wave = win.points_per_wave;
+    c.params.leaf_size = win.leaf_size;
+    c.params.group_level = win.group_level;
+    c.params.list_stack = win.list_stack;
+    c.params.list_tile = win.list_tile;
+    c.params.list_blocks = win.list_blocks;
     push_params(c, 0.0);
     OVG_HIP(hipMemcpyAsync(c.counter64.p, keep, 16, hipMemcpyHostToDevice, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));

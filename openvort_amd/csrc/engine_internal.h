@@ -93,6 +93,7 @@ constexpr int kEvalPlain = 0, kEvalStep = 1, kEvalStage = 2;
 
 struct Ctx {
     int device = 0;
+    int n_cu = 256;  // compute units of the device (persistent list grids)
     hipStream_t stream = nullptr;
     ovg_params params{};
     bool have_params = false;

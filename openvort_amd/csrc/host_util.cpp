@@ -46,6 +46,7 @@ extern "C" const char *ovg_last_error(void) { return ovg::g_last_error.c_str(); 
 extern "C" const char *ovg_version(void) { return "openvort-mi355x 0.1 (gfx950)"; }
 
 extern "C" size_t ovg_params_sizeof(void) { return sizeof(ovg_params); }
+extern "C" size_t ovg_tune_sizeof(void) { return sizeof(ovg_tune); }
 
 extern "C" size_t ovg_stats_sizeof(void) { return sizeof(ovg_stats); }
 

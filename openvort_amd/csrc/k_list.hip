@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     const int *__restrict__ n_groups_p, const TangleParams *__restrict__ Pp,
     const int *__restrict__ roots, double *__restrict__ px, double *__restrict__ py, double *__restrict__ pz,
     double *__restrict__ ix, double *__restrict__ iy, double *__restrict__ iz,
-    unsigned long long *__restrict__ icount)
+    unsigned long long *__restrict__ icount, int stack_cap, int far_tile)
 {
     static_assert(64 % PPW == 0 && 4 * 64 / PPW <= kListFar, "slices of a full group");
     // kMirror: copies may mirror.  A mirrored copy of segment (x, q) is the
@@ -199,11 +199,13 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     }
     const double near2 = P.near2;
     // lane = (slice, target) over the group's ny targets: S = 64 / ny slices
-    // (at most 32, so a flush round of 4 S entries fits the far list), the
-    // lanes from S * ny on idle.  A far flush holds whole rounds of 4 S.
+    // (at most 32 and at most far_tile / 4, so a flush round of 4 S entries
+    // fits the far list), the lanes from S * ny on idle.  A far flush holds
+    // whole rounds of 4 S; far_tile (96 or 128 <= kListFar) leaves room for
+    // a round of 64 entries and the remainder of a flush for every ny.
     const int ny = min(max(grp.y, 1), PPW);
-    const int S = min(64 / ny, 32);
-    const int far_cap = kListFar / (4 * S) * (4 * S);
+    const int S = min(min(64 / ny, 32), far_tile / 4);
+    const int far_cap = far_tile / (4 * S) * (4 * S);
     const bool valid = grp.y > 0 && lane < S * ny;
     const int slice = valid ? lane / ny : 0;
     const int t = grp.x + lane % ny;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (far_n + 64 > far_cap) flush_far(false);
         if (near_n + 64 > kListNear) flush_near();
         // each popped range pushes at most two: keep kStackMargin free
-        const int room = kListStack - kStackMargin - sp;
+        const int room = stack_cap - kStackMargin - sp;
         const int kp = min(min(sp, 64), max(1, room));
         const bool act = lane < kp;
         // every lane reads a range and a node: an idle lane takes the stack's
@@ -699,6 +701,24 @@ int list_points_per_wave(const ovg_params &p)
     return p.points_per_wave == 0 ? 16 : p.points_per_wave;
 }
 
+// The launch shape (validated in ovg_set_params: stack 128..384, tile 96 or
+// 128, grid 0..64 wavefronts per CU).
+static int list_stack_cap(const ovg_params &p)
+{
+    return p.list_stack == 0 ? kListStack : std::min(p.list_stack, kListStack);
+}
+
+static int list_far_tile(const ovg_params &p)
+{
+    return p.list_tile == 96 ? 96 : kListFar;
+}
+
+static int list_grid(const Ctx &c, int groups)
+{
+    const int b = c.params.list_blocks;
+    return b > 0 ? std::max(1, std::min(groups, b * c.n_cu)) : groups;
+}
+
 int launch_bh_list(Ctx &c, const double *const x[3])
 {
     if (c.n_live == 0) return OVG_OK;
@@ -723,18 +743,20 @@ int launch_bh_list(Ctx &c, const double *const x[3])
         }
     }
     // groups are counted on the device; blocks stride over them, so any
-    // count is covered (typically 1.0-1.2 x n / ppw)
+    // count is covered (typically 1.0-1.2 x n / ppw).  list_blocks > 0: a
+    // persistent grid of that many wavefronts per CU.
     const int full = (c.n_live + ppw - 1) / ppw;
-    const int blocks = full + full / 4 + 8;
-    // records: one slot of kRecCap entries for each of the first `blocks`
+    const int slots = full + full / 4 + 8;
+    const int blocks = list_grid(c, slots);
+    // records: one slot of kRecCap entries for each of the first `slots`
     // groups (a group beyond them, or one whose entries do not fit, walks)
     int rec_groups = 0, rec_cap = 0;
     if (mode == kRecord) {
-        OVG_TRY(T.rec.ensure(sizeof(unsigned) * (size_t)blocks * kRecCap));
-        OVG_TRY(T.rec_count.ensure(sizeof(int) * (size_t)blocks));
-        rec_groups = blocks;
+        OVG_TRY(T.rec.ensure(sizeof(unsigned) * (size_t)slots * kRecCap));
+        OVG_TRY(T.rec_count.ensure(sizeof(int) * (size_t)slots));
+        rec_groups = slots;
         rec_cap = kRecCap;
-        c.rec_groups = blocks;
+        c.rec_groups = slots;
     } else if (mode == kReplay) {
         rec_groups = c.rec_groups;
         rec_cap = kRecCap;
@@ -750,7 +772,8 @@ int launch_bh_list(Ctx &c, const double *const x[3])
                            T.counters.as<int>() + kGroupCounter, dparams(c),
                            T.roots.as<int>(), c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
-                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
+                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt,
+                           list_stack_cap(c.params), list_far_tile(c.params));
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
     auto by_mode = [&](auto kw, auto kr, auto kp, auto km) {
@@ -837,7 +860,7 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     OVG_HIP(hipMemcpyAsync(T.sub_count.p, &ng, sizeof(int), hipMemcpyHostToDevice, c.stream));
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ng), dim3(64), 0, c.stream, nullptr, nullptr, 0, 0, d_grp,
+        hipLaunchKernelGGL(kern, dim3(list_grid(c, ng)), dim3(64), 0, c.stream, nullptr, nullptr, 0, 0, d_grp,
                            d_tgt, x[0], x[1],
                            x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                            T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
@@ -845,7 +868,8 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
                            T.sub_count.as<int>(), dparams(c), T.roots.as<int>(),
                            c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
-                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt);
+                           c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt,
+                           list_stack_cap(c.params), list_far_tile(c.params));
     };
     const bool mixed = c.params.precision == OVG_PREC_MIXED;
     bool mirrored = false;

@@ -66,6 +66,7 @@ def test_dropin_exports_reference_prototypes():
 def test_struct_sizes_match():
     assert ov.lib().ovg_params_sizeof() == C.sizeof(ov.Params)
     assert ov.lib().ovg_stats_sizeof() == C.sizeof(ov.Stats)
+    assert ov.lib().ovg_tune_sizeof() == C.sizeof(ov.Tune)
 
 
 def test_defaults_follow_reference_globals():

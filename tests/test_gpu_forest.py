@@ -190,8 +190,9 @@ def test_forest_autotuned_equals_alone():
     say(f"forest autotune pick: {rows[best]}")
     for j, (t, p) in enumerate(ms):
         pj = ov.Params.from_buffer_copy(p)
-        pj.points_per_wave, pj.leaf_size, pj.group_level = (q.points_per_wave, q.leaf_size,
-                                                            q.group_level)
+        for f in ("points_per_wave", "leaf_size", "group_level", "list_stack", "list_tile",
+                  "list_blocks"):
+            setattr(pj, f, getattr(q, f))
         with ov.Engine(pj) as a:
             a.upload(t)
             a.update_tangle(0.0)

@@ -173,6 +173,41 @@ def test_c2_list_traversal(c2, ppw, leaf):
     assert s64.interactions_fp32 == 0
 
 
+@pytest.mark.parametrize("stack,tile,blocks", [(128, 96, 0), (384, 128, 8), (256, 96, 16),
+                                               (192, 128, 1)])
+def test_c2_list_launch_shapes(c2, stack, tile, blocks):
+    """The launch shape the autotuner searches (stack depth, far-field flush
+    tile, persistent grid) changes the walk order and the flush batches,
+    never the set of accepted entries: the same pair count, FP64 results
+    equal to rounding, mixed results within the FP32 far field's rounding,
+    and a persistent grid alone changes nothing at all."""
+    g, t, p = c2
+    say(f"c2 list traversal, stack {stack}, tile {tile}, {blocks} wavefronts per CU")
+    kw = dict(traverse=ov.TRAVERSE_LIST, leaf_size=1, points_per_wave=16)
+    shape = dict(list_stack=stack, list_tile=tile, list_blocks=blocks)
+    live = t.live
+    for prec in (ov.PREC_FP64, ov.PREC_MIXED):
+        o0, s0 = run_c2(t, p, precision=prec, **kw)
+        o1, s1 = run_c2(t, p, precision=prec, **kw, **shape)
+        og, _ = run_c2(t, p, precision=prec, **kw, list_blocks=blocks)
+        assert s1.interactions == s0.interactions
+        assert np.array_equal(og["vs"][live], o0["vs"][live])
+        err = rel_l2(o1["vs"][live], o0["vs"][live])
+        say(f"precision {prec}: shape vs default {err:.3e}")
+        assert err < (1e-13 if prec == ov.PREC_FP64 else 1e-6)
+
+
+def test_list_launch_shape_validation():
+    t, p, _ = configs.make("c2", rings=4, traverse=ov.TRAVERSE_LIST, leaf_size=1)
+    for field, bad in (("list_stack", 100), ("list_stack", 448), ("list_stack", 200),
+                       ("list_tile", 64), ("list_blocks", 65), ("list_blocks", -1)):
+        q = ov.Params.from_buffer_copy(p)
+        setattr(q, field, bad)
+        with pytest.raises(ov.OvgError) as ei:
+            ov.Engine(q)
+        assert ei.value.code == ov.EINVAL, (field, bad)
+
+
 def test_c2_list_small_theta_matches_point(c2):
     """theta -> 0: both walks reduce to the same segment sum (every source
     visited once), so a missing or repeated subtree shows at O(1e-3).  At
@@ -722,8 +757,15 @@ def test_autotune_picks_fastest_and_keeps_results_valid(c2):
     say("autotune of the list traversal on c2")
     with engine(t, q) as e:
         rows, best = e.autotune(repeats=1)
-        assert len(rows) == 48 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
+        assert len(rows) == 48 + 18 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
         assert rows[best]["ms"] == min(r["ms"] for r in rows)
+        # stage two varies the launch shape only: same setting, same pairs
+        first = min(range(48), key=lambda i: rows[i]["ms"])
+        for r in rows[48:]:
+            assert all(r[f] == rows[first][f] for f in ("points_per_wave", "leaf_size",
+                                                        "group_level"))
+            assert r["pairs"] == rows[first]["pairs"]
+        say(f"autotune pick {rows[best]}")
         e.update_tangle(0.0)
         o = e.download()
         s = e.stats()
