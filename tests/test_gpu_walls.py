@@ -212,9 +212,14 @@ def test_pinned_velocity_at_matches_oracle(oracle):
         assert np.abs(vels[i] - wv).max() <= 1e-12 * max(1.0, np.abs(wv).max()), k
 
 
-def test_pinned_small_lines_vs_oracle(oracle):
+@pytest.mark.parametrize("lengths,orders", [
+    ([2, 3, 5, 6, 7, 40, 4], None),
+    ([3, 5, 6, 4, 6, 9, 5], ["mid", "up", "mid", "up", "down", "mid", "up"])])
+def test_pinned_small_lines_vs_oracle(oracle, lengths, orders):
+    """Lines whose lowest slot is their DOWN end, their UP end or mid-line
+    (the pass then counts back from its start point before erasing)."""
     from test_walls import lines_and_rings
-    t, p = lines_and_rings([2, 3, 5, 6, 7, 40, 4])
+    t, p = lines_and_rings(lengths, seed=5 if orders is None else 11, orders=orders)
     say(f"small-loop removal with pinned lines: {len(t.live)} points")
     w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
     want = oracle.eliminate_small_loops(w, 6)

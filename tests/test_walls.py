@@ -247,25 +247,32 @@ def test_mirror_image_orientation_follows_reflection_parity(oracle, faces, rever
     assert rel_l2(o["vs"][:nl], base["vs"][live]) < 1e-12
 
 
-def lines_and_rings(lengths, seed=5, rings=3):
+def lines_and_rings(lengths, seed=5, rings=3, orders=None):
     """Rings plus lines of the given point counts from the DOWN wall to the
-    UP wall, pinned at both ends (parallel-planes geometry)."""
+    UP wall, pinned at both ends (parallel-planes geometry).  orders[i] puts
+    line i's lowest slot at its DOWN end ("down", the default), at its UP
+    end ("up": slots in reverse) or mid-line ("mid": slots rotated)."""
     D, dxi = 0.1, 2e-3
     t = rings_inside(seed, rings, D, 0.012, dxi)
     rng = np.random.default_rng(seed)
     xs, conns, sts = [t.vnodes], [t.conn], [t.status]
     off = t.n
-    for m in lengths:
+    for i, m in enumerate(lengths):
         x0, y0 = rng.uniform(0.02, 0.08, 2)
         z = np.linspace(0.0, D, m)
-        xs.append(np.stack([np.full(m, x0), np.full(m, y0), z], 1))
         j = np.arange(m)
-        conns.append(np.stack([np.where(j < m - 1, off + j + 1, -1),
-                               np.where(j > 0, off + j - 1, -1)], 1))
+        order = orders[i] if orders else "down"
+        # pos[s]: position along the line (0 = DOWN end) held by slot off + s
+        pos = {"down": j, "up": m - 1 - j, "mid": (j + m // 2) % m}[order]
+        slot = np.empty(m, dtype=np.int64)
+        slot[pos] = off + j
+        xs.append(np.stack([np.full(m, x0), np.full(m, y0), z[pos]], 1))
+        conns.append(np.stack([np.where(pos < m - 1, slot[np.minimum(pos + 1, m - 1)], -1),
+                               np.where(pos > 0, slot[np.maximum(pos - 1, 0)], -1)], 1))
         st = np.zeros((m, 2), dtype=np.int32)
         st[:, 1] = -1
-        st[0] = (ov.PINNED, ov.FACE_DOWN)
-        st[-1] = (ov.PINNED_SLIP, ov.FACE_UP)
+        st[pos == 0] = (ov.PINNED, ov.FACE_DOWN)
+        st[pos == m - 1] = (ov.PINNED_SLIP, ov.FACE_UP)
         sts.append(st)
         off += m
     t = ov.Tangle(np.concatenate(xs), np.concatenate(conns).astype(np.int32),
@@ -290,3 +297,20 @@ def test_small_open_lines_removed_as_reference(oracle, ref):
     assert np.array_equal(w.conn, want_conn)
     assert np.array_equal(w.status[:, 0], want_st[:, 0])
     assert removed == 2 + 3 + 5 + 6 + 4  # lines of at most 6 points
+
+
+def test_small_open_lines_started_mid_line_or_far_end(oracle, ref):
+    """The same pass on short lines whose lowest slot is mid-line or at the
+    UP end, so the visit counts back from its start point before erasing
+    forward from the far end (src/tangle.c:1081-1093)."""
+    lengths = [3, 5, 6, 4, 6, 9, 5]
+    orders = ["mid", "up", "mid", "up", "down", "mid", "up"]
+    t, p = lines_and_rings(lengths, seed=11, orders=orders)
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.eliminate_small_loops(6)
+    want_conn, want_st = r.get()["conn"], r.status()
+    w = ov.Tangle(t.vnodes.copy(), t.conn.copy(), t.status.copy())
+    removed = oracle.eliminate_small_loops(w, 6)
+    assert np.array_equal(w.conn, want_conn)
+    assert np.array_equal(w.status[:, 0], want_st[:, 0])
+    assert removed == 3 + 5 + 6 + 4 + 6 + 5  # every line but the 9-point one
