@@ -185,6 +185,10 @@ typedef struct ovg_params {
     int32_t list_blocks;          /* 0: one wavefront per target group; n > 0:
                                      a persistent grid of n wavefronts per CU
                                      (1..64) striding over the groups           */
+    int32_t pin_mode;             /* pin_mode (src/initial_conditions.c:54): the
+                                     status of the nodes wall clipping creates,
+                                     OVG_PINNED or OVG_PINNED_SLIP (0 = PINNED) */
+    int32_t pad_;
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a
@@ -377,6 +381,20 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
  * slots its own edits reach, and sends only the changed slots back (the
  * whole state when the slot count doubled). */
 int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
+
+/* enforce_wall_boundaries, src/tangle.c:966-1016, at time t: for every
+ * solid wall (not OPEN, not PERIODIC) in face order and every FREE point in
+ * slot order within rec_dist of it (check_wall, :822-840), the point's
+ * tangent and direct-sum velocity (update_velocity with tree == NULL, left
+ * in the context as the reference leaves them) and connect_to_wall
+ * (:876-958): unless the point moves away from the wall or the wall
+ * segments would be longer, it and its nearer neighbour are clipped to the
+ * wall through two new nodes of status pin_mode (the reference's allocator,
+ * doubling when full); then every FREE point behind a solid wall is removed.
+ * The parallel test takes abs() of the cosine as an int, as the reference
+ * does (:892), so only |cos| >= 1 counts.  *n_clipped / *n_removed: clips
+ * and removals. */
+int ovg_enforce_walls(ovg_ctx *ctx, double t, int *n_clipped, int *n_removed);
 
 /* smooth, src/vortex_smooth.c:459-462 (low_pass_filter per loop and
  * coordinate, src/vortex_smooth.c:134-175,183-453). */

@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
         ("reuse_lists", C.c_int32), ("image_reflect", C.c_int32 * 26),
         ("list_stack", C.c_int32), ("list_tile", C.c_int32), ("list_blocks", C.c_int32),
+        ("pin_mode", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -132,6 +133,7 @@ _sig("ovg_reconnect", C.c_int, C.c_void_p, C.c_double, _ip)
 _sig("ovg_smooth", C.c_int, C.c_void_p)
 _sig("ovg_eliminate_small_loops", C.c_int, C.c_void_p, C.c_int, _ip)
 _sig("ovg_remesh", C.c_int, C.c_void_p, C.c_double, C.c_double, _ip, _ip)
+_sig("ovg_enforce_walls", C.c_int, C.c_void_p, C.c_double, _ip, _ip)
 _sig("ovg_loop_points", C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip)
 _sig("ovg_inject_loop_draw", C.c_int, C.c_double, C.c_double, _dp, _dp, _P(C.c_ushort), _dp, _dp,
      _dp, _dp, _ip)
@@ -372,6 +374,14 @@ class Engine:
     def remesh(self, min_dist, max_dist):
         a, r = C.c_int32(0), C.c_int32(0)
         _check(_lib.ovg_remesh(self._h, min_dist, max_dist, C.byref(a), C.byref(r)), "ovg_remesh")
+        self.n = _lib.ovg_num_slots(self._h)
+        return a.value, r.value
+
+    def enforce_walls(self, t=0.0):
+        """enforce_wall_boundaries (src/tangle.c:966-1016): returns (clipped,
+        removed)."""
+        a, r = C.c_int32(0), C.c_int32(0)
+        _check(_lib.ovg_enforce_walls(self._h, t, C.byref(a), C.byref(r)), "ovg_enforce_walls")
         self.n = _lib.ovg_num_slots(self._h)
         return a.value, r.value
 

@@ -516,6 +516,8 @@ static int validate_params(const ovg_params *p)
         return set_error(OVG_EINVAL, "list_tile must be 0, 96 or 128");
     if (p->list_blocks < 0 || p->list_blocks > 64)
         return set_error(OVG_EINVAL, "list_blocks must be in [0, 64]");
+    if (p->pin_mode != 0 && p->pin_mode != OVG_PINNED && p->pin_mode != OVG_PINNED_SLIP)
+        return set_error(OVG_EINVAL, "pin_mode must be 0, OVG_PINNED or OVG_PINNED_SLIP");
     if (p->traverse == OVG_TRAVERSE_POINT &&
         (p->leaf_size != 1 || p->precision != OVG_PREC_FP64) && p->method == OVG_METHOD_BH)
         return set_error(OVG_EINVAL,
@@ -1437,6 +1439,40 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
     return launch_status();
 }
 
+// The host doubled the slot count (expand_tangle, src/tangle.c:68-101):
+// reallocate, keep every old slot's vels, vs, tangent and normal (the
+// reference's realloc keeps them; the new slots read zero here), and upload
+// the positions and the topology.
+static int grow_slots(Ctx &c, int n_new, const double *X)
+{
+    const int n_old = c.n;
+    DBuf *f[12] = {&c.vels[0], &c.vels[1], &c.vels[2], &c.vs[0], &c.vs[1], &c.vs[2],
+                   &c.tan[0],  &c.tan[1],  &c.tan[2],  &c.nor[0], &c.nor[1], &c.nor[2]};
+    const size_t b_old = sizeof(double) * (size_t)n_old;
+    DBuf keep;
+    int r = keep.ensure(12 * b_old);
+    for (int i = 0; i < 12 && r == OVG_OK; ++i)
+        if (hipMemcpyAsync(keep.as<unsigned char>() + i * b_old, f[i]->p, b_old,
+                           hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+            r = set_error(OVG_EDEVICE, "state copy failed");
+    if (r == OVG_OK) r = alloc_state(c, n_new);
+    if (r == OVG_OK) {
+        c.n = n_new;
+        const size_t b_new = sizeof(double) * (size_t)n_new;
+        for (int i = 0; i < 12 && r == OVG_OK; ++i)
+            if (hipMemcpyAsync(f[i]->p, keep.as<unsigned char>() + i * b_old, b_old,
+                               hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+                hipMemsetAsync(f[i]->as<unsigned char>() + b_old, 0, b_new - b_old,
+                               c.stream) != hipSuccess)
+                r = set_error(OVG_EDEVICE, "state copy failed");
+    }
+    if (r == OVG_OK) r = upload_positions(c, X);  // synchronises the copies above
+    if (r == OVG_OK) r = upload_topology(c, c.h_conn.data(), c.h_status.data());
+    hipStreamSynchronize(c.stream);
+    keep.release();
+    return r;
+}
+
 // ---- remesh (src/tangle.c:1018-1061) on the host mirrors ----------------
 namespace {
 
@@ -1553,6 +1589,288 @@ int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
 
 }  // namespace
 
+// tangents, normals, vs and vels of m slots from AoS records of 12 doubles
+__global__ void k_scatter_state(int m, const int *__restrict__ slots,
+                                const double *__restrict__ v, double *tx, double *ty, double *tz,
+                                double *nx, double *ny, double *nz, double *sx, double *sy,
+                                double *sz, double *ex, double *ey, double *ez)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int k = slots[i];
+    const double *r = v + 12 * (size_t)i;
+    tx[k] = r[0]; ty[k] = r[1]; tz[k] = r[2];
+    nx[k] = r[3]; ny[k] = r[4]; nz[k] = r[5];
+    sx[k] = r[6]; sy[k] = r[7]; sz[k] = r[8];
+    ex[k] = r[9]; ey[k] = r[10]; ez[k] = r[11];
+}
+
+// Pin walls on the device from the host status mirror, kept while the
+// tangle has pinned nodes.
+static int refresh_pins(Ctx &c)
+{
+    std::vector<int> pw((size_t)c.n);
+    bool any = false;
+    for (int k = 0; k < c.n; ++k) {
+        const int s = c.h_status[2 * (size_t)k];
+        const bool pinned = s == OVG_PINNED || s == OVG_PINNED_SLIP;
+        pw[k] = pinned ? c.h_status[2 * (size_t)k + 1] : -1;
+        any |= pinned;
+    }
+    c.has_pinned = any;
+    if (!any) {
+        c.pinw.release();
+        return OVG_OK;
+    }
+    c.loops_full_scan = true;
+    OVG_TRY(c.pinw.ensure(sizeof(int) * (size_t)c.n));
+    OVG_HIP(hipMemcpyAsync(c.pinw.p, pw.data(), sizeof(int) * (size_t)c.n, hipMemcpyHostToDevice,
+                           c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}
+
+// enforce_wall_boundaries, src/tangle.c:966-1016, on the host mirrors.
+// Wall by wall (solid ones: not OPEN, not PERIODIC) and in slot order, every
+// FREE point within rec_dist of the wall (check_wall, :822-840) that no clip
+// of this pass has flagged gets its tangent and its direct-sum velocity
+// (update_tangent_normal, update_velocity with tree == NULL) and is clipped
+// to the wall (connect_to_wall, :876-958) unless it moves away from the wall,
+// runs parallel to it, or the two new wall segments would be longer; then
+// every FREE point behind a solid wall is removed (:996-1015).
+// A clip moves no existing point: the velocities are the direct sums at the
+// pass's start, computed for every candidate in one launch, plus the exact
+// change (all images, mirrors unreflected as the direct path has them) of
+// each segment the clips so far replaced or added.
+static int walls_pass(Ctx &c, double t, int *n_clip, int *n_rem)
+{
+    *n_clip = *n_rem = 0;
+    const ovg_params &pr = c.params;
+    bool solid[6], any = false;
+    for (int f = 0; f < 6; ++f) {
+        solid[f] = pr.wall[f] != OVG_WALL_OPEN && pr.wall[f] != OVG_WALL_PERIODIC;
+        any |= solid[f];
+    }
+    if (!any || c.n_live == 0) return OVG_OK;
+    const double *Xp = nullptr;
+    OVG_TRY(host_positions(c, &Xp));
+    HostTangle h(c.n, Xp, c.h_conn, c.h_status, c.next_empty);
+    const TangleParams P = host_tparams(pr, t);
+    const double rd = pr.rec_dist;
+    auto free_at = [&](int k) { return h.st[2 * (size_t)k] == OVG_FREE; };
+    // wall_dist, :848-866 (axis of face w: w / 2; LEFT, BACK, DOWN even)
+    auto wall_dist = [&](int k, int w) {
+        const int a = w >> 1;
+        const double x = h.X[3 * (size_t)k + a];
+        return (w & 1) ? pr.box_hi[a] - x : x - pr.box_lo[a];
+    };
+    // check_wall, :822-840
+    auto check_wall = [&](int k, int w) {
+        const int a = w >> 1;
+        const double x = h.X[3 * (size_t)k + a];
+        return (w & 1) ? x > pr.box_hi[a] - rd : x < rd + pr.box_lo[a];
+    };
+    std::vector<int> cand;
+    for (int k = 0; k < c.n; ++k) {
+        if (!free_at(k)) continue;
+        bool near = false;
+        for (int w = 0; w < 6 && !near; ++w) near = solid[w] && check_wall(k, w);
+        if (near) cand.push_back(k);
+    }
+    std::vector<double> bsp, bsi;
+    std::vector<int> where((size_t)c.n, -1);
+    const int m = (int)cand.size();
+    if (m) {
+        for (int i = 0; i < m; ++i) where[cand[i]] = i;
+        push_params(c, t);
+        Triple x = tri(c.x);
+        launch_prep_segments(c, x.v);
+        OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)m));
+        OVG_HIP(hipMemcpyAsync(c.rc_tmp.p, cand.data(), sizeof(int) * (size_t)m,
+                               hipMemcpyHostToDevice, c.stream));
+        Triple bp = tri(c.bs_prim), bi = tri(c.bs_img);
+        const ovg_params saved = c.params;
+        c.params.precision = OVG_PREC_FP64;
+        const int r = launch_direct(c, x.v, m, c.rc_tmp.as<int>(), bp.v, bi.v, 0);
+        c.params = saved;
+        OVG_TRY(r);
+        OVG_TRY(c.partial.ensure(sizeof(double) * 3 * (size_t)m));
+        bsp.resize(3 * (size_t)m);
+        bsi.resize(3 * (size_t)m);
+        for (int o = 0; o < 2; ++o) {
+            const Triple &src = o == 0 ? bp : bi;
+            hipLaunchKernelGGL(k_gather_aos, dim3((m + 255) / 256), dim3(256), 0, c.stream, m,
+                               c.rc_tmp.as<int>(), src.v[0], src.v[1], src.v[2],
+                               c.partial.as<double>());
+            OVG_HIP(hipMemcpyAsync((o == 0 ? bsp : bsi).data(), c.partial.p,
+                                   sizeof(double) * 3 * (size_t)m, hipMemcpyDeviceToHost,
+                                   c.stream));
+            OVG_HIP(hipStreamSynchronize(c.stream));
+        }
+    }
+    struct Change {
+        int seg, old_f, new_f;
+    };
+    std::vector<Change> changes;
+    std::vector<int> recalc((size_t)c.n, 0);
+    std::vector<int> visited, vis_at((size_t)c.n, -1);  // slots update_velocity wrote
+    std::vector<double> state;          // their tangent, normal, vs, vels
+    std::vector<int> touched, inserted;
+    const int pin = pr.pin_mode ? pr.pin_mode : OVG_PINNED;
+    const double cut = std::cos(pr.rec_angle);
+    auto connect = [&](int k, int w) -> bool {
+        double tk[3], nk[3], vs[3], vels[3], bs[3];
+        host::tangent_normal(P, h.X, h.conn.data(), k, tk, nk, h.st.data());
+        const int e = where[k];
+        for (int a = 0; a < 3; ++a) bs[a] = bsp[3 * (size_t)e + a] + bsi[3 * (size_t)e + a];
+        for (const Change &ch : changes) {
+            double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+            if (ch.old_f >= 0) host::segment_all_images(P, h.X, ch.seg, ch.old_f, k, vo, false);
+            if (ch.new_f >= 0) host::segment_all_images(P, h.X, ch.seg, ch.new_f, k, vn, false);
+            for (int a = 0; a < 3; ++a) bs[a] += vn[a] - vo[a];
+        }
+        host::compose(P, h.X, h.conn.data(), k, tk, nk, bs, vs, vels, h.st.data());
+        // the last visit of a point (a corner point may be tested for two
+        // walls) leaves its state
+        if (vis_at[k] < 0) {
+            vis_at[k] = (int)visited.size();
+            visited.push_back(k);
+            state.resize(state.size() + 12);
+        }
+        double *rec = &state[12 * (size_t)vis_at[k]];
+        for (const double *v : {tk, nk, vs, vels})
+            for (int a = 0; a < 3; ++a) *rec++ = v[a];
+        double nw[3];
+        host::wall_normal(w, nw);
+        const double d0 = wall_dist(k, w);
+        if (d0 > 0 && host::dot3(vels, nw) > 0) return false;  // moving away
+        // parallel to the wall: the reference takes abs() of the cosine, an
+        // int, so only |cos| >= 1 counts (:892)
+        const double nd = host::dot3(tk, nw) / host::len3(tk) / host::len3(nw);
+        if (std::abs((int)nd) > cut) return false;
+        const int next = h.conn[2 * (size_t)k], prev = h.conn[2 * (size_t)k + 1];
+        const double d1 = wall_dist(next, w), dm1 = wall_dist(prev, w);
+        auto seg_len = [&](int a, int b) {
+            double q[3] = {h.X[3 * (size_t)b], h.X[3 * (size_t)b + 1], h.X[3 * (size_t)b + 2]};
+            host::pwrap(P, &h.X[3 * (size_t)a], q);
+            return host::dist3(&h.X[3 * (size_t)a], q);
+        };
+        const double vd1 = seg_len(k, next), vdm1 = seg_len(k, prev);
+        if (vd1 < d0 + d1 && vdm1 < d0 + dm1) return false;  // total length would grow
+        const int p1 = h.next_empty_node();
+        h.st[2 * (size_t)p1] = pin;
+        h.st[2 * (size_t)p1 + 1] = w;
+        const int p2 = h.next_empty_node();
+        h.st[2 * (size_t)p2] = pin;
+        h.st[2 * (size_t)p2 + 1] = w;
+        if ((int)recalc.size() < h.N) recalc.resize((size_t)h.N, 0);
+        for (int a = 0; a < 3; ++a)
+            h.X[3 * (size_t)p1 + a] = h.X[3 * (size_t)k + a] + nw[a] * -d0;
+        for (int sl : {k, p1, p2, next, prev}) recalc[sl]++;
+        if (d1 < dm1) {
+            for (int a = 0; a < 3; ++a)
+                h.X[3 * (size_t)p2 + a] = h.X[3 * (size_t)next + a] + nw[a] * -d1;
+            h.conn[2 * (size_t)k] = p1;
+            h.conn[2 * (size_t)p1] = -1;
+            h.conn[2 * (size_t)p1 + 1] = k;
+            h.conn[2 * (size_t)p2] = next;
+            h.conn[2 * (size_t)p2 + 1] = -1;
+            h.conn[2 * (size_t)next + 1] = p2;
+            changes.push_back({k, next, p1});
+            changes.push_back({p2, -1, next});
+        } else {
+            for (int a = 0; a < 3; ++a)
+                h.X[3 * (size_t)p2 + a] = h.X[3 * (size_t)prev + a] + nw[a] * -dm1;
+            h.conn[2 * (size_t)k + 1] = p1;
+            h.conn[2 * (size_t)p1] = k;
+            h.conn[2 * (size_t)p1 + 1] = -1;
+            h.conn[2 * (size_t)p2] = -1;
+            h.conn[2 * (size_t)p2 + 1] = prev;
+            h.conn[2 * (size_t)prev] = p2;
+            changes.push_back({prev, k, p2});
+            changes.push_back({p1, -1, k});
+        }
+        for (int sl : {k, next, prev, p1, p2}) touched.push_back(sl);
+        inserted.push_back(p1);
+        inserted.push_back(p2);
+        return true;
+    };
+    int clips = 0, removed = 0;
+    for (int w = 0; w < 6; ++w) {
+        if (!solid[w]) continue;
+        for (int k : cand) {
+            if (!free_at(k) || recalc[k]) continue;
+            if (check_wall(k, w) && connect(k, w)) ++clips;
+        }
+    }
+    // remove_point (:342-355) for every FREE point behind a solid wall
+    for (int w = 0; w < 6; ++w) {
+        if (!solid[w]) continue;
+        for (int k = 0; k < h.N; ++k) {
+            if (!free_at(k) || !(wall_dist(k, w) < 0)) continue;
+            const int pv = h.conn[2 * (size_t)k + 1], nx = h.conn[2 * (size_t)k];
+            if (pv >= 0) h.conn[2 * (size_t)pv] = nx;
+            if (nx >= 0) h.conn[2 * (size_t)nx + 1] = pv;
+            h.conn[2 * (size_t)k] = h.conn[2 * (size_t)k + 1] = -1;
+            h.st[2 * (size_t)k] = OVG_EMPTY;
+            for (int sl : {k, pv, nx})
+                if (sl >= 0) touched.push_back(sl);
+            ++removed;
+        }
+    }
+    c.next_empty = h.next_empty;
+    *n_clip = clips;
+    *n_rem = removed;
+    if (clips || removed) {
+        if (h.grown) {
+            OVG_TRY(grow_slots(c, h.N, h.X));
+        } else {
+            for (int sl : touched) mark_dirty(c, sl);
+            for (int nw : inserted) {
+                if (c.h_status[2 * (size_t)nw] == OVG_EMPTY) continue;
+                c.new_pos.push_back(nw);
+                for (int q = 0; q < 3; ++q) c.new_xyz.push_back(h.X[3 * (size_t)nw + q]);
+            }
+            OVG_TRY(commit_topology(c));
+        }
+        OVG_TRY(refresh_pins(c));
+    }
+    // the state update_velocity left at every visited point
+    const int nv = (int)visited.size();
+    if (nv) {
+        OVG_TRY(c.topo_stage.ensure(sizeof(int) * (size_t)nv + sizeof(double) * 12 * (size_t)nv +
+                                    256));
+        unsigned char *d = c.topo_stage.as<unsigned char>();
+        const size_t off = ((sizeof(int) * (size_t)nv) + 255) & ~(size_t)255;
+        OVG_HIP(hipMemcpyAsync(d, visited.data(), sizeof(int) * (size_t)nv, hipMemcpyHostToDevice,
+                               c.stream));
+        OVG_HIP(hipMemcpyAsync(d + off, state.data(), sizeof(double) * 12 * (size_t)nv,
+                               hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_scatter_state, dim3((nv + 255) / 256), dim3(256), 0, c.stream, nv,
+                           reinterpret_cast<const int *>(d),
+                           reinterpret_cast<const double *>(d + off), dp(c.tan[0]), dp(c.tan[1]),
+                           dp(c.tan[2]), dp(c.nor[0]), dp(c.nor[1]), dp(c.nor[2]), dp(c.vs[0]),
+                           dp(c.vs[1]), dp(c.vs[2]), dp(c.vels[0]), dp(c.vels[1]),
+                           dp(c.vels[2]));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+    }
+    return launch_status();
+}
+
+extern "C" int ovg_enforce_walls(ovg_ctx *ctx, double t, int *n_clipped, int *n_removed)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    int a = 0, b = 0;
+    const int r = walls_pass(c, t, &a, &b);
+    if (n_clipped) *n_clipped = a;
+    if (n_removed) *n_removed = b;
+    return r;
+}
+
 // The reference visits every slot in index order (src/tangle.c:1021).  A
 // slot whose links are unchanged when the loop reaches it decides exactly as
 // it would with the links before the pass (positions of existing points do
@@ -1643,10 +1961,7 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     if (!added && !removed) return OVG_OK;
     if (h.grown) {
         // the tangle doubled: reallocate and upload the whole state
-        OVG_TRY(alloc_state(c, h.N));
-        c.n = h.N;
-        OVG_TRY(upload_positions(c, h.X));
-        OVG_TRY(upload_topology(c, c.h_conn.data(), c.h_status.data()));
+        OVG_TRY(grow_slots(c, h.N, h.X));
     } else {
         for (int sl : touched) mark_dirty(c, sl);
         for (int nw : inserted) {
@@ -1709,10 +2024,7 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
     h.conn[2 * (size_t)first + 1] = cur;
     c.next_empty = h.next_empty;
     if (h.grown) {
-        OVG_TRY(alloc_state(c, h.N));
-        c.n = h.N;
-        OVG_TRY(upload_positions(c, h.X));
-        OVG_TRY(upload_topology(c, c.h_conn.data(), c.h_status.data()));
+        OVG_TRY(grow_slots(c, h.N, h.X));
     } else {
         for (int sl : slots) {
             mark_dirty(c, sl);

@@ -82,13 +82,54 @@ inline void segment_all_images(const TangleParams &P, const double *X, int m, in
     }
 }
 
-// update_tangent_normal, src/tangle.c:423-502, with FREE links
-inline void tangent_normal(const TangleParams &P, const double *X, const int *conn, int k,
-                           double *t, double *nrm)
+// step_node, src/tangle.c:155-194, as a position: `where` links from k on
+// the host mirrors (st: status, pin_wall per slot).  A pinned node at its
+// open end walks the other way instead and the result is mirrored through
+// its wall (mirror_shift, src/boundaries.c:159-193), innermost mirror
+// first; the device's step_pos in the same steps.
+inline void step_pos(const TangleParams &P, const double *X, const int *conn, const int *st,
+                     int k, int where, double *o)
 {
-    int i1 = conn[2 * k], i2 = conn[2 * i1], im1 = conn[2 * k + 1], im2 = conn[2 * im1 + 1];
-    const double *s0 = X + 3 * (size_t)k, *s1 = X + 3 * (size_t)i1, *s2 = X + 3 * (size_t)i2;
-    const double *m1 = X + 3 * (size_t)im1, *m2 = X + 3 * (size_t)im2;
+    int walls[4], nw = 0, i = k;
+    for (int guard = 0; where != 0 && guard < 8; ++guard) {
+        const int s = st[2 * i];
+        const int nx = where > 0 ? conn[2 * i] : conn[2 * i + 1];
+        if ((s == OVG_PINNED || s == OVG_PINNED_SLIP) && nx < 0) {
+            if (nw < 4) walls[nw++] = st[2 * i + 1];
+            where = -where;
+            continue;
+        }
+        i = nx;
+        where += where > 0 ? -1 : 1;
+    }
+    for (int a = 0; a < 3; ++a) o[a] = X[3 * (size_t)i + a];
+    if (where != 0) o[0] = o[1] = o[2] = std::nan("");
+    for (int j = nw - 1; j >= 0; --j) {
+        const int w = walls[j], a = w >> 1;
+        const double wall = (w & 1) ? P.hi[a] : P.lo[a];
+        o[a] = 2 * wall - o[a];
+    }
+}
+
+// update_tangent_normal, src/tangle.c:423-502; st == nullptr: FREE links
+// only (every neighbour a link away), else pinned ends as step_node has them
+inline void tangent_normal(const TangleParams &P, const double *X, const int *conn, int k,
+                           double *t, double *nrm, const int *st = nullptr)
+{
+    double q[4][3];
+    const double *s1, *s2, *m1, *m2;
+    if (st) {
+        step_pos(P, X, conn, st, k, 1, q[0]);
+        step_pos(P, X, conn, st, k, 2, q[1]);
+        step_pos(P, X, conn, st, k, -1, q[2]);
+        step_pos(P, X, conn, st, k, -2, q[3]);
+        s1 = q[0]; s2 = q[1]; m1 = q[2]; m2 = q[3];
+    } else {
+        int i1 = conn[2 * k], i2 = conn[2 * i1], im1 = conn[2 * k + 1], im2 = conn[2 * im1 + 1];
+        s1 = X + 3 * (size_t)i1; s2 = X + 3 * (size_t)i2;
+        m1 = X + 3 * (size_t)im1; m2 = X + 3 * (size_t)im2;
+    }
+    const double *s0 = X + 3 * (size_t)k;
     double w[4][3];
     const double *ends[4] = {s2, s1, m1, m2};
     for (int j = 0; j < 4; ++j) {
@@ -123,15 +164,25 @@ inline void tangent_normal(const TangleParams &P, const double *X, const int *co
     }
 }
 
-// update_velocity composition (src/tangle.c:617-716) given the Biot-Savart
-// sum bs at slot k.
+// update_velocity composition (src/tangle.c:600-726) given the Biot-Savart
+// sum bs at slot k; st == nullptr: FREE links only, else the neighbours of
+// step_node and the pinned cases (:608-615, :718-725).
 inline void compose(const TangleParams &P, const double *X, const int *conn, int k,
                     const double *t, const double *nr, const double *bs, double *vs,
-                    double *vels)
+                    double *vels, const int *st = nullptr)
 {
     const double *p = X + 3 * (size_t)k;
-    const double *next = X + 3 * (size_t)conn[2 * k];
-    const double *prev = X + 3 * (size_t)conn[2 * k + 1];
+    double qn[3], qp[3];
+    const double *next, *prev;
+    if (st) {
+        step_pos(P, X, conn, st, k, 1, qn);
+        step_pos(P, X, conn, st, k, -1, qp);
+        next = qn;
+        prev = qp;
+    } else {
+        next = X + 3 * (size_t)conn[2 * k];
+        prev = X + 3 * (size_t)conn[2 * k + 1];
+    }
     double wf[3] = {next[0], next[1], next[2]}, wr[3] = {p[0], p[1], p[2]};
     pwrap(P, p, wf);
     pwrap(P, prev, wr);
@@ -151,6 +202,29 @@ inline void compose(const TangleParams &P, const double *X, const int *conn, int
         cross3(nn, t, m);
         for (int a = 0; a < 3; ++a) vels[a] = vels[a] + nn[a] * -P.alpha_p;
     }
+    const int sk = st ? st[2 * k] : OVG_FREE;
+    if (sk == OVG_PINNED) {
+        // no self-induced motion, the boundary's velocity
+        for (int a = 0; a < 3; ++a) {
+            vs[a] = 0;
+            vels[a] = P.evb[a];
+        }
+    } else if (sk == OVG_PINNED_SLIP) {
+        // the velocity through the wall removed; boundary_normals
+        // (src/boundaries.c:11-18) are +-e_axis, inward
+        const int w = st[2 * k + 1], ax = w >> 1;
+        double n3[3] = {0, 0, 0};
+        n3[ax] = (w & 1) ? -1.0 : 1.0;
+        const double nv = dot3(n3, vels);
+        for (int a = 0; a < 3; ++a) vels[a] = vels[a] - n3[a] * nv;
+    }
+}
+
+// Inward unit normal of a wall (boundary_normals, src/boundaries.c:11-18).
+inline void wall_normal(int w, double *n)
+{
+    n[0] = n[1] = n[2] = 0;
+    n[w >> 1] = (w & 1) ? -1.0 : 1.0;
 }
 
 }  // namespace host

@@ -189,6 +189,7 @@ class RefLib:
         L.ref_bh_field.argtypes = [vp, C.c_int, _dp, C.c_double, _dp]
         L.ref_eliminate_small_loops.argtypes = [vp, C.c_int]
         L.ref_remesh.argtypes = [vp, C.c_double, C.c_double]
+        L.ref_enforce_walls.argtypes = [vp, C.c_double, C.c_int]
         L.ref_num_slots.argtypes = [vp]
         L.ref_save_tangle.argtypes = [vp, C.c_char_p]
         L.ref_load_tangle.argtypes = [C.c_char_p, C.c_void_p]
@@ -274,6 +275,12 @@ class RefTangle:
 
     def remesh(self, min_dist, max_dist):
         self.ref.lib.ref_remesh(self.h, min_dist, max_dist)
+        self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def enforce_walls(self, time=0.0, pin_mode=1):
+        """enforce_wall_boundaries (src/tangle.c:966-1016); pin_mode: status
+        of the new wall nodes."""
+        self.ref.lib.ref_enforce_walls(self.h, time, pin_mode)
         self.n = self.ref.lib.ref_num_slots(self.h)
 
     def add_loop(self, center, direction, r, dl_min, dl_max):

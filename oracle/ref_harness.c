@@ -232,6 +232,14 @@ void ref_remesh(void *h, double min_dist, double max_dist)
     remesh((struct tangle_state *)h, min_dist, max_dist, 0.0);
 }
 
+/* enforce_wall_boundaries (src/tangle.c:966-1016) at time t, new wall nodes
+ * of status pin (the pin_mode global, src/initial_conditions.c:54) */
+void ref_enforce_walls(void *h, double t, int pin)
+{
+    pin_mode = pin;
+    enforce_wall_boundaries((struct tangle_state *)h, t);
+}
+
 int ref_num_slots(void *h) { return ((struct tangle_state *)h)->N; }
 
 /* save_tangle / load_tangle, src/utils.c:132-199; the load goes into a

@@ -282,3 +282,52 @@ def test_wall_list_truncation_vs_exact_sum(config, leaf):
     assert e_point > 1e-6
     assert e_list <= e_point
     assert e_mixed <= e_point
+
+
+@pytest.mark.parametrize("pin", [ov.PINNED, ov.PINNED_SLIP])
+def test_wall_clipping_vs_reference(ref, pin):
+    """ovg_enforce_walls against the compiled reference's
+    enforce_wall_boundaries (src/tangle.c:966-1016) on rings crossing and
+    approaching the two walls of the parallel-planes geometry: the same
+    clips (new pinned nodes in the same slots, through a slot doubling), the
+    same removals, links, statuses and positions exactly, the velocities the
+    pass leaves at the points it tested within 1e-10, and the evaluation of
+    the clipped tangle within 1e-10 of the reference's."""
+    from test_walls import rings_at_walls
+    t, p = rings_at_walls()
+    p.pin_mode = pin
+    say(f"wall clipping, pin mode {pin}: {len(t.live)} points in {t.n} slots")
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.enforce_walls(0.0, pin)
+    want, want_st = r.get(), r.status()
+    with ov.Engine(p) as e:
+        e.upload(t)
+        clipped, removed = e.enforce_walls(0.0)
+        assert e.n == r.n
+        got = e.download()
+        conn, st = e.download_connections()
+        e.update_tangle(0.0)
+        ev = e.download()
+    say(f"clipped {clipped}, removed {removed}, slots {e.n}")
+    assert clipped > 0 and removed > 0
+    assert np.array_equal(conn, want["conn"])
+    assert np.array_equal(st, want_st)
+    live = st[:, 0] != ov.EMPTY
+    assert np.array_equal(got["vnodes"][live], want["vnodes"][live])
+    # the state the pass wrote, at the points that existed before it (the
+    # reference's new slots come from realloc and hold no values)
+    old = np.zeros(len(live), dtype=bool)
+    old[t.live] = True
+    old &= live
+    for key in ("vels", "vs", "tangents", "normals"):
+        scale = np.abs(want[key][old]).max()
+        assert scale > 0, key
+        assert np.abs(got[key][old] - want[key][old]).max() <= 1e-10 * scale, key
+    r.update_tangle(0.0)
+    w2 = r.get()
+    # two-node lines clipped onto one wall point have no stencil (0/0) in
+    # the reference as here: the same NaN rows
+    for key in ("vels", "tangents"):
+        fin = np.isfinite(w2[key][live]).all(1)
+        assert np.array_equal(fin, np.isfinite(ev[key][live]).all(1)), key
+        assert rel_l2(ev[key][live][fin], w2[key][live][fin]) < 1e-10, key

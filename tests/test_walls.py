@@ -48,6 +48,62 @@ def rings_inside(seed, n, D, R, dxi):
     return ov.Tangle(x, np.concatenate(conns).astype(np.int32), st)
 
 
+def rings_at_walls(seed=3, n_cross=6, n_inside=3, D=0.1, R=0.012, dxi=2e-3, spare=16):
+    """Rings inside the box plus rings whose centres lie within 0.9 R of the
+    DOWN or UP plane (so they cross it or come within rec_dist of it), and
+    `spare` EMPTY slots after them (the allocator's first choices)."""
+    rng = np.random.default_rng(seed)
+    t = rings_inside(seed, n_inside, D, R, dxi)
+    xs, conns = [t.vnodes], [t.conn]
+    off = t.n
+    for i in range(n_cross):
+        z = rng.uniform(-0.3 * R, 0.9 * R)
+        c = np.array([rng.uniform(0, D), rng.uniform(0, D), z if i % 2 == 0 else D - z])
+        nrm = rng.normal(size=3)
+        nrm[2] *= 0.3  # mostly upright rings, which cut the plane
+        nrm /= np.linalg.norm(nrm)
+        u = np.cross(nrm, [1.0, 0, 0] if abs(nrm[0]) < 0.9 else [0, 1.0, 0])
+        u /= np.linalg.norm(u)
+        v = np.cross(nrm, u)
+        m = int(2 * np.pi * R / dxi)
+        ph = 2 * np.pi * np.arange(m) / m
+        xs.append(c + R * (np.cos(ph)[:, None] * u + np.sin(ph)[:, None] * v))
+        conns.append(np.stack([off + (np.arange(m) + 1) % m, off + (np.arange(m) - 1) % m], 1))
+        off += m
+    xs.append(np.zeros((spare, 3)))
+    conns.append(np.full((spare, 2), -1))
+    st = np.zeros((off + spare, 2), dtype=np.int32)
+    st[:, 1] = -1
+    st[off:, 0] = ov.EMPTY
+    t = ov.Tangle(np.concatenate(xs), np.concatenate(conns).astype(np.int32), st)
+    p = configs.params(D, dxi, configs.T16, 0.5, method=ov.METHOD_DIRECT)
+    for f, w in enumerate(WALLS[ov.IMAGES_WALL_PARALLEL_PLANES]):
+        p.wall[f] = w
+    ov.set_wall_images(p, ov.IMAGES_WALL_PARALLEL_PLANES)
+    return t, p
+
+
+def test_reference_wall_clipping_scenario(ref):
+    """The scenario of the engine's wall-clipping parity test clips and
+    removes in the compiled reference (enforce_wall_boundaries,
+    src/tangle.c:966-1016), for both pin modes: new nodes pinned to the DOWN
+    and UP walls, open lines ending on them, nothing left behind a wall."""
+    t, p = rings_at_walls()
+    for pin in (ov.PINNED, ov.PINNED_SLIP):
+        r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+        r.enforce_walls(0.0, pin)
+        st, g = r.status(), r.get()
+        live = st[:, 0] != ov.EMPTY
+        pinned = st[:, 0] == pin
+        assert pinned.sum() >= 4 and pinned.sum() % 2 == 0
+        assert set(st[pinned, 1]) <= {ov.FACE_DOWN, ov.FACE_UP}
+        z = g["vnodes"][live, 2]
+        assert (z >= 0).all() and (z <= p.box_hi[2]).all()
+        # every pinned node ends a line: exactly one of its links is -1
+        c = g["conn"][pinned]
+        assert ((c[:, 0] < 0) ^ (c[:, 1] < 0)).all()
+
+
 def wall_case(config, method, seed=7, n=6, theta=0.4):
     D, dxi = 0.1, 2e-3
     t = rings_inside(seed, n, D, 0.012, dxi)
