@@ -372,8 +372,11 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
 /* remesh, src/tangle.c:1018-1061: remove a point whose forward or reverse
  * segment is shorter than min_dist when both together are shorter than
  * max_dist; insert a point on the fitted circle (add_point, :261-335) after
- * every point whose forward segment is longer than max_dist; wrap into the
- * box if points were added.  New points take the slots the reference's
+ * every point whose forward segment is longer than max_dist; if points were
+ * added, wrap into the box and clip to the walls (ovg_enforce_walls at time
+ * 0, as the reference driver calls remesh, src/main.c:131).  Lines pinned
+ * to walls are taken: an open end is never removed and its stencil is
+ * step_node's (src/tangle.c:155-194).  New points take the slots the reference's
  * allocator would (src/tangle.c:201-253), doubling the slot count when full
  * (expand_tangle, :68-101); ovg_num_slots reports the new count.  The
  * device flags the slots whose test (with the links before the pass) can

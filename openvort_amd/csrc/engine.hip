@@ -1545,8 +1545,9 @@ struct HostTangle {
     }
 };
 
-// add_point, src/tangle.c:261-335
-int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
+// add_point, src/tangle.c:261-335 (st: status mirror when the tangle has
+// pinned nodes, whose stencils step_node takes through the wall)
+int add_point(HostTangle &h, const ovg::TangleParams &P, int p, const int *st)
 {
     using namespace ovg::host;
     int nw = h.next_empty_node();
@@ -1554,8 +1555,8 @@ int add_point(HostTangle &h, const ovg::TangleParams &P, int p)
     h.st[2 * (size_t)nw + 1] = -1;
     int next = h.conn[2 * (size_t)p];
     double t0[3], n0[3], t1[3], n1[3];
-    tangent_normal(P, h.X, h.conn.data(), p, t0, n0);
-    tangent_normal(P, h.X, h.conn.data(), next, t1, n1);
+    tangent_normal(P, h.X, h.conn.data(), p, t0, n0, st);
+    tangent_normal(P, h.X, h.conn.data(), next, t1, n1, st);
     double s0[3] = {h.X[3 * (size_t)p], h.X[3 * (size_t)p + 1], h.X[3 * (size_t)p + 2]};
     double s1[3] = {h.X[3 * (size_t)next], h.X[3 * (size_t)next + 1], h.X[3 * (size_t)next + 2]};
     pwrap(P, s0, s1);
@@ -1884,9 +1885,6 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     clear_error();
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
-                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_added) *n_added = 0;
@@ -1948,7 +1946,7 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
         if (next >= 0 && lf > max_dist) {
             ++added;
             const int nxt = h.conn[2 * (size_t)k];
-            const int nw = add_point(h, P, k);
+            const int nw = add_point(h, P, k, c.has_pinned ? h.st.data() : nullptr);
             inserted.push_back(nw);
             touch(nw);
             touch(k);
@@ -1972,8 +1970,13 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
         OVG_TRY(commit_topology(c));
     }
     if (added) {
-        launch_wrap(c);  // enforce_periodic_boundaries after insertions (:1057-1060)
+        // enforce_periodic_boundaries and enforce_wall_boundaries after
+        // insertions (:1057-1060), the latter at time 0 as the reference
+        // driver calls remesh (src/main.c:131)
+        launch_wrap(c);
         OVG_HIP(hipStreamSynchronize(c.stream));
+        int clipped = 0, behind = 0;
+        OVG_TRY(walls_pass(c, 0.0, &clipped, &behind));
     }
     return launch_status();
 }

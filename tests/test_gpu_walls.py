@@ -183,10 +183,10 @@ def test_pinned_lines_refuse_topology_passes():
     from test_walls import pinned_case
     t, p = pinned_case(ov.METHOD_BH, False)
     p.traverse = ov.TRAVERSE_GROUP
-    say("pinned lines: reconnect / remesh / smooth refused")
+    say("pinned lines: reconnect / smooth refused")
     with ov.Engine(p) as e:
         e.upload(t)
-        for call in (lambda: e.reconnect(0.0), lambda: e.remesh(1e-3, 3e-3), e.smooth):
+        for call in (lambda: e.reconnect(0.0), e.smooth):
             with pytest.raises(ov.OvgError) as ei:
                 call()
             assert ei.value.code == ov.EUNSUPPORTED
@@ -331,3 +331,38 @@ def test_wall_clipping_vs_reference(ref, pin):
         fin = np.isfinite(w2[key][live]).all(1)
         assert np.array_equal(fin, np.isfinite(ev[key][live]).all(1)), key
         assert rel_l2(ev[key][live][fin], w2[key][live][fin]) < 1e-10, key
+
+
+@pytest.mark.parametrize("case", ["lines", "clipped"])
+def test_remesh_pinned_vs_reference(ref, case):
+    """ovg_remesh on tangles with lines pinned to the walls against the
+    compiled reference's remesh (src/tangle.c:1018-1061): lines from wall to
+    wall whose segments are split (and the new points split again later in
+    the same pass), and a tangle just clipped to the walls, whose short wall
+    segments are removed and whose insertions are followed by the wall
+    clipping remesh runs after them.  Links, statuses and positions exact."""
+    from test_walls import lines_and_rings, rings_at_walls
+    if case == "lines":
+        t, p = lines_and_rings([2, 3, 5, 6, 7, 40, 4])
+        bounds = (1.5e-3, 2.4e-3)
+    else:
+        t, p = rings_at_walls()
+        bounds = (1.0e-3, 2.03e-3)
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        if case == "clipped":
+            r.enforce_walls(0.0, ov.PINNED)
+            e.enforce_walls(0.0)
+        added, removed = e.remesh(*bounds)
+        conn, st = e.download_connections()
+        got = e.download()
+    r.remesh(*bounds)
+    want, want_st = r.get(), r.status()
+    say(f"remesh {case}: {added} added, {removed} removed, {e.n} slots")
+    assert added > 0
+    assert e.n == r.n
+    assert np.array_equal(conn, want["conn"])
+    assert np.array_equal(st, want_st)
+    live = st[:, 0] != ov.EMPTY
+    assert np.array_equal(got["vnodes"][live], want["vnodes"][live])
