@@ -1284,9 +1284,6 @@ extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs,
     clear_error();
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
-                                           "add_loop on tangles with pinned lines are not built");
     if (max_pairs < 0 || (max_pairs > 0 && !pairs))
         return set_error(OVG_EINVAL, "bad pair buffer");
     Ctx &c = ctx->c;
@@ -1299,9 +1296,6 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
     clear_error();
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
-                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int nrec = 0;
@@ -1316,8 +1310,8 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
     if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
-                                           "add_loop on tangles with pinned lines are not built");
+        return set_error(OVG_EUNSUPPORTED, "pinned nodes: smoothing of open lines "
+                                           "(src/vortex_smooth.c:300-453) is not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return launch_status(smooth_loops(c));
@@ -1991,9 +1985,6 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
     clear_error();
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: reconnection, remesh, smoothing and "
-                                           "add_loop on tangles with pinned lines are not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     if (n_points) *n_points = 0;

@@ -57,12 +57,15 @@ inline int do_reconnection(const TangleParams &P, const double *X, int *conn, in
 //   otherwise only of those within 25 rec_dist of the point.
 //   flag: n zeros, returned zeroed; mark(slot): called for every slot whose
 //   links may have changed.
+//   st: the status mirror (status, pin_wall per slot) when the tangle has
+//   pinned nodes: stencils and velocities as step_node has them there.
 // Returns the number of rewirings.
 inline int greedy_pass(const TangleParams &P, const double *X, int *conn, int n,
                        const std::vector<int> &ck, const std::vector<int> &cl,
                        const std::vector<int> &ends, const std::vector<int> &where,
                        const double *bsp, const double *bsi, double rec_angle, double rec_dist,
-                       bool exact, std::vector<int> &flag, const std::function<void(int)> &mark)
+                       bool exact, std::vector<int> &flag, const std::function<void(int)> &mark,
+                       const int *st = nullptr)
 {
     const int total = (int)ck.size();
     if ((int)flag.size() < n) flag.assign((size_t)n, 0);
@@ -112,10 +115,10 @@ inline int greedy_pass(const TangleParams &P, const double *X, int *conn, int n,
             for (int i = a; i < b; ++i) {
                 const int e = ends[i];
                 double *t = &pre[9 * (size_t)i], *nr = t + 3, *v = t + 6;
-                tangent_normal(P, X, conn, e, t, nr);
+                tangent_normal(P, X, conn, e, t, nr, st);
                 double bs[3], vs[3];
                 for (int q = 0; q < 3; ++q) bs[q] = bsp[3 * (size_t)i + q] + bsi[3 * (size_t)i + q];
-                compose(P, X, conn, e, t, nr, bs, vs, v);
+                compose(P, X, conn, e, t, nr, bs, vs, v, st);
             }
         };
         const int hw = (int)std::thread::hardware_concurrency();
@@ -138,7 +141,7 @@ inline int greedy_pass(const TangleParams &P, const double *X, int *conn, int n,
                 nr[a] = s0[3 + a];
             }
         } else {
-            tangent_normal(P, X, conn, q, t, nr);
+            tangent_normal(P, X, conn, q, t, nr, st);
         }
     };
     std::vector<int> flagged;
@@ -248,7 +251,7 @@ inline int greedy_pass(const TangleParams &P, const double *X, int *conn, int n,
             return;
         }
         double vs[3];
-        compose(P, X, conn, p, tn, nr, bs, vs, vels);
+        compose(P, X, conn, p, tn, nr, bs, vs, vels, st);
     };
     int nrec = 0;
     int i = 0;

@@ -100,9 +100,10 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
                              const int *__restrict__ sorted, const double *__restrict__ X,
                              const double *__restrict__ Y, const double *__restrict__ Z,
                              const int *__restrict__ fwd, const int *__restrict__ rev,
-                             const TangleParams *__restrict__ Pp, Grid g, double rec_dist,
-                             const int *__restrict__ start, const int *__restrict__ end,
-                             int *__restrict__ count, int2 *__restrict__ out, int cap)
+                             const int *__restrict__ status, const TangleParams *__restrict__ Pp,
+                             Grid g, double rec_dist, const int *__restrict__ start,
+                             const int *__restrict__ end, int *__restrict__ count,
+                             int2 *__restrict__ out, int cap)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_live) return;
@@ -110,6 +111,7 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
     const double r2 = rec_dist * rec_dist;
     const double far2 = r2 * (1.0 + 0x1.0p-40), near2 = r2 * (1.0 - 0x1.0p-40);
     int k = live[i];
+    if (status[k] != OVG_FREE) return;  // pinned ends never reconnect (:201-204)
     double xk = X[k], yk = Y[k], zk = Z[k];
     int fk = fwd[k], rk = rev[k];
     int cx = cell_coord(xk, P.lo[0], g.cs[0], g.nc[0]);
@@ -146,7 +148,7 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
                 }
                 for (int s = s0; s < s1; ++s) {
                     int l = sorted[s];
-                    if (l <= k || l == fk || l == rk) continue;
+                    if (l <= k || l == fk || l == rk || status[l] != OVG_FREE) continue;
                     double wx = X[l], wy = Y[l], wz = Z[l];
                     pwrap(P, xk, yk, zk, wx, wy, wz);
                     double ex = xk - wx, ey = yk - wy, ez = zk - wz;
@@ -197,7 +199,8 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
                            0);
         hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
                            T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
-                           dparams(c), g, c.params.rec_dist, start, end, c.rc_count.as<int>(),
+                           c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
+                           c.rc_count.as<int>(),
                            c.rc_pairs.as<int2>(), c.rc_cap);
         OVG_HIP(hipMemcpyAsync(&total, c.rc_count.p, sizeof(int), hipMemcpyDeviceToHost,
                                c.stream));
@@ -326,7 +329,7 @@ int reconnect_pass(Ctx &c, double t, int *n_rec)
     const int nrec = host::greedy_pass(
         P, X, c.h_conn.data(), n, ck, cl, ends, where, bsp.data(), bsi.data(),
         c.params.rec_angle, c.params.rec_dist, c.params.rec_velocity != OVG_RECVEL_TREE, c.rc_flag,
-        [&](int sl) { mark_dirty(c, sl); });
+        [&](int sl) { mark_dirty(c, sl); }, c.has_pinned ? c.h_status.data() : nullptr);
     *n_rec = nrec;
     for (int i = 0; i < m; ++i) where[ends[i]] = -1;
     if (nrec) OVG_TRY(commit_topology(c));  // the rewired links only

@@ -183,10 +183,10 @@ def test_pinned_lines_refuse_topology_passes():
     from test_walls import pinned_case
     t, p = pinned_case(ov.METHOD_BH, False)
     p.traverse = ov.TRAVERSE_GROUP
-    say("pinned lines: reconnect / smooth refused")
+    say("pinned lines: smoothing of open lines refused")
     with ov.Engine(p) as e:
         e.upload(t)
-        for call in (lambda: e.reconnect(0.0), e.smooth):
+        for call in (e.smooth,):
             with pytest.raises(ov.OvgError) as ei:
                 call()
             assert ei.value.code == ov.EUNSUPPORTED
@@ -366,3 +366,32 @@ def test_remesh_pinned_vs_reference(ref, case):
     assert np.array_equal(st, want_st)
     live = st[:, 0] != ov.EMPTY
     assert np.array_equal(got["vnodes"][live], want["vnodes"][live])
+
+
+def test_reconnect_pinned_vs_reference(ref):
+    """ovg_reconnect on a tangle just clipped to the walls (open lines ending
+    on pinned nodes, rings crossing them) against the compiled reference's
+    reconnect (src/vortex_dynamics.c:186-288) with its direct-sum velocities:
+    pinned nodes never reconnect, stencils next to a pinned end use
+    step_node's mirrored ghosts; the same rewirings, links exact."""
+    from test_walls import rings_at_walls
+    t, p = rings_at_walls(seed=9, n_cross=30, n_inside=20)
+    p.rec_dist = 0.003
+    p.rec_velocity = ov.RECVEL_DIRECT
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.enforce_walls(0.0, ov.PINNED)
+    with ov.Engine(p) as e:
+        e.upload(t)
+        e.enforce_walls(0.0)
+        cand = e.reconnect_candidates()
+        n = e.reconnect(0.0)
+        conn, st = e.download_connections()
+    want_n = r.reconnect(0.0)
+    want = r.get()
+    say(f"pinned reconnection: {len(cand)} candidate pairs, {n} reconnections "
+        f"(reference {want_n})")
+    assert n == want_n and n > 0
+    assert np.array_equal(conn, want["conn"])
+    assert np.array_equal(st, r.status())
+    free = st[:, 0] == ov.FREE
+    assert all(free[k] and free[l] for k, l in cand)
