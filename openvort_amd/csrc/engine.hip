@@ -1309,9 +1309,6 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
     clear_error();
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
-    if (ctx->c.has_pinned)
-        return set_error(OVG_EUNSUPPORTED, "pinned nodes: smoothing of open lines "
-                                           "(src/vortex_smooth.c:300-453) is not built");
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return launch_status(smooth_loops(c));

@@ -171,6 +171,8 @@ struct Ctx {
     DBuf loop_off, loop_slots;          // smoothing loop table (CSR over slots)
     DBuf lt_buf;                        // loop table construction scratch
     std::vector<int> h_loop_off;
+    std::vector<int> h_loop_open;       // per loop: 1 = an open line (ends on walls)
+    DBuf loop_open;
     DBuf sm_fft, sm_big;                // transform arena and big-loop list + chunk maps
     DBuf sm_gstate;                     // per big loop state, per chunk wrap sums
     DBuf sm_tw;                         // e^{-2 pi i k / 2^sm_tw_log}, k < 2^(sm_tw_log-1)

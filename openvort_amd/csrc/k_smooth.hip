@@ -127,8 +127,9 @@ __device__ void block_detrend(double *const xs[3], int n, double *coef, double *
 }
 
 // (src/vortex_smooth.c:286-291) then write back along the loop (:301-309)
-__device__ void block_finish(double *const xs[3], int n, bool span, const double *coef,
-                             const double *x0, const int *slots, double *X, double *Y, double *Z)
+__device__ void block_finish(double *const xs[3], int n, bool span, bool open,
+                             const double *coef, const double *x0, const int *slots, double *X,
+                             double *Y, double *Z)
 {
     if (span) {
         for (int j = threadIdx.x; j < n; j += blockDim.x)
@@ -136,6 +137,7 @@ __device__ void block_finish(double *const xs[3], int n, bool span, const double
         __syncthreads();
     }
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        if (open && (j == 0 || j == n - 1)) continue;  // the ends stay (:435-444)
         int m = slots[j];
         X[m] = xs[0][j];
         Y[m] = xs[1][j];
@@ -161,8 +163,9 @@ __device__ bool kept_band(int N, double fcut, int &kr, int &ki)
 // ---- small loops: DFT of the kept band -------------------------------------
 __global__ __launch_bounds__(kSmBlock) void k_smooth(
     int n_loops, const int *__restrict__ loop_off, const int *__restrict__ loop_slots,
-    double *__restrict__ X, double *__restrict__ Y, double *__restrict__ Z,
-    const TangleParams *__restrict__ Pp, double fcut, double *__restrict__ work)
+    const int *__restrict__ loop_open, double *__restrict__ X, double *__restrict__ Y,
+    double *__restrict__ Z, const TangleParams *__restrict__ Pp, double fcut,
+    double *__restrict__ work)
 {
     int lp = blockIdx.x;
     if (lp >= n_loops) return;
@@ -170,6 +173,11 @@ __global__ __launch_bounds__(kSmBlock) void k_smooth(
     const int off = loop_off[lp];
     const int n = loop_off[lp + 1] - off;
     if (n > kSmallLoop) return;  // FFT paths
+    // an open line (ends pinned to walls, src/vortex_smooth.c:316-444) is
+    // detrended and odd-extended like a box-spanning loop and keeps its end
+    // points; one of fewer than 3 points has nothing to move
+    const bool open = loop_open[lp] != 0;
+    if (open && n < 3) return;
     const int *slots = loop_slots + off;
     // per loop 15n + 6 doubles from 15*off + 6*lp: coordinates 3 x 2n,
     // coefficients 3 x (2n + 2), output 3 x n
@@ -182,7 +190,7 @@ __global__ __launch_bounds__(kSmBlock) void k_smooth(
     __shared__ int s_carry, s_span;
     __shared__ double s_coef[3], s_x0[3];
     block_unwrap(P, slots, n, X, Y, Z, xs, sh, &s_carry);
-    if (threadIdx.x == 0) s_span = loop_spans(P, xs, n);
+    if (threadIdx.x == 0) s_span = open || loop_spans(P, xs, n);
     __syncthreads();
     const bool span = s_span;
     if (span) block_detrend(xs, n, s_coef, s_x0);
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(kSmBlock) void k_smooth(
         }
         __syncthreads();
     }
-    block_finish(xs, n, span, s_coef, s_x0, slots, X, Y, Z);
+    block_finish(xs, n, span, open, s_coef, s_x0, slots, X, Y, Z);
 }
 
 // ---- loops longer than kSmallLoop: one batched Bluestein transform ---------
@@ -268,7 +276,7 @@ struct BsLoop {
     int logMmax;      // log2 of the convolution length for the spanning length 2n-2
     int chunk0;       // first point chunk
     int tile0;        // first tile of its 2^logMmax-element signals
-    int pad;
+    int open;         // an open line: detrended, its ends kept
     long long xs;     // first double of the unwrapped coordinates, 3 x n
     long long chirp;  // first double2 of c_j, j < N
     long long bhat;   // first double2 of b, then DIF(b): 2^logMmax
@@ -457,7 +465,7 @@ __global__ void k_bs_setup(int nb, const BsLoop *__restrict__ L, const double *_
 #pragma unroll
     for (int a = 0; a < 3; ++a) e[a] = xs[(size_t)a * n] - xs[(size_t)a * n + n - 1];
     BsState s;
-    s.span = sqrt(dot3(e[0], e[1], e[2], e[0], e[1], e[2])) > (P.hi[0] - P.lo[0]) / 2;
+    s.span = B.open || sqrt(dot3(e[0], e[1], e[2], e[0], e[1], e[2])) > (P.hi[0] - P.lo[0]) / 2;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         s.coef[a] = s.span ? (xs[(size_t)a * n + n - 1] - xs[(size_t)a * n]) / (n - 1) : 0.0;
@@ -692,6 +700,7 @@ __global__ __launch_bounds__(kChunk) void k_bs_finish(
     const BsLoop B = L[b];
     const int j = (blockIdx.x - B.chunk0) * kChunk + threadIdx.x;
     if (j >= B.n) return;
+    if (B.open && (j == 0 || j == B.n - 1)) return;  // an open line keeps its ends
     const BsState s = S[b];
     const double *xs = (const double *)arena + B.xs;
     const double inv = 1.0 / (1 << s.logM);
@@ -738,7 +747,7 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
         B.loop = lp;
         B.n = n;
         B.logMmax = logM;
-        B.pad = 0;
+        B.open = c.h_loop_open.empty() ? 0 : c.h_loop_open[lp];
         B.chunk0 = (int)chunk_loop.size();
         B.tile0 = (int)tile_loop.size();
         chunk_loop.insert(chunk_loop.end(), (n + kChunk - 1) / kChunk, i);
@@ -901,30 +910,53 @@ __global__ void k_lt_scatter(int n, const int *__restrict__ lab, const int *__re
     if (pos >= 0 && pos < len[m]) slots[hoff[m] + pos] = i;  // always, for consistent links
 }
 
-// Loop table from the host link mirror (chains with open ends).
+__global__ void k_fill_zero(int n, int *a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// Loop table from the host link mirror (chains with open ends): loops in
+// the order of their smallest slot i; a closed loop listed forward from i,
+// an open line forward from its start, the point is_vortex_open reaches by
+// the reverse links from i (src/vortex_smooth.c:24-39,198-205).
 static int build_loops_host(Ctx &c)
 {
     int n = c.n;
-    std::vector<int> off, slots;
+    std::vector<int> off, slots, open;
     std::vector<char> seen(n, 0);
     off.push_back(0);
     for (int i = 0; i < n; ++i) {
         if (seen[i] || c.h_status[2 * i] == OVG_EMPTY) continue;
-        int k = i;
+        int s = i, guard = 0;
+        int r = c.h_conn[2 * s + 1];
+        while (r != i && r >= 0 && r < n && ++guard <= n) {
+            s = r;
+            r = c.h_conn[2 * s + 1];
+        }
+        const bool is_open = r < 0 || r >= n;
+        const int start = is_open ? s : i;
+        int k = start;
         do {
             seen[k] = 1;
             slots.push_back(k);
             k = c.h_conn[2 * k];
-        } while (k != i && k >= 0 && !seen[k]);
+        } while (k != start && k >= 0 && k < n && !seen[k]);
         off.push_back((int)slots.size());
+        open.push_back(is_open ? 1 : 0);
     }
     c.n_loops = (int)off.size() - 1;
     c.h_loop_off = off;
+    c.h_loop_open = open;
     OVG_TRY(c.loop_off.ensure(sizeof(int) * off.size()));
     OVG_TRY(c.loop_slots.ensure(sizeof(int) * (slots.size() + 1)));
+    OVG_TRY(c.loop_open.ensure(sizeof(int) * (open.size() + 1)));
     OVG_HIP(hipMemcpy(c.loop_off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
     if (!slots.empty())
         OVG_HIP(hipMemcpy(c.loop_slots.p, slots.data(), sizeof(int) * slots.size(),
+                          hipMemcpyHostToDevice));
+    if (!open.empty())
+        OVG_HIP(hipMemcpy(c.loop_open.p, open.data(), sizeof(int) * open.size(),
                           hipMemcpyHostToDevice));
     c.loops_version = c.topo_version;
     return OVG_OK;
@@ -979,6 +1011,10 @@ static int build_loops(Ctx &c)
     hipLaunchKernelGGL(k_lt_scatter, dim3(g), dim3(256), 0, c.stream, n, lab, d, L, flag, hoff,
                        c.loop_off.as<int>(), c.loop_slots.as<int>());
     c.n_loops = n_loops;
+    c.h_loop_open.clear();  // closed loops only
+    OVG_TRY(c.loop_open.ensure(sizeof(int) * ((size_t)n_loops + 1)));
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(n_loops + 1, 256)), dim3(256), 0, c.stream,
+                       n_loops + 1, c.loop_open.as<int>());
     c.h_loop_off.resize((size_t)n_loops + 1);
     OVG_HIP(hipMemcpyAsync(c.h_loop_off.data(), c.loop_off.p, sizeof(int) * ((size_t)n_loops + 1),
                            hipMemcpyDeviceToHost, c.stream));
@@ -999,7 +1035,8 @@ int smooth_loops(Ctx &c)
     for (int lp = 0; lp < c.n_loops; ++lp)
         if (c.h_loop_off[lp + 1] - c.h_loop_off[lp] > kSmallLoop) big.push_back(lp);
     hipLaunchKernelGGL(k_smooth, dim3(c.n_loops), dim3(kSmBlock), 0, c.stream, c.n_loops,
-                       c.loop_off.as<int>(), c.loop_slots.as<int>(), c.x[0].as<double>(),
+                       c.loop_off.as<int>(), c.loop_slots.as<int>(), c.loop_open.as<int>(),
+                       c.x[0].as<double>(),
                        c.x[1].as<double>(), c.x[2].as<double>(), dparams(c),
                        c.params.freq_to_cutoff, c.sm_buf.as<double>());
     std::vector<char> hmeta;

@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY (the checker for ovg_smooth).
 
 Follows smooth -> high_freq_cutoff -> low_pass_filter
-(src/vortex_smooth.c:134-175,183-453,459-462) for closed loops.  The
+(src/vortex_smooth.c:134-175,183-453,459-462) for closed loops and for
+open lines ending on walls.  The
 reference calls FFTW (fftw_plan_r2r_1d, :142,163), a third-party library
 that is not vendored and not installed here; the thesis names FFTW 3.3.9
 (section 1.3.1 p.11).  Its halfcomplex R2HC layout is restated with
@@ -101,10 +102,25 @@ def pwrap_next(prev, cur, L, periodic):
     return out
 
 
+def line_start(conn, i):
+    """is_vortex_open (src/vortex_smooth.c:24-39): walking the reverse links
+    from i, the point whose reverse link is -1 for an open line (its
+    start), None for a closed loop."""
+    k = i
+    while conn[k, 1] != i and conn[k, 1] != -1:
+        k = conn[k, 1]
+    return k if conn[k, 1] == -1 else None
+
+
 def smooth_tangle(vnodes, conn, status, box_lo, box_hi, periodic, freq_to_cutoff):
-    """high_freq_cutoff for closed loops; returns new positions (unwrapped
-    along each loop, as the reference leaves them before
-    enforce_periodic_boundaries)."""
+    """high_freq_cutoff (src/vortex_smooth.c:183-453); returns new positions
+    (unwrapped along each loop, as the reference leaves them before
+    enforce_periodic_boundaries).  Loops are visited from their smallest
+    slot; a closed loop is listed forward from there, an open line (ends
+    pinned to walls) forward from its start (:316-352).  Open lines are
+    always detrended and odd-extended (:354-381, as box-spanning loops are,
+    :245-292), filtered at length 2m - 2 and retrended (:421-433), and keep
+    their two end points (:435-444)."""
     vnodes = np.array(vnodes, dtype=np.float64)
     n = len(vnodes)
     L = np.asarray(box_hi, dtype=np.float64) - np.asarray(box_lo, dtype=np.float64)
@@ -113,20 +129,24 @@ def smooth_tangle(vnodes, conn, status, box_lo, box_hi, periodic, freq_to_cutoff
     for i in range(n):
         if seen[i] or status[i, 0] == -1:
             continue
+        start = line_start(conn, i)
+        is_open = start is not None
         slots = []
-        k = i
+        k = start if is_open else i
         while True:
             seen[k] = True
             slots.append(k)
             k = conn[k, 0]
-            if k == i:
+            if k == slots[0] or k == -1:
                 break
         m = len(slots)
         xs = np.zeros((m, 3))
         xs[0] = vnodes[slots[0]]
         for j in range(1, m):
             xs[j] = pwrap_next(xs[j - 1], vnodes[slots[j]], L, periodic)
-        span = spans_box(xs, L)
+        if is_open and m < 3:
+            continue  # no interior point to move
+        span = is_open or spans_box(xs, L)
         ys = np.zeros_like(xs)
         for c in range(3):
             if span:
@@ -134,5 +154,8 @@ def smooth_tangle(vnodes, conn, status, box_lo, box_hi, periodic, freq_to_cutoff
                 ys[:, c] = retrend(low_pass(ext, freq_to_cutoff), m, coef, x0)
             else:
                 ys[:, c] = low_pass(xs[:, c], freq_to_cutoff)
-        out[slots] = ys
+        if is_open:
+            out[slots[1:m - 1]] = ys[1:m - 1]
+        else:
+            out[slots] = ys
     return out

@@ -179,17 +179,34 @@ def test_pinned_lines_vs_oracle(oracle, method, traverse, slip):
     assert np.abs(x - w.vnodes[live]).max() < 1e-14
 
 
-def test_pinned_lines_refuse_topology_passes():
-    from test_walls import pinned_case
-    t, p = pinned_case(ov.METHOD_BH, False)
-    p.traverse = ov.TRAVERSE_GROUP
-    say("pinned lines: smoothing of open lines refused")
+@pytest.mark.parametrize("orders", [None, "mixed"])
+def test_smooth_open_lines_vs_oracle(orders):
+    """ovg_smooth on rings and lines pinned to the walls (2 to 300 points:
+    the kept-band DFT and the batched transform) against the oracle's
+    high_freq_cutoff: an open line is found from its smallest slot by its
+    reverse links, detrended and odd-extended, filtered and retrended, and
+    keeps its end points (src/vortex_smooth.c:316-444)."""
+    from oracle import smooth as S
+    from test_walls import lines_and_rings
+    lengths = [2, 3, 5, 6, 7, 40, 4, 300]
+    od = None if orders is None else ["mid", "up", "down", "mid", "up", "mid", "up", "mid"]
+    t, p = lines_and_rings(lengths, orders=od)
+    rng = np.random.default_rng(8)
+    st = t.status[:, 0]
+    free = np.where(st == ov.FREE)[0]
+    t.vnodes[free] += rng.normal(scale=3e-4, size=(len(free), 3))
+    say(f"smoothing with open lines: {len(t.live)} points")
     with ov.Engine(p) as e:
         e.upload(t)
-        for call in (e.smooth,):
-            with pytest.raises(ov.OvgError) as ei:
-                call()
-            assert ei.value.code == ov.EUNSUPPORTED
+        e.smooth()
+        got = e.download()["vnodes"]
+    want = S.smooth_tangle(t.vnodes, t.conn, t.status, p.box_lo[:], p.box_hi[:],
+                           [1, 1, 0], p.freq_to_cutoff)
+    live = t.live
+    assert np.abs(got[live] - want[live]).max() < 1e-14
+    pinned = (st == ov.PINNED) | (st == ov.PINNED_SLIP)
+    assert np.array_equal(got[pinned], t.vnodes[pinned])
+    assert np.abs(got[live] - t.vnodes[live]).max() > 1e-5
 
 
 def test_pinned_velocity_at_matches_oracle(oracle):

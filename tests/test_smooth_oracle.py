@@ -143,3 +143,68 @@ def test_circle_loop_unchanged():
     st = np.zeros((n, 2), dtype=np.int32)
     y = S.smooth_tangle(x, conn, st, [0, 0, 0], [0.1, 0.1, 0.1], [1, 1, 1], 0.8)
     assert np.abs(y - x).max() < 1e-15
+
+
+def open_line(m, z_mode=None, amp=0.0, k=0):
+    """m points from the DOWN wall (z = 0) to the UP wall (z = 0.1) at
+    x = y = 0.05, slots in order, both ends pinned; optionally x displaced
+    by amp sin(pi k j / (m - 1)) (a mode of the odd extension)."""
+    j = np.arange(m)
+    x = np.stack([np.full(m, 0.05), np.full(m, 0.05), 0.1 * j / (m - 1)], 1)
+    if amp:
+        x[:, 0] += amp * np.sin(np.pi * k * j / (m - 1))
+    conn = np.stack([np.where(j < m - 1, j + 1, -1), np.where(j > 0, j - 1, -1)], 1)
+    st = np.zeros((m, 2), dtype=np.int64)
+    st[0] = (1, 4)
+    st[-1] = (1, 5)
+    return x, conn, st
+
+
+def smooth_lines(x, conn, st, fcut=0.25):
+    return S.smooth_tangle(x, conn, st, [0, 0, 0], [0.1, 0.1, 0.1], [1, 1, 0], fcut)
+
+
+def test_open_line_straight_unchanged():
+    x, conn, st = open_line(40)
+    y = smooth_lines(x, conn, st)
+    assert np.abs(y - x).max() < 1e-16
+
+
+def test_open_line_low_mode_kept_high_mode_removed():
+    """The open branch (src/vortex_smooth.c:316-444) filters the odd
+    extension of length 2m - 2, whose modes are sin(pi k j / (m - 1)): a low
+    k passes unchanged, one in the cut band is removed; the end points stay
+    exactly where they were."""
+    m = 33
+    x0, conn, st = open_line(m)
+    lo, c1, s1 = open_line(m, amp=1e-3, k=2)
+    hi, _, _ = open_line(m, amp=1e-3, k=m - 3)
+    ylo = smooth_lines(lo, c1, s1)
+    yhi = smooth_lines(hi, c1, s1)
+    assert np.abs(ylo - lo).max() < 1e-15
+    assert np.abs(yhi - x0).max() < 1e-15
+    for y, src in ((ylo, lo), (yhi, hi)):
+        assert np.array_equal(y[[0, m - 1]], src[[0, m - 1]])
+
+
+def test_open_line_is_the_spanning_branch_with_fixed_ends():
+    """An open line is filtered as a box-spanning loop of the same points is
+    (detrend, odd extension, low pass, retrend), except that its end points
+    are not written; the line is found from a mid-line slot by its reverse
+    links (is_vortex_open, :24-39)."""
+    rng = np.random.default_rng(4)
+    m = 21
+    x, conn, st = open_line(m)
+    x[1:-1] += rng.normal(scale=2e-4, size=(m - 2, 3))
+    perm = np.roll(np.arange(m), 7)  # slot s holds line point perm[s]
+    inv = np.argsort(perm)
+    xs = x[perm]
+    cs = np.stack([np.where(conn[perm, 0] >= 0, inv[np.maximum(conn[perm, 0], 0)], -1),
+                   np.where(conn[perm, 1] >= 0, inv[np.maximum(conn[perm, 1], 0)], -1)], 1)
+    y = smooth_lines(xs, cs, st[perm])[inv]
+    want = np.zeros_like(x)
+    for c in range(3):
+        ext, coef, x0 = S.detrend_odd(x[:, c])
+        want[:, c] = S.retrend(S.low_pass(ext, 0.25), m, coef, x0)
+    assert np.abs(y[1:-1] - want[1:-1]).max() < 1e-16
+    assert np.array_equal(y[[0, m - 1]], x[[0, m - 1]])
