@@ -483,9 +483,18 @@ int ovg_synchronize(ovg_ctx *ctx);
  * empty slots in the reference's order (the slots double first while fewer
  * are empty than needed), linked in order and closed.  *n_points = points
  * added.  Loops with no point are refused (OVG_EINVAL).
- * Injection of straight line pairs (inject_line_pairs) pins the lines to
- * walls: pinned lines are evaluated and stepped by this engine, but the
- * topology passes refuse them, so the injection is the reference's.
+ * ovg_inject_line_pairs_draw: inject_line_pairs' schedule (:760-800) with
+ * the same caller-held state: when t - *last_injection >= 1/frequency, for
+ * each of n_pairs pairs x1, x2 and then y1, y2 (polarized: y1 in the upper,
+ * y2 in the lower half of the box's y range) into xy[4 k .. 4 k + 3] =
+ * (x1, y1, x2, y2), *last_injection = t and *due = 1; else nothing is drawn
+ * and *due = 0.  Host only.
+ * ovg_add_line_z: add_line_in_zdir (:267-318) on a resident tangle: a
+ * straight line at (x, y) from the DOWN to the UP wall (direction > 0) or
+ * back, (int)(2 (zmax - zmin) / (dl_max - dl_min)) points in the next empty
+ * slots (doubling first while fewer are empty), its two ends of status
+ * pin_mode pinned to the walls.  A pair of the reference's injection is
+ * ovg_add_line_z(x1, y1, +1) then ovg_add_line_z(x2, y2, -1).
  */
 int ovg_loop_points(const double center[3], const double dir[3], double r, double dl_min,
                     double dl_max, double *xyz, int cap, int *n_points);
@@ -494,6 +503,12 @@ int ovg_inject_loop_draw(double t, double frequency, const double box_lo[3],
                          double center[3], double dir[3], double *r, int *due);
 int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double dir[3], double r,
                  double dl_min, double dl_max, int *n_points);
+int ovg_inject_line_pairs_draw(double t, double frequency, int n_pairs, int polarized,
+                               const double box_lo[3], const double box_hi[3],
+                               unsigned short rng[3], double *last_injection, double *xy,
+                               int *due);
+int ovg_add_line_z(ovg_ctx *ctx, double x, double y, int direction, double dl_min,
+                   double dl_max, int *n_points);
 
 /* ---- stream markers (no counterpart in the reference) ----------------- */
 /*

@@ -138,6 +138,10 @@ _sig("ovg_loop_points", C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _
 _sig("ovg_inject_loop_draw", C.c_int, C.c_double, C.c_double, _dp, _dp, _P(C.c_ushort), _dp, _dp,
      _dp, _dp, _ip)
 _sig("ovg_add_loop", C.c_int, C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_double, _ip)
+_sig("ovg_inject_line_pairs_draw", C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp,
+     _P(C.c_ushort), _dp, _dp, _ip)
+_sig("ovg_add_line_z", C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double,
+     C.c_double, _ip)
 _sig("ovg_velocity_at", C.c_int, C.c_void_p, C.c_int, _ip, C.c_double, _dp, _dp)
 _sig("ovg_velocity_at_noalloc", C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double,
      C.c_void_p, C.c_void_p)
@@ -388,6 +392,15 @@ class Engine:
     def smooth(self):
         _check(_lib.ovg_smooth(self._h), "ovg_smooth")
 
+    def add_line_z(self, x, y, direction, dl_min, dl_max):
+        """add_line_in_zdir (src/vortex_injection.c:267-318): a line from
+        wall to wall with pinned ends; returns the number of points."""
+        n = C.c_int32(0)
+        _check(_lib.ovg_add_line_z(self._h, x, y, direction, dl_min, dl_max, C.byref(n)),
+               "ovg_add_line_z")
+        self.n = _lib.ovg_num_slots(self._h)
+        return n.value
+
     def add_loop(self, center, direction, r, dl_min, dl_max):
         """add_loop (src/vortex_injection.c:44-103) on the resident tangle;
         returns the number of points added."""
@@ -538,6 +551,22 @@ def inject_loop_draw(t, frequency, box_lo, box_hi, rng, last_injection):
                                      rng.ctypes.data_as(_P(C.c_ushort)), C.byref(last), _d(c),
                                      _d(d), C.byref(r), C.byref(due)), "ovg_inject_loop_draw")
     return bool(due.value), c, d, r.value, last.value
+
+
+def inject_line_pairs_draw(t, frequency, n_pairs, polarized, box_lo, box_hi, rng,
+                           last_injection):
+    """inject_line_pairs' schedule (src/vortex_injection.c:760-800) with
+    explicit state.  Returns (due, xy of shape (n_pairs, 4) as x1, y1, x2,
+    y2, last_injection)."""
+    lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+    hi = np.ascontiguousarray(box_hi, dtype=np.float64)
+    assert rng.dtype == np.uint16 and rng.shape == (3,) and rng.flags.c_contiguous
+    last = C.c_double(last_injection)
+    xy, due = np.zeros((max(n_pairs, 1), 4)), C.c_int32(0)
+    _check(_lib.ovg_inject_line_pairs_draw(t, frequency, n_pairs, int(polarized), _d(lo), _d(hi),
+                                           rng.ctypes.data_as(_P(C.c_ushort)), C.byref(last),
+                                           _d(xy), C.byref(due)), "ovg_inject_line_pairs_draw")
+    return bool(due.value), xy[:n_pairs], last.value
 
 
 def write_tangle_file(path, t, vels=None, tangents=None, normals=None, digits=15):

@@ -2028,6 +2028,80 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
     return launch_status();
 }
 
+// add_line_in_zdir, src/vortex_injection.c:267-318: a straight line at
+// (x, y) from the DOWN wall to the UP wall (direction > 0) or back, of
+// (int)(2 (zmax - zmin) / (dl_max - dl_min)) points (the reference divides
+// by the difference here, by the sum in add_line), its ends of status
+// pin_mode pinned to the walls; the tangle doubles first while it has fewer
+// empty slots than points.
+extern "C" int ovg_add_line_z(ovg_ctx *ctx, double x, double y, int direction, double dl_min,
+                              double dl_max, int *n_points)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx));
+    OVG_TRY(single_tangle(ctx->c, __func__));
+    Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (n_points) *n_points = 0;
+    const double zmin = c.params.box_lo[2], zmax = c.params.box_hi[2];
+    if (direction == 0 || !(dl_max > dl_min))
+        return set_error(OVG_EINVAL, "add_line_z: direction must be +-1 and dl_max > dl_min");
+    const int npts = (int)(2 * (zmax - zmin) / (dl_max - dl_min));
+    if (npts < 2)
+        return set_error(OVG_EINVAL, "add_line_z: %d points for a box of height %g", npts,
+                         zmax - zmin);
+    const double *Xp = nullptr;
+    OVG_TRY(host_positions(c, &Xp));
+    HostTangle h(c.n, Xp, c.h_conn, c.h_status, c.next_empty);
+    while (h.empty_count() < npts) h.expand();
+    const double dz = (zmax - zmin) / npts;
+    const double zstart = direction > 0 ? zmin : zmax, zend = direction > 0 ? zmax : zmin;
+    const int pin = c.params.pin_mode ? c.params.pin_mode : OVG_PINNED;
+    std::vector<int> slots;
+    auto place = [&](int sl, double z) {
+        h.X[3 * (size_t)sl] = x;
+        h.X[3 * (size_t)sl + 1] = y;
+        h.X[3 * (size_t)sl + 2] = z;
+        slots.push_back(sl);
+    };
+    int cur = h.next_empty_node();  // the first point, pinned
+    place(cur, zstart);
+    h.st[2 * (size_t)cur] = pin;
+    h.st[2 * (size_t)cur + 1] = direction > 0 ? OVG_FACE_DOWN : OVG_FACE_UP;
+    h.conn[2 * (size_t)cur + 1] = -1;
+    for (int k = 1; k < npts - 1; ++k) {
+        const int last = cur;
+        cur = h.next_empty_node();
+        place(cur, zstart + direction * k * dz);
+        h.st[2 * (size_t)cur] = OVG_FREE;
+        h.st[2 * (size_t)cur + 1] = -1;
+        h.conn[2 * (size_t)cur + 1] = last;
+        h.conn[2 * (size_t)last] = cur;
+    }
+    const int last = cur;  // the last point, pinned
+    cur = h.next_empty_node();
+    place(cur, zend);
+    h.st[2 * (size_t)cur] = pin;
+    h.st[2 * (size_t)cur + 1] = direction > 0 ? OVG_FACE_UP : OVG_FACE_DOWN;
+    h.conn[2 * (size_t)cur + 1] = last;
+    h.conn[2 * (size_t)cur] = -1;
+    h.conn[2 * (size_t)last] = cur;
+    c.next_empty = h.next_empty;
+    if (h.grown) {
+        OVG_TRY(grow_slots(c, h.N, h.X));
+    } else {
+        for (int sl : slots) {
+            mark_dirty(c, sl);
+            c.new_pos.push_back(sl);
+            for (int q = 0; q < 3; ++q) c.new_xyz.push_back(h.X[3 * (size_t)sl + q]);
+        }
+        OVG_TRY(commit_topology(c));
+    }
+    OVG_TRY(refresh_pins(c));
+    if (n_points) *n_points = npts;
+    return launch_status();
+}
+
 // ---- autotuning of the list traversal --------------------------------------
 extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
 {

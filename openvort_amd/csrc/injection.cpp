@@ -1,7 +1,8 @@
-// Vortex-loop injection (src/vortex_injection.c): the geometry of add_loop
-// (:44-103, with perpendicular :11-35) and the random schedule of
-// inject_loop (:711-745), host-only.  ovg_add_loop (engine.hip) places the
-// points of a loop in a device-resident tangle.
+// Vortex injection (src/vortex_injection.c): the geometry of add_loop
+// (:44-103, with perpendicular :11-35) and the random schedules of
+// inject_loop (:711-745) and inject_line_pairs (:760-800), host-only.
+// ovg_add_loop and ovg_add_line_z (engine.hip) place the points of a loop
+// or of a line pinned to the DOWN and UP walls in a device-resident tangle.
 //
 // The reference draws from the process-wide drand48 stream and keeps the
 // last injection time in a static; here both are the caller's: the 48-bit
@@ -137,5 +138,37 @@ extern "C" int ovg_inject_loop_draw(double t, double frequency, const double box
     const double rmin = 0.05 * D, rmax = 0.25 * D;
     *r = rmin + (rmax - rmin) * rand48(rng);
     for (int q = 0; q < 3; ++q) dir[q] = dv.p[q];
+    return OVG_OK;
+}
+
+extern "C" int ovg_inject_line_pairs_draw(double t, double frequency, int n_pairs, int polarized,
+                                          const double box_lo[3], const double box_hi[3],
+                                          unsigned short rng[3], double *last_injection,
+                                          double *xy, int *due)
+{
+    if (!box_lo || !box_hi || !rng || !last_injection || !due || n_pairs < 0 ||
+        (n_pairs > 0 && !xy))
+        return OVG_EINVAL;
+    *due = 0;
+    if (t - *last_injection < 1 / frequency) return OVG_OK;  // :772-773
+    *last_injection = t;
+    *due = 1;
+    const double left = box_lo[0], right = box_hi[0], back = box_lo[1], front = box_hi[1];
+    for (int k = 0; k < n_pairs; ++k) {  // :776-799
+        const double x1 = left + rand48(rng) * (right - left);
+        const double x2 = left + rand48(rng) * (right - left);
+        double y1, y2;
+        if (polarized) {
+            y1 = back + (1 + rand48(rng)) * (front - back) / 2;  // positive line in (0.5, 1)
+            y2 = back + rand48(rng) * (front - back) / 2;        // negative line in (0, 0.5)
+        } else {
+            y1 = back + rand48(rng) * (front - back);
+            y2 = back + rand48(rng) * (front - back);
+        }
+        xy[4 * (size_t)k] = x1;
+        xy[4 * (size_t)k + 1] = y1;
+        xy[4 * (size_t)k + 2] = x2;
+        xy[4 * (size_t)k + 3] = y2;
+    }
     return OVG_OK;
 }

@@ -199,6 +199,8 @@ class RefLib:
         L.ref_add_loop.argtypes = [vp, _dp, _dp, C.c_double, C.c_double, C.c_double]
         L.ref_inject_loops.argtypes = [vp, C.c_long, _dp, C.c_int, C.c_double, C.c_double,
                                        C.c_double]
+        L.ref_inject_line_pairs.argtypes = [vp, C.c_long, _dp, C.c_int, C.c_double, C.c_int,
+                                            C.c_int, C.c_double, C.c_double, C.c_int]
 
     def image_table(self, n):
         out = np.zeros((26, 3), dtype=np.int32)
@@ -294,6 +296,14 @@ class RefTangle:
         injection time persists for the life of the library)."""
         ts = np.ascontiguousarray(times, dtype=np.float64)
         self.ref.lib.ref_inject_loops(self.h, seed, _d(ts), len(ts), frequency, dl_min, dl_max)
+        self.n = self.ref.lib.ref_num_slots(self.h)
+
+    def inject_line_pairs(self, seed, times, frequency, n_pairs, polarized, dl_min, dl_max,
+                          pin_mode=1):
+        """srand48(seed), then inject_line_pairs at each time."""
+        ts = np.ascontiguousarray(times, dtype=np.float64)
+        self.ref.lib.ref_inject_line_pairs(self.h, seed, _d(ts), len(ts), frequency, n_pairs,
+                                           int(polarized), dl_min, dl_max, pin_mode)
         self.n = self.ref.lib.ref_num_slots(self.h)
 
     def save(self, path):

@@ -336,6 +336,20 @@ void ref_add_loop(void *h, const double *center, const double *dir, double r, do
 /* srand48(seed), then inject_loop (src/vortex_injection.c:711-745) at each
  * of the n times.  inject_loop keeps its last injection time in a static
  * that starts at 0 when the library is loaded and persists across calls. */
+/* srand48(seed), then inject_line_pairs (src/vortex_injection.c:760-800) at
+ * each time with global_dl_min / global_dl_max set (its static last
+ * injection time persists for the life of the library) */
+void ref_inject_line_pairs(void *h, long seed, const double *times, int n, double frequency,
+                           int n_pairs, int polarized, double dl_min, double dl_max, int pin)
+{
+    struct tangle_state *t = (struct tangle_state *)h;
+    global_dl_min = dl_min;
+    global_dl_max = dl_max;
+    pin_mode = pin;
+    srand48(seed);
+    for (int i = 0; i < n; ++i) inject_line_pairs(t, times[i], frequency, n_pairs, polarized);
+}
+
 void ref_inject_loops(void *h, long seed, const double *times, int n, double frequency,
                       double dl_min, double dl_max)
 {

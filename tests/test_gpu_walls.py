@@ -412,3 +412,42 @@ def test_reconnect_pinned_vs_reference(ref):
     assert np.array_equal(st, r.status())
     free = st[:, 0] == ov.FREE
     assert all(free[k] and free[l] for k, l in cand)
+
+
+def test_inject_line_pairs_vs_reference(ref):
+    """The reference's line-pair injection (inject_line_pairs,
+    src/vortex_injection.c:760-800) through ovg_inject_line_pairs_draw and
+    ovg_add_line_z on a resident tangle with holes (the slots double on the
+    way): slots, links, statuses and positions bit for bit; the tangle with
+    its new wall-to-wall lines then evaluates as the reference's does."""
+    from test_walls import rings_at_walls
+    t, p = rings_at_walls(spare=40)
+    dl_min, dl_max, freq, seed, n_pairs = 5e-4, 1.5e-3, 1000.0, 4242, 2
+    times = np.arange(1, 6) * 4e-4 + 5.0  # after any earlier call's static
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    r.inject_line_pairs(seed, times, freq, n_pairs, 0, dl_min, dl_max)
+    rng, last = ov.rand48_state(seed), 0.0
+    added = 0
+    with ov.Engine(p) as e:
+        e.upload(t)
+        for tm in times:
+            due, xy, last = ov.inject_line_pairs_draw(tm, freq, n_pairs, 0, p.box_lo[:],
+                                                      p.box_hi[:], rng, last)
+            if due:
+                for x1, y1, x2, y2 in xy:
+                    added += e.add_line_z(x1, y1, +1, dl_min, dl_max)
+                    added += e.add_line_z(x2, y2, -1, dl_min, dl_max)
+        conn, st = e.download_connections()
+        got = e.download()["vnodes"]
+        e.update_tangle(0.0)
+        ev = e.download()
+    say(f"line injection: {added} points in {e.n} slots")
+    assert e.n == r.n and added > 0
+    want = r.get()
+    assert np.array_equal(conn, want["conn"])
+    assert np.array_equal(st, r.status())
+    live = st[:, 0] != ov.EMPTY
+    assert np.array_equal(got[live], want["vnodes"][live])
+    r.update_tangle(0.0)
+    w2 = r.get()
+    assert rel_l2(ev["vels"][live], w2["vels"][live]) < 1e-10

@@ -66,3 +66,41 @@ def test_inject_schedule_matches_reference(ref):
         assert np.array_equal(pts, want)
         first += len(pts)
     assert (rt.status()[first:, 0] == ov.EMPTY).all()
+
+
+@pytest.mark.parametrize("polarized", [0, 1])
+def test_inject_line_pairs_schedule_matches_reference(ref, polarized):
+    """inject_line_pairs (src/vortex_injection.c:760-800) from the same
+    srand48 seed: the frequency gate and the (x1, y1, x2, y2) draws of each
+    pair are the positions of the lines the reference's add_line_in_zdir
+    (:267-318) placed, up from the DOWN wall then down from the UP wall, in
+    order; both ends pinned, the count 2 L / (dl_max - dl_min)."""
+    _, p, _ = configs.make("c1", rings=1)
+    dl_min, dl_max, freq, seed, n_pairs = 5e-4, 1.5e-3, 1000.0, 777 + polarized, 2
+    times = np.arange(1, 9) * 4e-4 + 1.0 * (1 + polarized)  # after any earlier call's static
+    rt = ref.tangle(empty_tangle(64), p)
+    rt.inject_line_pairs(seed, times, freq, n_pairs, polarized, dl_min, dl_max)
+    rng, last, draws = ov.rand48_state(seed), 0.0, []
+    for t in times:
+        due, xy, last = ov.inject_line_pairs_draw(t, freq, n_pairs, polarized, p.box_lo[:],
+                                                  p.box_hi[:], rng, last)
+        if due:
+            draws.extend(xy)
+    assert 3 <= len(draws) // n_pairs < len(times)
+    o, st = rt.get(), rt.status()
+    L = p.box_hi[2] - p.box_lo[2]
+    npts = int(2 * L / (dl_max - dl_min))
+    first = 0
+    for x1, y1, x2, y2 in draws:
+        for (x, y, up) in ((x1, y1, True), (x2, y2, False)):
+            seg = np.arange(first, first + npts)
+            assert np.array_equal(o["vnodes"][seg, 0], np.full(npts, x))
+            assert np.array_equal(o["vnodes"][seg, 1], np.full(npts, y))
+            z = o["vnodes"][seg, 2]
+            assert (z[0], z[-1]) == ((p.box_lo[2], p.box_hi[2]) if up else
+                                      (p.box_hi[2], p.box_lo[2]))
+            assert (st[seg[0]] == (ov.PINNED, ov.FACE_DOWN if up else ov.FACE_UP)).all()
+            assert (st[seg[-1]] == (ov.PINNED, ov.FACE_UP if up else ov.FACE_DOWN)).all()
+            first += npts
+        if polarized:
+            assert y1 >= 0.5 * p.box_hi[1] and y2 <= 0.5 * p.box_hi[1]
