@@ -451,3 +451,55 @@ def test_inject_line_pairs_vs_reference(ref):
     r.update_tangle(0.0)
     w2 = r.get()
     assert rel_l2(ev["vels"][live], w2["vels"][live]) < 1e-10
+
+
+def test_wall_driver_steps_vs_reference(ref):
+    """Three steps of the reference driver's order with walls
+    (src/main.c:125-138 without injection and smoothing, whose FFTW is
+    absent): enforce_wall_boundaries, small loops, reconnect, small loops,
+    wrap, remesh, small loops, update_tangle, rk4_step, wrap; direct FP64
+    sums as the reference's update_velocity with tree == NULL.  The same
+    topology after every step and positions within 1e-10 cm."""
+    from test_walls import rings_at_walls
+    t, p = rings_at_walls(seed=9, n_cross=12, n_inside=6)
+    p.rec_dist = 0.003
+    p.rec_velocity = ov.RECVEL_DIRECT
+    dt, bounds = 1e-4, (1.0e-3, 2.6e-3)
+    r = ref.tangle(t, p, wall_images=ov.IMAGES_WALL_PARALLEL_PLANES)
+    stats = []
+    with ov.Engine(p) as e:
+        e.upload(t)
+        tm = 0.0
+        for s in range(3):
+            clip = e.enforce_walls(0.0)
+            e.eliminate_small_loops(5)
+            nrec = e.reconnect(0.0)
+            e.eliminate_small_loops(5)
+            e.enforce_periodic()
+            rem = e.remesh(*bounds)
+            e.eliminate_small_loops(5)
+            e.update_tangle(tm)
+            e.rk4_step(tm, dt)
+            e.enforce_periodic()
+            r.enforce_walls(0.0)
+            r.eliminate_small_loops(5)
+            want_rec = r.reconnect(0.0)
+            r.eliminate_small_loops(5)
+            r.enforce_periodic()
+            r.remesh(*bounds)
+            r.eliminate_small_loops(5)
+            r.update_tangle(tm)
+            r.rk4_step(tm, dt)
+            r.enforce_periodic()
+            tm += dt
+            conn, st = e.download_connections()
+            got = e.download()["vnodes"]
+            want = r.get()
+            stats.append((clip, nrec, rem))
+            assert nrec == want_rec, s
+            assert e.n == r.n, s
+            assert np.array_equal(conn, want["conn"]), s
+            assert np.array_equal(st, r.status()), s
+            live = st[:, 0] != ov.EMPTY
+            assert np.abs(got[live] - want["vnodes"][live]).max() < 1e-10, s
+    say(f"wall driver steps (clipped/removed, reconnections, remesh added/removed): {stats}")
