@@ -40,7 +40,7 @@ extern "C" {
 #define OVG_EINVAL       1   /* bad argument, shape or state */
 #define OVG_ENOMEM       2   /* device or host allocation failed / cap exceeded */
 #define OVG_EDEVICE      3   /* HIP runtime error, no device, missing code object */
-#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. topology passes on pinned lines) */
+#define OVG_EUNSUPPORTED 4   /* feature outside the engine's scope (e.g. topology passes on a forest) */
 #define OVG_ELIMIT       5   /* an internal fixed-size limit was exceeded */
 
 /* ---- node status and wall kinds (reference enums) -------------------- */
@@ -244,8 +244,9 @@ int ovg_get_params(const ovg_ctx *ctx, ovg_params *p);
  * host changed capacity or topology (the reference's expand_tangle
  * reallocates, src/tangle.c:68-101).  FREE slots need both links; PINNED
  * and PINNED_SLIP slots (pin_wall 0..5, OVG_FACE_*) may end a line
- * (link -1).  A tangle with pinned slots is evaluated and stepped; its
- * topology passes return OVG_EUNSUPPORTED.
+ * (link -1).  A tangle with pinned slots is evaluated, stepped, clipped
+ * to its walls (ovg_enforce_walls), reconnected, remeshed and smoothed as
+ * the reference does (open lines: src/vortex_smooth.c:316-444).
  */
 int ovg_upload(ovg_ctx *ctx, int n, const double *vnodes,
                const int *connections, const int *status);
@@ -357,7 +358,9 @@ int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs, int *n_pai
  * reconnect, src/vortex_dynamics.c:186-288: the candidate scan on the GPU,
  * then the reference's greedy pass in (k,l) order with the angle test,
  * the approach test on direct-sum velocities and the length test.  The
- * updated connections stay on the device.
+ * updated connections stay on the device.  Pinned nodes are never
+ * candidates (:198-212); stencils next to a pinned end step through the
+ * wall as step_node does.
  */
 int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections);
 
@@ -400,7 +403,10 @@ int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int
 int ovg_enforce_walls(ovg_ctx *ctx, double t, int *n_clipped, int *n_removed);
 
 /* smooth, src/vortex_smooth.c:459-462 (low_pass_filter per loop and
- * coordinate, src/vortex_smooth.c:134-175,183-453). */
+ * coordinate, src/vortex_smooth.c:134-175,183-453): closed loops (those
+ * spanning the box detrended and odd-extended), and open lines ending on
+ * walls, which are always detrended and odd-extended and keep their ends
+ * (:316-444). */
 int ovg_smooth(ovg_ctx *ctx);
 
 /*
