@@ -101,6 +101,26 @@ extern "C" {
 #define OVG_RECVEL_DIRECT  0
 #define OVG_RECVEL_TREE    1
 
+/* Where the topology passes run (ovg_params.topology).
+ * OVG_TOPO_HOST: the sequential passes walk host mirrors of the links and
+ *   send the changed slots back (every tangle, walls and pinned lines
+ *   included).
+ * OVG_TOPO_DEVICE: reconnection (candidates, endpoint velocities and the
+ *   greedy pass), small-loop removal and remesh run as device kernels on
+ *   the links in HBM and queue without waiting for the host; results are
+ *   identical to OVG_TOPO_HOST.  Taken by one periodic tangle without
+ *   pinned nodes or mirror images whose reconnection velocities are the
+ *   direct sum or the list traversal; any other tangle runs the host
+ *   passes.  The point count and the counts the passes return are read
+ *   back when a later call needs them (one wait per step in the reference
+ *   driver's order), or at once when a count pointer is given.  A device
+ *   pass that meets more than its capacity of candidate pairs (grown
+ *   automatically from the counts seen, up to 32768 pairs and 65536
+ *   endpoints) or finds no empty slot for an insertion stops and the next
+ *   call that waits returns OVG_ELIMIT. */
+#define OVG_TOPO_HOST    0
+#define OVG_TOPO_DEVICE  1
+
 /* External (prescribed) flow, include/external_velocity.h:11-19. */
 typedef struct ovg_flow {
     int32_t type;                 /* OVG_FLOW_*                               */
@@ -188,7 +208,9 @@ typedef struct ovg_params {
     int32_t pin_mode;             /* pin_mode (src/initial_conditions.c:54): the
                                      status of the nodes wall clipping creates,
                                      OVG_PINNED or OVG_PINNED_SLIP (0 = PINNED) */
-    int32_t pad_;
+    int32_t topology;             /* where ovg_reconnect, ovg_eliminate_small_loops
+                                     and ovg_remesh run: OVG_TOPO_HOST (0) or
+                                     OVG_TOPO_DEVICE (see below)               */
 } ovg_params;
 
 /* Fill p with the reference defaults (src/initial_conditions.c:12-91) for a

@@ -38,6 +38,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 TRAVERSE_POINT, TRAVERSE_GROUP, TRAVERSE_LIST = 0, 1, 2
 FIELD_VELS, FIELD_VS, FIELD_TANGENTS, FIELD_NORMALS = 0, 1, 2, 3
 RECVEL_DIRECT, RECVEL_TREE = 0, 1
+TOPO_HOST, TOPO_DEVICE = 0, 1
 
 
 class Flow(C.Structure):
@@ -63,7 +64,7 @@ class Params(C.Structure):
         ("points_per_wave", C.c_int32), ("group_level", C.c_int32),
         ("reuse_lists", C.c_int32), ("image_reflect", C.c_int32 * 26),
         ("list_stack", C.c_int32), ("list_tile", C.c_int32), ("list_blocks", C.c_int32),
-        ("pin_mode", C.c_int32), ("pad_", C.c_int32),
+        ("pin_mode", C.c_int32), ("topology", C.c_int32),
     ]
 
 
@@ -364,22 +365,26 @@ class Engine:
                "ovg_reconnect_candidates")
         return pairs[: n.value].copy()
 
-    def reconnect(self, t=0.0):
+    # The topology passes return their counts; counts=False passes no count
+    # pointer, so that with topology = TOPO_DEVICE the pass queues on the
+    # GPU without waiting (returns None).
+    def reconnect(self, t=0.0, counts=True):
         n = C.c_int32(0)
-        _check(_lib.ovg_reconnect(self._h, t, C.byref(n)), "ovg_reconnect")
-        return n.value
+        _check(_lib.ovg_reconnect(self._h, t, C.byref(n) if counts else None), "ovg_reconnect")
+        return n.value if counts else None
 
-    def eliminate_small_loops(self, cutoff=5):
+    def eliminate_small_loops(self, cutoff=5, counts=True):
         n = C.c_int32(0)
-        _check(_lib.ovg_eliminate_small_loops(self._h, cutoff, C.byref(n)),
+        _check(_lib.ovg_eliminate_small_loops(self._h, cutoff, C.byref(n) if counts else None),
                "ovg_eliminate_small_loops")
-        return n.value
+        return n.value if counts else None
 
-    def remesh(self, min_dist, max_dist):
+    def remesh(self, min_dist, max_dist, counts=True):
         a, r = C.c_int32(0), C.c_int32(0)
-        _check(_lib.ovg_remesh(self._h, min_dist, max_dist, C.byref(a), C.byref(r)), "ovg_remesh")
+        _check(_lib.ovg_remesh(self._h, min_dist, max_dist, C.byref(a) if counts else None,
+                               C.byref(r) if counts else None), "ovg_remesh")
         self.n = _lib.ovg_num_slots(self._h)
-        return a.value, r.value
+        return (a.value, r.value) if counts else None
 
     def enforce_walls(self, t=0.0):
         """enforce_wall_boundaries (src/tangle.c:966-1016): returns (clipped,

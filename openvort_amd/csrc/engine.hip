@@ -616,6 +616,11 @@ static int upload_topology(Ctx &c, const int *conn, const int *status)
     int n = c.n;
     c.rec_ok = false;
     c.topo_version++;
+    // the host's arrays replace whatever the device passes left
+    c.dt_unread = false;
+    c.live_pending = false;
+    c.mirrors_stale = false;
+    c.hint_on_device = false;
     if (conn != c.h_conn.data()) c.h_conn.assign(conn, conn + 2 * (size_t)n);
     if (status != c.h_status.data()) c.h_status.assign(status, status + 2 * (size_t)n);
     // the whole state is on the device now; every loop is new to the
@@ -826,6 +831,7 @@ extern "C" int ovg_graph_begin(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     if (c.capturing) return set_error(OVG_EINVAL, "a capture is already open on this context");
     if (c.timing)
@@ -915,6 +921,7 @@ extern "C" int ovg_upload_positions(ovg_ctx *ctx, const double *vnodes)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return launch_status(upload_positions(c, vnodes));
@@ -924,6 +931,7 @@ extern "C" int ovg_upload_connections(ovg_ctx *ctx, const int *connections, cons
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     OVG_TRY(validate_topology(c.n, connections, status));
@@ -935,6 +943,7 @@ extern "C" int ovg_upload_field(ovg_ctx *ctx, int field, const double *values)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     if (!values) return set_error(OVG_EINVAL, "values is NULL");
     Ctx &c = ctx->c;
     DBuf *dst;
@@ -977,6 +986,7 @@ extern "C" int ovg_download(ovg_ctx *ctx, double *vnodes, double *vels, double *
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int n = c.n;
@@ -998,6 +1008,7 @@ extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *sta
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int n = c.n;
@@ -1015,12 +1026,21 @@ extern "C" int ovg_download_connections(ovg_ctx *ctx, int *connections, int *sta
 
 extern "C" int ovg_num_slots(const ovg_ctx *ctx) { return ctx ? ctx->c.n : -1; }
 
-extern "C" int ovg_num_points(const ovg_ctx *ctx) { return ctx ? ctx->c.n_live : 0; }
+// The live count after device topology passes is read from the device
+// (waits); a failed read reports 0 with the error in ovg_last_error.
+extern "C" int ovg_num_points(const ovg_ctx *ctx)
+{
+    if (!ctx) return 0;
+    Ctx &c = const_cast<ovg_ctx *>(ctx)->c;
+    if (c.dt_unread && sync_live(c) != OVG_OK) return 0;
+    return c.n_live;
+}
 
 extern "C" int ovg_update_tangents_normals(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     Triple x = tri(c.x), t = tri(c.tan), nrm = tri(c.nor);
@@ -1041,6 +1061,7 @@ extern "C" int ovg_update_velocities(ovg_ctx *ctx, double t)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return launch_status(velocities(c, t, false));
@@ -1050,6 +1071,7 @@ extern "C" int ovg_update_tangle(ovg_ctx *ctx, double t)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     return launch_status(velocities(c, t, true));
@@ -1059,6 +1081,7 @@ extern "C" int ovg_rk4_step(ovg_ctx *ctx, double t, double dt)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     double h4 = c.params.rk4_classic ? dt : dt / 2;
@@ -1086,6 +1109,7 @@ extern "C" int ovg_enforce_periodic(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     launch_wrap(c);
@@ -1097,6 +1121,7 @@ extern "C" int ovg_velocity_at(ovg_ctx *ctx, int m, const int *slots, double t, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !slots)) return set_error(OVG_EINVAL, "bad slot list");
@@ -1161,6 +1186,7 @@ extern "C" int ovg_velocity_at_noalloc(ovg_ctx *ctx, int m, const int *d_slots, 
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     if (m < 0 || (m > 0 && !d_slots)) return set_error(OVG_EINVAL, "bad slot list");
@@ -1205,6 +1231,7 @@ extern "C" int ovg_line_length(ovg_ctx *ctx, double *length, int *n_points)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
     int gs = grid_for(c.n_live);
@@ -1228,6 +1255,7 @@ extern "C" int ovg_get_stats(ovg_ctx *ctx, ovg_stats *s)
     clear_error();
     if (!ctx || !s) return set_error(OVG_EINVAL, "NULL argument");
     Ctx &c = ctx->c;
+    OVG_TRY(sync_live(c));
     hipSetDevice(c.device);
     return launch_status(fetch_stats(c, s));
 }
@@ -1283,6 +1311,7 @@ extern "C" int ovg_reconnect_candidates(ovg_ctx *ctx, int max_pairs, int *pairs,
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     if (max_pairs < 0 || (max_pairs > 0 && !pairs))
         return set_error(OVG_EINVAL, "bad pair buffer");
@@ -1298,6 +1327,16 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
+    if (devtopo_applies(c)) {
+        OVG_TRY(devtopo_reconnect(c, t));
+        if (n_reconnections) {
+            int cnt[4];
+            OVG_TRY(devtopo_counts(c, cnt));
+            *n_reconnections = cnt[0];
+        }
+        return launch_status();
+    }
+    OVG_TRY(sync_mirrors(c));
     int nrec = 0;
     OVG_TRY(reconnect_pass(c, t, &nrec));
     if (n_reconnections) *n_reconnections = nrec;
@@ -1308,6 +1347,7 @@ extern "C" int ovg_smooth(ovg_ctx *ctx)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
@@ -1327,6 +1367,17 @@ extern "C" int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_remove
     OVG_TRY(check_ready(ctx));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
+    hipSetDevice(c.device);
+    if (devtopo_applies(c)) {
+        OVG_TRY(devtopo_small_loops(c, cutoff));
+        if (n_removed) {
+            int cnt[4];
+            OVG_TRY(devtopo_counts(c, cnt));
+            *n_removed = cnt[1];
+        }
+        return launch_status();
+    }
+    OVG_TRY(sync_mirrors(c));
     int n = c.n;
     std::vector<int> &conn = c.h_conn, &st = c.h_status;
     int removed = 0;
@@ -1462,6 +1513,19 @@ static int grow_slots(Ctx &c, int n_new, const double *X)
     hipStreamSynchronize(c.stream);
     keep.release();
     return r;
+}
+
+int ovg::grow_capacity(Ctx &c, int n_new)
+{
+    const int n_old = c.n;
+    if (n_new <= n_old) return OVG_OK;
+    const double *Xp = nullptr;
+    OVG_TRY(host_positions(c, &Xp));
+    std::vector<double> X(3 * (size_t)n_new, 0.0);
+    std::memcpy(X.data(), Xp, sizeof(double) * 3 * (size_t)n_old);
+    c.h_conn.resize(2 * (size_t)n_new, -1);
+    c.h_status.resize(2 * (size_t)n_new, -1);
+    return grow_slots(c, n_new, X.data());
 }
 
 // ---- remesh (src/tangle.c:1018-1061) on the host mirrors ----------------
@@ -1853,6 +1917,7 @@ extern "C" int ovg_enforce_walls(ovg_ctx *ctx, double t, int *n_clipped, int *n_
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
@@ -1880,6 +1945,17 @@ extern "C" int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n
     hipSetDevice(c.device);
     if (n_added) *n_added = 0;
     if (n_removed) *n_removed = 0;
+    if (devtopo_applies(c)) {
+        OVG_TRY(devtopo_remesh(c, min_dist, max_dist));
+        if (n_added || n_removed) {
+            int cnt[4];
+            OVG_TRY(devtopo_counts(c, cnt));
+            if (n_added) *n_added = cnt[2];
+            if (n_removed) *n_removed = cnt[3];
+        }
+        return launch_status();
+    }
+    OVG_TRY(sync_mirrors(c));
     std::vector<int> cands;
     OVG_TRY(remesh_candidates(c, min_dist, max_dist, cands));
     if (cands.empty()) return OVG_OK;
@@ -1981,6 +2057,7 @@ extern "C" int ovg_add_loop(ovg_ctx *ctx, const double center[3], const double d
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
@@ -2039,6 +2116,7 @@ extern "C" int ovg_add_line_z(ovg_ctx *ctx, double x, double y, int direction, d
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_mirrors(ctx->c));
     OVG_TRY(single_tangle(ctx->c, __func__));
     Ctx &c = ctx->c;
     hipSetDevice(c.device);
@@ -2107,6 +2185,7 @@ extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
 {
     clear_error();
     OVG_TRY(check_ready(ctx));
+    OVG_TRY(sync_live(ctx->c));
     if (!out || repeats < 1) return set_error(OVG_EINVAL, "bad arguments to ovg_autotune");
     Ctx &c = ctx->c;
     if (c.params.method != OVG_METHOD_BH || c.params.traverse != OVG_TRAVERSE_LIST)

@@ -177,6 +177,22 @@ struct Ctx {
     DBuf sm_gstate;                     // per big loop state, per chunk wrap sums
     DBuf sm_tw;                         // e^{-2 pi i k / 2^sm_tw_log}, k < 2^(sm_tw_log-1)
     int sm_tw_log = 0;
+    // Device-resident topology (ovg_params.topology == OVG_TOPO_DEVICE):
+    // the reconnection, small-loop and remesh passes edit the device links,
+    // status and positions in place and queue without waiting; the live
+    // count and the host mirrors catch up when a call needs them
+    // (sync_live, sync_mirrors).
+    DBuf dt_state;                      // int counters of the device passes (k_devtopo.hip)
+    DBuf dt_where;                      // int per slot: endpoint index, -1 between passes
+    DBuf dt_mark;                       // int per slot, 0 between passes
+    DBuf dt_pairs, dt_ends, dt_chg;     // reconnection pass scratch
+    bool dt_unread = false;             // device passes ran since the counters were read
+    bool live_pending = false;          // the device's live list is newer than n_live
+    bool mirrors_stale = false;         // h_conn / h_status / next_empty are older
+    bool hint_on_device = false;        // the device holds the allocator hint
+    int dt_cap = 0;                     // candidate pairs a device pass takes
+    int dt_seen = 0;                    // most candidate pairs a device pass met
+    int dt_ends_seen = 0;               // most endpoints a device pass met
     ovg_stats stats{};
     hipEvent_t ev[8] = {};
     // per-evaluation timing events, resolved when statistics are read so
@@ -198,12 +214,20 @@ void launch_compose(Ctx &c, const double *const x[3], bool own_tangents, int sta
 void launch_stage_positions(Ctx &c, const double *const kv[3], double h,
                             double *const out[3]);
 void launch_wrap(Ctx &c);
+// enforce_periodic_boundaries over every slot that is not EMPTY, queued
+// only when the device int *d_gate is nonzero (no wait).
+void launch_wrap_slots(Ctx &c, const int *d_gate);
 void launch_line_length(Ctx &c, double *d_out);
 void launch_check_finite(Ctx &c, const double *const x[3]);
 
 // k_direct.hip
 int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
                   double *const out_prim[3], double *const out_img[3], int chunks);
+// The same sum for the targets d_tgt_slots[0 .. *d_ntgt) counted on the
+// device, *d_ntgt <= n_bound; source chunks planned for about n_est targets.
+int launch_direct_counted(Ctx &c, const double *const x[3], int n_bound, int n_est,
+                          const int *d_tgt_slots, const int *d_ntgt, double *const out_prim[3],
+                          double *const out_img[3]);
 // The same sum with its chunk partials in the device's static scratch array
 // (no allocation): d_slots is checked into scratch first (a slot out of
 // range or EMPTY becomes -1 and sets bit 4 of the device flags); *checked
@@ -236,6 +260,11 @@ int list_points_per_wave(const ovg_params &p);
 // The list traversal for m live slots (device list) against the tree built
 // at x: grouped by tree order within level-L0 cells.
 int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d_slots);
+// The list traversal of target groups already built on the device: d_grp
+// (first index into d_tgt, count) per group, their number at *d_ngroups,
+// at most max_groups.
+int launch_bh_list_groups(Ctx &c, const double *const x[3], const int2 *d_grp, const int *d_tgt,
+                          const int *d_ngroups, int max_groups);
 
 // k_topo.hip
 // Pinned host copy of the positions (AoS, c.n slots), valid until the next
@@ -255,9 +284,39 @@ int commit_topology(Ctx &c);
 // Remesh candidates: live slots whose original-link decision may remove or
 // insert (a superset of the slots the sequential pass acts on first).
 int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out);
+// The same candidates queued on the device: the ascending slot list at
+// *d_list, its length at *d_count (no wait).
+int remesh_candidates_queue(Ctx &c, double min_dist, double max_dist, const int **d_list,
+                            const int **d_count);
 
 // k_recon.hip
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);
+// Queue the candidate scan: pairs at c.rc_pairs (unordered, at most
+// c.rc_cap of them), their full count at c.rc_count[0]; no wait.
+int candidates_queue(Ctx &c);
+
+// k_topo.hip: ordered live list over every slot, rebuilt on the device;
+// the count goes to d_count (device int).  No wait.
+int rebuild_live(Ctx &c, int *d_count);
+
+// k_devtopo.hip: the device-resident topology passes
+// (ovg_params.topology == OVG_TOPO_DEVICE)
+bool devtopo_applies(const Ctx &c);   // the tangle qualifies (one periodic tangle, no pins)
+int devtopo_reconnect(Ctx &c, double t);
+int devtopo_small_loops(Ctx &c, int cutoff);
+int devtopo_remesh(Ctx &c, double min_dist, double max_dist);
+// The counters of the last device passes (waits): reconnections, small-loop
+// points removed, remesh points added and removed.
+int devtopo_counts(Ctx &c, int out[4]);
+// engine.hip: grow the slots to n_new (> n) with the host mirrors
+// current, the new slots EMPTY; every old slot keeps its state.
+int grow_capacity(Ctx &c, int n_new);
+// n_live / n_src from the device after device passes changed the live set
+// (waits once); reports an error a device pass left.
+int sync_live(Ctx &c);
+// h_conn / h_status / next_empty from the device after device passes
+// (waits); implies sync_live.
+int sync_mirrors(Ctx &c);
 
 // k_smooth.hip
 int smooth_loops(Ctx &c);

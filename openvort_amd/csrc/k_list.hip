@@ -858,14 +858,26 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     OVG_HIP(hipMemcpyAsync(d_tgt, tgt.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice,
                            c.stream));
     OVG_HIP(hipMemcpyAsync(T.sub_count.p, &ng, sizeof(int), hipMemcpyHostToDevice, c.stream));
+    OVG_TRY(launch_bh_list_groups(c, x, d_grp, d_tgt, T.sub_count.as<int>(), ng));
+    // the host vectors must outlive the copies above
+    OVG_HIP(hipStreamSynchronize(c.stream));
+    return OVG_OK;
+}
+
+int launch_bh_list_groups(Ctx &c, const double *const x[3], const int2 *d_grp, const int *d_tgt,
+                          const int *d_ngroups, int max_groups)
+{
+    if (max_groups <= 0) return OVG_OK;
+    TreeBufs &T = c.tree;
+    const int ppw = list_points_per_wave(c.params);
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(list_grid(c, ng)), dim3(64), 0, c.stream, nullptr, nullptr, 0, 0, d_grp,
-                           d_tgt, x[0], x[1],
+        hipLaunchKernelGGL(kern, dim3(list_grid(c, max_groups)), dim3(64), 0, c.stream, nullptr,
+                           nullptr, 0, 0, d_grp, d_tgt, x[0], x[1],
                            x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                            T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
                            T.sqy.as<double>(), T.sqz.as<double>(), T.counters.as<int>(),
-                           T.sub_count.as<int>(), dparams(c), T.roots.as<int>(),
+                           d_ngroups, dparams(c), T.roots.as<int>(),
                            c.bs_prim[0].as<double>(), c.bs_prim[1].as<double>(),
                            c.bs_prim[2].as<double>(), c.bs_img[0].as<double>(),
                            c.bs_img[1].as<double>(), c.bs_img[2].as<double>(), cnt,
@@ -883,9 +895,7 @@ int launch_bh_list_targets(Ctx &c, const double *const x[3], int m, const int *d
     default: mixed ? OVG_LIST_WALK(true, 64) : OVG_LIST_WALK(false, 64); break;
     }
 #undef OVG_LIST_WALK
-    // the host vectors must outlive the copies above
-    OVG_HIP(hipStreamSynchronize(c.stream));
-    return OVG_OK;
+    return launch_status();
 }
 
 }  // namespace ovg

@@ -161,6 +161,42 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
             }
 }
 
+int candidates_queue(Ctx &c)
+{
+    const int n = c.n_live;
+    OVG_TRY(c.rc_count.ensure(sizeof(int) * 64));
+    if (c.rc_cap == 0) c.rc_cap = 1 << 16;
+    OVG_TRY(c.rc_pairs.ensure(sizeof(int2) * (size_t)c.rc_cap));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, 1, c.rc_count.as<int>(), 0);
+    if (n == 0) return OVG_OK;
+    Grid g = make_grid(c.params);
+    int ncell = g.nc[0] * g.nc[1] * g.nc[2];
+    TreeBufs &T = c.tree;
+    for (int b = 0; b < 2; ++b) {
+        OVG_TRY(T.keys[b].ensure(sizeof(uint64_t) * (n + 1)));
+        OVG_TRY(T.vals[b].ensure(sizeof(int) * (n + 1)));
+    }
+    OVG_TRY(c.rc_start.ensure(sizeof(int) * 2 * (size_t)ncell));
+    int bs = 256, gs = (n + bs - 1) / bs;
+    ++c.tree_id;  // the sort below reuses the tree's key buffers
+    const double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
+    hipLaunchKernelGGL(k_cell_keys, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(), X, Y,
+                       Z, dparams(c), g, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
+    OVG_TRY(radix_sort(c, n, 0, 24));
+    int *start = c.rc_start.as<int>(), *end = start + ncell;
+    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, ncell,
+                       start, INT_MAX);
+    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, ncell, end,
+                       0);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
+                       T.keys[0].as<uint64_t>(), start, end);
+    hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
+                       T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
+                       c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
+                       c.rc_count.as<int>(), c.rc_pairs.as<int2>(), c.rc_cap);
+    return OVG_OK;
+}
+
 // Candidate pairs in the reference's scan order (k ascending, then l).
 static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
 {

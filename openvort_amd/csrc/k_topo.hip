@@ -143,6 +143,12 @@ __global__ void k_live_flags(int n, const int *__restrict__ status, int *__restr
     if (i < n) flag[i] = status[i] != OVG_EMPTY;
 }
 
+__global__ void k_zero_slots(int n, int *__restrict__ a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 __global__ void k_live_scatter(int n, const int *__restrict__ status,
                                const int *__restrict__ pos, int *__restrict__ live,
                                int *__restrict__ src)
@@ -151,6 +157,21 @@ __global__ void k_live_scatter(int n, const int *__restrict__ status,
     if (i >= n || status[i] == OVG_EMPTY) return;
     live[pos[i]] = i;
     src[pos[i]] = i;
+}
+
+int rebuild_live(Ctx &c, int *d_count)
+{
+    const int n = c.n;
+    OVG_TRY(c.dt_mark.ensure(sizeof(int) * (size_t)(n + 1)));
+    int *pos = c.dt_mark.as<int>();
+    hipLaunchKernelGGL(k_live_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.status.as<int>(), pos);
+    OVG_TRY(launch_exclusive_scan(c, pos, pos, n, d_count, c.tree.scan_tmp));
+    hipLaunchKernelGGL(k_live_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       c.status.as<int>(), pos, c.live.as<int>(), c.src.as<int>());
+    // dt_mark is zero between passes
+    hipLaunchKernelGGL(k_zero_slots, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, pos);
+    return OVG_OK;
 }
 
 int commit_topology(Ctx &c)
@@ -193,11 +214,7 @@ int commit_topology(Ctx &c)
     // ordered live list on the device
     OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)(n + 1)));
     int *flag = c.rc_tmp.as<int>();
-    hipLaunchKernelGGL(k_live_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
-                       c.status.as<int>(), flag);
-    OVG_TRY(launch_exclusive_scan(c, flag, flag, n, flag + n, c.tree.scan_tmp));
-    hipLaunchKernelGGL(k_live_scatter, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
-                       c.status.as<int>(), flag, c.live.as<int>(), c.src.as<int>());
+    OVG_TRY(rebuild_live(c, flag + n));
     OVG_HIP(hipMemcpyAsync(&live, flag + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
     c.n_live = live;
@@ -256,11 +273,10 @@ __global__ void k_compact(int n, const int *__restrict__ flag, const int *__rest
     if (k < n && flag[k]) out[pos[k]] = k;
 }
 
-int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out)
+int remesh_candidates_queue(Ctx &c, double min_dist, double max_dist, const int **d_list,
+                            const int **d_count)
 {
     const int n = c.n;
-    out.clear();
-    if (n == 0) return OVG_OK;
     OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)(n + 1)));
     OVG_TRY(c.rc_count.ensure(sizeof(int) * (size_t)(n + 1)));
     OVG_TRY(c.rc_sorted.ensure(sizeof(int) * (size_t)(n + 1)));
@@ -273,6 +289,19 @@ int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int>
     OVG_TRY(launch_exclusive_scan(c, flag, pos, n, pos + n, c.tree.scan_tmp));
     hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, flag, pos,
                        lst);
+    *d_list = lst;
+    *d_count = pos + n;
+    return OVG_OK;
+}
+
+int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out)
+{
+    const int n = c.n;
+    out.clear();
+    if (n == 0) return OVG_OK;
+    const int *lst = nullptr, *cntp = nullptr;
+    OVG_TRY(remesh_candidates_queue(c, min_dist, max_dist, &lst, &cntp));
+    const int *pos = cntp - n;
     int cnt = 0;
     OVG_HIP(hipMemcpyAsync(&cnt, pos + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
