@@ -52,15 +52,17 @@ def run_steps(e, n_steps, t0, dt, step0, smooth_every, reconnect, remesh=None):
     e.marker_record(0)
     for s in range(n_steps):
         k = step0 + s
+        # no count pointers: with device topology the passes queue without
+        # waiting for the host
         if reconnect:
-            e.reconnect(tm)
-            e.eliminate_small_loops(5)
+            e.reconnect(tm, counts=False)
+            e.eliminate_small_loops(5, counts=False)
         if smooth_every and k % smooth_every == 0:
             e.smooth()
         e.enforce_periodic()
         if remesh:
-            e.remesh(*remesh)
-            e.eliminate_small_loops(5)
+            e.remesh(*remesh, counts=False)
+            e.eliminate_small_loops(5, counts=False)
         e.update_tangle(tm)
         e.rk4_step(tm, dt)
         e.enforce_periodic()
@@ -453,6 +455,10 @@ def main():
     ap.add_argument("--blocks", type=int, default=0,
                     help="list traversal persistent grid, wavefronts per CU (0 = one per group)")
     ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])
+    ap.add_argument("--topology", default="host", choices=["device", "host"],
+                    help="where reconnection, small-loop removal and remesh run")
+    ap.add_argument("--other-topo-steps", type=int, default=3,
+                    help="also time this many steps with the other topology mode (0 = skip)")
     ap.add_argument("--smooth-every", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--reuse-steps", type=int, default=5,
@@ -523,6 +529,7 @@ def main():
     recv = ov.RECVEL_DIRECT if args.rec_velocity == "direct" else ov.RECVEL_TREE
     t, p, dt = configs.make(args.workload, precision=prec, traverse=trav, leaf_size=args.leaf)
     p.rec_velocity = recv
+    p.topology = ov.TOPO_DEVICE if args.topology == "device" else ov.TOPO_HOST
     p.points_per_wave = args.ppw
     p.group_level = args.group_level
     p.list_stack, p.list_tile, p.list_blocks = args.stack, args.tile, args.blocks
@@ -599,6 +606,34 @@ def main():
                          "(acceptance decided at the step's first stage; not the "
                          "headline value)"}
         log(f"reuse_lists: {reuse['ms_per_step']:.1f} ms/step")
+
+    # ---- the same path with the topology passes in the other mode -------
+    other_topo = None
+    if args.other_topo_steps > 0:
+        qh = ov.Params.from_buffer_copy(p_run)
+        dev = p_run.topology != ov.TOPO_DEVICE
+        qh.topology = ov.TOPO_DEVICE if dev else ov.TOPO_HOST
+        e5 = ov.Engine(qh)
+        e5.upload(t)
+        e5.synchronize()
+        _, tm5 = run_steps(e5, args.warmup, 0.0, dt, 0, args.smooth_every, True, rb)
+        e5.synchronize()
+        a = time.perf_counter()
+        lat5, _ = run_steps(e5, args.other_topo_steps, tm5, dt, args.warmup, args.smooth_every,
+                            True, rb)
+        e5.synchronize()
+        w5 = time.perf_counter() - a
+        e5.close()
+        other_topo = {"value": n_pts * args.other_topo_steps / w5,
+                      "steps": args.other_topo_steps,
+                      "ms_per_step": w5 / args.other_topo_steps * 1e3,
+                      "p50_step_ms": pct(lat5, 50), "p99_step_ms": pct(lat5, 99),
+                      "topology": "device" if dev else "host",
+                      "path": "as the headline path, with reconnection, small-loop removal and "
+                              + ("remesh as device kernels that queue without waiting for the "
+                                 "host (topology = device)" if dev else
+                                 "remesh as sequential passes on host mirrors (topology = host)")}
+        log(f"{other_topo['topology']} topology: {other_topo['ms_per_step']:.1f} ms/step")
 
     # ---- the same path with the reference's reconnection velocities ------
     rec_direct = None
@@ -693,7 +728,8 @@ def main():
                    "leaf_size": p_run.leaf_size, "points_per_wave": p_run.points_per_wave,
                    "group_level": p_run.group_level, "list_stack": p_run.list_stack or 384,
                    "list_tile": p_run.list_tile or 128, "list_blocks": p_run.list_blocks,
-                   "rec_velocity": args.rec_velocity, "autotune": tune},
+                   "rec_velocity": args.rec_velocity, "topology": args.topology,
+                   "autotune": tune},
         "p50_step_ms": pct(lat, 50), "p99_step_ms": pct(lat, 99),
         "interactions_per_s": interactions_per_s,
         "direct_equivalent_interactions_per_s": 27.0 * n_pts * n_pts * 4 * args.steps / wall,
@@ -701,6 +737,7 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "reuse_lists": reuse,
+        "other_topology": other_topo,
         "rec_velocity_direct": rec_direct,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,

@@ -410,6 +410,17 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
  * whole state when the slot count doubled). */
 int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
 
+/* The 16 counters of the device topology passes (OVG_TOPO_DEVICE), read
+ * after the passes queued so far (waits): [0] live points, [1] error bits,
+ * [2] allocator hint, [3] scratch, [4] reconnections, [5] points removed by
+ * small-loop removal, [6] points added and [7] removed by remesh, [8]
+ * candidate pairs taken and [9] found, [10] their endpoints, [11] rewired
+ * segments, [12] endpoint target groups, [13] slots remesh visited, [14]
+ * endpoint velocities the greedy pass recomposed, [15] endpoints whose
+ * rewirable neighbourhood overflowed its list.  Diagnostics, no
+ * counterpart in the reference. */
+int ovg_topology_counters(ovg_ctx *ctx, int *out);
+
 /* enforce_wall_boundaries, src/tangle.c:966-1016, at time t: for every
  * solid wall (not OPEN, not PERIODIC) in face order and every FREE point in
  * slot order within rec_dist of it (check_wall, :822-840), the point's

@@ -134,6 +134,7 @@ _sig("ovg_reconnect", C.c_int, C.c_void_p, C.c_double, _ip)
 _sig("ovg_smooth", C.c_int, C.c_void_p)
 _sig("ovg_eliminate_small_loops", C.c_int, C.c_void_p, C.c_int, _ip)
 _sig("ovg_remesh", C.c_int, C.c_void_p, C.c_double, C.c_double, _ip, _ip)
+_sig("ovg_topology_counters", C.c_int, C.c_void_p, _ip)
 _sig("ovg_enforce_walls", C.c_int, C.c_void_p, C.c_double, _ip, _ip)
 _sig("ovg_loop_points", C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip)
 _sig("ovg_inject_loop_draw", C.c_int, C.c_double, C.c_double, _dp, _dp, _P(C.c_ushort), _dp, _dp,
@@ -385,6 +386,12 @@ class Engine:
                                C.byref(r) if counts else None), "ovg_remesh")
         self.n = _lib.ovg_num_slots(self._h)
         return (a.value, r.value) if counts else None
+
+    def topology_counters(self):
+        """The device topology passes' 16 counters (ovg_topology_counters)."""
+        out = np.zeros(16, dtype=np.int32)
+        _check(_lib.ovg_topology_counters(self._h, _i(out)), "ovg_topology_counters")
+        return out
 
     def enforce_walls(self, t=0.0):
         """enforce_wall_boundaries (src/tangle.c:966-1016): returns (clipped,

@@ -28,7 +28,7 @@ __device__ __forceinline__ double dist3(double ax, double ay, double az, double 
 // is mirrored through its wall (mirror_shift, src/boundaries.c:159-193);
 // the mirrors of nested ends apply innermost first.  A pinned node with no
 // link at all has no stencil: NaN.
-inline __device__ void step_pos(const TangleParams &P, const double *__restrict__ X,
+static __device__ void step_pos(const TangleParams &P, const double *__restrict__ X,
                          const double *__restrict__ Y, const double *__restrict__ Z,
                          const int *__restrict__ fwd, const int *__restrict__ rev,
                          const int *__restrict__ st, const int *__restrict__ pinw, int k,
@@ -60,7 +60,7 @@ inline __device__ void step_pos(const TangleParams &P, const double *__restrict_
 // update_tangent_normal's arithmetic, src/tangle.c:446-496, from the
 // positions of the five stencil points s0 (the point), s1, s2 (one and two
 // links forward) and m1, m2 (backward).
-__device__ __forceinline__ void tangent_normal_pts(
+static __device__ void tangent_normal_pts(
     const TangleParams &P, double s0x, double s0y, double s0z, double s1x, double s1y,
     double s1z, double s2x, double s2y, double s2z, double m1x, double m1y, double m1z,
     double m2x, double m2y, double m2z, double t[3], double nrm[3])
@@ -114,7 +114,7 @@ __device__ __forceinline__ void tangent_normal_pts(
 
 // update_tangent_normal, src/tangle.c:423-502.  pinw == nullptr: no pinned
 // nodes, every neighbour is a link away.
-inline __device__ void tangent_normal(const TangleParams &P, const double *__restrict__ X,
+static __device__ void tangent_normal(const TangleParams &P, const double *__restrict__ X,
                                const double *__restrict__ Y, const double *__restrict__ Z,
                                const int *__restrict__ fwd, const int *__restrict__ rev,
                                const int *__restrict__ st, const int *__restrict__ pinw,

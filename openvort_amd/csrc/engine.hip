@@ -1343,6 +1343,15 @@ extern "C" int ovg_reconnect(ovg_ctx *ctx, double t, int *n_reconnections)
     return launch_status();
 }
 
+extern "C" int ovg_topology_counters(ovg_ctx *ctx, int *out)
+{
+    clear_error();
+    OVG_TRY(check_ready(ctx, false));
+    if (!out) return set_error(OVG_EINVAL, "out is NULL");
+    hipSetDevice(ctx->c.device);
+    return launch_status(devtopo_counters(ctx->c, out));
+}
+
 extern "C" int ovg_smooth(ovg_ctx *ctx)
 {
     clear_error();

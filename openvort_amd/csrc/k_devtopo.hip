@@ -42,12 +42,15 @@ enum {
     DT_ENDS,       // their distinct endpoints
     DT_CHANGES,    // segments rewired by the last reconnection pass
     DT_GROUPS,     // endpoint target groups of the list traversal
+    DT_VISITS,     // slots the last remesh visited
+    DT_SLOW,       // endpoint velocities the last greedy pass recomposed
+    DT_NEAR_OVER,  // endpoints with more rewirable pairs near them than kNear
     DT_COUNT = 16
 };
 constexpr int kErrPairs = 1, kErrEnds = 2, kErrSlots = 4, kErrHeap = 8;
 constexpr int kMaxPairs = 32768, kMaxEnds = 65536;
-constexpr int kNear = 8;          // tree mode: rewirable pairs listed per endpoint
-constexpr int kHeapLds = 8192;    // remesh: top of the heap of touched slots, in LDS
+constexpr int kNear = 32;         // tree mode: rewirable pairs listed per endpoint
+constexpr int kHeapLds = 4096;    // remesh: top of the heap of touched slots, in LDS
 constexpr int kGroupSort = 8192;  // endpoints grouped by tree order in one LDS sort
 
 template <class T>
@@ -73,6 +76,12 @@ __global__ void k_dt_fill(int n, int *__restrict__ a, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+__global__ void k_dt_iota(int n, int *__restrict__ a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
 }
 
 // one atomic per wavefront for a per-lane count
@@ -124,10 +133,18 @@ __global__ void k_dt_loop_erase(int n, const int *__restrict__ flag, int *__rest
 // lowest EMPTY slot (n when none): the start of get_next_empty_node's scan
 __global__ void k_dt_first_empty(int n, const int *__restrict__ st, int *__restrict__ state)
 {
+    __shared__ int wmin[4];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long b = __ballot(k < n && st[k] == OVG_EMPTY);
-    if (b && (threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1)
-        atomicMin(state + DT_LO, k - (int)(threadIdx.x & 63) + __ffsll((long long)b) - 1);
+    if (lane == 0) wmin[w] = b ? k + __ffsll((long long)b) - 1 : INT_MAX;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = INT_MAX;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = min(m, wmin[i]);
+        // blocks run roughly in order: most find a lower value already there
+        if (m < INT_MAX && m < ld(state + DT_LO)) atomicMin(state + DT_LO, m);
+    }
 }
 
 struct RemeshArgs {
@@ -143,41 +160,161 @@ struct RemeshArgs {
     int *state;
 };
 
-__device__ __forceinline__ void dt_pos(const RemeshArgs &a, int k, double &x, double &y,
-                                       double &z)
-{
-    x = ld(a.X + k);
-    y = ld(a.Y + k);
-    z = ld(a.Z + k);
-}
+// The slot state the one-wavefront passes read and write, cached in LDS:
+// direct-mapped by slot, write-through (every store goes to the cache and
+// to global memory), so a miss always finds the current value in global
+// memory.
+constexpr int kCache = 2048;
+struct SlotCache {
+    int slot[kCache], fwd[kCache], rev[kCache], st[kCache], claim[kCache];
+    double x[kCache], y[kCache], z[kCache];
+};
 
-// update_tangent_normal (src/tangle.c:423-502) with the current links
-__device__ void dt_stencil(const RemeshArgs &a, const TangleParams &P, int k, double t[3],
-                           double nr[3])
-{
-    const int i1 = ld(a.fwd + k), im1 = ld(a.rev + k);
-    const int i2 = ld(a.fwd + i1), im2 = ld(a.rev + im1);
-    double p[5][3];
-    const int s[5] = {k, i1, i2, im1, im2};
-    for (int j = 0; j < 5; ++j) dt_pos(a, s[j], p[j][0], p[j][1], p[j][2]);
-    tangent_normal_pts(P, p[0][0], p[0][1], p[0][2], p[1][0], p[1][1], p[1][2], p[2][0], p[2][1],
-                       p[2][2], p[3][0], p[3][1], p[3][2], p[4][0], p[4][1], p[4][2], t, nr);
-}
+struct RemeshCtx {
+    const RemeshArgs &a;
+    SlotCache &C;
+    // entry of slot s (loaded on a miss)
+    __device__ __forceinline__ int get(int s)
+    {
+        const int h = s & (kCache - 1);
+        if (C.slot[h] != s) {
+            const int f = ld(a.fwd + s), r = ld(a.rev + s), q = ld(a.st + s);
+            const double x = ld(a.X + s), y = ld(a.Y + s), z = ld(a.Z + s);
+            C.slot[h] = s;
+            C.fwd[h] = f;
+            C.rev[h] = r;
+            C.st[h] = q;
+            C.x[h] = x;
+            C.y[h] = y;
+            C.z[h] = z;
+        }
+        return h;
+    }
+    __device__ __forceinline__ int fwd(int s) { return C.fwd[get(s)]; }
+    __device__ __forceinline__ int rev(int s) { return C.rev[get(s)]; }
+    __device__ __forceinline__ int st(int s) { return C.st[get(s)]; }
+    __device__ __forceinline__ void pos(int s, double &x, double &y, double &z)
+    {
+        const int h = get(s);
+        x = C.x[h];
+        y = C.y[h];
+        z = C.z[h];
+    }
+    __device__ __forceinline__ void set_fwd(int s, int v)
+    {
+        C.fwd[get(s)] = v;
+        a.fwd[s] = v;
+    }
+    __device__ __forceinline__ void set_rev(int s, int v)
+    {
+        C.rev[get(s)] = v;
+        a.rev[s] = v;
+    }
+    __device__ __forceinline__ void set_st(int s, int v)
+    {
+        C.st[get(s)] = v;
+        a.st[s] = v;
+    }
+    __device__ __forceinline__ void set_pos(int s, double x, double y, double z)
+    {
+        const int h = get(s);
+        C.x[h] = x;
+        C.y[h] = y;
+        C.z[h] = z;
+        a.X[s] = x;
+        a.Y[s] = y;
+        a.Z[s] = z;
+    }
+    // update_tangent_normal (src/tangle.c:423-502) with the current links
+    __device__ void stencil(const TangleParams &P, int k, double t[3], double nr[3])
+    {
+        const int i1 = fwd(k), im1 = rev(k);
+        const int i2 = fwd(i1), im2 = rev(im1);
+        double p[5][3];
+        const int sl[5] = {k, i1, i2, im1, im2};
+        for (int j = 0; j < 5; ++j) pos(sl[j], p[j][0], p[j][1], p[j][2]);
+        tangent_normal_pts(P, p[0][0], p[0][1], p[0][2], p[1][0], p[1][1], p[1][2], p[2][0],
+                           p[2][1], p[2][2], p[3][0], p[3][1], p[3][2], p[4][0], p[4][1],
+                           p[4][2], t, nr);
+    }
+    // Load the neighbourhoods of candidates cand[c0 .. c0 + 64) (rev^2 .. fwd^3,
+    // what a visit and add_point's two stencils read) into the cache, one
+    // candidate per lane.  Lanes that stage different slots for one entry
+    // leave the entry to one of them whole.
+    __device__ void prefetch(int c0, int n_cand, int *cbuf)
+    {
+        const int lane = threadIdx.x;
+        const int ci = c0 + lane;
+        cbuf[lane] = ci < n_cand ? a.cand[ci] : INT_MAX;
+        int sl[6];
+        int f[6], r[6], q[6];
+        double x[6], y[6], z[6];
+        for (int j = 0; j < 6; ++j) sl[j] = -1;
+        if (ci < n_cand) {
+            const int k = a.cand[ci];
+            sl[0] = k;
+            for (int j = 0; j < 6; ++j) {
+                // order: k, fwd k, rev k, fwd^2 k, rev^2 k, fwd^3 k
+                if (j == 1) sl[1] = f[0];
+                if (j == 2) sl[2] = r[0];
+                if (j == 3) sl[3] = sl[1] >= 0 ? f[1] : -1;
+                if (j == 4) sl[4] = sl[2] >= 0 ? r[2] : -1;
+                if (j == 5) sl[5] = sl[3] >= 0 ? f[3] : -1;
+                const int s = sl[j];
+                if (s < 0 || s >= a.n) {
+                    sl[j] = -1;
+                    f[j] = r[j] = -1;
+                    continue;
+                }
+                f[j] = ld(a.fwd + s);
+                r[j] = ld(a.rev + s);
+                q[j] = ld(a.st + s);
+                x[j] = ld(a.X + s);
+                y[j] = ld(a.Y + s);
+                z[j] = ld(a.Z + s);
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < 6; ++j)
+            if (sl[j] >= 0) C.claim[sl[j] & (kCache - 1)] = sl[j];
+        __syncthreads();
+        for (int j = 0; j < 6; ++j) {
+            const int s = sl[j];
+            if (s < 0) continue;
+            const int h = s & (kCache - 1);
+            if (C.claim[h] != s) continue;
+            C.slot[h] = s;
+            C.fwd[h] = f[j];
+            C.rev[h] = r[j];
+            C.st[h] = q[j];
+            C.x[h] = x[j];
+            C.y[h] = y[j];
+            C.z[h] = z[j];
+        }
+        __syncthreads();
+    }
+};
 
 // The sequential remesh (src/tangle.c:1022-1056) over the candidate slots
 // and the slots the pass's own edits reach ahead of its cursor, in slot
-// order, as the host pass (ovg_remesh).  One wavefront; lanes share only
-// the allocator's scan for an EMPTY slot.
+// order, as the host pass (ovg_remesh).  One wavefront; lanes share the
+// prefetch of each 64 candidates' neighbourhoods and the allocator's scan
+// for an EMPTY slot.
 __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
 {
+    __shared__ SlotCache C;
+    __shared__ int cbuf[64];  // the candidates of the current prefetch
     // a binary min-heap whose first kHeapLds entries (its top levels) live
     // in LDS and the rest in global memory
     __shared__ int lheap[kHeapLds];
     const TangleParams &P = *a.P;
     const int lane = threadIdx.x;
     const int n = a.n, n_cand = *a.n_cand;
+    for (int i = lane; i < kCache; i += 64) C.slot[i] = -1;
+    __syncthreads();
+    RemeshCtx S{a, C};
     int hsize = 0, hint = a.state[DT_HINT], lo = a.state[DT_LO];
-    int cursor = -1, ci = 0, added = 0, removed = 0, err = 0;
+    int cursor = -1, ci = 0, added = 0, removed = 0, err = 0, fetched = 0, visits = 0;
     auto hget = [&](int i) { return i < kHeapLds ? lheap[i] : ld(a.gheap + (i - kHeapLds)); };
     auto hset = [&](int i, int v) {
         if (i < kHeapLds) lheap[i] = v;
@@ -216,7 +353,8 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         if (sl >= 0 && sl > cursor) push(sl);
     };
     // search_next_empty_node (src/tangle.c:201-234); slots below lo are
-    // known not to be EMPTY, so the scan from slot 0 starts there
+    // known not to be EMPTY, so the scan from slot 0 starts there (global
+    // memory holds every status: the cache writes through)
     auto alloc = [&]() -> int {
         if (hint > 0) {
             if (hint < n && ld(a.st + hint) == OVG_EMPTY) {
@@ -238,7 +376,11 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         return -1;
     };
     for (;;) {
-        const int av = ci < n_cand ? a.cand[ci] : INT_MAX;
+        if (ci >= fetched && ci < n_cand) {
+            S.prefetch(ci, n_cand, cbuf);
+            fetched = ci + 64;
+        }
+        const int av = ci < n_cand ? cbuf[ci - (fetched - 64)] : INT_MAX;
         const int bv = hsize ? lheap[0] : INT_MAX;
         const int k = av < bv ? av : bv;
         if (k == INT_MAX || err) break;
@@ -246,30 +388,31 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         while (hsize && lheap[0] == k) pop();
         if (k <= cursor) continue;
         cursor = k;
-        if (k >= n || ld(a.st + k) == OVG_EMPTY) continue;
-        const int next = ld(a.fwd + k), prev = ld(a.rev + k);
+        ++visits;
+        if (k >= n || S.st(k) == OVG_EMPTY) continue;
+        const int next = S.fwd(k), prev = S.rev(k);
         double x, y, z;
-        dt_pos(a, k, x, y, z);
+        S.pos(k, x, y, z);
         double lf = 0, lr = 0;
         if (next >= 0) {
             double bx, by, bz;
-            dt_pos(a, next, bx, by, bz);
+            S.pos(next, bx, by, bz);
             lf = dt_dist(P, x, y, z, bx, by, bz);
         }
         if (prev >= 0) {
             double bx, by, bz;
-            dt_pos(a, prev, bx, by, bz);
+            S.pos(prev, bx, by, bz);
             lr = dt_dist(P, x, y, z, bx, by, bz);
         }
         if (next >= 0 && prev >= 0 &&
             ((lf < a.min_dist || lr < a.min_dist) && (lf + lr) < a.max_dist)) {
             // remove_point (:342-355)
-            const int pv = ld(a.rev + k), nx = ld(a.fwd + k);
-            if (pv >= 0) a.fwd[pv] = nx;
-            if (nx >= 0) a.rev[nx] = pv;
-            a.fwd[k] = -1;
-            a.rev[k] = -1;
-            a.st[k] = OVG_EMPTY;
+            const int pv = S.rev(k), nx = S.fwd(k);
+            if (pv >= 0) S.set_fwd(pv, nx);
+            if (nx >= 0) S.set_rev(nx, pv);
+            S.set_fwd(k, -1);
+            S.set_rev(k, -1);
+            S.set_st(k, OVG_EMPTY);
             if (k < lo) lo = k;
             ++removed;
             touch(k);
@@ -278,20 +421,20 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         }
         if (next >= 0 && lf > a.max_dist) {
             // add_point (:261-335)
-            const int nxt = ld(a.fwd + k);
+            const int nxt = S.fwd(k);
             const int nw = alloc();
             if (nw < 0) {
                 err |= kErrSlots;
                 break;
             }
-            a.st[nw] = OVG_FREE;
-            const int p = k, nn = ld(a.fwd + p);
+            S.set_st(nw, OVG_FREE);
+            const int p = k, nn = S.fwd(p);
             double t0[3], n0[3], t1[3], n1[3];
-            dt_stencil(a, P, p, t0, n0);
-            dt_stencil(a, P, nn, t1, n1);
+            S.stencil(P, p, t0, n0);
+            S.stencil(P, nn, t1, n1);
             double s0[3], s1[3];
-            dt_pos(a, p, s0[0], s0[1], s0[2]);
-            dt_pos(a, nn, s1[0], s1[1], s1[2]);
+            S.pos(p, s0[0], s0[1], s0[2]);
+            S.pos(nn, s1[0], s1[1], s1[2]);
             pwrap(P, s0[0], s0[1], s0[2], s1[0], s1[1], s1[2]);
             const double l = dist3(s0[0], s0[1], s0[2], s1[0], s1[1], s1[2]);
             double nv[3], np[3];
@@ -313,17 +456,15 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
                 for (int q = 0; q < 3; ++q) np[q] = s0[q] + s1[q];
                 for (int q = 0; q < 3; ++q) np[q] = np[q] * 0.5;
             }
-            a.X[nw] = np[0];
-            a.Y[nw] = np[1];
-            a.Z[nw] = np[2];
+            S.set_pos(nw, np[0], np[1], np[2]);
             for (int q = 0; q < 3; ++q) {
                 a.vs[q][nw] = 0.0;
                 a.vels[q][nw] = 0.0;
             }
-            a.rev[nw] = p;
-            a.fwd[nw] = nn;
-            a.fwd[p] = nw;
-            a.rev[nn] = nw;
+            S.set_rev(nw, p);
+            S.set_fwd(nw, nn);
+            S.set_fwd(p, nw);
+            S.set_rev(nn, nw);
             ++added;
             touch(nw);
             touch(k);
@@ -334,6 +475,7 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         a.state[DT_HINT] = hint;
         a.state[DT_ADDED] = added;
         a.state[DT_REMOVED] = removed;
+        a.state[DT_VISITS] = visits;
         if (err) atomicOr(a.state + DT_ERR, err);
     }
 }
@@ -519,11 +661,25 @@ __device__ __forceinline__ bool dt_angle_reject(const double *tk, const double *
     return ca > cutoff;
 }
 
+// |a - b|^2 with periodic axes folded to the nearest image, the host pass's
+// `near` test
+__device__ __forceinline__ double dt_near2(const TangleParams &P, double ax, double ay, double az,
+                                           double bx, double by, double bz)
+{
+    double d[3] = {ax - bx, ay - by, az - bz};
+    double d2 = 0;
+    for (int q = 0; q < 3; ++q) {
+        if (P.periodic[q]) d[q] -= P.L[q] * rint(d[q] / P.L[q]);
+        d2 += d[q] * d[q];
+    }
+    return d2;
+}
+
 __global__ void k_dt_pair_prep(int cap, const int *__restrict__ state, PairRec *__restrict__ pr,
                                const EndRec *__restrict__ er, const int *__restrict__ where,
                                const double *__restrict__ X, const double *__restrict__ Y,
                                const double *__restrict__ Z, const TangleParams *__restrict__ Pp,
-                               double cutoff)
+                               double cutoff, double4 *__restrict__ sphere)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cap || j >= state[DT_PAIRS] || state[DT_ENDS] == 0) return;
@@ -553,29 +709,27 @@ __global__ void k_dt_pair_prep(int cap, const int *__restrict__ state, PairRec *
     const double cf2 = l_kl + l_krlf - l_kkr - l_llf;
     const int kind = (cf1 > 0 && cf2 > 0) ? 0 : (cf1 < cf2 ? 1 : 2);
     R.info = kind | (rej ? 4 : 0);
+    // a sphere about x_k holding the four points where a segment this pair
+    // rewires can start (k, l and their reverse neighbours), for the near lists
+    double rad2 = 0;
+    const int segs[3] = {l, kr, lr};
+    for (int q = 0; q < 3; ++q) {
+        const int g = segs[q];
+        rad2 = fmax(rad2, dt_near2(P, X[g], Y[g], Z[g], xk, yk, zk));
+    }
+    sphere[j] = make_double4(xk, yk, zk, sqrt(rad2));
 }
 
-// |a - b|^2 with periodic axes folded to the nearest image, the host pass's
-// `near` test
-__device__ __forceinline__ double dt_near2(const TangleParams &P, double ax, double ay, double az,
-                                           double bx, double by, double bz)
-{
-    double d[3] = {ax - bx, ay - by, az - bz};
-    double d2 = 0;
-    for (int q = 0; q < 3; ++q) {
-        if (P.periodic[q]) d[q] -= P.L[q] * rint(d[q] / P.L[q]);
-        d2 += d[q] * d[q];
-    }
-    return d2;
-}
 
 // Tree mode: for every endpoint, the pairs whose rewiring could change a
 // segment within the correction radius of it (a rewired segment starts at
-// k, l or their reverse neighbours), ascending; more than kNear: -1.
-__global__ void k_dt_near(int bound, const int *__restrict__ state, EndRec *__restrict__ er,
-                          const PairRec *__restrict__ pr, const double *__restrict__ X,
-                          const double *__restrict__ Y, const double *__restrict__ Z,
-                          const TangleParams *__restrict__ Pp, double rc2)
+// k, l or their reverse neighbours), ascending; more than kNear: -1.  Pairs
+// are screened by a sphere about x_k that holds those four points.
+__global__ void k_dt_near(int bound, int *__restrict__ state, EndRec *__restrict__ er,
+                          const PairRec *__restrict__ pr, const double4 *__restrict__ sphere,
+                          const double *__restrict__ X, const double *__restrict__ Y,
+                          const double *__restrict__ Z, const TangleParams *__restrict__ Pp,
+                          double rc)
 {
     __shared__ int cnt;
     __shared__ int lst[kNear];
@@ -587,15 +741,17 @@ __global__ void k_dt_near(int bound, const int *__restrict__ state, EndRec *__re
     const int s = er[e].slot;
     const double px = X[s], py = Y[s], pz = Z[s];
     const int np = state[DT_PAIRS];
-    const double lim = rc2 * (1 + 1e-9);
+    const double lim = rc * rc * (1 + 1e-9);
     for (int j = threadIdx.x; j < np; j += blockDim.x) {
+        const double4 b = sphere[j];
+        const double reach = rc * (1 + 1e-9) + b.w * (1 + 1e-12);
+        if (dt_near2(P, b.x, b.y, b.z, px, py, pz) >= reach * reach) continue;
         const PairRec &R = pr[j];
-        const EndRec &K = er[R.ek], &L = er[R.el];
-        const int segs[4] = {R.k, R.l, K.rev, L.rev};
+        const int segs[4] = {R.k, R.l, er[R.ek].rev, er[R.el].rev};
         bool hit = false;
         for (int q = 0; q < 4 && !hit; ++q) {
             const int g = segs[q];
-            if (g >= 0) hit = dt_near2(P, X[g], Y[g], Z[g], px, py, pz) < lim;
+            hit = dt_near2(P, X[g], Y[g], Z[g], px, py, pz) < lim;
         }
         if (hit) {
             const int at = atomicAdd(&cnt, 1);
@@ -607,6 +763,7 @@ __global__ void k_dt_near(int bound, const int *__restrict__ state, EndRec *__re
         const int c = cnt;
         if (c > kNear) {
             er[e].near_n = -1;
+            atomicAdd(state + DT_NEAR_OVER, 1);
         } else {
             for (int i = 1; i < c; ++i)  // ascending pair order
                 for (int q = i; q > 0 && lst[q - 1] > lst[q]; --q) {
@@ -667,30 +824,79 @@ __global__ __launch_bounds__(1024) void k_dt_groups(
             }
             __syncthreads();
         }
-    if (t == 0) {
-        int ng = 0, cur = 0;
-        uint64_t prev = 0;
-        for (int i = 0; i < m; ++i) {
-            const int r = (int)(sk[i] >> 32);
-            const uint64_t ky = keys[r];
-            bool cut = i == 0 || cur == ppw;
-            if (!cut) {
-                // common 3-bit levels of one tangle's 48-bit Morton keys
-                const int lv = ky == prev ? 16 : (__clzll((long long)(prev ^ ky)) - 15) / 3;
-                cut = lv < L0;
-            }
-            if (cut) {
-                grp[ng++] = make_int2(i, 1);
-                cur = 1;
-            } else {
-                grp[ng - 1].y++;
-                ++cur;
+    // A group starts where two neighbours in tree order share fewer than L0
+    // levels of one tangle's 48-bit Morton keys (3 bits a level), and every
+    // ppw targets after such a start; the sequential rule of the host path
+    // (a new group when the current one holds ppw) cuts at the same places.
+    // Each thread takes 8 consecutive targets (m <= kGroupSort = 8 * 1024).
+    __shared__ int tmax[1024], tcnt[1024];
+    const int i0 = 8 * t;
+    int seg = -1;  // last segment start at or before i
+    bool cut[8];
+    for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q;
+        bool sp = false;
+        if (i < m) {
+            sp = i == 0;
+            if (!sp) {
+                const uint64_t a = keys[(int)(sk[i - 1] >> 32)], b = keys[(int)(sk[i] >> 32)];
+                const int lv = a == b ? 16 : (__clzll((long long)(a ^ b)) - 15) / 3;
+                sp = lv < L0;
             }
             tgt[i] = (int)(sk[i] & 0xffffffffu);
-            prev = ky;
         }
-        *n_groups = ng;
+        cut[q] = sp;
+        if (sp) seg = i;
     }
+    // inclusive max-scan of segment starts over threads
+    tmax[t] = seg;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? tmax[t - off] : -1;
+        __syncthreads();
+        if (v > tmax[t]) tmax[t] = v;
+        __syncthreads();
+    }
+    int run = t > 0 ? tmax[t - 1] : -1;
+    int cnt = 0;
+    for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q;
+        if (i >= m) {
+            cut[q] = false;
+            continue;
+        }
+        if (cut[q]) run = i;
+        cut[q] = (i - run) % ppw == 0;
+        cnt += cut[q];
+    }
+    // exclusive scan of group starts over threads
+    tcnt[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? tcnt[t - off] : 0;
+        __syncthreads();
+        tcnt[t] += v;
+        __syncthreads();
+    }
+    int g = tcnt[t] - cnt;
+    for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q;
+        if (i < m && cut[q]) {
+            int len = 1;
+            while (i + len < m && len < ppw) {
+                // the next group start ends this group
+                const int jn = i + len;
+                bool sp = false;
+                const uint64_t a = keys[(int)(sk[jn - 1] >> 32)], b = keys[(int)(sk[jn] >> 32)];
+                const int lv = a == b ? 16 : (__clzll((long long)(a ^ b)) - 15) / 3;
+                sp = lv < L0;
+                if (sp) break;
+                ++len;
+            }
+            grp[g++] = make_int2(i, len);
+        }
+    }
+    if (t == 1023) *n_groups = tcnt[1023];
 }
 
 struct GreedyArgs {
@@ -733,14 +939,22 @@ __device__ void dt_segment_all(const GreedyArgs &a, const TangleParams &P, int m
 }
 
 // The sequential greedy pass of src/vortex_dynamics.c:224-284 in (k, l)
-// order, as host_recon.h greedy_pass: one wavefront.
+// order, as host_recon.h greedy_pass: one wavefront.  Pairs are taken in
+// batches of 64 whose records, and their endpoints' values from before the
+// pass, are staged in LDS together; a pair whose endpoints no rewiring has
+// reached since is decided from those alone.
 __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
 {
-    __shared__ unsigned fbits[kGreedyWords], sbits[kGreedyWords];
+    __shared__ unsigned fbits[kGreedyWords], sbits[kGreedyWords], rbits[kMaxPairs / 32];
     __shared__ double sd[64][3];
     __shared__ int sfound[64];
+    __shared__ int bk[64], bl[64], bek[64], bel[64], binfo[64], bnk[64], bnl[64];
+    __shared__ int bnear[64][2][kNear];
+    __shared__ double bdx[64][3], bvk[64][3], bvl[64][3];
     const TangleParams &P = *a.P;
     const int lane = threadIdx.x;
+    // tree mode with every near list whole: the cluster kernel takes the pass
+    if (!a.exact && a.state[DT_NEAR_OVER] == 0) return;
     const int np = a.state[DT_PAIRS], m = a.state[DT_ENDS];
     if (np == 0 || m == 0) {
         if (lane == 0) {
@@ -753,12 +967,13 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
         fbits[i] = 0;
         sbits[i] = 0;
     }
+    for (int i = lane; i < kMaxPairs / 32; i += 64) rbits[i] = 0;
     __syncthreads();
     auto bit = [&](const unsigned *b, int e) { return e >= 0 && ((b[e >> 5] >> (e & 31)) & 1); };
     auto set = [&](unsigned *b, int e) {
         if (e >= 0) b[e >> 5] |= 1u << (e & 31);
     };
-    int nch = 0, nrec = 0;
+    int nch = 0, nrec = 0, slow = 0;
     // tangent and normal of endpoint e: before the pass unless a rewiring
     // changed a link of its stencil, else from the current links
     auto tangent = [&](int e, double t[3], double nr[3]) {
@@ -795,7 +1010,24 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
                 for (int q = 0; q < 3; ++q) bs[q] += sd[i][q];
         }
     };
-    auto velocity = [&](int e, const double t[3], const double nr[3], double v[3]) {
+    // velocity of endpoint e; vel0 / near_n / near: its values before the pass
+    auto velocity = [&](int e, const double t[3], const double nr[3], const double *vel0,
+                        int near_n, const int *near, double v[3]) {
+        const bool st_e = bit(sbits, e);
+        if (!st_e) {
+            bool clean;
+            if (a.exact) {
+                clean = nch == 0;
+            } else {
+                clean = near_n >= 0;
+                for (int i = 0; i < near_n && clean; ++i) clean = !bit(rbits, near[i]);
+            }
+            if (clean) {
+                for (int q = 0; q < 3; ++q) v[q] = vel0[q];
+                return;
+            }
+        }
+        ++slow;
         EndRec &E = a.er[e];
         const int p = E.slot;
         double bs[3] = {ld(&E.bs[0]), ld(&E.bs[1]), ld(&E.bs[2])};
@@ -823,10 +1055,10 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
             const double px = a.X[p], py = a.Y[p], pz = a.Z[p];
             int cnt = 0;
             __syncthreads();
-            if (E.near_n >= 0) {
-                for (int i = 0; i < E.near_n; ++i) {
-                    const int c0 = ld(&a.pr[E.near[i]].chg);
-                    if (c0 < 0) continue;
+            if (near_n >= 0) {
+                for (int i = 0; i < near_n; ++i) {
+                    if (!bit(rbits, near[i])) continue;
+                    const int c0 = ld(&a.pr[near[i]].chg);
                     for (int c = c0; c < c0 + 2; ++c) {
                         const int g = ld(&a.chg[c].x);
                         if (dt_near2(P, a.X[g], a.Y[g], a.Z[g], px, py, pz) < a.rc2) {
@@ -857,8 +1089,8 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
             }
             found = cnt > 0;
         }
-        if (!found && !bit(sbits, e)) {
-            for (int q = 0; q < 3; ++q) v[q] = E.vel[q];
+        if (!found && !st_e) {
+            for (int q = 0; q < 3; ++q) v[q] = vel0[q];
             return;
         }
         dt_compose(P, E.f_lia, t, nr, bs, v);
@@ -867,20 +1099,319 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
     int cur_k = -1;
     bool skip = false, have_t = false, have_v = false;
     double tk[3], nk[3], vk[3];
-    for (int j = 0; j < np; ++j) {
+    for (int b0 = 0; b0 < np; b0 += 64) {
+        const int nbat = min(64, np - b0);
+        __syncthreads();
+        if (lane < nbat) {
+            const PairRec &R = a.pr[b0 + lane];
+            const int ek = R.ek, el = R.el;
+            bk[lane] = R.k;
+            bl[lane] = R.l;
+            bek[lane] = ek;
+            bel[lane] = el;
+            binfo[lane] = R.info;
+            const EndRec &K = a.er[ek], &L = a.er[el];
+            bnk[lane] = K.near_n;
+            bnl[lane] = L.near_n;
+            for (int q = 0; q < 3; ++q) {
+                bdx[lane][q] = R.dx[q];
+                bvk[lane][q] = K.vel[q];
+                bvl[lane][q] = L.vel[q];
+            }
+            if (!a.exact)
+                for (int i = 0; i < kNear; ++i) {
+                    bnear[lane][0][i] = K.near[i];
+                    bnear[lane][1][i] = L.near[i];
+                }
+        }
+        __syncthreads();
+        for (int jb = 0; jb < nbat; ++jb) {
+            const int j = b0 + jb;
+            const int k = bk[jb], l = bl[jb], ek = bek[jb], el = bel[jb];
+            if (k != cur_k) {
+                cur_k = k;
+                skip = bit(fbits, ek);
+                have_t = false;
+                have_v = false;
+            }
+            if (skip || bit(fbits, el)) continue;
+            const int info = binfo[jb];
+            double tl[3], nl[3];
+            bool have_tl = false;
+            if (!bit(sbits, ek) && !bit(sbits, el)) {
+                if (info & 4) continue;
+            } else {
+                if (!have_t) {
+                    tangent(ek, tk, nk);
+                    have_t = true;
+                }
+                tangent(el, tl, nl);
+                have_tl = true;
+                if (dt_angle_reject(tk, tl, a.cutoff)) continue;
+            }
+            if (!have_v) {
+                if (!have_t) {
+                    tangent(ek, tk, nk);
+                    have_t = true;
+                }
+                velocity(ek, tk, nk, bvk[jb], bnk[jb], bnear[jb][0], vk);
+                have_v = true;
+            }
+            if (!have_tl) tangent(el, tl, nl);
+            double vl[3];
+            velocity(el, tl, nl, bvl[jb], bnl[jb], bnear[jb][1], vl);
+            double dv[3];
+            for (int q = 0; q < 3; ++q) dv[q] = vk[q] - vl[q];
+            if (dot3(bdx[jb][0], bdx[jb][1], bdx[jb][2], dv[0], dv[1], dv[2]) > 0) continue;
+            const int kind = info & 3;
+            if (!kind) continue;
+            // do_reconnection's rewiring (:153-172) and the changed segments
+            const EndRec &K = a.er[ek], &L = a.er[el];
+            const int kf = K.fwd, kr = K.rev, lf = L.fwd, lr = L.rev;
+            if (nch + 2 > a.max_chg) break;  // cannot happen: two per pair
+            int4 c0, c1;
+            if (kind == 1) {
+                a.fwd[k] = l;
+                a.rev[l] = k;
+                a.rev[kf] = lr;
+                a.fwd[lr] = kf;
+                c0 = make_int4(k, kf, l, j);
+                c1 = make_int4(lr, l, kf, j);
+            } else {
+                a.fwd[l] = k;
+                a.rev[k] = l;
+                a.rev[lf] = kr;
+                a.fwd[kr] = lf;
+                c0 = make_int4(l, lf, k, j);
+                c1 = make_int4(kr, k, lf, j);
+            }
+            a.chg[nch] = c0;
+            a.chg[nch + 1] = c1;
+            a.pr[j].chg = nch;
+            set(rbits, j);
+            nch += 2;
+            ++nrec;
+            // flags: the pair, its old neighbours (and the new ones, which are
+            // among them)
+            set(fbits, ek);
+            set(fbits, el);
+            set(fbits, K.wf);
+            set(fbits, K.wr);
+            set(fbits, L.wf);
+            set(fbits, L.wr);
+            // stale stencils: the six slots and their current neighbours,
+            // one slot per lane
+            {
+                const int six[6] = {k, l, kf, kr, lf, lr};
+                int w0 = -1, w1 = -1, w2 = -1;
+                if (lane < 6) {
+                    const int sl = six[lane];
+                    const int f = ld(a.fwd + sl), r = ld(a.rev + sl);
+                    w0 = a.where[sl];
+                    w1 = f >= 0 ? a.where[f] : -1;
+                    w2 = r >= 0 ? a.where[r] : -1;
+                }
+                __syncthreads();
+                for (int q = 0; q < 6; ++q) {
+                    set(sbits, __shfl(w0, q, 64));
+                    set(sbits, __shfl(w1, q, 64));
+                    set(sbits, __shfl(w2, q, 64));
+                }
+                __syncthreads();
+            }
+            skip = true;
+        }
+    }
+    if (lane == 0) {
+        a.state[DT_REC] = nrec;
+        a.state[DT_CHANGES] = nch;
+        a.state[DT_SLOW] = slow;
+    }
+}
+
+// ---- the greedy pass by clusters (tree-mode velocities) -----------------
+// A rewiring reaches later pairs only through the flags and stale stencils
+// it sets (slots within two links of its endpoints), the links it writes
+// (within one link) and, in tree mode, the field corrections within the
+// correction radius (the endpoints' near lists).  Joining each pair with
+// every endpoint within two links of its own endpoints and with the pairs
+// on its endpoints' near lists splits the pairs into clusters that no
+// other cluster's rewiring reaches; each is processed alone in (k, l)
+// order, one thread per cluster, which gives the sequential pass's result.
+// Union-find nodes: pairs [0, cap), endpoints [cap, cap + ebound).
+
+__device__ __forceinline__ int uf_find(int *parent, int x)
+{
+    for (;;) {
+        const int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        const int gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp != p) atomicCAS(parent + x, p, gp);  // path halving
+        x = p;
+    }
+}
+
+__device__ __forceinline__ void uf_union(int *parent, int a, int b)
+{
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        // the larger root goes under the smaller
+        if (atomicCAS(parent + a, a, b) == a) return;
+    }
+}
+
+__global__ void k_cl_pairs(int cap, const int *__restrict__ state, const PairRec *__restrict__ pr,
+                           const EndRec *__restrict__ er, const int *__restrict__ where,
+                           const int *__restrict__ fwd, const int *__restrict__ rev,
+                           int *__restrict__ parent)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cap || j >= state[DT_PAIRS] || state[DT_ENDS] == 0) return;
+    const PairRec &R = pr[j];
+    uf_union(parent, j, cap + R.ek);
+    uf_union(parent, j, cap + R.el);
+    for (int side = 0; side < 2; ++side) {
+        const EndRec &E = er[side ? R.el : R.ek];
+        const int f = E.fwd, r = E.rev;
+        const int sl[4] = {f, r, f >= 0 ? fwd[f] : -1, r >= 0 ? rev[r] : -1};
+        for (int q = 0; q < 4; ++q)
+            if (sl[q] >= 0) {
+                const int e = where[sl[q]];
+                if (e >= 0) uf_union(parent, j, cap + e);
+            }
+    }
+}
+
+__global__ void k_cl_near(int cap, int bound, const int *__restrict__ state,
+                          const EndRec *__restrict__ er, int *__restrict__ parent)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= bound || e >= state[DT_ENDS] || state[DT_NEAR_OVER]) return;
+    const EndRec &E = er[e];
+    for (int i = 0; i < E.near_n; ++i) uf_union(parent, cap + e, E.near[i]);
+}
+
+// pair j -> (cluster root << 15 | j), sortable; the rest sort last
+__global__ void k_cl_keys(int cap, const int *__restrict__ state, int *__restrict__ parent,
+                          uint64_t *__restrict__ keys, int *__restrict__ vals)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cap) return;
+    const bool on = j < state[DT_PAIRS] && state[DT_ENDS] > 0;
+    keys[j] = on ? ((uint64_t)uf_find(parent, j) << 15) | (uint64_t)j : ~0ull;
+    vals[j] = j;
+}
+
+struct ClusterArgs {
+    int cap;
+    const uint64_t *keys;   // pairs by (cluster, j)
+    PairRec *pr;
+    EndRec *er;
+    const int *where;
+    int *fwd, *rev;
+    const double *X, *Y, *Z;
+    int4 *chg;               // pair j's changes at 2 j, 2 j + 1
+    unsigned char *flag, *stale;  // per endpoint
+    const TangleParams *P;
+    double cutoff, rc2;
+    int *state;
+};
+
+// host_recon.h greedy_pass restricted to one cluster, one thread
+__global__ void k_cl_greedy(ClusterArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int np = a.state[DT_PAIRS];
+    if (i >= a.cap || i >= np || a.state[DT_ENDS] == 0 || a.state[DT_NEAR_OVER]) return;
+    const uint64_t root = a.keys[i] >> 15;
+    if (i > 0 && (a.keys[i - 1] >> 15) == root) return;  // not the cluster's first pair
+    const TangleParams &P = *a.P;
+    auto tangent = [&](int e, double t[3], double nr[3]) {
+        const EndRec &E = a.er[e];
+        if (!a.stale[e]) {
+            for (int q = 0; q < 3; ++q) {
+                t[q] = E.t[q];
+                nr[q] = E.nr[q];
+            }
+            return;
+        }
+        const int s = E.slot;
+        const int i1 = a.fwd[s], im1 = a.rev[s];
+        const int i2 = a.fwd[i1], im2 = a.rev[im1];
+        tangent_normal_pts(P, a.X[s], a.Y[s], a.Z[s], a.X[i1], a.Y[i1], a.Z[i1], a.X[i2],
+                           a.Y[i2], a.Z[i2], a.X[im1], a.Y[im1], a.Z[im1], a.X[im2], a.Y[im2],
+                           a.Z[im2], t, nr);
+    };
+    // the field of segment (m -> f) at slot p, all images (pure shifts)
+    auto seg_all = [&](int m, int f, int p, double acc[3]) {
+        const double xm = a.X[m], ym = a.Y[m], zm = a.Z[m];
+        double bx = a.X[f], by = a.Y[f], bz = a.Z[f];
+        pwrap(P, xm, ym, zm, bx, by, bz);
+        const double rx = a.X[p], ry = a.Y[p], rz = a.Z[p];
+        for (int im = -1; im < P.n_images; ++im) {
+            if (im < 0 && (m == p || f == p)) continue;
+            double qx = rx, qy = ry, qz = rz;
+            if (im >= 0) image_point(P, im, rx, ry, rz, qx, qy, qz);
+            double vx, vy, vz;
+            if (seg_field64(xm - qx, ym - qy, zm - qz, bx - qx, by - qy, bz - qz, P.f_bs, vx,
+                            vy, vz)) {
+                acc[0] += vx;
+                acc[1] += vy;
+                acc[2] += vz;
+            }
+        }
+    };
+    auto velocity = [&](int e, const double t[3], const double nr[3], double v[3]) {
+        const EndRec &E = a.er[e];
+        const int p = E.slot;
+        double bs[3] = {E.bs[0], E.bs[1], E.bs[2]};
+        bool found = false;
+        const double px = a.X[p], py = a.Y[p], pz = a.Z[p];
+        for (int q = 0; q < E.near_n; ++q) {
+            const int c0 = a.pr[E.near[q]].chg;
+            if (c0 < 0) continue;
+            for (int c = c0; c < c0 + 2; ++c) {
+                const int4 ch = a.chg[c];
+                if (!(dt_near2(P, a.X[ch.x], a.Y[ch.x], a.Z[ch.x], px, py, pz) < a.rc2))
+                    continue;
+                double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+                seg_all(ch.x, ch.y, p, vo);
+                seg_all(ch.x, ch.z, p, vn);
+                for (int d = 0; d < 3; ++d) bs[d] += vn[d] - vo[d];
+                found = true;
+            }
+        }
+        if (!found && !a.stale[e]) {
+            for (int q = 0; q < 3; ++q) v[q] = E.vel[q];
+            return;
+        }
+        dt_compose(P, E.f_lia, t, nr, bs, v);
+    };
+    int cur_k = -1, nrec = 0;
+    bool skip = false, have_t = false, have_v = false;
+    double tk[3], nk[3], vk[3];
+    for (int t = i; t < np && (a.keys[t] >> 15) == root; ++t) {
+        const int j = (int)(a.keys[t] & 0x7fff);
         PairRec &R = a.pr[j];
         const int k = R.k, l = R.l, ek = R.ek, el = R.el;
         if (k != cur_k) {
             cur_k = k;
-            skip = bit(fbits, ek);
+            skip = a.flag[ek];
             have_t = false;
             have_v = false;
         }
-        if (skip || bit(fbits, el)) continue;
+        if (skip || a.flag[el]) continue;
         const int info = R.info;
         double tl[3], nl[3];
         bool have_tl = false;
-        if (!bit(sbits, ek) && !bit(sbits, el)) {
+        if (!a.stale[ek] && !a.stale[el]) {
             if (info & 4) continue;
         } else {
             if (!have_t) {
@@ -907,54 +1438,38 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
         if (dot3(R.dx[0], R.dx[1], R.dx[2], dv[0], dv[1], dv[2]) > 0) continue;
         const int kind = info & 3;
         if (!kind) continue;
-        // do_reconnection's rewiring (:153-172) and the changed segments
         const EndRec &K = a.er[ek], &L = a.er[el];
         const int kf = K.fwd, kr = K.rev, lf = L.fwd, lr = L.rev;
-        if (nch + 2 > a.max_chg) break;  // cannot happen: two per pair
-        int4 c0, c1;
         if (kind == 1) {
             a.fwd[k] = l;
             a.rev[l] = k;
             a.rev[kf] = lr;
             a.fwd[lr] = kf;
-            c0 = make_int4(k, kf, l, j);
-            c1 = make_int4(lr, l, kf, j);
+            a.chg[2 * j] = make_int4(k, kf, l, j);
+            a.chg[2 * j + 1] = make_int4(lr, l, kf, j);
         } else {
             a.fwd[l] = k;
             a.rev[k] = l;
             a.rev[lf] = kr;
             a.fwd[kr] = lf;
-            c0 = make_int4(l, lf, k, j);
-            c1 = make_int4(kr, k, lf, j);
+            a.chg[2 * j] = make_int4(l, lf, k, j);
+            a.chg[2 * j + 1] = make_int4(kr, k, lf, j);
         }
-        a.chg[nch] = c0;
-        a.chg[nch + 1] = c1;
-        R.chg = nch;
-        nch += 2;
+        R.chg = 2 * j;
         ++nrec;
-        // flags: the pair, its old neighbours (and the new ones, which are
-        // among them)
-        set(fbits, ek);
-        set(fbits, el);
-        set(fbits, K.wf);
-        set(fbits, K.wr);
-        set(fbits, L.wf);
-        set(fbits, L.wr);
-        // stale stencils: the six slots and their current neighbours
+        const int fl[6] = {ek, el, K.wf, K.wr, L.wf, L.wr};
+        for (int q = 0; q < 6; ++q)
+            if (fl[q] >= 0) a.flag[fl[q]] = 1;
         const int six[6] = {k, l, kf, kr, lf, lr};
         for (int q = 0; q < 6; ++q) {
             const int sl = six[q];
-            set(sbits, a.where[sl]);
-            const int f = ld(a.fwd + sl), r = ld(a.rev + sl);
-            if (f >= 0) set(sbits, a.where[f]);
-            if (r >= 0) set(sbits, a.where[r]);
+            const int w[3] = {a.where[sl], a.where[a.fwd[sl]], a.where[a.rev[sl]]};
+            for (int d = 0; d < 3; ++d)
+                if (w[d] >= 0) a.stale[w[d]] = 1;
         }
         skip = true;
     }
-    if (lane == 0) {
-        a.state[DT_REC] = nrec;
-        a.state[DT_CHANGES] = nch;
-    }
+    if (nrec) atomicAdd(a.state + DT_REC, nrec);
 }
 
 __global__ void k_dt_clear_where(int bound, const int *__restrict__ state,
@@ -1083,15 +1598,23 @@ int sync_mirrors(Ctx &c)
 
 int devtopo_counts(Ctx &c, int out[4])
 {
-    OVG_TRY(dt_state_ready(c));
-    OVG_TRY(sync_live(c));
     int st[DT_COUNT];
-    OVG_HIP(hipMemcpyAsync(st, c.dt_state.p, sizeof(st), hipMemcpyDeviceToHost, c.stream));
-    OVG_HIP(hipStreamSynchronize(c.stream));
+    OVG_TRY(devtopo_counters(c, st));
     out[0] = st[DT_REC];
     out[1] = st[DT_LOOPS];
     out[2] = st[DT_ADDED];
     out[3] = st[DT_REMOVED];
+    return OVG_OK;
+}
+
+int devtopo_counters(Ctx &c, int out[16])
+{
+    static_assert(DT_COUNT == 16, "ovg_topology_counters returns 16 counters");
+    OVG_TRY(dt_state_ready(c));
+    OVG_TRY(sync_live(c));
+    OVG_HIP(hipMemcpyAsync(out, c.dt_state.p, sizeof(int) * DT_COUNT, hipMemcpyDeviceToHost,
+                           c.stream));
+    OVG_HIP(hipStreamSynchronize(c.stream));
     return OVG_OK;
 }
 
@@ -1182,6 +1705,9 @@ int devtopo_reconnect(Ctx &c, double t)
     OVG_TRY(dt_state_ready(c));
     int *state = c.dt_state.as<int>();
     hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_REC, 0);
+    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_NEAR_OVER, 0);
+    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_CHANGES, 0);
+    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_SLOW, 0);
     const int n = c.n;
     if (c.n_live == 0 || n == 0) return OVG_OK;
     // capacity: four times the most pairs seen, at least 4096
@@ -1204,7 +1730,7 @@ int devtopo_reconnect(Ctx &c, double t)
         OVG_TRY(T.vals[b].ensure(sizeof(int) * (size_t)cap));
     }
     ++c.tree_id;
-    OVG_TRY(c.dt_pairs.ensure(sizeof(PairRec) * (size_t)cap));
+    OVG_TRY(c.dt_pairs.ensure((sizeof(PairRec) + sizeof(double4)) * (size_t)cap));
     OVG_TRY(c.dt_ends.ensure(sizeof(EndRec) * (size_t)ebound + sizeof(int) * (size_t)ebound + 256));
     OVG_TRY(c.dt_chg.ensure(sizeof(int4) * 2 * (size_t)cap + 64));
     OVG_TRY(c.dt_mark.ensure(sizeof(int) * (size_t)(n + 1)));
@@ -1265,12 +1791,51 @@ int devtopo_reconnect(Ctx &c, double t)
     hipLaunchKernelGGL(k_dt_end_vel, dim3(nb256(ebound)), dim3(256), 0, c.stream, ebound, state,
                        er, op[0], op[1], op[2], oi[0], oi[1], oi[2], dparams(c));
     const double cutoff = std::cos(c.params.rec_angle);
+    double4 *sphere = reinterpret_cast<double4 *>(pr + cap);
     hipLaunchKernelGGL(k_dt_pair_prep, dim3(nb256(cap)), dim3(256), 0, c.stream, cap, state, pr,
-                       er, where, X, Y, Z, dparams(c), cutoff);
+                       er, where, X, Y, Z, dparams(c), cutoff, sphere);
     const double rc = 25 * c.params.rec_dist, rc2 = rc * rc;
     if (!exact)
         hipLaunchKernelGGL(k_dt_near, dim3(ebound), dim3(256), 0, c.stream, ebound, state, er, pr,
-                           X, Y, Z, dparams(c), rc2);
+                           sphere, X, Y, Z, dparams(c), rc);
+    if (!exact) {
+        // clusters of pairs that no other cluster's rewiring reaches
+        const int nodes = cap + ebound;
+        OVG_TRY(c.dt_cl.ensure(sizeof(int) * (size_t)nodes + 2 * (size_t)ebound + 64));
+        int *parent = c.dt_cl.as<int>();
+        unsigned char *flag = reinterpret_cast<unsigned char *>(parent + nodes);
+        unsigned char *stale = flag + ebound;
+        hipLaunchKernelGGL(k_dt_iota, dim3(nb256(nodes)), dim3(256), 0, c.stream, nodes, parent);
+        hipLaunchKernelGGL(k_dt_fill, dim3(nb256(ebound / 2)), dim3(256), 0, c.stream,
+                           ebound / 2, reinterpret_cast<int *>(flag), 0);
+        hipLaunchKernelGGL(k_cl_pairs, dim3(nb256(cap)), dim3(256), 0, c.stream, cap, state, pr,
+                           er, where, c.fwd.as<int>(), c.rev.as<int>(), parent);
+        hipLaunchKernelGGL(k_cl_near, dim3(nb256(ebound)), dim3(256), 0, c.stream, cap, ebound,
+                           state, er, parent);
+        hipLaunchKernelGGL(k_cl_keys, dim3(nb256(cap)), dim3(256), 0, c.stream, cap, state,
+                           parent, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
+        ++c.tree_id;
+        OVG_TRY(radix_sort(c, cap, 0, 24));
+        ClusterArgs ca{};
+        ca.cap = cap;
+        ca.keys = T.keys[0].as<uint64_t>();
+        ca.pr = pr;
+        ca.er = er;
+        ca.where = where;
+        ca.fwd = c.fwd.as<int>();
+        ca.rev = c.rev.as<int>();
+        ca.X = X;
+        ca.Y = Y;
+        ca.Z = Z;
+        ca.chg = c.dt_chg.as<int4>();
+        ca.flag = flag;
+        ca.stale = stale;
+        ca.P = dparams(c);
+        ca.cutoff = cutoff;
+        ca.rc2 = rc2;
+        ca.state = state;
+        hipLaunchKernelGGL(k_cl_greedy, dim3(nb256(cap)), dim3(256), 0, c.stream, ca);
+    }
     GreedyArgs g{};
     g.pr = pr;
     g.er = er;
