@@ -114,6 +114,9 @@ def serve_bench(args):
         (lambda fast: serve.c4_jobs(n=args.jobs, steps=max(1, int(round(args.steps_scale * 100))),
                                     fast=fast))
     fast_jobs = make(True)
+    import openvort_amd as ov
+    for j in fast_jobs:
+        j.topology = ov.TOPO_DEVICE if args.topology == "device" else ov.TOPO_HOST
     pts = sum(ov_points(j) for j in fast_jobs)
     log(f"{args.workload}: {len(fast_jobs)} jobs, {pts} points at admission")
     rep, res = serve.run(fast_jobs, workers=args.workers, max_active=args.max_active,
@@ -145,6 +148,7 @@ def serve_bench(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": (f"{args.workload}: first {len(fast_jobs)} seeded jobs, steps "
                                 f"scaled by {args.steps_scale}, all admitted at t = 0"),
+                   "topology": args.topology,
                    "workers": args.workers, "max_active": args.max_active,
                    "hbm_cap_bytes": args.hbm_cap},
         "p50_step_ms": rep["step_p50_ms"], "p99_step_ms": rep["step_p99_ms"],

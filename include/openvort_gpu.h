@@ -410,15 +410,21 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
  * whole state when the slot count doubled). */
 int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
 
-/* The 16 counters of the device topology passes (OVG_TOPO_DEVICE), read
+/* The 20 counters of the device topology passes (OVG_TOPO_DEVICE), read
  * after the passes queued so far (waits): [0] live points, [1] error bits,
  * [2] allocator hint, [3] scratch, [4] reconnections, [5] points removed by
  * small-loop removal, [6] points added and [7] removed by remesh, [8]
  * candidate pairs taken and [9] found, [10] their endpoints, [11] rewired
- * segments, [12] endpoint target groups, [13] slots remesh visited, [14]
- * endpoint velocities the greedy pass recomposed, [15] endpoints whose
- * rewirable neighbourhood overflowed its list.  Diagnostics, no
- * counterpart in the reference. */
+ * segments (one-wavefront greedy pass), [12] endpoint target groups, [13]
+ * slots the one-wavefront remesh visited, [14] endpoint velocities that
+ * pass recomposed, [15] endpoints whose rewirable neighbourhood overflowed
+ * its list, [16] why the last remesh left the cluster pass for the
+ * one-wavefront pass (0: it did not; bits: 1 two clusters reached one
+ * slot, 2 a cluster's touched slots overflowed, 4 the new-point table
+ * overflowed, 8 a new point would have acted when visited, 16 more
+ * candidates than the pass takes), [17] the allocator hint after the
+ * cluster pass, [18] its new points, [19] remesh candidates.  Diagnostics,
+ * no counterpart in the reference. */
 int ovg_topology_counters(ovg_ctx *ctx, int *out);
 
 /* enforce_wall_boundaries, src/tangle.c:966-1016, at time t: for every

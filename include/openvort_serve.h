@@ -43,6 +43,8 @@ typedef struct ovs_job {
     int32_t  points_per_wave;
     int32_t  rec_velocity;/* OVG_RECVEL_*                                   */
     double   theta;
+    int32_t  topology;    /* OVG_TOPO_* (openvort_gpu.h)                    */
+    int32_t  pad_;
 } ovs_job;
 
 typedef struct ovs_result {

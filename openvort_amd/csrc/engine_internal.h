@@ -187,6 +187,8 @@ struct Ctx {
     DBuf dt_mark;                       // int per slot, 0 between passes
     DBuf dt_pairs, dt_ends, dt_chg;     // reconnection pass scratch
     DBuf dt_cl;                         // its clusters: union-find parents, flags
+    DBuf prem_buf;                      // remesh by clusters: parents, owners, copies, table
+    int prem_seen = 0;                  // most remesh candidates a device pass met
     bool dt_unread = false;             // device passes ran since the counters were read
     bool live_pending = false;          // the device's live list is newer than n_live
     bool mirrors_stale = false;         // h_conn / h_status / next_empty are older
@@ -309,8 +311,8 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist);
 // The counters of the last device passes (waits): reconnections, small-loop
 // points removed, remesh points added and removed.
 int devtopo_counts(Ctx &c, int out[4]);
-// All 16 device counters (ovg_topology_counters).
-int devtopo_counters(Ctx &c, int out[16]);
+// The 20 public device counters (ovg_topology_counters).
+int devtopo_counters(Ctx &c, int out[20]);
 // engine.hip: grow the slots to n_new (> n) with the host mirrors
 // current, the new slots EMPTY; every old slot keeps its state.
 int grow_capacity(Ctx &c, int n_new);

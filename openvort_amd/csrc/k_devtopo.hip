@@ -45,7 +45,12 @@ enum {
     DT_VISITS,     // slots the last remesh visited
     DT_SLOW,       // endpoint velocities the last greedy pass recomposed
     DT_NEAR_OVER,  // endpoints with more rewirable pairs near them than kNear
-    DT_COUNT = 16
+    DT_PREM_FAIL,       // remesh by clusters: reasons it handed the pass to the sequential kernel
+    DT_PREM_HINT,       // the allocator hint after its replay
+    DT_PREM_NEW,        // new points of the cluster pass
+    DT_PREM_CAND,       // remesh candidates of the last pass
+    DT_PUBLIC,          // the counters ovg_topology_counters returns
+    DT_COUNT = 32
 };
 constexpr int kErrPairs = 1, kErrEnds = 2, kErrSlots = 4, kErrHeap = 8;
 constexpr int kMaxPairs = 32768, kMaxEnds = 65536;
@@ -57,6 +62,13 @@ template <class T>
 __device__ __forceinline__ T ld(const T *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// the same at device scope: sees other lanes' atomics
+template <class T>
+__device__ __forceinline__ T ld_agent(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double dt_dist(const TangleParams &P, double ax, double ay, double az,
@@ -149,6 +161,7 @@ __global__ void k_dt_first_empty(int n, const int *__restrict__ st, int *__restr
 
 struct RemeshArgs {
     int n;  // slots
+    const int *gate;  // runs only when *gate is nonzero (null: always)
     const int *cand, *n_cand;
     int *gheap;       // heap entries past the LDS part
     int gheap_cap;
@@ -307,6 +320,7 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
     // a binary min-heap whose first kHeapLds entries (its top levels) live
     // in LDS and the rest in global memory
     __shared__ int lheap[kHeapLds];
+    if (a.gate && *a.gate == 0) return;
     const TangleParams &P = *a.P;
     const int lane = threadIdx.x;
     const int n = a.n, n_cand = *a.n_cand;
@@ -478,6 +492,505 @@ __global__ __launch_bounds__(64) void k_dt_remesh(RemeshArgs a)
         a.state[DT_VISITS] = visits;
         if (err) atomicOr(a.state + DT_ERR, err);
     }
+}
+
+// union-find with path halving; the larger root goes under the smaller
+__device__ __forceinline__ int uf_find(int *parent, int x)
+{
+    for (;;) {
+        const int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        const int gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp != p) atomicCAS(parent + x, p, gp);  // path halving
+        x = p;
+    }
+}
+
+__device__ __forceinline__ void uf_union(int *parent, int a, int b)
+{
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        // the larger root goes under the smaller
+        if (atomicCAS(parent + a, a, b) == a) return;
+    }
+}
+
+// ------------------------------------------------------ remesh by clusters
+// The sequential remesh made parallel without changing its result.  Visits
+// interact through the links (a visit reads its point's neighbours, add_point
+// two stencils: at most three links away) and through the allocator; the
+// first is split off by clusters, the second replayed afterwards:
+//  1. candidates within kPremChain links of each other form a cluster; one
+//     thread per cluster runs the visits of its candidates, and of the slots
+//     their edits reach ahead of the cursor, in slot order, on the links in
+//     place; points it inserts are new points numbered in a table (links to
+//     them hold -2 - id) that are never visited;
+//  2. one wavefront replays get_next_empty_node over the insertions in visit
+//     order: at visit c a slot is EMPTY when it was so before the pass, or a
+//     visit before c removed it (a removal happens at its own slot), and no
+//     earlier insertion took it;
+//  3. the new points move to their slots and the links to them are
+//     resolved.
+// The reference would visit a new point whose slot lies ahead of the cursor
+// that inserted it.  Such a visit changes nothing when the point's links
+// are still those of its insertion and its remesh test fails; otherwise, as
+// when two clusters reach the same slot, a cluster holds more touched slots
+// than its heap or the table overflows, the links are restored from a copy
+// and the one-wavefront pass (k_dt_remesh) runs instead.
+constexpr int kPremChain = 6;
+constexpr int kPremHeap = 32;
+constexpr int kFailClash = 1, kFailHeap = 2, kFailTable = 4, kFailVisit = 8, kFailCand = 16;
+
+struct NewPt {
+    int fwd, rev;   // links; another new point j as -2 - j
+    int cursor;     // the visit that inserted it
+    int slot;       // its slot, from the allocator replay
+    double x, y, z;
+    int relinked;   // a later visit changed its links
+    int pad;
+};
+
+struct PremArgs {
+    int n;                     // slots
+    const int *cand, *n_cand;  // candidate slots, ascending
+    int cand_cap;              // candidates a cluster pass takes
+    int sbits, ibits;          // key bits of a slot and of a new point's number
+    int *fwd, *rev, *st;
+    const double *X, *Y, *Z;
+    double *vs[3], *vels[3];
+    double *WX, *WY, *WZ;      // positions, written for the new points only
+    const TangleParams *P;
+    double min_dist, max_dist;
+    int *cwhere;               // slot -> candidate index, -1
+    int *parent;               // union-find over candidates
+    const uint64_t *ckeys;     // candidates by (cluster, slot)
+    int *owner;                // slot -> cluster that touched it, -1
+    unsigned *removed;         // bitmap: removed by this pass
+    unsigned *taken;           // bitmap: allocated by the replay
+    unsigned *origE;           // bitmap: EMPTY before the pass
+    int words;                 // 32-bit words of each bitmap
+    NewPt *np;
+    int np_cap;
+    int *state;
+};
+
+__global__ void k_prem_where(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = *a.n_cand;
+    if (i == 0) {
+        a.state[DT_PREM_CAND] = nc;
+        if (nc > a.cand_cap) atomicOr(a.state + DT_PREM_FAIL, kFailCand);
+    }
+    if (i >= nc || i >= a.cand_cap) return;
+    a.cwhere[a.cand[i]] = i;
+    a.parent[i] = i;
+}
+
+__global__ void k_prem_union(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = *a.n_cand;
+    if (i >= nc || i >= a.cand_cap || a.state[DT_PREM_FAIL]) return;
+    const int k = a.cand[i];
+    for (int dir = 0; dir < 2; ++dir) {
+        int s = k;
+        for (int h = 0; h < kPremChain; ++h) {
+            s = dir ? a.rev[s] : a.fwd[s];
+            if (s < 0 || s == k) break;
+            const int j = a.cwhere[s];
+            if (j >= 0) uf_union(a.parent, i, j);
+        }
+    }
+}
+
+__global__ void k_prem_keys(PremArgs a, uint64_t *__restrict__ keys, int *__restrict__ vals)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.cand_cap) return;
+    const int nc = *a.n_cand;
+    const bool on = i < nc && !a.state[DT_PREM_FAIL];
+    keys[i] = on ? ((uint64_t)uf_find(a.parent, i) << a.sbits) | (uint64_t)(unsigned)a.cand[i]
+                 : ~0ull;
+    vals[i] = i;
+}
+
+// One cluster's visits (k_dt_remesh's, new points never visited).
+__global__ void k_prem_cluster(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = *a.n_cand;
+    if (i >= nc || i >= a.cand_cap || a.state[DT_PREM_FAIL]) return;
+    const uint64_t root = a.ckeys[i] >> a.sbits;
+    if (i > 0 && (a.ckeys[i - 1] >> a.sbits) == root) return;
+    const TangleParams &P = *a.P;
+    const int cid = (int)root;
+    const uint64_t smask = (1ull << a.sbits) - 1;
+    int fail = 0;
+    // the cluster's own record of the slots it reaches
+    auto tag = [&](int s) {
+        if (s < 0) return;
+        const int o = atomicCAS(a.owner + s, -1, cid);
+        if (o != -1 && o != cid) fail |= kFailClash;
+    };
+    auto f_of = [&](int s) { return s >= 0 ? a.fwd[s] : a.np[-2 - s].fwd; };
+    auto r_of = [&](int s) { return s >= 0 ? a.rev[s] : a.np[-2 - s].rev; };
+    auto set_f = [&](int s, int v) {
+        if (s >= 0) {
+            a.fwd[s] = v;
+        } else {
+            a.np[-2 - s].fwd = v;
+            a.np[-2 - s].relinked = 1;
+        }
+    };
+    auto set_r = [&](int s, int v) {
+        if (s >= 0) {
+            a.rev[s] = v;
+        } else {
+            a.np[-2 - s].rev = v;
+            a.np[-2 - s].relinked = 1;
+        }
+    };
+    auto pos = [&](int s, double &x, double &y, double &z) {
+        if (s >= 0) {
+            x = a.X[s];
+            y = a.Y[s];
+            z = a.Z[s];
+        } else {
+            const NewPt &q = a.np[-2 - s];
+            x = q.x;
+            y = q.y;
+            z = q.z;
+        }
+    };
+    auto stencil = [&](int k, double t[3], double nr[3]) {
+        const int i1 = f_of(k), im1 = r_of(k);
+        const int i2 = f_of(i1), im2 = r_of(im1);
+        tag(i1);
+        tag(im1);
+        tag(i2);
+        tag(im2);
+        double p[5][3];
+        const int sl[5] = {k, i1, i2, im1, im2};
+        for (int j = 0; j < 5; ++j) pos(sl[j], p[j][0], p[j][1], p[j][2]);
+        tangent_normal_pts(P, p[0][0], p[0][1], p[0][2], p[1][0], p[1][1], p[1][2], p[2][0],
+                           p[2][1], p[2][2], p[3][0], p[3][1], p[3][2], p[4][0], p[4][1],
+                           p[4][2], t, nr);
+    };
+    int heap[kPremHeap];
+    int hsize = 0, cursor = -1, removed = 0;
+    auto touch = [&](int sl) {
+        if (sl < 0 || sl <= cursor) return;  // new points (< -1) are never visited
+        if (hsize == kPremHeap) {
+            fail |= kFailHeap;
+            return;
+        }
+        int j = hsize++;
+        while (j > 0 && heap[(j - 1) >> 1] > sl) {
+            heap[j] = heap[(j - 1) >> 1];
+            j = (j - 1) >> 1;
+        }
+        heap[j] = sl;
+    };
+    auto pop = [&]() {
+        const int v = heap[--hsize];
+        int j = 0;
+        for (;;) {
+            int ch = 2 * j + 1;
+            if (ch >= hsize) break;
+            if (ch + 1 < hsize && heap[ch + 1] < heap[ch]) ++ch;
+            if (v <= heap[ch]) break;
+            heap[j] = heap[ch];
+            j = ch;
+        }
+        if (hsize > 0) heap[j] = v;
+    };
+    int ci = i;
+    for (;;) {
+        const int av = (ci < nc && (a.ckeys[ci] >> a.sbits) == root) ? (int)(a.ckeys[ci] & smask)
+                                                                       : INT_MAX;
+        const int bv = hsize ? heap[0] : INT_MAX;
+        const int k = av < bv ? av : bv;
+        if (k == INT_MAX || fail) break;
+        if (av == k) ++ci;
+        while (hsize && heap[0] == k) pop();
+        if (k <= cursor) continue;
+        cursor = k;
+        tag(k);
+        if (a.st[k] == OVG_EMPTY) continue;
+        const int next = a.fwd[k], prev = a.rev[k];
+        tag(next);
+        tag(prev);
+        double x, y, z;
+        pos(k, x, y, z);
+        double lf = 0, lr = 0;
+        if (next != -1) {
+            double bx, by, bz;
+            pos(next, bx, by, bz);
+            lf = dt_dist(P, x, y, z, bx, by, bz);
+        }
+        if (prev != -1) {
+            double bx, by, bz;
+            pos(prev, bx, by, bz);
+            lr = dt_dist(P, x, y, z, bx, by, bz);
+        }
+        if (next != -1 && prev != -1 &&
+            ((lf < a.min_dist || lr < a.min_dist) && (lf + lr) < a.max_dist)) {
+            // remove_point (:342-355) of slot k, at visit k
+            const int pv = a.rev[k], nx = a.fwd[k];
+            if (pv != -1) set_f(pv, nx);
+            if (nx != -1) set_r(nx, pv);
+            a.fwd[k] = -1;
+            a.rev[k] = -1;
+            a.st[k] = OVG_EMPTY;
+            atomicOr(a.removed + (k >> 5), 1u << (k & 31));
+            ++removed;
+            touch(k);
+            touch(pv);
+            touch(nx);
+        }
+        if (next != -1 && lf > a.max_dist) {
+            // add_point (:261-335) into a numbered new point
+            const int nxt = a.fwd[k];
+            const int id = atomicAdd(a.state + DT_PREM_NEW, 1);
+            if (id >= a.np_cap) {
+                fail |= kFailTable;
+                break;
+            }
+            const int p = k, nn = a.fwd[p];
+            double t0[3], n0[3], t1[3], n1[3];
+            stencil(p, t0, n0);
+            stencil(nn, t1, n1);
+            double s0[3], s1[3];
+            pos(p, s0[0], s0[1], s0[2]);
+            pos(nn, s1[0], s1[1], s1[2]);
+            pwrap(P, s0[0], s0[1], s0[2], s1[0], s1[1], s1[2]);
+            const double l = dist3(s0[0], s0[1], s0[2], s1[0], s1[1], s1[2]);
+            double nv[3], npos[3];
+            for (int q = 0; q < 3; ++q) nv[q] = n0[q] + n1[q];
+            for (int q = 0; q < 3; ++q) nv[q] = nv[q] * 0.5;
+            const double lnv = sqrt(dot3(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+            if (lnv > 1e-5) {
+                const double R = 1 / lnv;
+                const double dR = R * R - l * l / 4;
+                const double delta = dR > 0 ? R - sqrt(dR) : R;
+                for (int q = 0; q < 3; ++q) nv[q] = nv[q] * (1 / lnv);
+                for (int q = 0; q < 3; ++q) nv[q] = nv[q] * -1;
+                double aa[3], bb[3];
+                for (int q = 0; q < 3; ++q) aa[q] = s0[q] + s1[q];
+                for (int q = 0; q < 3; ++q) aa[q] = aa[q] * 0.5;
+                for (int q = 0; q < 3; ++q) bb[q] = nv[q] * delta;
+                for (int q = 0; q < 3; ++q) npos[q] = aa[q] + bb[q];
+            } else {
+                for (int q = 0; q < 3; ++q) npos[q] = s0[q] + s1[q];
+                for (int q = 0; q < 3; ++q) npos[q] = npos[q] * 0.5;
+            }
+            NewPt &q = a.np[id];
+            q.fwd = nn;
+            q.rev = p;
+            q.cursor = k;
+            q.slot = -1;
+            q.x = npos[0];
+            q.y = npos[1];
+            q.z = npos[2];
+            q.relinked = 0;
+            set_f(p, -2 - id);
+            set_r(nn, -2 - id);
+            touch(k);
+            touch(nxt);
+        }
+    }
+    if (removed) atomicAdd(a.state + DT_REMOVED, removed);
+    if (fail) atomicOr(a.state + DT_PREM_FAIL, fail);
+}
+
+// new points by the visit that inserted them: (cursor, id)
+__global__ void k_prem_event_keys(PremArgs a, uint64_t *__restrict__ keys,
+                                  int *__restrict__ vals)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.np_cap) return;
+    const int nn = min(a.state[DT_PREM_NEW], a.np_cap);
+    const bool on = i < nn && !a.state[DT_PREM_FAIL];
+    keys[i] = on ? ((uint64_t)(unsigned)a.np[i].cursor << a.ibits) | (unsigned)i : ~0ull;
+    vals[i] = i;
+}
+
+// EMPTY before the pass, one 32-bit word of slots per thread
+__global__ void k_prem_origE(int n, int words, const int *__restrict__ st,
+                             unsigned *__restrict__ origE)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    unsigned b = 0;
+    for (int j = 0; j < 32; ++j) {
+        const int s = 32 * w + j;
+        if (s < n && st[s] == OVG_EMPTY) b |= 1u << j;
+    }
+    origE[w] = b;
+}
+
+// get_next_empty_node (src/tangle.c:201-253) for every insertion in visit
+// order; one wavefront, lanes share the scans.  At visit c a slot is EMPTY
+// when it was before the pass, or a visit before c removed it (a removal
+// happens at its own slot, at visit = slot), and no earlier insertion took
+// it.  The scan's answer is the lower of the first slot EMPTY before the
+// pass and not taken, and the first slot removed before c and not taken.
+__global__ __launch_bounds__(64) void k_prem_alloc(PremArgs a, const uint64_t *__restrict__ ev)
+{
+    if (a.state[DT_PREM_FAIL]) return;
+    const int lane = threadIdx.x;
+    const int n = a.n;
+    const int nn = min(a.state[DT_PREM_NEW], a.np_cap);
+    int hint = a.state[DT_HINT];
+    auto bit = [&](const unsigned *b, int s) {
+        return (ld_agent(b + (s >> 5)) >> (s & 31)) & 1u;
+    };
+    auto empty_at = [&](int s, int c) {
+        if (s < 0 || s >= n || bit(a.taken, s)) return false;
+        return bit(a.origE, s) || (bit(a.removed, s) && s < c);
+    };
+    // first slot >= from set in `bits` and not taken (n when none); 64
+    // words (2048 slots) per round
+    auto first = [&](const unsigned *bits, int from) {
+        for (int w0 = from >> 5; w0 < a.words; w0 += 64) {
+            const int w = w0 + lane;
+            unsigned v = 0;
+            if (w < a.words) {
+                v = ld_agent(bits + w) & ~ld_agent(a.taken + w);
+                if (w == (from >> 5)) v &= ~0u << (from & 31);
+            }
+            const unsigned long long m = __ballot(v != 0);
+            if (m) {
+                const int l = __ffsll((long long)m) - 1;
+                const unsigned vw = __shfl(v, l, 64);
+                const int sl = 32 * (w0 + l) + __ffs(vw) - 1;
+                return sl < n ? sl : n;
+            }
+        }
+        return n;
+    };
+    int po = first(a.origE, 0), pr = first(a.removed, 0);
+    for (int e = 0; e < nn; ++e) {
+        const uint64_t key = ev[e];
+        const int id = (int)(key & ((1ull << a.ibits) - 1)), c = (int)(key >> a.ibits);
+        int got = -1;
+        if (hint > 0 && hint < n && empty_at(hint, c)) {
+            got = hint;
+        } else {
+            if (po < n && bit(a.taken, po)) po = first(a.origE, po);
+            if (pr < n && bit(a.taken, pr)) pr = first(a.removed, pr);
+            got = po;
+            if (pr < c && pr < got) got = pr;
+            if (got >= n) got = -1;
+        }
+        if (got < 0) {
+            if (lane == 0) atomicOr(a.state + DT_ERR, kErrSlots);
+            if (lane == 0) atomicOr(a.state + DT_PREM_FAIL, kFailTable);
+            return;
+        }
+        hint = empty_at(got + 1, c) ? got + 1 : -1;
+        if (lane == 0) {
+            atomicOr(a.taken + (got >> 5), 1u << (got & 31));
+            a.np[id].slot = got;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) a.state[DT_PREM_HINT] = hint;
+}
+
+__device__ __forceinline__ int prem_slot(const PremArgs &a, int v)
+{
+    return v <= -2 ? a.np[-2 - v].slot : v;
+}
+
+// The reference visits a new point whose slot is ahead of its insertion's
+// cursor: that visit must change nothing.
+__global__ void k_prem_check(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = min(a.state[DT_PREM_NEW], a.np_cap);
+    if (i >= nn || a.state[DT_PREM_FAIL]) return;
+    const NewPt &q = a.np[i];
+    if (q.slot <= q.cursor) return;
+    bool acts = q.relinked != 0;
+    if (!acts) {
+        const TangleParams &P = *a.P;
+        double bx, by, bz, cx, cy, cz;
+        auto pos = [&](int s, double &x, double &y, double &z) {
+            if (s >= 0) {
+                x = a.X[s]; y = a.Y[s]; z = a.Z[s];
+            } else {
+                const NewPt &w = a.np[-2 - s];
+                x = w.x; y = w.y; z = w.z;
+            }
+        };
+        pos(q.fwd, bx, by, bz);
+        pos(q.rev, cx, cy, cz);
+        const double lf = dt_dist(P, q.x, q.y, q.z, bx, by, bz);
+        const double lr = dt_dist(P, q.x, q.y, q.z, cx, cy, cz);
+        acts = ((lf < a.min_dist || lr < a.min_dist) && (lf + lr) < a.max_dist) ||
+               lf > a.max_dist;
+    }
+    if (acts) atomicOr(a.state + DT_PREM_FAIL, kFailVisit);
+}
+
+__global__ void k_prem_apply(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = min(a.state[DT_PREM_NEW], a.np_cap);
+    if (i == 0 && !a.state[DT_PREM_FAIL]) {
+        a.state[DT_ADDED] = nn;
+        a.state[DT_HINT] = a.state[DT_PREM_HINT];
+    }
+    if (i >= nn || a.state[DT_PREM_FAIL]) return;
+    const NewPt &q = a.np[i];
+    const int s = q.slot;
+    a.WX[s] = q.x;
+    a.WY[s] = q.y;
+    a.WZ[s] = q.z;
+    for (int d = 0; d < 3; ++d) {
+        a.vs[d][s] = 0.0;
+        a.vels[d][s] = 0.0;
+    }
+    a.st[s] = OVG_FREE;
+    const int f = prem_slot(a, q.fwd), r = prem_slot(a, q.rev);
+    a.fwd[s] = f;
+    a.rev[s] = r;
+    if (q.fwd >= 0) a.rev[q.fwd] = s;
+    if (q.rev >= 0) a.fwd[q.rev] = s;
+}
+
+// fallback: the links and status before the pass
+__global__ void k_prem_restore(int n, const int *__restrict__ state, int *__restrict__ fwd,
+                               int *__restrict__ rev, int *__restrict__ st,
+                               const int *__restrict__ sf, const int *__restrict__ sr,
+                               const int *__restrict__ ss)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !state[DT_PREM_FAIL]) return;
+    fwd[i] = sf[i];
+    rev[i] = sr[i];
+    st[i] = ss[i];
+}
+
+__global__ void k_prem_unwhere(PremArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *a.n_cand && i < a.cand_cap) a.cwhere[a.cand[i]] = -1;
+}
+
+__global__ void k_prem_copy(int n, const int *__restrict__ a, int *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
 }
 
 // ------------------------------------------------------------ reconnection
@@ -1240,33 +1753,6 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
 // order, one thread per cluster, which gives the sequential pass's result.
 // Union-find nodes: pairs [0, cap), endpoints [cap, cap + ebound).
 
-__device__ __forceinline__ int uf_find(int *parent, int x)
-{
-    for (;;) {
-        const int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (p == x) return x;
-        const int gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (gp != p) atomicCAS(parent + x, p, gp);  // path halving
-        x = p;
-    }
-}
-
-__device__ __forceinline__ void uf_union(int *parent, int a, int b)
-{
-    for (;;) {
-        a = uf_find(parent, a);
-        b = uf_find(parent, b);
-        if (a == b) return;
-        if (a < b) {
-            const int t = a;
-            a = b;
-            b = t;
-        }
-        // the larger root goes under the smaller
-        if (atomicCAS(parent + a, a, b) == a) return;
-    }
-}
-
 __global__ void k_cl_pairs(int cap, const int *__restrict__ state, const PairRec *__restrict__ pr,
                            const EndRec *__restrict__ er, const int *__restrict__ where,
                            const int *__restrict__ fwd, const int *__restrict__ rev,
@@ -1368,13 +1854,16 @@ __global__ void k_cl_greedy(ClusterArgs a)
             }
         }
     };
+    int nrec = 0;
     auto velocity = [&](int e, const double t[3], const double nr[3], double v[3]) {
         const EndRec &E = a.er[e];
         const int p = E.slot;
         double bs[3] = {E.bs[0], E.bs[1], E.bs[2]};
         bool found = false;
         const double px = a.X[p], py = a.Y[p], pz = a.Z[p];
-        for (int q = 0; q < E.near_n; ++q) {
+        // corrections come from this cluster's rewirings only
+        const int nq = nrec > 0 ? E.near_n : 0;
+        for (int q = 0; q < nq; ++q) {
             const int c0 = a.pr[E.near[q]].chg;
             if (c0 < 0) continue;
             for (int c = c0; c < c0 + 2; ++c) {
@@ -1394,7 +1883,7 @@ __global__ void k_cl_greedy(ClusterArgs a)
         }
         dt_compose(P, E.f_lia, t, nr, bs, v);
     };
-    int cur_k = -1, nrec = 0;
+    int cur_k = -1;
     bool skip = false, have_t = false, have_v = false;
     double tk[3], nk[3], vk[3];
     for (int t = i; t < np && (a.keys[t] >> 15) == root; ++t) {
@@ -1533,6 +2022,7 @@ int sync_live(Ctx &c)
         c.n_src = st[DT_LIVE];
     }
     c.dt_seen = std::max(c.dt_seen, st[DT_FOUND]);
+    c.prem_seen = std::max(c.prem_seen, st[DT_PREM_CAND]);
     c.dt_ends_seen = std::max(c.dt_ends_seen, st[DT_ENDS]);
     if (st[DT_ERR]) {
         hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, c.dt_state.as<int>(), DT_ERR,
@@ -1607,12 +2097,12 @@ int devtopo_counts(Ctx &c, int out[4])
     return OVG_OK;
 }
 
-int devtopo_counters(Ctx &c, int out[16])
+int devtopo_counters(Ctx &c, int out[20])
 {
-    static_assert(DT_COUNT == 16, "ovg_topology_counters returns 16 counters");
+    static_assert(DT_PUBLIC == 20, "ovg_topology_counters returns 20 counters");
     OVG_TRY(dt_state_ready(c));
     OVG_TRY(sync_live(c));
-    OVG_HIP(hipMemcpyAsync(out, c.dt_state.p, sizeof(int) * DT_COUNT, hipMemcpyDeviceToHost,
+    OVG_HIP(hipMemcpyAsync(out, c.dt_state.p, sizeof(int) * DT_PUBLIC, hipMemcpyDeviceToHost,
                            c.stream));
     OVG_HIP(hipStreamSynchronize(c.stream));
     return OVG_OK;
@@ -1634,6 +2124,108 @@ int devtopo_small_loops(Ctx &c, int cutoff)
     device_edited(c);
     c.live_pending = true;
     return launch_status();
+}
+
+// The cluster pass (remesh by clusters, above); leaves DT_PREM_FAIL nonzero
+// with the links and status as before the pass when it hands the pass back.
+static int prem_queue(Ctx &c, const int *cand, const int *n_cand, double min_dist,
+                      double max_dist)
+{
+    int *state = c.dt_state.as<int>();
+    const int n = c.n;
+    int cap = 16384;
+    while (cap < (1 << 22) && cap < 2 * c.prem_seen) cap *= 2;
+    const int np_cap = cap;
+    const int words = (n + 31) / 32 + 1;
+    OVG_TRY(c.dt_where.ensure(sizeof(int) * (size_t)(n + 1)));
+    OVG_TRY(c.prem_buf.ensure(sizeof(int) * ((size_t)cap + 4 * (size_t)n + 3 * (size_t)words + 64) +
+                              sizeof(NewPt) * (size_t)np_cap + 256));
+    int *parent = c.prem_buf.as<int>();
+    int *owner = parent + cap;
+    int *snap = owner + n;  // fwd, rev, status
+    unsigned *removed = reinterpret_cast<unsigned *>(snap + 3 * (size_t)n);
+    unsigned *taken = removed + words;
+    unsigned *origE = taken + words;
+    NewPt *np = reinterpret_cast<NewPt *>(
+        (reinterpret_cast<uintptr_t>(origE + words) + 63) & ~(uintptr_t)63);
+    TreeBufs &T = c.tree;
+    for (int b = 0; b < 2; ++b) {
+        OVG_TRY(T.keys[b].ensure(sizeof(uint64_t) * (size_t)cap));
+        OVG_TRY(T.vals[b].ensure(sizeof(int) * (size_t)cap));
+    }
+    ++c.tree_id;
+    hipLaunchKernelGGL(k_dt_fill, dim3(nb256(n)), dim3(256), 0, c.stream, n,
+                       c.dt_where.as<int>(), -1);
+    for (int d : {DT_PREM_FAIL, DT_PREM_NEW, DT_ADDED, DT_REMOVED})
+        hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, d, 0);
+    const int *fw = c.fwd.as<int>(), *rv = c.rev.as<int>(), *ss = c.status.as<int>();
+    hipLaunchKernelGGL(k_prem_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, fw, snap);
+    hipLaunchKernelGGL(k_prem_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, rv, snap + n);
+    hipLaunchKernelGGL(k_prem_copy, dim3(nb256(n)), dim3(256), 0, c.stream, n, ss,
+                       snap + 2 * (size_t)n);
+    hipLaunchKernelGGL(k_dt_fill, dim3(nb256(n)), dim3(256), 0, c.stream, n, owner, -1);
+    hipLaunchKernelGGL(k_dt_fill, dim3(nb256(2 * words)), dim3(256), 0, c.stream, 2 * words,
+                       reinterpret_cast<int *>(removed), 0);
+    hipLaunchKernelGGL(k_prem_origE, dim3(nb256(words)), dim3(256), 0, c.stream, n, words, ss,
+                       origE);
+    PremArgs a{};
+    a.n = n;
+    a.cand = cand;
+    a.n_cand = n_cand;
+    a.cand_cap = cap;
+    a.sbits = 1;
+    while ((1LL << a.sbits) < (long long)n) ++a.sbits;
+    a.ibits = 1;
+    while ((1 << a.ibits) < cap) ++a.ibits;  // candidates and new points: both below cap
+    a.fwd = c.fwd.as<int>();
+    a.rev = c.rev.as<int>();
+    a.st = c.status.as<int>();
+    a.X = c.x[0].as<double>();
+    a.Y = c.x[1].as<double>();
+    a.Z = c.x[2].as<double>();
+    a.WX = c.x[0].as<double>();
+    a.WY = c.x[1].as<double>();
+    a.WZ = c.x[2].as<double>();
+    for (int q = 0; q < 3; ++q) {
+        a.vs[q] = c.vs[q].as<double>();
+        a.vels[q] = c.vels[q].as<double>();
+    }
+    a.P = dparams(c);
+    a.min_dist = min_dist;
+    a.max_dist = max_dist;
+    a.cwhere = c.dt_where.as<int>();
+    a.parent = parent;
+    a.ckeys = T.keys[0].as<uint64_t>();
+    a.owner = owner;
+    a.removed = removed;
+    a.taken = taken;
+    a.origE = origE;
+    a.words = words;
+    a.np = np;
+    a.np_cap = np_cap;
+    a.state = state;
+    hipLaunchKernelGGL(k_prem_where, dim3(nb256(cap)), dim3(256), 0, c.stream, a);
+    hipLaunchKernelGGL(k_prem_union, dim3(nb256(cap)), dim3(256), 0, c.stream, a);
+    hipLaunchKernelGGL(k_prem_keys, dim3(nb256(cap)), dim3(256), 0, c.stream, a,
+                       T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
+    // (cluster, slot): the root is a candidate index below cap
+    int passes = (a.sbits + a.ibits + 7) / 8;
+    if (passes & 1) ++passes;
+    OVG_TRY(radix_sort(c, cap, 0, 8 * (passes - 1)));
+    hipLaunchKernelGGL(k_prem_cluster, dim3(nb256(cap)), dim3(256), 0, c.stream, a);
+    hipLaunchKernelGGL(k_prem_event_keys, dim3(nb256(np_cap)), dim3(256), 0, c.stream, a,
+                       T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
+    // (cursor, number): the cursor is a slot
+    OVG_TRY(radix_sort(c, np_cap, 0, 8 * (passes - 1)));
+    hipLaunchKernelGGL(k_prem_alloc, dim3(1), dim3(64), 0, c.stream, a,
+                       T.keys[0].as<const uint64_t>());
+    hipLaunchKernelGGL(k_prem_check, dim3(nb256(np_cap)), dim3(256), 0, c.stream, a);
+    hipLaunchKernelGGL(k_prem_apply, dim3(nb256(np_cap)), dim3(256), 0, c.stream, a);
+    hipLaunchKernelGGL(k_prem_restore, dim3(nb256(n)), dim3(256), 0, c.stream, n, state,
+                       c.fwd.as<int>(), c.rev.as<int>(), c.status.as<int>(), snap, snap + n,
+                       snap + 2 * (size_t)n);
+    hipLaunchKernelGGL(k_prem_unwhere, dim3(nb256(cap)), dim3(256), 0, c.stream, a);
+    return OVG_OK;
 }
 
 int devtopo_remesh(Ctx &c, double min_dist, double max_dist)
@@ -1663,6 +2255,8 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist)
     OVG_TRY(positions_change(c));
     const int *cand = nullptr, *n_cand = nullptr;
     OVG_TRY(remesh_candidates_queue(c, min_dist, max_dist, &cand, &n_cand));
+    OVG_TRY(prem_queue(c, cand, n_cand, min_dist, max_dist));
+    // the one-wavefront pass, for the passes the clusters hand back
     hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_LO, n);
     hipLaunchKernelGGL(k_dt_first_empty, dim3(nb256(n)), dim3(256), 0, c.stream, n,
                        c.status.as<int>(), state);
@@ -1672,6 +2266,7 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist)
     OVG_TRY(c.dt_chg.ensure(std::max(c.dt_chg.bytes, sizeof(int) * gcap)));
     RemeshArgs a{};
     a.n = n;
+    a.gate = state + DT_PREM_FAIL;
     a.cand = cand;
     a.n_cand = n_cand;
     a.gheap = c.dt_chg.as<int>();

@@ -59,6 +59,7 @@ void job_params(const ovs_job &j, ovg_params &p)
     p.freq_to_cutoff = 0.8;
     p.mixed_near = 2 * j.dxi;
     p.rec_velocity = j.rec_velocity;
+    p.topology = j.topology;
 }
 
 int create_job(const ovs_job &j, Active &a)
@@ -84,16 +85,18 @@ int create_job(const ovs_job &j, Active &a)
 }
 
 // One step in the order of src/main.c:127-138.
+// The passes get no count pointers, so that with device topology they queue
+// without waiting for the host.
 int run_step(const ovs_job &j, Active &a)
 {
-    int r, na, nr, nrec, nel;
+    int r;
     const int k = a.steps_done;
-    if ((r = ovg_reconnect(a.ctx, a.t, &nrec))) return r;
-    if ((r = ovg_eliminate_small_loops(a.ctx, 5, &nel))) return r;
+    if ((r = ovg_reconnect(a.ctx, a.t, nullptr))) return r;
+    if ((r = ovg_eliminate_small_loops(a.ctx, 5, nullptr))) return r;
     if (j.smooth_every > 0 && k % j.smooth_every == 0 && (r = ovg_smooth(a.ctx))) return r;
     if ((r = ovg_enforce_periodic(a.ctx))) return r;
-    if ((r = ovg_remesh(a.ctx, j.dxi / 2, 1.5 * j.dxi, &na, &nr))) return r;
-    if ((r = ovg_eliminate_small_loops(a.ctx, 5, &nel))) return r;
+    if ((r = ovg_remesh(a.ctx, j.dxi / 2, 1.5 * j.dxi, nullptr, nullptr))) return r;
+    if ((r = ovg_eliminate_small_loops(a.ctx, 5, nullptr))) return r;
     if ((r = ovg_update_tangle(a.ctx, a.t))) return r;
     if ((r = ovg_rk4_step(a.ctx, a.t, j.dt))) return r;
     if ((r = ovg_enforce_periodic(a.ctx))) return r;

@@ -26,7 +26,7 @@ class Job(C.Structure):
                 ("smooth_every", C.c_int32), ("precision", C.c_int32),
                 ("traverse", C.c_int32), ("leaf_size", C.c_int32),
                 ("points_per_wave", C.c_int32), ("rec_velocity", C.c_int32),
-                ("theta", C.c_double)]
+                ("theta", C.c_double), ("topology", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Result(C.Structure):

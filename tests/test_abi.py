@@ -35,7 +35,7 @@ def test_serve_symbols_exported_and_structs_match():
     lib = C.CDLL(ov.LIB_PATH)
     assert all(hasattr(lib, n) for n in names)
     # field layouts restated in Python: 8-byte aligned, sizes as declared
-    assert C.sizeof(serve.Job) == 8 + 4 * 2 + 8 * 9 + 4 * 6 + 8
+    assert C.sizeof(serve.Job) == 8 + 4 * 2 + 8 * 9 + 4 * 6 + 8 + 4 * 2
     assert C.sizeof(serve.Report) % 8 == 0
 
 
