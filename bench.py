@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--blocks", type=int, default=0,
                     help="list traversal persistent grid, wavefronts per CU (0 = one per group)")
     ap.add_argument("--rec-velocity", default="tree", choices=["direct", "tree"])
-    ap.add_argument("--topology", default="host", choices=["device", "host"],
+    ap.add_argument("--topology", default="device", choices=["device", "host"],
                     help="where reconnection, small-loop removal and remesh run")
     ap.add_argument("--other-topo-steps", type=int, default=3,
                     help="also time this many steps with the other topology mode (0 = skip)")

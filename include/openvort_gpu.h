@@ -410,7 +410,7 @@ int ovg_eliminate_small_loops(ovg_ctx *ctx, int cutoff, int *n_removed);
  * whole state when the slot count doubled). */
 int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int *n_removed);
 
-/* The 20 counters of the device topology passes (OVG_TOPO_DEVICE), read
+/* The 22 counters of the device topology passes (OVG_TOPO_DEVICE), read
  * after the passes queued so far (waits): [0] live points, [1] error bits,
  * [2] allocator hint, [3] scratch, [4] reconnections, [5] points removed by
  * small-loop removal, [6] points added and [7] removed by remesh, [8]
@@ -423,8 +423,11 @@ int ovg_remesh(ovg_ctx *ctx, double min_dist, double max_dist, int *n_added, int
  * slot, 2 a cluster's touched slots overflowed, 4 the new-point table
  * overflowed, 8 a new point would have acted when visited, 16 more
  * candidates than the pass takes), [17] the allocator hint after the
- * cluster pass, [18] its new points, [19] remesh candidates.  Diagnostics,
- * no counterpart in the reference. */
+ * cluster pass, [18] its new points, [19] remesh candidates, [20] clusters
+ * of the last reconnection pass (tree-mode velocities: pairs that no other
+ * cluster's rewiring reaches, one wavefront each) and [21] the candidate
+ * pairs of its largest cluster.  Diagnostics, no counterpart in the
+ * reference. */
 int ovg_topology_counters(ovg_ctx *ctx, int *out);
 
 /* enforce_wall_boundaries, src/tangle.c:966-1016, at time t: for every

@@ -388,8 +388,8 @@ class Engine:
         return (a.value, r.value) if counts else None
 
     def topology_counters(self):
-        """The device topology passes' 20 counters (ovg_topology_counters)."""
-        out = np.zeros(20, dtype=np.int32)
+        """The device topology passes' 22 counters (ovg_topology_counters)."""
+        out = np.zeros(22, dtype=np.int32)
         _check(_lib.ovg_topology_counters(self._h, _i(out)), "ovg_topology_counters")
         return out
 

@@ -312,7 +312,7 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist);
 // points removed, remesh points added and removed.
 int devtopo_counts(Ctx &c, int out[4]);
 // The 20 public device counters (ovg_topology_counters).
-int devtopo_counters(Ctx &c, int out[20]);
+int devtopo_counters(Ctx &c, int out[22]);
 // engine.hip: grow the slots to n_new (> n) with the host mirrors
 // current, the new slots EMPTY; every old slot keeps its state.
 int grow_capacity(Ctx &c, int n_new);

@@ -49,6 +49,8 @@ enum {
     DT_PREM_HINT,       // the allocator hint after its replay
     DT_PREM_NEW,        // new points of the cluster pass
     DT_PREM_CAND,       // remesh candidates of the last pass
+    DT_CL_N,            // clusters of the last greedy pass by clusters
+    DT_CL_MAX,          // candidate pairs in its largest cluster
     DT_PUBLIC,          // the counters ovg_topology_counters returns
     DT_COUNT = 32
 };
@@ -1795,33 +1797,50 @@ __global__ void k_cl_keys(int cap, const int *__restrict__ state, int *__restric
     vals[j] = j;
 }
 
+// cluster starts: the first pair of each cluster, in any order
+__global__ void k_cl_starts(int cap, const uint64_t *__restrict__ keys, int *__restrict__ state,
+                            int *__restrict__ starts)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int np = state[DT_PAIRS];
+    if (i >= cap || i >= np || state[DT_ENDS] == 0 || state[DT_NEAR_OVER]) return;
+    if (i > 0 && (keys[i - 1] >> 15) == (keys[i] >> 15)) return;
+    starts[atomicAdd(state + DT_CL_N, 1)] = i;
+}
+
 struct ClusterArgs {
     int cap;
     const uint64_t *keys;   // pairs by (cluster, j)
+    const int *starts;      // first pair of each cluster (state[DT_CL_N] of them)
     PairRec *pr;
     EndRec *er;
     const int *where;
     int *fwd, *rev;
     const double *X, *Y, *Z;
     int4 *chg;               // pair j's changes at 2 j, 2 j + 1
-    unsigned char *flag, *stale;  // per endpoint
+    int *flag, *stale;       // per endpoint
     const TangleParams *P;
     double cutoff, rc2;
     int *state;
 };
 
-// host_recon.h greedy_pass restricted to one cluster, one thread
-__global__ void k_cl_greedy(ClusterArgs a)
+// host_recon.h greedy_pass restricted to one cluster, one wavefront per
+// cluster.  The lanes run the pass in step (every lane holds the same
+// values and makes the same writes; what the pass wrote is read back with
+// ld); a field correction evaluates its two segments' 27 image terms on
+// separate lanes and adds them in image order, as the sequential sum does.
+__global__ __launch_bounds__(64) void k_cl_greedy(ClusterArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sv[2][32][3];
+    __shared__ int sok[2][32];
+    const int lane = threadIdx.x;
     const int np = a.state[DT_PAIRS];
-    if (i >= a.cap || i >= np || a.state[DT_ENDS] == 0 || a.state[DT_NEAR_OVER]) return;
-    const uint64_t root = a.keys[i] >> 15;
-    if (i > 0 && (a.keys[i - 1] >> 15) == root) return;  // not the cluster's first pair
+    const int ncl = a.state[DT_CL_N];
+    if (a.state[DT_ENDS] == 0 || a.state[DT_NEAR_OVER]) return;
     const TangleParams &P = *a.P;
     auto tangent = [&](int e, double t[3], double nr[3]) {
         const EndRec &E = a.er[e];
-        if (!a.stale[e]) {
+        if (!ld(a.stale + e)) {
             for (int q = 0; q < 3; ++q) {
                 t[q] = E.t[q];
                 nr[q] = E.nr[q];
@@ -1829,136 +1848,151 @@ __global__ void k_cl_greedy(ClusterArgs a)
             return;
         }
         const int s = E.slot;
-        const int i1 = a.fwd[s], im1 = a.rev[s];
-        const int i2 = a.fwd[i1], im2 = a.rev[im1];
+        const int i1 = ld(a.fwd + s), im1 = ld(a.rev + s);
+        const int i2 = ld(a.fwd + i1), im2 = ld(a.rev + im1);
         tangent_normal_pts(P, a.X[s], a.Y[s], a.Z[s], a.X[i1], a.Y[i1], a.Z[i1], a.X[i2],
                            a.Y[i2], a.Z[i2], a.X[im1], a.Y[im1], a.Z[im1], a.X[im2], a.Y[im2],
                            a.Z[im2], t, nr);
     };
-    // the field of segment (m -> f) at slot p, all images (pure shifts)
-    auto seg_all = [&](int m, int f, int p, double acc[3]) {
-        const double xm = a.X[m], ym = a.Y[m], zm = a.Z[m];
-        double bx = a.X[f], by = a.Y[f], bz = a.Z[f];
-        pwrap(P, xm, ym, zm, bx, by, bz);
-        const double rx = a.X[p], ry = a.Y[p], rz = a.Z[p];
-        for (int im = -1; im < P.n_images; ++im) {
-            if (im < 0 && (m == p || f == p)) continue;
+    // bs += field of (m -> nf) - field of (m -> of) at slot p, each summed
+    // over the images in order (primary box skipped when p is a segment end)
+    auto correct = [&](int m, int of, int nf, int p, double bs[3]) {
+        const int half = lane >> 5, im = (lane & 31) - 1;
+        const int f = half ? nf : of;
+        double v[3] = {0, 0, 0};
+        int ok = 0;
+        if (im < P.n_images && !(im < 0 && (m == p || f == p))) {
+            const double xm = a.X[m], ym = a.Y[m], zm = a.Z[m];
+            double bx = a.X[f], by = a.Y[f], bz = a.Z[f];
+            pwrap(P, xm, ym, zm, bx, by, bz);
+            const double rx = a.X[p], ry = a.Y[p], rz = a.Z[p];
             double qx = rx, qy = ry, qz = rz;
             if (im >= 0) image_point(P, im, rx, ry, rz, qx, qy, qz);
-            double vx, vy, vz;
-            if (seg_field64(xm - qx, ym - qy, zm - qz, bx - qx, by - qy, bz - qz, P.f_bs, vx,
-                            vy, vz)) {
-                acc[0] += vx;
-                acc[1] += vy;
-                acc[2] += vz;
-            }
+            ok = seg_field64(xm - qx, ym - qy, zm - qz, bx - qx, by - qy, bz - qz, P.f_bs, v[0],
+                             v[1], v[2]);
         }
+        __syncthreads();
+        for (int q = 0; q < 3; ++q) sv[half][lane & 31][q] = v[q];
+        sok[half][lane & 31] = ok;
+        __syncthreads();
+        double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+        for (int k = 0; k <= P.n_images; ++k) {
+            if (sok[0][k])
+                for (int q = 0; q < 3; ++q) vo[q] += sv[0][k][q];
+            if (sok[1][k])
+                for (int q = 0; q < 3; ++q) vn[q] += sv[1][k][q];
+        }
+        for (int q = 0; q < 3; ++q) bs[q] += vn[q] - vo[q];
     };
-    int nrec = 0;
-    auto velocity = [&](int e, const double t[3], const double nr[3], double v[3]) {
-        const EndRec &E = a.er[e];
-        const int p = E.slot;
-        double bs[3] = {E.bs[0], E.bs[1], E.bs[2]};
-        bool found = false;
-        const double px = a.X[p], py = a.Y[p], pz = a.Z[p];
-        // corrections come from this cluster's rewirings only
-        const int nq = nrec > 0 ? E.near_n : 0;
-        for (int q = 0; q < nq; ++q) {
-            const int c0 = a.pr[E.near[q]].chg;
-            if (c0 < 0) continue;
-            for (int c = c0; c < c0 + 2; ++c) {
-                const int4 ch = a.chg[c];
-                if (!(dt_near2(P, a.X[ch.x], a.Y[ch.x], a.Z[ch.x], px, py, pz) < a.rc2))
-                    continue;
-                double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
-                seg_all(ch.x, ch.y, p, vo);
-                seg_all(ch.x, ch.z, p, vn);
-                for (int d = 0; d < 3; ++d) bs[d] += vn[d] - vo[d];
-                found = true;
+    for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
+        const int i = a.starts[ci];
+        const uint64_t root = a.keys[i] >> 15;
+        int nrec = 0;
+        auto velocity = [&](int e, const double t[3], const double nr[3], double v[3]) {
+            const EndRec &E = a.er[e];
+            const int p = E.slot;
+            double bs[3] = {E.bs[0], E.bs[1], E.bs[2]};
+            bool found = false;
+            const double px = a.X[p], py = a.Y[p], pz = a.Z[p];
+            // corrections come from this cluster's rewirings only
+            const int nq = nrec > 0 ? E.near_n : 0;
+            for (int q = 0; q < nq; ++q) {
+                const int c0 = ld(&a.pr[E.near[q]].chg);
+                if (c0 < 0) continue;
+                for (int c = c0; c < c0 + 2; ++c) {
+                    const int m = ld(&a.chg[c].x), of = ld(&a.chg[c].y), nf = ld(&a.chg[c].z);
+                    if (!(dt_near2(P, a.X[m], a.Y[m], a.Z[m], px, py, pz) < a.rc2)) continue;
+                    correct(m, of, nf, p, bs);
+                    found = true;
+                }
             }
-        }
-        if (!found && !a.stale[e]) {
-            for (int q = 0; q < 3; ++q) v[q] = E.vel[q];
-            return;
-        }
-        dt_compose(P, E.f_lia, t, nr, bs, v);
-    };
-    int cur_k = -1;
-    bool skip = false, have_t = false, have_v = false;
-    double tk[3], nk[3], vk[3];
-    for (int t = i; t < np && (a.keys[t] >> 15) == root; ++t) {
-        const int j = (int)(a.keys[t] & 0x7fff);
-        PairRec &R = a.pr[j];
-        const int k = R.k, l = R.l, ek = R.ek, el = R.el;
-        if (k != cur_k) {
-            cur_k = k;
-            skip = a.flag[ek];
-            have_t = false;
-            have_v = false;
-        }
-        if (skip || a.flag[el]) continue;
-        const int info = R.info;
-        double tl[3], nl[3];
-        bool have_tl = false;
-        if (!a.stale[ek] && !a.stale[el]) {
-            if (info & 4) continue;
-        } else {
-            if (!have_t) {
-                tangent(ek, tk, nk);
-                have_t = true;
+            if (!found && !ld(a.stale + e)) {
+                for (int q = 0; q < 3; ++q) v[q] = E.vel[q];
+                return;
             }
-            tangent(el, tl, nl);
-            have_tl = true;
-            if (dt_angle_reject(tk, tl, a.cutoff)) continue;
-        }
-        if (!have_v) {
-            if (!have_t) {
-                tangent(ek, tk, nk);
-                have_t = true;
+            dt_compose(P, E.f_lia, t, nr, bs, v);
+        };
+        int cur_k = -1, npairs = 0;
+        bool skip = false, have_t = false, have_v = false;
+        double tk[3], nk[3], vk[3];
+        for (int t = i; t < np && (a.keys[t] >> 15) == root; ++t) {
+            ++npairs;
+            const int j = (int)(a.keys[t] & 0x7fff);
+            PairRec &R = a.pr[j];
+            const int k = R.k, l = R.l, ek = R.ek, el = R.el;
+            if (k != cur_k) {
+                cur_k = k;
+                skip = ld(a.flag + ek);
+                have_t = false;
+                have_v = false;
             }
-            velocity(ek, tk, nk, vk);
-            have_v = true;
+            if (skip || ld(a.flag + el)) continue;
+            const int info = R.info;
+            double tl[3], nl[3];
+            bool have_tl = false;
+            if (!ld(a.stale + ek) && !ld(a.stale + el)) {
+                if (info & 4) continue;
+            } else {
+                if (!have_t) {
+                    tangent(ek, tk, nk);
+                    have_t = true;
+                }
+                tangent(el, tl, nl);
+                have_tl = true;
+                if (dt_angle_reject(tk, tl, a.cutoff)) continue;
+            }
+            if (!have_v) {
+                if (!have_t) {
+                    tangent(ek, tk, nk);
+                    have_t = true;
+                }
+                velocity(ek, tk, nk, vk);
+                have_v = true;
+            }
+            if (!have_tl) tangent(el, tl, nl);
+            double vl[3];
+            velocity(el, tl, nl, vl);
+            double dv[3];
+            for (int q = 0; q < 3; ++q) dv[q] = vk[q] - vl[q];
+            if (dot3(R.dx[0], R.dx[1], R.dx[2], dv[0], dv[1], dv[2]) > 0) continue;
+            const int kind = info & 3;
+            if (!kind) continue;
+            const EndRec &K = a.er[ek], &L = a.er[el];
+            const int kf = K.fwd, kr = K.rev, lf = L.fwd, lr = L.rev;
+            if (kind == 1) {
+                a.fwd[k] = l;
+                a.rev[l] = k;
+                a.rev[kf] = lr;
+                a.fwd[lr] = kf;
+                a.chg[2 * j] = make_int4(k, kf, l, j);
+                a.chg[2 * j + 1] = make_int4(lr, l, kf, j);
+            } else {
+                a.fwd[l] = k;
+                a.rev[k] = l;
+                a.rev[lf] = kr;
+                a.fwd[kr] = lf;
+                a.chg[2 * j] = make_int4(l, lf, k, j);
+                a.chg[2 * j + 1] = make_int4(kr, k, lf, j);
+            }
+            R.chg = 2 * j;
+            ++nrec;
+            const int fl[6] = {ek, el, K.wf, K.wr, L.wf, L.wr};
+            for (int q = 0; q < 6; ++q)
+                if (fl[q] >= 0) a.flag[fl[q]] = 1;
+            const int six[6] = {k, l, kf, kr, lf, lr};
+            for (int q = 0; q < 6; ++q) {
+                const int sl = six[q];
+                const int w[3] = {a.where[sl], a.where[ld(a.fwd + sl)], a.where[ld(a.rev + sl)]};
+                for (int d = 0; d < 3; ++d)
+                    if (w[d] >= 0) a.stale[w[d]] = 1;
+            }
+            skip = true;
         }
-        if (!have_tl) tangent(el, tl, nl);
-        double vl[3];
-        velocity(el, tl, nl, vl);
-        double dv[3];
-        for (int q = 0; q < 3; ++q) dv[q] = vk[q] - vl[q];
-        if (dot3(R.dx[0], R.dx[1], R.dx[2], dv[0], dv[1], dv[2]) > 0) continue;
-        const int kind = info & 3;
-        if (!kind) continue;
-        const EndRec &K = a.er[ek], &L = a.er[el];
-        const int kf = K.fwd, kr = K.rev, lf = L.fwd, lr = L.rev;
-        if (kind == 1) {
-            a.fwd[k] = l;
-            a.rev[l] = k;
-            a.rev[kf] = lr;
-            a.fwd[lr] = kf;
-            a.chg[2 * j] = make_int4(k, kf, l, j);
-            a.chg[2 * j + 1] = make_int4(lr, l, kf, j);
-        } else {
-            a.fwd[l] = k;
-            a.rev[k] = l;
-            a.rev[lf] = kr;
-            a.fwd[kr] = lf;
-            a.chg[2 * j] = make_int4(l, lf, k, j);
-            a.chg[2 * j + 1] = make_int4(kr, k, lf, j);
+        if (lane == 0) {
+            if (nrec) atomicAdd(a.state + DT_REC, nrec);
+            atomicMax(a.state + DT_CL_MAX, npairs);
         }
-        R.chg = 2 * j;
-        ++nrec;
-        const int fl[6] = {ek, el, K.wf, K.wr, L.wf, L.wr};
-        for (int q = 0; q < 6; ++q)
-            if (fl[q] >= 0) a.flag[fl[q]] = 1;
-        const int six[6] = {k, l, kf, kr, lf, lr};
-        for (int q = 0; q < 6; ++q) {
-            const int sl = six[q];
-            const int w[3] = {a.where[sl], a.where[a.fwd[sl]], a.where[a.rev[sl]]};
-            for (int d = 0; d < 3; ++d)
-                if (w[d] >= 0) a.stale[w[d]] = 1;
-        }
-        skip = true;
     }
-    if (nrec) atomicAdd(a.state + DT_REC, nrec);
 }
 
 __global__ void k_dt_clear_where(int bound, const int *__restrict__ state,
@@ -2097,9 +2131,9 @@ int devtopo_counts(Ctx &c, int out[4])
     return OVG_OK;
 }
 
-int devtopo_counters(Ctx &c, int out[20])
+int devtopo_counters(Ctx &c, int out[22])
 {
-    static_assert(DT_PUBLIC == 20, "ovg_topology_counters returns 20 counters");
+    static_assert(DT_PUBLIC == 22, "ovg_topology_counters returns 22 counters");
     OVG_TRY(dt_state_ready(c));
     OVG_TRY(sync_live(c));
     OVG_HIP(hipMemcpyAsync(out, c.dt_state.p, sizeof(int) * DT_PUBLIC, hipMemcpyDeviceToHost,
@@ -2396,13 +2430,16 @@ int devtopo_reconnect(Ctx &c, double t)
     if (!exact) {
         // clusters of pairs that no other cluster's rewiring reaches
         const int nodes = cap + ebound;
-        OVG_TRY(c.dt_cl.ensure(sizeof(int) * (size_t)nodes + 2 * (size_t)ebound + 64));
+        OVG_TRY(c.dt_cl.ensure(sizeof(int) * ((size_t)nodes + 2 * (size_t)ebound + cap) + 64));
         int *parent = c.dt_cl.as<int>();
-        unsigned char *flag = reinterpret_cast<unsigned char *>(parent + nodes);
-        unsigned char *stale = flag + ebound;
+        int *flag = parent + nodes;
+        int *stale = flag + ebound;
+        int *starts = stale + ebound;
         hipLaunchKernelGGL(k_dt_iota, dim3(nb256(nodes)), dim3(256), 0, c.stream, nodes, parent);
-        hipLaunchKernelGGL(k_dt_fill, dim3(nb256(ebound / 2)), dim3(256), 0, c.stream,
-                           ebound / 2, reinterpret_cast<int *>(flag), 0);
+        hipLaunchKernelGGL(k_dt_fill, dim3(nb256(2 * ebound)), dim3(256), 0, c.stream,
+                           2 * ebound, flag, 0);
+        hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_CL_N, 0);
+        hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_CL_MAX, 0);
         hipLaunchKernelGGL(k_cl_pairs, dim3(nb256(cap)), dim3(256), 0, c.stream, cap, state, pr,
                            er, where, c.fwd.as<int>(), c.rev.as<int>(), parent);
         hipLaunchKernelGGL(k_cl_near, dim3(nb256(ebound)), dim3(256), 0, c.stream, cap, ebound,
@@ -2411,9 +2448,12 @@ int devtopo_reconnect(Ctx &c, double t)
                            parent, T.keys[0].as<uint64_t>(), T.vals[0].as<int>());
         ++c.tree_id;
         OVG_TRY(radix_sort(c, cap, 0, 24));
+        hipLaunchKernelGGL(k_cl_starts, dim3(nb256(cap)), dim3(256), 0, c.stream, cap,
+                           T.keys[0].as<uint64_t>(), state, starts);
         ClusterArgs ca{};
         ca.cap = cap;
         ca.keys = T.keys[0].as<uint64_t>();
+        ca.starts = starts;
         ca.pr = pr;
         ca.er = er;
         ca.where = where;
@@ -2429,7 +2469,8 @@ int devtopo_reconnect(Ctx &c, double t)
         ca.cutoff = cutoff;
         ca.rc2 = rc2;
         ca.state = state;
-        hipLaunchKernelGGL(k_cl_greedy, dim3(nb256(cap)), dim3(256), 0, c.stream, ca);
+        hipLaunchKernelGGL(k_cl_greedy, dim3(std::min(cap, 16 * c.n_cu)), dim3(64), 0, c.stream,
+                           ca);
     }
     GreedyArgs g{};
     g.pr = pr;

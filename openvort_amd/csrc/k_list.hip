@@ -181,7 +181,11 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     // and root node (loaded again when the tangle changes)
     const bool forest = Pp[0].n_tangles > 1;
     int cur_tangle = -1;
-    for (int g = xcd_group(blockIdx.x, gridDim.x); g < n_groups; g += gridDim.x) {
+    // a grid sized for more groups than the device counted (a capacity
+    // bound) maps only its first n_groups blocks, so every XCD gets a share
+    const int nb = min((int)gridDim.x, n_groups);
+    if ((int)blockIdx.x >= nb) return;
+    for (int g = xcd_group(blockIdx.x, nb); g < n_groups; g += nb) {
     const int2 grp = groups[g];
     const int tg = forest ? __builtin_amdgcn_readfirstlane(Pp[0].tid[spts[grp.x]]) : 0;
     const TangleParams &P = Pp[tg];
@@ -870,9 +874,12 @@ int launch_bh_list_groups(Ctx &c, const double *const x[3], const int2 *d_grp, c
     if (max_groups <= 0) return OVG_OK;
     TreeBufs &T = c.tree;
     const int ppw = list_points_per_wave(c.params);
+    // the count may be a capacity bound (groups counted on the device):
+    // blocks stride over the groups, at most 16 wavefronts per CU
+    const int grid = std::min(list_grid(c, max_groups), 16 * c.n_cu);
     unsigned long long *cnt = c.counter64.as<unsigned long long>();
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(list_grid(c, max_groups)), dim3(64), 0, c.stream, nullptr,
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, c.stream, nullptr,
                            nullptr, 0, 0, d_grp, d_tgt, x[0], x[1],
                            x[2], T.nodes.as<Node>(), T.meta.as<NodeMeta>(), T.sx.as<double>(),
                            T.sy.as<double>(), T.sz.as<double>(), T.sqx.as<double>(),
