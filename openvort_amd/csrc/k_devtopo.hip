@@ -7,8 +7,11 @@
 // for the host: the passes leave their counts in a small array of device
 // counters (DT_*) that the host reads when a call needs them (sync_live).
 //
-// The sequential parts run in one wavefront and keep the reference's
-// visiting order.  They make the same decisions as the host passes
+// The sequential parts keep the reference's visiting order: the greedy
+// reconnection and the remesh visit run per independent cluster (pairs or
+// slots that no other cluster's edits can reach), one thread or wavefront
+// each, with a one-wavefront kernel over the whole pass where clusters do
+// not apply.  They make the same decisions as the host passes
 // (host_recon.h, engine.hip) with the same FP64 operation order, so the two
 // give identical links, status and positions; what runs in parallel around
 // them (candidate search, endpoint stencils and velocities, the remesh

@@ -96,6 +96,7 @@ __global__ void k_cell_ranges(int n_live, const uint64_t *__restrict__ keys,
 // dist = sqrt(e.e) > rec_dist is decided on e.e against rec_dist^2 with a
 // relative margin of 2^-40 (far wider than the roundings of e.e, rec_dist^2
 // and the square root) and evaluated exactly only inside that margin.
+// live: the targets, taken in the order given (the cell order)
 __global__ void k_candidates(int n_live, const int *__restrict__ live,
                              const int *__restrict__ sorted, const double *__restrict__ X,
                              const double *__restrict__ Y, const double *__restrict__ Z,
@@ -190,7 +191,8 @@ int candidates_queue(Ctx &c)
                        0);
     hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), start, end);
-    hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
+    // targets in cell order too: a wavefront's neighbourhoods overlap
+    hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, T.vals[0].as<int>(),
                        T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
                        c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
                        c.rc_count.as<int>(), c.rc_pairs.as<int2>(), c.rc_cap);
@@ -233,8 +235,9 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, 1, c.rc_count.as<int>(),
                            0);
-        hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, c.live.as<int>(),
-                           T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
+        hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n,
+                           T.vals[0].as<int>(), T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(),
+                           c.rev.as<int>(),
                            c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
                            c.rc_count.as<int>(),
                            c.rc_pairs.as<int2>(), c.rc_cap);
