@@ -187,6 +187,7 @@ struct Ctx {
     DBuf dt_mark;                       // int per slot, 0 between passes
     DBuf dt_pairs, dt_ends, dt_chg;     // reconnection pass scratch
     DBuf dt_cl;                         // its clusters: union-find parents, flags
+    DBuf dt_delta;                      // direct mode: field change of each change at each end
     DBuf prem_buf;                      // remesh by clusters: parents, owners, copies, table
     int prem_seen = 0;                  // most remesh candidates a device pass met
     bool dt_unread = false;             // device passes ran since the counters were read

@@ -55,6 +55,7 @@ enum {
     DT_CL_N,            // clusters of the last greedy pass by clusters
     DT_CL_MAX,          // candidate pairs in its largest cluster
     DT_PUBLIC,          // the counters ovg_topology_counters returns
+    DT_DELTA_ON = DT_PUBLIC,  // direct mode: the field-change table holds every change
     DT_COUNT = 32
 };
 constexpr int kErrPairs = 1, kErrEnds = 2, kErrSlots = 4, kErrHeap = 8;
@@ -62,6 +63,7 @@ constexpr int kMaxPairs = 32768, kMaxEnds = 65536;
 constexpr int kNear = 32;         // tree mode: rewirable pairs listed per endpoint
 constexpr int kHeapLds = 4096;    // remesh: top of the heap of touched slots, in LDS
 constexpr int kGroupSort = 8192;  // endpoints grouped by tree order in one LDS sort
+constexpr size_t kDeltaBytes = size_t(4) << 30;  // direct mode's field-change table
 
 template <class T>
 __device__ __forceinline__ T ld(const T *p)
@@ -1428,6 +1430,10 @@ struct GreedyArgs {
     double cutoff, rc2;
     int exact, max_chg;
     int *state;
+    const double *delta;  // direct mode: field change of pair change r at endpoint e,
+                          // 3 doubles at (r * ends + e) * 3 (state[DT_DELTA_ON])
+    double *bsr;          // with the table: each endpoint's Biot-Savart with every
+                          // rewiring so far, component q at q * ends + e
 };
 
 constexpr int kGreedyWords = kMaxEnds / 32;
@@ -1453,6 +1459,56 @@ __device__ void dt_segment_all(const GreedyArgs &a, const TangleParams &P, int m
             acc[1] += vy;
             acc[2] += vz;
         }
+    }
+}
+
+// Direct mode: every rewiring changes every endpoint's velocity, by the
+// field of its two new segments minus that of the two old ones.  The
+// changes a pair would make are fixed by the links before the pass
+// (do_reconnection's case and the endpoints' recorded links), so the table
+// of each candidate change's field change at each endpoint is computed up
+// front, in parallel, with the arithmetic the sequential pass uses
+// (dt_segment_all, new minus old); the pass then only adds table entries in
+// rewiring order.  Row 2 j + s is pair j's change s (s = 0: the segment
+// from k, or from l; s = 1: the other), as the pass numbers its changes.
+// Tables that would exceed `cap` entries are not filled (DT_DELTA_ON = 0)
+// and the pass evaluates the changes itself.  With the table, the pass
+// keeps every endpoint's running sum (bsr, from the pre-pass values) and
+// adds each rewiring's two rows to all of them as it happens, so an
+// endpoint's sum at any point is the one the sequential pass forms, with
+// the additions in the same order.
+__global__ void k_dt_delta(GreedyArgs a, size_t cap, double *__restrict__ delta)
+{
+    const int np = a.state[DT_PAIRS], ne = a.state[DT_ENDS];
+    const size_t total = 2 * (size_t)np * (size_t)ne;
+    const bool on = total > 0 && total <= cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.state[DT_DELTA_ON] = on;
+    if (!on) return;
+    const TangleParams &P = *a.P;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x)
+        for (int q = 0; q < 3; ++q) a.bsr[(size_t)q * ne + e] = a.er[e].bs[q];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / (size_t)ne), e = (int)(i - (size_t)r * ne);
+        const PairRec &R = a.pr[r >> 1];
+        const int kind = R.info & 3;
+        if (!kind) continue;  // never rewired: the row is never read
+        const EndRec &K = a.er[R.ek], &L = a.er[R.el];
+        int m, of, nf;
+        if (kind == 1) {
+            m = (r & 1) ? L.rev : R.k;
+            of = (r & 1) ? R.l : K.fwd;
+            nf = (r & 1) ? K.fwd : R.l;
+        } else {
+            m = (r & 1) ? K.rev : R.l;
+            of = (r & 1) ? R.k : L.fwd;
+            nf = (r & 1) ? L.fwd : R.k;
+        }
+        const int p = a.er[e].slot;
+        double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+        dt_segment_all(a, P, m, of, p, vo);
+        dt_segment_all(a, P, m, nf, p, vn);
+        for (int q = 0; q < 3; ++q) delta[3 * i + q] = vn[q] - vo[q];
     }
 }
 
@@ -1512,16 +1568,26 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
     };
     // bs += (field of the new segment - field of the old one), change by
     // change in order, for the cnt changes listed in sfound
-    auto apply = [&](int cnt, int p, double bs[3]) {
+    // (the table, when filled, holds the same differences: k_dt_delta)
+    const bool use_delta = a.exact && a.state[DT_DELTA_ON];
+    const int n_ends = a.state[DT_ENDS];
+    auto apply = [&](int cnt, int e, int p, double bs[3]) {
         for (int b0 = 0; b0 < cnt; b0 += 64) {
             const int nb = min(64, cnt - b0);
             __syncthreads();
             if (lane < nb) {
-                const int4 ch = a.chg[sfound[b0 + lane]];
-                double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
-                dt_segment_all(a, P, ch.x, ch.y, p, vo);
-                dt_segment_all(a, P, ch.x, ch.z, p, vn);
-                for (int q = 0; q < 3; ++q) sd[lane][q] = vn[q] - vo[q];
+                const int c = sfound[b0 + lane];
+                if (use_delta) {
+                    const int r = 2 * ld(&a.chg[c].w) + (c & 1);
+                    const double *d = a.delta + 3 * ((size_t)r * n_ends + e);
+                    for (int q = 0; q < 3; ++q) sd[lane][q] = d[q];
+                } else {
+                    const int4 ch = a.chg[c];
+                    double vo[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+                    dt_segment_all(a, P, ch.x, ch.y, p, vo);
+                    dt_segment_all(a, P, ch.x, ch.z, p, vn);
+                    for (int q = 0; q < 3; ++q) sd[lane][q] = vn[q] - vo[q];
+                }
             }
             __syncthreads();
             for (int i = 0; i < nb; ++i)
@@ -1550,7 +1616,11 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
         const int p = E.slot;
         double bs[3] = {ld(&E.bs[0]), ld(&E.bs[1]), ld(&E.bs[2])};
         bool found = false;
-        if (a.exact) {
+        if (use_delta) {
+            // the running sum, kept up to date at every rewiring
+            found = nch > 0;
+            for (int q = 0; q < 3; ++q) bs[q] = ld_agent(a.bsr + (size_t)q * n_ends + e);
+        } else if (a.exact) {
             // every rewiring so far, added once into the endpoint's running sum
             found = nch > 0;
             int app = ld(&E.applied);
@@ -1559,7 +1629,7 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
                 __syncthreads();
                 if (lane < nb) sfound[lane] = app + lane;
                 __syncthreads();
-                apply(nb, p, bs);
+                apply(nb, e, p, bs);
                 app += nb;
             }
             if (ld(&E.applied) != app) {
@@ -1586,7 +1656,7 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
                     }
                 }
                 __syncthreads();
-                apply(cnt, p, bs);
+                apply(cnt, e, p, bs);
             } else {
                 for (int c0 = 0; c0 < nch; c0 += 64) {
                     const int c = c0 + lane;
@@ -1601,7 +1671,7 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
                     __syncthreads();
                     if (hit) sfound[__popcll(b & ((1ull << lane) - 1))] = c;
                     __syncthreads();
-                    apply(nb, p, bs);
+                    apply(nb, e, p, bs);
                     cnt += nb;
                 }
             }
@@ -1709,6 +1779,17 @@ __global__ __launch_bounds__(64) void k_dt_greedy(GreedyArgs a)
             set(rbits, j);
             nch += 2;
             ++nrec;
+            if (use_delta) {
+                // both changes into every endpoint's running sum, in order
+                const double *d0 = a.delta + 3 * (size_t)(2 * j) * n_ends;
+                const double *d1 = d0 + 3 * (size_t)n_ends;
+                for (int e = lane; e < n_ends; e += 64)
+                    for (int q = 0; q < 3; ++q) {
+                        double *b = a.bsr + (size_t)q * n_ends + e;
+                        *b = (ld_agent(b) + d0[3 * e + q]) + d1[3 * e + q];
+                    }
+                __syncthreads();
+            }
             // flags: the pair, its old neighbours (and the new ones, which are
             // among them)
             set(fbits, ek);
@@ -2397,6 +2478,7 @@ int devtopo_reconnect(Ctx &c, double t)
                      c.bs_img[2].as<double>()};
     const bool exact = !(c.params.rec_velocity == OVG_RECVEL_TREE &&
                          c.params.method == OVG_METHOD_BH);
+    int ex_pairs = 0, ex_ends = 0;  // direct mode: the pass's counts
     if (!exact) {
         // the step's own Barnes-Hut evaluator at the endpoints
         OVG_TRY(build_tree(c, xv));
@@ -2411,12 +2493,20 @@ int devtopo_reconnect(Ctx &c, double t)
                            d_tgt, T.sub_count.as<int>());
         OVG_TRY(launch_bh_list_groups(c, xv, d_grp, d_tgt, T.sub_count.as<int>(), ebound));
     } else {
-        // update_velocity(tree = NULL): the FP64 direct sum (:250-251)
+        // update_velocity(tree = NULL): the FP64 direct sum (:250-251).
+        // This mode waits once for the pass's pair and endpoint counts, so
+        // the direct sum is planned for the endpoints there are (enough
+        // source chunks to fill the device) and the field-change table is
+        // sized to them.
+        int cnt[DT_COUNT];
+        OVG_HIP(hipMemcpyAsync(cnt, state, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        ex_pairs = std::max(0, std::min(cnt[DT_PAIRS], cap));
+        ex_ends = std::max(0, std::min(cnt[DT_ENDS], ebound));
         launch_prep_segments(c, xv);
         ovg_params saved = c.params;
         c.params.precision = OVG_PREC_FP64;
-        const int rd = launch_direct_counted(c, xv, ebound, std::max(256, c.dt_ends_seen), eslot,
-                                             state + DT_ENDS, op, oi);
+        const int rd = launch_direct(c, xv, ex_ends, eslot, op, oi, 0);
         c.params = saved;
         OVG_TRY(rd);
     }
@@ -2491,6 +2581,19 @@ int devtopo_reconnect(Ctx &c, double t)
     g.exact = exact ? 1 : 0;
     g.max_chg = 2 * cap;
     g.state = state;
+    if (exact) {
+        // the table of field changes, when it fits in kDeltaBytes
+        size_t entries = 2 * (size_t)ex_pairs * (size_t)ex_ends;
+        if (entries * 3 * sizeof(double) > kDeltaBytes ||
+            c.dt_delta.ensure((entries + ex_ends) * 3 * sizeof(double) + 64) != OVG_OK) {
+            clear_error();
+            entries = 0;
+        }
+        g.delta = c.dt_delta.as<double>();
+        g.bsr = c.dt_delta.p ? c.dt_delta.as<double>() + 3 * entries : nullptr;
+        hipLaunchKernelGGL(k_dt_delta, dim3(8 * c.n_cu), dim3(256), 0, c.stream, g, entries,
+                           c.dt_delta.as<double>());
+    }
     hipLaunchKernelGGL(k_dt_greedy, dim3(1), dim3(64), 0, c.stream, g);
     hipLaunchKernelGGL(k_dt_clear_where, dim3(nb256(ebound)), dim3(256), 0, c.stream, ebound,
                        state, er, where);
