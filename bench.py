@@ -206,7 +206,7 @@ def forest_bench(args):
         rows, best = e.autotune(repeats=2)
         tune = dict(rows[best], candidates=len(rows),
                     shapes=[[r["list_stack"], r["list_tile"], r["list_blocks"], round(r["ms"], 3)]
-                            for r in rows[48:]])
+                            for r in rows[-18:]])
         log(f"forest autotune: {tune}")
     step(e, 0.0)
     # the traversal's launch time and pairs: two eager steps with timing
@@ -555,7 +555,7 @@ def main():
         rows, best = e.autotune(repeats=2)
         tune = dict(rows[best], seconds=time.perf_counter() - a, candidates=len(rows),
                     shapes=[[r["list_stack"], r["list_tile"], r["list_blocks"], round(r["ms"], 3)]
-                            for r in rows[48:]])
+                            for r in rows[-18:]])
         log(f"autotune: {tune}")
     p_run = e.get_params()  # the tuned setting that the timed steps use
 

@@ -169,8 +169,8 @@ typedef struct ovg_params {
                                      Barnes-Hut evaluator)                     */
     double  mixed_near;           /* OVG_PREC_MIXED: segment (leaf) pairs closer
                                      than this (cm) stay FP64; 0 = none        */
-    int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 8, 16, 32 or
-                                     64 (0 = 16)                                */
+    int32_t points_per_wave;      /* OVG_TRAVERSE_LIST group size: 8, 16, 21,
+                                     32 or 64 (0 = 16)                          */
     int32_t group_level;          /* OVG_TRAVERSE_LIST: groups never span two
                                      octree cells of this level (1..21); 0 =
                                      the coarsest level whose cells are at most
@@ -576,7 +576,7 @@ int ovg_marker_elapsed(ovg_ctx *ctx, int id0, int id1, double *ms);
  * Time the list traversal (OVG_TRAVERSE_LIST) on the context's current
  * tangle (or forest), one warm-up and `repeats` timed launches each on a
  * tree built once, and keep the fastest in the context's parameters.  Stage
- * one: every combination of points_per_wave {8, 16, 32, 64}, leaf_size
+ * one: every combination of points_per_wave {8, 16, 21, 32, 64}, leaf_size
  * {1, 2, 4, 8} and group_level {L-1, L, L+1} (L: the automatic level) at
  * the context's launch shape.  Stage two, at the best of stage one: the
  * launch shape, list_stack {128, 256, 384} x list_tile {96, 128} x

@@ -504,8 +504,8 @@ static int validate_params(const ovg_params *p)
         p->traverse != OVG_TRAVERSE_LIST)
         return set_error(OVG_EINVAL, "unknown traversal %d", p->traverse);
     if (p->points_per_wave != 0 && p->points_per_wave != 8 && p->points_per_wave != 16 &&
-        p->points_per_wave != 32 && p->points_per_wave != 64)
-        return set_error(OVG_EINVAL, "points_per_wave must be 0, 8, 16, 32 or 64");
+        p->points_per_wave != 21 && p->points_per_wave != 32 && p->points_per_wave != 64)
+        return set_error(OVG_EINVAL, "points_per_wave must be 0, 8, 16, 21, 32 or 64");
     if (p->group_level < 0 || p->group_level > 21)
         return set_error(OVG_EINVAL, "group_level must be in [0, 21]");
     if (p->reuse_lists != 0 && p->reuse_lists != 1)
@@ -2246,7 +2246,7 @@ extern "C" int ovg_autotune(ovg_ctx *ctx, int repeats, ovg_tune *out)
         return OVG_OK;
     };
     // stage one: group size, bucket size and group level at the given shape
-    const int ppws[4] = {8, 16, 32, 64}, leaves[4] = {1, 2, 4, 8};
+    const int ppws[5] = {8, 16, 21, 32, 64}, leaves[4] = {1, 2, 4, 8};
     for (int pw : ppws)
         for (int lf : leaves)
             for (int dl = -1; dl <= 1; ++dl) {

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
     double *__restrict__ ix, double *__restrict__ iy, double *__restrict__ iz,
     unsigned long long *__restrict__ icount, int stack_cap, int far_tile)
 {
-    static_assert(64 % PPW == 0 && 4 * 64 / PPW <= kListFar, "slices of a full group");
+    static_assert(PPW <= 64 && 4 * (64 / PPW) <= kListFar, "slices of a full group");
     // kMirror: copies may mirror.  A mirrored copy of segment (x, q) is the
     // segment reflected through the walls, reversed when the copy mirrors an
     // odd number of axes (the reference's reflected target and reflected
@@ -742,6 +742,7 @@ int launch_bh_list(Ctx &c, const double *const x[3])
         switch (ppw) {
         case 8: OVG_TRY(build_groups<8>(c)); break;
         case 16: OVG_TRY(build_groups<16>(c)); break;
+        case 21: OVG_TRY(build_groups<21>(c)); break;
         case 32: OVG_TRY(build_groups<32>(c)); break;
         default: OVG_TRY(build_groups<64>(c)); break;
         }
@@ -792,6 +793,7 @@ int launch_bh_list(Ctx &c, const double *const x[3])
     switch (ppw) {
     case 8: mixed ? OVG_LIST_MODES(true, 8) : OVG_LIST_MODES(false, 8); break;
     case 16: mixed ? OVG_LIST_MODES(true, 16) : OVG_LIST_MODES(false, 16); break;
+    case 21: mixed ? OVG_LIST_MODES(true, 21) : OVG_LIST_MODES(false, 21); break;
     case 32: mixed ? OVG_LIST_MODES(true, 32) : OVG_LIST_MODES(false, 32); break;
     default: mixed ? OVG_LIST_MODES(true, 64) : OVG_LIST_MODES(false, 64); break;
     }
@@ -898,6 +900,7 @@ int launch_bh_list_groups(Ctx &c, const double *const x[3], const int2 *d_grp, c
     switch (ppw) {
     case 8: mixed ? OVG_LIST_WALK(true, 8) : OVG_LIST_WALK(false, 8); break;
     case 16: mixed ? OVG_LIST_WALK(true, 16) : OVG_LIST_WALK(false, 16); break;
+    case 21: mixed ? OVG_LIST_WALK(true, 21) : OVG_LIST_WALK(false, 21); break;
     case 32: mixed ? OVG_LIST_WALK(true, 32) : OVG_LIST_WALK(false, 32); break;
     default: mixed ? OVG_LIST_WALK(true, 64) : OVG_LIST_WALK(false, 64); break;
     }

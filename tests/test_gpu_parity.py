@@ -149,7 +149,7 @@ def test_c2_mixed_precision_error(c2):
 
 
 @pytest.mark.parametrize("ppw,leaf", [(8, 16), (16, 16), (32, 16), (64, 16), (16, 1), (8, 1),
-                                      (32, 4)])
+                                      (32, 4), (21, 1), (21, 16)])
 def test_c2_list_traversal(c2, ppw, leaf):
     g, t, p = c2
     say(f"c2 BH list traversal, {ppw} points per wave, leaf {leaf}: FP64 and mixed")
@@ -757,11 +757,11 @@ def test_autotune_picks_fastest_and_keeps_results_valid(c2):
     say("autotune of the list traversal on c2")
     with engine(t, q) as e:
         rows, best = e.autotune(repeats=1)
-        assert len(rows) == 48 + 18 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
+        assert len(rows) == 60 + 18 and all(r["ms"] > 0 and r["pairs"] > 0 for r in rows)
         assert rows[best]["ms"] == min(r["ms"] for r in rows)
         # stage two varies the launch shape only: same setting, same pairs
-        first = min(range(48), key=lambda i: rows[i]["ms"])
-        for r in rows[48:]:
+        first = min(range(60), key=lambda i: rows[i]["ms"])
+        for r in rows[60:]:
             assert all(r[f] == rows[first][f] for f in ("points_per_wave", "leaf_size",
                                                         "group_level"))
             assert r["pairs"] == rows[first]["pairs"]
