@@ -194,8 +194,12 @@ __global__ __launch_bounds__(64, 4) void k_bh_list(
         if (lane < 27) {
             const bool img = lane >= 1 && lane <= P.n_images;
             const int m = MIR && img ? P.refl[lane - 1] : 0;
+            // a zero made here: a constant hoisted out of the group loop
+            // would stay live across it and be spilled
+            double zero;
+            asm volatile("v_mov_b64 %0, 0" : "=v"(zero));
             for (int q = 0; q < 3; ++q)
-                shL[lane][q] = !img ? 0.0 : (m >> q & 1) ? P.W2[lane - 1][q] : P.Ls[lane - 1][q];
+                shL[lane][q] = !img ? zero : (m >> q & 1) ? P.W2[lane - 1][q] : P.Ls[lane - 1][q];
             if (MIR) shM[lane] = m;
         }
         __syncthreads();
