@@ -139,6 +139,7 @@ struct Ctx {
     int rec_groups = 0;
     // reconnection / smoothing scratch
     DBuf rc_cell, rc_sorted, rc_start, rc_pairs, rc_count, rc_tmp;
+    DBuf rc_room;                       // device remesh: its insertion bound (one u64)
     int rc_cap = 0;                 // int2 pairs that rc_pairs holds
     DBuf sm_buf;
 
@@ -289,9 +290,10 @@ int commit_topology(Ctx &c);
 // insert (a superset of the slots the sequential pass acts on first).
 int remesh_candidates(Ctx &c, double min_dist, double max_dist, std::vector<int> &out);
 // The same candidates queued on the device: the ascending slot list at
-// *d_list, its length at *d_count (no wait).
+// *d_list, its length at *d_count (no wait).  d_room, when given, gains a
+// bound on the points the pass can insert.
 int remesh_candidates_queue(Ctx &c, double min_dist, double max_dist, const int **d_list,
-                            const int **d_count);
+                            const int **d_count, unsigned long long *d_room = nullptr);
 
 // k_recon.hip
 int recon_candidates(Ctx &c, int max_pairs, int *h_pairs, int *n_pairs);

@@ -2353,13 +2353,32 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist)
     // room for the insertions: the reference doubles the slots when none is
     // EMPTY (get_next_empty_node, src/tangle.c:241-253); doubling earlier
     // leaves every allocation the same slot, since the scan finds the old
-    // slots first either way.  The live count here may be an over-estimate.
+    // slots first either way.  With fewer free slots than a quarter of the
+    // points (the live count here may be an over-estimate), the pass's
+    // insertion bound is read once, and the slots double only when it may
+    // not fit.
     const int free_slots = c.n - c.n_live;
-    if (free_slots < std::max(4096, c.n_live / 4)) {
-        OVG_TRY(sync_mirrors(c));
-        int grown = 2 * c.n;
-        while (grown - c.n_live < std::max(4096, c.n_live / 4)) grown *= 2;
-        OVG_TRY(grow_capacity(c, grown));
+    const int *cand = nullptr, *n_cand = nullptr;
+    if (c.n > 0 && free_slots < std::max(4096, c.n_live / 4)) {
+        OVG_TRY(positions_change(c));
+        OVG_TRY(c.rc_room.ensure(sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_dt_fill, dim3(1), dim3(64), 0, c.stream, 2, c.rc_room.as<int>(), 0);
+        OVG_TRY(remesh_candidates_queue(c, min_dist, max_dist, &cand, &n_cand,
+                                        c.rc_room.as<unsigned long long>()));
+        unsigned long long room = 0;
+        OVG_HIP(hipMemcpyAsync(&room, c.rc_room.p, sizeof(room), hipMemcpyDeviceToHost,
+                               c.stream));
+        OVG_HIP(hipStreamSynchronize(c.stream));
+        if ((unsigned long long)std::max(free_slots, 0) < room + 1024) {
+            OVG_TRY(sync_mirrors(c));
+            long long grown = 2 * (long long)c.n;
+            while (grown - c.n_live < (long long)room + 1024) grown *= 2;
+            if (grown > INT32_MAX / 2)
+                return set_error(OVG_ELIMIT, "device remesh: %llu possible insertions exceed "
+                                             "the slot limit", room);
+            OVG_TRY(grow_capacity(c, (int)grown));
+            cand = n_cand = nullptr;  // scanned again over the grown slots
+        }
     }
     const int n = c.n;
     hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_ADDED, 0);
@@ -2371,8 +2390,7 @@ int devtopo_remesh(Ctx &c, double min_dist, double max_dist)
         c.hint_on_device = true;
     }
     OVG_TRY(positions_change(c));
-    const int *cand = nullptr, *n_cand = nullptr;
-    OVG_TRY(remesh_candidates_queue(c, min_dist, max_dist, &cand, &n_cand));
+    if (!cand) OVG_TRY(remesh_candidates_queue(c, min_dist, max_dist, &cand, &n_cand));
     OVG_TRY(prem_queue(c, cand, n_cand, min_dist, max_dist));
     // the one-wavefront pass, for the passes the clusters hand back
     hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, state, DT_LO, n);

@@ -233,11 +233,13 @@ __global__ void k_remesh_flags(int n, const int *__restrict__ status, const int 
                                const int *__restrict__ rev, const double *__restrict__ X,
                                const double *__restrict__ Y, const double *__restrict__ Z,
                                const TangleParams *__restrict__ Pp, double min_dist,
-                               double max_dist, int *__restrict__ flag)
+                               double max_dist, int *__restrict__ flag,
+                               unsigned long long *__restrict__ room)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int f = 0;
+    unsigned long long need = 0;
     if (status[k] != OVG_EMPTY) {
         const TangleParams &P = *Pp;
         const int nx = fwd[k], pv = rev[k];
@@ -262,8 +264,14 @@ __global__ void k_remesh_flags(int n, const int *__restrict__ status, const int 
         // a non-finite length decides nothing in the reference either way;
         // let the host look at it
         f = rm || ad || !(lf == lf) || !(lr == lr);
+        // room: a bound on the points the pass can insert for this segment
+        // (each split leaves pieces at most 1/sqrt(2) of the chord, so a
+        // chain of splits of one segment is shorter than 2 lf / max_dist)
+        if (ad) need = lf < 1e6 * max_dist ? (unsigned long long)(2.0 * lf / max_dist) + 1 : 1ull << 20;
+        else if (f) need = 1;
     }
     flag[k] = f;
+    if (room && need) atomicAdd(room, need);
 }
 
 __global__ void k_compact(int n, const int *__restrict__ flag, const int *__restrict__ pos,
@@ -274,7 +282,7 @@ __global__ void k_compact(int n, const int *__restrict__ flag, const int *__rest
 }
 
 int remesh_candidates_queue(Ctx &c, double min_dist, double max_dist, const int **d_list,
-                            const int **d_count)
+                            const int **d_count, unsigned long long *d_room)
 {
     const int n = c.n;
     OVG_TRY(c.rc_tmp.ensure(sizeof(int) * (size_t)(n + 1)));
@@ -285,7 +293,7 @@ int remesh_candidates_queue(Ctx &c, double min_dist, double max_dist, const int 
     hipLaunchKernelGGL(k_remesh_flags, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
                        c.status.as<int>(), c.fwd.as<int>(), c.rev.as<int>(), c.x[0].as<double>(),
                        c.x[1].as<double>(), c.x[2].as<double>(), dparams(c), min_dist, max_dist,
-                       flag);
+                       flag, d_room);
     OVG_TRY(launch_exclusive_scan(c, flag, pos, n, pos + n, c.tree.scan_tmp));
     hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, flag, pos,
                        lst);
