@@ -256,15 +256,16 @@ __global__ __launch_bounds__(kSmBlock) void k_smooth(
 // All loops longer than kSmallLoop are transformed together: the three
 // coordinates of every loop are signals of their own and every launch covers
 // every signal, a block finding its loop through a host-built chunk map.
-// Stages of span >= kTile run as grid-wide radix-4 passes (two stages per
-// pass); the stages of span < kTile run as one pass over kTile-element tiles
+// Stages of span >= kTile run as grid-wide passes of up to four radix-2
+// stages each (16-element groups in registers); the stages of span < kTile run as one pass over kTile-element tiles
 // in LDS, which for the coordinate signals also takes the product with
 // DIF(b) and the first inverse stages.
 
 constexpr int kTileLog = 12;
 constexpr int kTile = 1 << kTileLog;
 constexpr int kChunk = 1024;       // points per block in the unwrap and the write-back
-constexpr int kPassBlock = 256;    // kTile/4 radix-4 butterflies per block, 4 per thread
+constexpr int kPassBlock = 256;    // threads per block of the grid-wide passes (one tile each)
+constexpr int kPassStages = 4;     // radix-2 stages per grid-wide pass (16-element groups)
 constexpr int kTileBlock = 512;
 // The convolution length is at least kTile (any M >= 2N-1 gives the same
 // cyclic convolution result), so every signal is whole tiles and a loop of up
@@ -531,83 +532,85 @@ __global__ __launch_bounds__(256) void k_bs_tables(const BsLoop *__restrict__ L,
 }
 
 // One grid-wide pass over the global stages (span >= kTile) of every signal:
-// stages 2*pass and 2*pass+1 as one radix-4 step, or the last stage alone
-// (radix 2) when a signal has an odd count.  Forward (DIF): stage s has span
-// M/2^(s+1); inverse (DIT): span kTile*2^s.  The twiddle of a span-h
-// butterfly at position p is e^{-+2 pi i p/(2h)} = tw[p << (logT-1-log2 h)].
+// stages 4*pass .. 4*pass+3, or the rest when a signal has fewer.  Forward
+// (DIF): stage s has span M/2^(s+1); inverse (DIT): span kTile*2^s.  The
+// twiddle of a span-h butterfly at position p is
+// e^{-+2 pi i p/(2h)} = tw[p << (logT-1-log2 h)].
 template <bool INV>
 __global__ __launch_bounds__(kPassBlock) void k_bs_pass(
     const BsLoop *__restrict__ L, const BsState *__restrict__ S, const int *__restrict__ tile_loop,
     double2 *__restrict__ arena, const double2 *__restrict__ tw, int logT, int sig0, int pass)
 {
+    // up to four radix-2 stages of span >= kTile per pass: the elements
+    // i0 + j hs (j < 2^ns) of a group stay in registers through them.
+    // Forward (decimation in frequency): spans h, h/2, ..., from the
+    // largest; inverse (decimation in time): spans hs, 2 hs, ..., from
+    // kTile.  Each stage is the radix-2 step of the LDS tiles, element by
+    // element, with the twiddle of the element's position in its span.
     const int b = tile_loop[blockIdx.x];
     const int logM = S[b].logM;
     if (!S[b].filt) return;
     const BsLoop B = L[b];
     const int tl = blockIdx.x - B.tile0;
     const int G = logM - kTileLog;
-    const int s0 = 2 * pass;
+    const int s0 = kPassStages * pass;
     if (s0 >= G || (tl << kTileLog) >= (1 << logM)) return;
+    const int ns = min(kPassStages, G - s0);
+    const int R = 1 << ns;
     double2 *a = bs_signal(arena, B, sig0 + blockIdx.y);
-    const int lh = INV ? kTileLog + s0 : logM - 1 - s0;  // log2 of the first stage's span
-    if (s0 + 1 < G) {
+    // log2 of the smallest span of the pass (the element stride)
+    const int ls = INV ? kTileLog + s0 : logM - s0 - ns;
+    const int hs = 1 << ls;
+    for (int k = 0; k < (kTile >> ns) / kPassBlock; ++k) {
+        const int q = (tl << (kTileLog - ns)) + k * kPassBlock + threadIdx.x;
+        const int p = q & (hs - 1);
+        const int i0 = ((q >> ls) << (ls + ns)) + p;
+        double2 e[1 << kPassStages];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int q = (tl << (kTileLog - 2)) + k * kPassBlock + threadIdx.x;
-            if (!INV) {
-                // spans h and h/2: elements i0 + {0, 1, 2, 3} h/2
-                const int hh = 1 << (lh - 1);
-                const int p = q & (hh - 1);
-                const int i0 = ((q >> (lh - 1)) << (lh + 1)) + p;
-                const double2 w1 = tw[p << (logT - 1 - lh)];
-                const double2 w2 = make_double2(w1.y, -w1.x);  // position p + h/2: w1 (-i)
-                const double2 w3 = tw[p << (logT - lh)];
-                const double2 a0 = a[i0], a1 = a[i0 + hh], a2 = a[i0 + 2 * hh], a3 = a[i0 + 3 * hh];
-                const double2 b0 = cadd(a0, a2), b2 = cmul(csub(a0, a2), w1);
-                const double2 b1 = cadd(a1, a3), b3 = cmul(csub(a1, a3), w2);
-                a[i0] = cadd(b0, b1);
-                a[i0 + hh] = cmul(csub(b0, b1), w3);
-                a[i0 + 2 * hh] = cadd(b2, b3);
-                a[i0 + 3 * hh] = cmul(csub(b2, b3), w3);
-            } else {
-                // spans h and 2h: elements i0 + {0, 1, 2, 3} h
-                const int h = 1 << lh;
-                const int p = q & (h - 1);
-                const int i0 = ((q >> lh) << (lh + 2)) + p;
-                const double2 W1 = conj2(tw[p << (logT - 1 - lh)]);
-                const double2 W2 = conj2(tw[p << (logT - 2 - lh)]);
-                const double2 W3 = make_double2(-W2.y, W2.x);  // position p + h of span 2h: W2 i
-                const double2 a0 = a[i0], a1 = a[i0 + h], a2 = a[i0 + 2 * h], a3 = a[i0 + 3 * h];
-                double2 t = cmul(W1, a1);
-                const double2 b0 = cadd(a0, t), b1 = csub(a0, t);
-                t = cmul(W1, a3);
-                const double2 b2 = cadd(a2, t), b3 = csub(a2, t);
-                t = cmul(W2, b2);
-                a[i0] = cadd(b0, t);
-                a[i0 + 2 * h] = csub(b0, t);
-                t = cmul(W3, b3);
-                a[i0 + h] = cadd(b1, t);
-                a[i0 + 3 * h] = csub(b1, t);
+        for (int j = 0; j < (1 << kPassStages); ++j)
+            if (j < R) e[j] = a[i0 + j * hs];
+        if (!INV) {
+            // stage t of the pass pairs elements 2^u apart, u = ns - 1 - t
+            // (u runs down, so the distances stay compile-time constants)
+#pragma unroll
+            for (int u = kPassStages - 1; u >= 0; --u) {
+                if (u < ns) {
+                    const int D = 1 << u;      // pair distance in the group
+                    const int lspan = ls + u;  // log2 of this stage's span
+#pragma unroll
+                    for (int j = 0; j < (1 << kPassStages); ++j) {
+                        if (j < R && !(j & D)) {
+                            const double2 w = tw[(p + (j & (D - 1)) * hs) << (logT - 1 - lspan)];
+                            const double2 u = e[j], v = e[j + D];
+                            e[j] = cadd(u, v);
+                            e[j + D] = cmul(csub(u, v), w);
+                        }
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < kPassStages; ++t) {
+                if (t < ns) {
+                    const int D = 1 << t;
+                    const int lspan = ls + t;
+#pragma unroll
+                    for (int j = 0; j < (1 << kPassStages); ++j) {
+                        if (j < R && !(j & D)) {
+                            const double2 w =
+                                conj2(tw[(p + (j & (D - 1)) * hs) << (logT - 1 - lspan)]);
+                            const double2 u = e[j];
+                            const double2 v = cmul(w, e[j + D]);
+                            e[j] = cadd(u, v);
+                            e[j + D] = csub(u, v);
+                        }
+                    }
+                }
             }
         }
-    } else {
-        const int h = 1 << lh;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int q = (tl << (kTileLog - 1)) + k * kPassBlock + threadIdx.x;
-            const int p = q & (h - 1);
-            const int i = ((q >> lh) << (lh + 1)) + p;
-            const double2 w = tw[p << (logT - 1 - lh)];
-            const double2 u = a[i], v = a[i + h];
-            if (!INV) {
-                a[i] = cadd(u, v);
-                a[i + h] = cmul(csub(u, v), w);
-            } else {
-                const double2 t = cmul(conj2(w), v);
-                a[i] = cadd(u, t);
-                a[i + h] = csub(u, t);
-            }
-        }
+        for (int j = 0; j < (1 << kPassStages); ++j)
+            if (j < R) a[i0 + j * hs] = e[j];
     }
 }
 
@@ -795,7 +798,7 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
     const int logT = c.sm_tw_log;
     const int *loff = c.loop_off.as<int>(), *lsl = c.loop_slots.as<int>();
     double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
-    const int passes = (logMall - kTileLog + 1) / 2;
+    const int passes = (logMall - kTileLog + kPassStages - 1) / kPassStages;
     const dim3 one((unsigned)nt, 1), three((unsigned)nt, 3);
 
     hipLaunchKernelGGL(k_bs_wraps, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dC, loff,
