@@ -45,6 +45,25 @@ def test_mix_equals_alone():
     assert rep["step_p99_ms"] >= rep["step_p50_ms"] > 0
 
 
+def test_device_topology_jobs_equal_host_topology_jobs():
+    """The same jobs with the topology passes on the device (queued, no
+    host wait inside a step) and on the host, concurrently: every job's
+    final positions, links and status are the same bit for bit."""
+    from openvort_amd import serve
+    jobs = small_jobs(n=6, steps=8)
+    res = {}
+    for mode in (ov.TOPO_HOST, ov.TOPO_DEVICE):
+        for j in jobs:
+            j.topology = mode
+        say(f"{len(jobs)} jobs on 3 workers, topology {mode}")
+        rep, res[mode] = serve.run(jobs, workers=3, max_active=6, realtime=False)
+        assert rep["jobs_done"] == len(jobs) and rep["jobs_failed"] == 0
+    for a, b in zip(res[ov.TOPO_HOST], res[ov.TOPO_DEVICE]):
+        assert a.steps_done == b.steps_done == jobs[0].steps
+        assert a.checksum == b.checksum and a.points_final == b.points_final
+        assert a.line_length == b.line_length
+
+
 def test_admission_respects_cap():
     from openvort_amd import serve
     jobs = small_jobs()
