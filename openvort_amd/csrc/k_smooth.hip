@@ -281,7 +281,7 @@ struct BsLoop {
     long long xs;     // first double of the unwrapped coordinates, 3 x n
     long long chirp;  // first double2 of c_j, j < N
     long long bhat;   // first double2 of b, then DIF(b): 2^logMmax
-    long long work;   // first double2 of the three coordinate signals, 3 x 2^logMmax
+    long long work;   // first double2 of the two coordinate signals (x + i y, z), 2 x 2^logMmax
 };
 
 struct BsState {
@@ -328,7 +328,7 @@ __device__ __forceinline__ double bs_coord(const double *x, int n, const BsState
     return j < n ? v : -v;
 }
 
-// signal sig of a loop: 0..2 the coordinates, 3 the b sequence
+// signal sig of a loop: 0 the packed x + i y, 1 z, 3 the b sequence
 __device__ __forceinline__ double2 *bs_signal(double2 *arena, const BsLoop &B, int sig)
 {
     return arena + (sig == 3 ? B.bhat : B.work + ((long long)sig << B.logMmax));
@@ -493,7 +493,7 @@ __global__ void k_bs_twiddles(int logT, double2 *__restrict__ tw)
 }
 
 // c_j (j < N), b_m (m < M) and the first convolution operands c_j u_j of the
-// three coordinates, zero padded to M; one tile per block
+// coordinates (x + i y, and z), zero padded to M; one tile per block
 __global__ __launch_bounds__(256) void k_bs_tables(const BsLoop *__restrict__ L,
                                                    const BsState *__restrict__ S,
                                                    const int *__restrict__ tile_loop,
@@ -519,15 +519,17 @@ __global__ __launch_bounds__(256) void k_bs_tables(const BsLoop *__restrict__ L,
             bv = chirp_at(M - m, N, 1.0);
         }
         bq[m] = bv;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double2 v = make_double2(0, 0);
-            if (m < N) {
-                const double u = bs_coord(xs + (size_t)c * n, n, s, c, m);
-                v = make_double2(u * cv.x, u * cv.y);
-            }
-            bs_signal(arena, B, c)[m] = v;
+        // x + i y in signal 0 and z in signal 1: the transforms are linear,
+        // so one complex transform carries two real coordinates
+        double2 v0 = make_double2(0, 0), v1 = make_double2(0, 0);
+        if (m < N) {
+            const double ux = bs_coord(xs, n, s, 0, m), uy = bs_coord(xs + n, n, s, 1, m);
+            const double uz = bs_coord(xs + 2 * (size_t)n, n, s, 2, m);
+            v0 = cmul(make_double2(ux, uy), cv);
+            v1 = make_double2(uz * cv.x, uz * cv.y);
         }
+        bs_signal(arena, B, 0)[m] = v0;
+        bs_signal(arena, B, 1)[m] = v1;
     }
 }
 
@@ -680,18 +682,40 @@ __global__ __launch_bounds__(256) void k_bs_mid(const BsLoop *__restrict__ L,
     double2 *a = bs_signal(arena, B, blockIdx.y);
     const double2 *chirp = arena + B.chirp;
     const double inv = 1.0 / M;
+    const int N = s.N;
     for (int m = m0 + threadIdx.x; m < m0 + kTile; m += blockDim.x) {
-        double2 v = make_double2(0, 0);
-        if (m < s.N) {
-            const double2 cv = chirp[m];
-            const double2 u = cmul(a[m], cv);
-            v = cmul(filter_conj(make_double2(u.x * inv, u.y * inv), m, s.N, s.kr, s.ki), cv);
+        if (m >= N) {
+            a[m] = make_double2(0, 0);
+            continue;
         }
-        a[m] = v;
+        const double2 cm = chirp[m];
+        if (blockIdx.y == 1) {  // z alone
+            const double2 u = cmul(a[m], cm);
+            a[m] = cmul(filter_conj(make_double2(u.x * inv, u.y * inv), m, N, s.kr, s.ki), cm);
+            continue;
+        }
+        // x + i y: Z_m = X_m + i Y_m with X, Y the spectra of real signals,
+        // X_m = (Z_m + conj Z_n) / 2 and Y_m = (Z_m - conj Z_n) / 2i for
+        // n = N - m.  The filter (real and imaginary half-complex parts kept
+        // separately) acts on X and Y; it maps a Hermitian spectrum to a
+        // Hermitian one, so X' + i Y' is again the transform of x' + i y'.
+        // The thread of the lower index of each pair (m, n) writes both.
+        const int n = m == 0 ? 0 : N - m;
+        if (n < m) continue;
+        const double2 cn = chirp[n];
+        const double2 zm = cmul(a[m], cm), zn = cmul(a[n], cn);
+        const double2 X = make_double2(0.5 * inv * (zm.x + zn.x), 0.5 * inv * (zm.y - zn.y));
+        const double2 Y = make_double2(0.5 * inv * (zm.y + zn.y), 0.5 * inv * (zn.x - zm.x));
+        const double2 Xf = conj2(filter_conj(X, m, N, s.kr, s.ki));
+        const double2 Yf = conj2(filter_conj(Y, m, N, s.kr, s.ki));
+        // Z'_m = X' + i Y', Z'_n = conj X' + i conj Y'; the second transform
+        // takes their conjugates times the chirp
+        a[m] = cmul(make_double2(Xf.x - Yf.y, -(Xf.y + Yf.x)), cm);
+        if (n != m) a[n] = cmul(make_double2(Xf.x + Yf.y, Xf.y - Yf.x), cn);
     }
 }
 
-// y_j = Re(c_j conv_j)/(M N), the detrend line added back, written along the
+// x_j, y_j, z_j from c_j conv_j / (M N), the detrend line added back, written along the
 // loop (src/vortex_smooth.c:286-309)
 __global__ __launch_bounds__(kChunk) void k_bs_finish(
     const BsLoop *__restrict__ L, const BsState *__restrict__ S, const int *__restrict__ chunk_loop,
@@ -708,18 +732,21 @@ __global__ __launch_bounds__(kChunk) void k_bs_finish(
     const double *xs = (const double *)arena + B.xs;
     const double inv = 1.0 / (1 << s.logM);
     double v[3];
+    if (s.filt) {
+        // the second transform of conj(X' + i Y') is N (x' - i y')
+        const double2 cj = arena[B.chirp + j];
+        const double2 u0 = cmul(bs_signal(arena, B, 0)[j], cj);
+        const double2 u1 = cmul(bs_signal(arena, B, 1)[j], cj);
+        v[0] = u0.x * inv / s.N;
+        v[1] = -u0.y * inv / s.N;
+        v[2] = u1.x * inv / s.N;
+    } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double y;
-        if (s.filt) {
-            const double2 u = cmul(bs_signal(arena, B, c)[j], arena[B.chirp + j]);
-            y = u.x * inv / s.N;
-        } else {
-            y = bs_coord(xs + (size_t)c * B.n, B.n, s, c, j);
-        }
-        if (s.span) y = y + s.coef[c] * j + s.x0[c];
-        v[c] = y;
+        for (int c = 0; c < 3; ++c) v[c] = bs_coord(xs + (size_t)c * B.n, B.n, s, c, j);
     }
+    if (s.span)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = v[c] + s.coef[c] * j + s.x0[c];
     const int m = loop_slots[loop_off[B.loop] + j];
     X[m] = v[0];
     Y[m] = v[1];
@@ -762,7 +789,7 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
         B.bhat = arena;
         arena = al16(arena + (1LL << logM));
         B.work = arena;
-        arena = al16(arena + (3LL << logM));
+        arena = al16(arena + (2LL << logM));
         logMall = std::max(logMall, logM);
     }
     if (tile_loop.size() > (size_t)INT32_MAX / 2)
@@ -799,7 +826,7 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
     const int *loff = c.loop_off.as<int>(), *lsl = c.loop_slots.as<int>();
     double *X = c.x[0].as<double>(), *Y = c.x[1].as<double>(), *Z = c.x[2].as<double>();
     const int passes = (logMall - kTileLog + kPassStages - 1) / kPassStages;
-    const dim3 one((unsigned)nt, 1), three((unsigned)nt, 3);
+    const dim3 one((unsigned)nt, 1), two((unsigned)nt, 2);
 
     hipLaunchKernelGGL(k_bs_wraps, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dC, loff,
                        lsl, X, Y, Z, dparams(c), csum);
@@ -816,15 +843,15 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
                        logT);
     for (int rep = 0; rep < 2; ++rep) {
         for (int q = 0; q < passes; ++q)
-            hipLaunchKernelGGL((k_bs_pass<false>), three, dim3(kPassBlock), 0, c.stream, dL, dS,
+            hipLaunchKernelGGL((k_bs_pass<false>), two, dim3(kPassBlock), 0, c.stream, dL, dS,
                                dT, ar, tw, logT, 0, q);
-        hipLaunchKernelGGL((k_bs_tile<true>), three, dim3(kTileBlock), 0, c.stream, dL, dS, dT,
+        hipLaunchKernelGGL((k_bs_tile<true>), two, dim3(kTileBlock), 0, c.stream, dL, dS, dT,
                            ar, tw, logT);
         for (int q = 0; q < passes; ++q)
-            hipLaunchKernelGGL((k_bs_pass<true>), three, dim3(kPassBlock), 0, c.stream, dL, dS,
+            hipLaunchKernelGGL((k_bs_pass<true>), two, dim3(kPassBlock), 0, c.stream, dL, dS,
                                dT, ar, tw, logT, 0, q);
         if (rep == 0)
-            hipLaunchKernelGGL(k_bs_mid, three, dim3(256), 0, c.stream, dL, dS, dT, ar);
+            hipLaunchKernelGGL(k_bs_mid, two, dim3(256), 0, c.stream, dL, dS, dT, ar);
     }
     hipLaunchKernelGGL(k_bs_finish, dim3((unsigned)nch), dim3(kChunk), 0, c.stream, dL, dS, dC,
                        loff, lsl, ar, X, Y, Z);
