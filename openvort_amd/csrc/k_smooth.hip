@@ -634,32 +634,59 @@ __global__ __launch_bounds__(kTileBlock) void k_bs_tile(
     __shared__ double2 sh[kTile];
     for (int i = threadIdx.x; i < kTile; i += kTileBlock) sh[i] = a[i];
     __syncthreads();
-    for (int lh = kTileLog - 1; lh >= 0; --lh) {
-        const int h = 1 << lh;
-        for (int q = threadIdx.x; q < kTile / 2; q += kTileBlock) {
-            const int p = q & (h - 1);
-            const int i = ((q >> lh) << (lh + 1)) + p;
-            const double2 w = tw[p << (logT - 1 - lh)];
-            const double2 u = sh[i], v = sh[i + h];
-            sh[i] = cadd(u, v);
-            sh[i + h] = cmul(csub(u, v), w);
+    // rounds of three radix-2 stages, one 8-element group per thread held in
+    // registers (element stride hs = the round's smallest span), the stage
+    // order and twiddles of the radix-2 sweeps: forward spans 2048 .. 1,
+    // inverse spans 1 .. 2048
+    static_assert(kTileLog % 3 == 0 && kTileBlock == kTile / 8, "8-element groups");
+    const int q = threadIdx.x;
+    for (int r = kTileLog / 3 - 1; r >= 0; --r) {
+        const int ls = 3 * r, hs = 1 << ls;
+        const int p = q & (hs - 1), i0 = ((q >> ls) << (ls + 3)) + p;
+        double2 e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = sh[i0 + j * hs];
+#pragma unroll
+        for (int u = 2; u >= 0; --u) {
+            const int D = 1 << u;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (!(j & D)) {
+                    const double2 w = tw[(p + (j & (D - 1)) * hs) << (logT - 1 - (ls + u))];
+                    const double2 x = e[j], y = e[j + D];
+                    e[j] = cadd(x, y);
+                    e[j + D] = cmul(csub(x, y), w);
+                }
         }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sh[i0 + j * hs] = e[j];
         __syncthreads();
     }
     if (CONV) {
         const double2 *bh = bs_signal(arena, B, 3) + ((size_t)tl << kTileLog);
         for (int i = threadIdx.x; i < kTile; i += kTileBlock) sh[i] = cmul(sh[i], bh[i]);
         __syncthreads();
-        for (int lh = 0; lh < kTileLog; ++lh) {
-            const int h = 1 << lh;
-            for (int q = threadIdx.x; q < kTile / 2; q += kTileBlock) {
-                const int p = q & (h - 1);
-                const int i = ((q >> lh) << (lh + 1)) + p;
-                const double2 t = cmul(conj2(tw[p << (logT - 1 - lh)]), sh[i + h]);
-                const double2 u = sh[i];
-                sh[i] = cadd(u, t);
-                sh[i + h] = csub(u, t);
+        for (int r = 0; r < kTileLog / 3; ++r) {
+            const int ls = 3 * r, hs = 1 << ls;
+            const int p = q & (hs - 1), i0 = ((q >> ls) << (ls + 3)) + p;
+            double2 e[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = sh[i0 + j * hs];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const int D = 1 << u;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (!(j & D)) {
+                        const double2 w =
+                            conj2(tw[(p + (j & (D - 1)) * hs) << (logT - 1 - (ls + u))]);
+                        const double2 x = e[j], y = cmul(w, e[j + D]);
+                        e[j] = cadd(x, y);
+                        e[j + D] = csub(x, y);
+                    }
             }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sh[i0 + j * hs] = e[j];
             __syncthreads();
         }
     }
