@@ -197,7 +197,6 @@ struct Ctx {
     bool hint_on_device = false;        // the device holds the allocator hint
     int dt_cap = 0;                     // candidate pairs a device pass takes
     int dt_seen = 0;                    // most candidate pairs a device pass met
-    int dt_ends_seen = 0;               // most endpoints a device pass met
     ovg_stats stats{};
     hipEvent_t ev[8] = {};
     // per-evaluation timing events, resolved when statistics are read so
@@ -228,11 +227,7 @@ void launch_check_finite(Ctx &c, const double *const x[3]);
 // k_direct.hip
 int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
                   double *const out_prim[3], double *const out_img[3], int chunks);
-// The same sum for the targets d_tgt_slots[0 .. *d_ntgt) counted on the
-// device, *d_ntgt <= n_bound; source chunks planned for about n_est targets.
-int launch_direct_counted(Ctx &c, const double *const x[3], int n_bound, int n_est,
-                          const int *d_tgt_slots, const int *d_ntgt, double *const out_prim[3],
-                          double *const out_img[3]);
+
 // The same sum with its chunk partials in the device's static scratch array
 // (no allocation): d_slots is checked into scratch first (a slot out of
 // range or EMPTY becomes -1 and sets bit 4 of the device flags); *checked

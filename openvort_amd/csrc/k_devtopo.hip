@@ -2141,7 +2141,6 @@ int sync_live(Ctx &c)
     }
     c.dt_seen = std::max(c.dt_seen, st[DT_FOUND]);
     c.prem_seen = std::max(c.prem_seen, st[DT_PREM_CAND]);
-    c.dt_ends_seen = std::max(c.dt_ends_seen, st[DT_ENDS]);
     if (st[DT_ERR]) {
         hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, c.stream, c.dt_state.as<int>(), DT_ERR,
                            0);

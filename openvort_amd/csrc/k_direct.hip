@@ -25,18 +25,12 @@ __global__ __launch_bounds__(kDirBlock) void k_direct(
     const double *__restrict__ s1z, const double *__restrict__ s2x,
     const double *__restrict__ s2y, const double *__restrict__ s2z,
     const int2 *__restrict__ ids, const TangleParams *__restrict__ Pp,
-    double *__restrict__ part, const int *__restrict__ d_ntgt)
+    double *__restrict__ part)
 {
     __shared__ double sh[6][kDirBlock];
     __shared__ int2 shid[kDirBlock];
     const TangleParams &P = *Pp;
     const int stride_tgt = n_tgt;  // the partials' row length
-    // d_ntgt: the targets counted on the device (n_tgt is then a bound)
-    if (d_ntgt) {
-        const int nt = *d_ntgt;
-        if ((int)blockIdx.x * kDirBlock >= nt) return;
-        n_tgt = min(n_tgt, nt);
-    }
     int i = blockIdx.x * kDirBlock + threadIdx.x;
     int k = -1;
     double rx = 0, ry = 0, rz = 0;
@@ -112,10 +106,10 @@ __global__ void k_direct_reduce(int n_tgt, int chunks, const int *__restrict__ t
                                 const double *__restrict__ part, double *__restrict__ px,
                                 double *__restrict__ py, double *__restrict__ pz,
                                 double *__restrict__ ix, double *__restrict__ iy,
-                                double *__restrict__ iz, const int *__restrict__ d_ntgt)
+                                double *__restrict__ iz)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_tgt || (d_ntgt && i >= *d_ntgt)) return;
+    if (i >= n_tgt) return;
     size_t stride = (size_t)n_tgt;
     double s[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < chunks; ++c) {
@@ -147,7 +141,7 @@ static void direct_plan(const Ctx &c, int n_tgt, int max_chunks, int &chunks, in
 
 static void direct_launch(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
                           double *const out_prim[3], double *const out_img[3], int chunks,
-                          int chunk_len, double *part, const int *d_ntgt = nullptr)
+                          int chunk_len, double *part)
 {
     const int tiles = (n_tgt + kDirBlock - 1) / kDirBlock;
     dim3 grid(tiles, chunks);
@@ -157,33 +151,17 @@ static void direct_launch(Ctx &c, const double *const x[3], int n_tgt, const int
                            d_tgt_slots, x[0], x[1], x[2], c.n_src, chunk_len,
                            c.seg[0].as<double>(), c.seg[1].as<double>(), c.seg[2].as<double>(),
                            c.seg[3].as<double>(), c.seg[4].as<double>(), c.seg[5].as<double>(),
-                           c.seg_ids.as<int2>(), dparams(c), part, d_ntgt);
+                           c.seg_ids.as<int2>(), dparams(c), part);
     else
         hipLaunchKernelGGL(k_direct<false>, grid, dim3(kDirBlock), 0, c.stream, n_tgt,
                            d_tgt_slots, x[0], x[1], x[2], c.n_src, chunk_len,
                            c.seg[0].as<double>(), c.seg[1].as<double>(), c.seg[2].as<double>(),
                            c.seg[3].as<double>(), c.seg[4].as<double>(), c.seg[5].as<double>(),
-                           c.seg_ids.as<int2>(), dparams(c), part, d_ntgt);
+                           c.seg_ids.as<int2>(), dparams(c), part);
     int bs = 256, gs = (n_tgt + bs - 1) / bs;
     hipLaunchKernelGGL(k_direct_reduce, dim3(gs), dim3(bs), 0, c.stream, n_tgt, chunks,
                        d_tgt_slots, part, out_prim[0], out_prim[1], out_prim[2], out_img[0],
-                       out_img[1], out_img[2], d_ntgt);
-}
-
-int launch_direct_counted(Ctx &c, const double *const x[3], int n_bound, int n_est,
-                          const int *d_tgt_slots, const int *d_ntgt, double *const out_prim[3],
-                          double *const out_img[3])
-{
-    if (n_bound <= 0) return OVG_OK;
-    // partials are laid out for n_bound targets: at most 256 MiB of them
-    const int max_chunks = (int)std::max<size_t>(1, ((size_t)256 << 20) / (48 * (size_t)n_bound));
-    int chunks, chunk_len;
-    direct_plan(c, n_est < 1 ? 1 : (n_est < n_bound ? n_est : n_bound), max_chunks, chunks,
-                chunk_len);
-    OVG_TRY(c.dpart.ensure(sizeof(double) * 6 * (size_t)n_bound * chunks));
-    direct_launch(c, x, n_bound, d_tgt_slots, out_prim, out_img, chunks, chunk_len,
-                  c.dpart.as<double>(), d_ntgt);
-    return launch_status();
+                       out_img[1], out_img[2]);
 }
 
 int launch_direct(Ctx &c, const double *const x[3], int n_tgt, const int *d_tgt_slots,
