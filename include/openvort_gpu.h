@@ -113,7 +113,11 @@ extern "C" {
  *   direct sum or the list traversal; any other tangle runs the host
  *   passes.  The point count and the counts the passes return are read
  *   back when a later call needs them (one wait per step in the reference
- *   driver's order), or at once when a count pointer is given.  A device
+ *   driver's order), or at once when a count pointer is given; with
+ *   direct-sum reconnection velocities the pass also waits once for its
+ *   pair and endpoint counts (to plan the direct sum), and a remesh on a
+ *   tangle with few free slots waits once for a bound on its insertions
+ *   (the slots double only when that may not fit).  A device
  *   pass that meets more than its capacity of candidate pairs (grown
  *   automatically from the counts seen, up to 32768 pairs and 65536
  *   endpoints) or finds no empty slot for an insertion stops and the next
