@@ -228,3 +228,26 @@ def test_devtopo_queued_counts_and_mirrors():
         assert d.reconnect(0.0) == h.reconnect(0.0)
         conn, st = h.download_connections()
         same_state(d, h.download()["vnodes"], conn, st, "host pass after device passes")
+
+
+@pytest.mark.parametrize("rec_velocity", [ov.RECVEL_TREE, ov.RECVEL_DIRECT])
+def test_devtopo_releases_its_buffers(rec_velocity):
+    """A context that ran the device passes (both endpoint-velocity modes)
+    gives back every device byte when it is destroyed: a serving loop
+    creates and destroys one per job."""
+    t, p = jittered(30, 5)
+    p.method = ov.METHOD_BH
+    p.precision = ov.PREC_MIXED
+    p.traverse = ov.TRAVERSE_LIST
+    p.rec_velocity = rec_velocity
+    before = ov.device_memory_in_use()
+    with engine(t, p, ov.TOPO_DEVICE) as e:
+        for step in range(2):
+            e.reconnect(0.0)
+            e.eliminate_small_loops(5)
+            e.smooth()
+            e.remesh(4e-4, 1.0e-3)
+            e.update_tangle(0.0)
+        assert ov.device_memory_in_use() > before
+    say(f"device bytes in use before {before}, after {ov.device_memory_in_use()}")
+    assert ov.device_memory_in_use() == before
