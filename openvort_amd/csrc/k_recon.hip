@@ -90,18 +90,32 @@ __global__ void k_cell_ranges(int n_live, const uint64_t *__restrict__ keys,
     if (i == n_live - 1 || (int)keys[i + 1] != c) end[c] = i + 1;
 }
 
+// The points in cell order: position and slot of sorted point s, the slot
+// -1 for a point that is not FREE (never a candidate).
+__global__ void k_cell_gather(int n_live, const int *__restrict__ sorted,
+                              const double *__restrict__ X, const double *__restrict__ Y,
+                              const double *__restrict__ Z, const int *__restrict__ status,
+                              double4 *__restrict__ sp, int *__restrict__ sslot)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_live) return;
+    const int l = sorted[s];
+    sp[s] = make_double4(X[l], Y[l], Z[l], 0.0);
+    sslot[s] = status[l] == OVG_FREE ? l : -1;
+}
+
 // Every pair (k, l > k) of the 27 neighbouring cells that passes the
 // reference predicate is appended at an atomic counter (order fixed later
 // by a sort); beyond cap only the count grows.  The distance test
 // dist = sqrt(e.e) > rec_dist is decided on e.e against rec_dist^2 with a
 // relative margin of 2^-40 (far wider than the roundings of e.e, rec_dist^2
 // and the square root) and evaluated exactly only inside that margin.
-// live: the targets, taken in the order given (the cell order)
-__global__ void k_candidates(int n_live, const int *__restrict__ live,
-                             const int *__restrict__ sorted, const double *__restrict__ X,
-                             const double *__restrict__ Y, const double *__restrict__ Z,
-                             const int *__restrict__ fwd, const int *__restrict__ rev,
-                             const int *__restrict__ status, const TangleParams *__restrict__ Pp,
+// Targets and neighbours are taken in cell order (sp, sslot: k_cell_gather),
+// so a wavefront's neighbourhoods overlap and each range is read
+// contiguously.
+__global__ void k_candidates(int n_live, const double4 *__restrict__ sp,
+                             const int *__restrict__ sslot, const int *__restrict__ fwd,
+                             const int *__restrict__ rev, const TangleParams *__restrict__ Pp,
                              Grid g, double rec_dist, const int *__restrict__ start,
                              const int *__restrict__ end, int *__restrict__ count,
                              int2 *__restrict__ out, int cap)
@@ -111,9 +125,10 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
     const TangleParams &P = *Pp;
     const double r2 = rec_dist * rec_dist;
     const double far2 = r2 * (1.0 + 0x1.0p-40), near2 = r2 * (1.0 - 0x1.0p-40);
-    int k = live[i];
-    if (status[k] != OVG_FREE) return;  // pinned ends never reconnect (:201-204)
-    double xk = X[k], yk = Y[k], zk = Z[k];
+    const int k = sslot[i];
+    if (k < 0) return;  // pinned ends never reconnect (:201-204)
+    const double4 pk = sp[i];
+    double xk = pk.x, yk = pk.y, zk = pk.z;
     int fk = fwd[k], rk = rev[k];
     int cx = cell_coord(xk, P.lo[0], g.cs[0], g.nc[0]);
     int cy = cell_coord(yk, P.lo[1], g.cs[1], g.nc[1]);
@@ -148,9 +163,10 @@ __global__ void k_candidates(int n_live, const int *__restrict__ live,
                     s1 = max(s1, end[row + x]);
                 }
                 for (int s = s0; s < s1; ++s) {
-                    int l = sorted[s];
-                    if (l <= k || l == fk || l == rk || status[l] != OVG_FREE) continue;
-                    double wx = X[l], wy = Y[l], wz = Z[l];
+                    const int l = sslot[s];  // -1 (not FREE) is below every k
+                    if (l <= k || l == fk || l == rk) continue;
+                    const double4 pl = sp[s];
+                    double wx = pl.x, wy = pl.y, wz = pl.z;
                     pwrap(P, xk, yk, zk, wx, wy, wz);
                     double ex = xk - wx, ey = yk - wy, ez = zk - wz;
                     const double d2 = dot3(ex, ey, ez, ex, ey, ez);
@@ -191,11 +207,14 @@ int candidates_queue(Ctx &c)
                        0);
     hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), start, end);
-    // targets in cell order too: a wavefront's neighbourhoods overlap
-    hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, T.vals[0].as<int>(),
-                       T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(), c.rev.as<int>(),
-                       c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
-                       c.rc_count.as<int>(), c.rc_pairs.as<int2>(), c.rc_cap);
+    OVG_TRY(c.rc_cell.ensure((sizeof(double4) + sizeof(int)) * (size_t)n + 64));
+    double4 *sp = c.rc_cell.as<double4>();
+    int *sslot = reinterpret_cast<int *>(sp + n);
+    hipLaunchKernelGGL(k_cell_gather, dim3(gs), dim3(bs), 0, c.stream, n, T.vals[0].as<int>(), X,
+                       Y, Z, c.status.as<int>(), sp, sslot);
+    hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, sp, sslot,
+                       c.fwd.as<int>(), c.rev.as<int>(), dparams(c), g, c.params.rec_dist, start,
+                       end, c.rc_count.as<int>(), c.rc_pairs.as<int2>(), c.rc_cap);
     return OVG_OK;
 }
 
@@ -232,15 +251,17 @@ static int candidates_device(Ctx &c, std::vector<int> &ck, std::vector<int> &cl)
     hipLaunchKernelGGL(k_cell_ranges, dim3(gs), dim3(bs), 0, c.stream, n,
                        T.keys[0].as<uint64_t>(), start, end);
     int total = 0;
+    OVG_TRY(c.rc_cell.ensure((sizeof(double4) + sizeof(int)) * (size_t)n + 64));
+    double4 *sp = c.rc_cell.as<double4>();
+    int *sslot = reinterpret_cast<int *>(sp + n);
+    hipLaunchKernelGGL(k_cell_gather, dim3(gs), dim3(bs), 0, c.stream, n, T.vals[0].as<int>(), X,
+                       Y, Z, c.status.as<int>(), sp, sslot);
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c.stream, 1, c.rc_count.as<int>(),
                            0);
-        hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n,
-                           T.vals[0].as<int>(), T.vals[0].as<int>(), X, Y, Z, c.fwd.as<int>(),
-                           c.rev.as<int>(),
-                           c.status.as<int>(), dparams(c), g, c.params.rec_dist, start, end,
-                           c.rc_count.as<int>(),
-                           c.rc_pairs.as<int2>(), c.rc_cap);
+        hipLaunchKernelGGL(k_candidates, dim3(gs), dim3(bs), 0, c.stream, n, sp, sslot,
+                           c.fwd.as<int>(), c.rev.as<int>(), dparams(c), g, c.params.rec_dist,
+                           start, end, c.rc_count.as<int>(), c.rc_pairs.as<int2>(), c.rc_cap);
         OVG_HIP(hipMemcpyAsync(&total, c.rc_count.p, sizeof(int), hipMemcpyDeviceToHost,
                                c.stream));
         OVG_HIP(hipStreamSynchronize(c.stream));
