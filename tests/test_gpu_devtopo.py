@@ -251,3 +251,24 @@ def test_devtopo_releases_its_buffers(rec_velocity):
         assert ov.device_memory_in_use() > before
     say(f"device bytes in use before {before}, after {ov.device_memory_in_use()}")
     assert ov.device_memory_in_use() == before
+
+
+def test_devtopo_remesh_chains_of_splits(oracle):
+    """Coarse rings (spacing 3e-3 cm) remeshed to max_dist 1e-3 cm with no
+    free slot: a segment takes a chain of insertions in one pass (a new point
+    whose slot lies ahead of the cursor is visited and split again), so the
+    clusters hand the pass to the sequential kernel.  The device reads its
+    insertion bound, grows its slots once, and places every point in the
+    slot the reference's doublings give."""
+    seed, _, D, rmin, rmax = configs.CONFIGS["c1"][:5]
+    t0 = ov.generate_rings(seed, 10, D, rmin, rmax, 3e-3)
+    t = ov.generate_rings(seed, 10, D, rmin, rmax, 3e-3, n_cap=len(t0.live))
+    _, p, _ = configs.make("c1", rings=10)
+    say(f"device remesh of {len(t.live)} coarse points with no free slot")
+    want, added, removed, _ = oracle.remesh(t, p, 0.3e-3, 1.0e-3)
+    with engine(t, p, ov.TOPO_DEVICE) as e:
+        a, r = e.remesh(0.3e-3, 1.0e-3)
+        say(f"  added {a} (oracle {added}), removed {r} (oracle {removed}), slots {e.n}")
+        assert (a, r) == (added, removed) and a > len(t.live)
+        assert e.n >= want.n
+        same_state(e, want.vnodes, want.conn, want.status, "chains of splits")
