@@ -460,6 +460,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
     for (auto e : c.ev_pool) hipEventDestroy(e);
     for (auto e : c.markers)
         if (e) hipEventDestroy(e);
+    if (c.sm_hmeta_done) hipEventDestroy(c.sm_hmeta_done);
     if (c.xpin_from) hipEventDestroy(c.xpin_from);
     if (c.xpin_done) hipEventDestroy(c.xpin_done);
     if (c.copy_stream) hipStreamDestroy(c.copy_stream);

@@ -178,6 +178,9 @@ struct Ctx {
     DBuf sm_gstate;                     // per big loop state, per chunk wrap sums
     DBuf sm_tw;                         // e^{-2 pi i k / 2^sm_tw_log}, k < 2^(sm_tw_log-1)
     int sm_tw_log = 0;
+    std::vector<char> sm_hmeta;         // host copy of the batched transform's tables,
+    hipEvent_t sm_hmeta_done = nullptr; // recorded after their upload: rewritten only
+                                        // once that upload has completed
     // Device-resident topology (ovg_params.topology == OVG_TOPO_DEVICE):
     // the reconnection, small-loop and remesh passes edit the device links,
     // status and positions in place and queue without waiting; the live

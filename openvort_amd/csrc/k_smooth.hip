@@ -783,8 +783,8 @@ __global__ __launch_bounds__(kChunk) void k_bs_finish(
 static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // Smooth the loops of the loop table listed in `big` (all longer than
-// kSmallLoop) with the batched transform.  Host vectors stay alive until
-// the caller's final stream synchronisation.
+// kSmallLoop) with the batched transform.  hmeta (the context's) holds the
+// uploaded tables until the next call.
 static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hmeta)
 {
     const int nb = (int)big.size();
@@ -836,11 +836,15 @@ static int smooth_big(Ctx &c, const std::vector<int> &big, std::vector<char> &hm
         hipLaunchKernelGGL(k_bs_twiddles, dim3(nblocks(1LL << (logMall - 1), 256)), dim3(256), 0,
                            c.stream, c.sm_tw_log, c.sm_tw.as<double2>());
     }
+    if (c.sm_hmeta_done) OVG_HIP(hipEventSynchronize(c.sm_hmeta_done));  // the last upload
     hmeta.assign(meta, 0);
     std::memcpy(hmeta.data(), L.data(), sizeof(BsLoop) * nb);
     std::memcpy(hmeta.data() + offC, chunk_loop.data(), sizeof(int) * nch);
     std::memcpy(hmeta.data() + offT, tile_loop.data(), sizeof(int) * nt);
     OVG_HIP(hipMemcpyAsync(c.sm_big.p, hmeta.data(), meta, hipMemcpyHostToDevice, c.stream));
+    if (!c.sm_hmeta_done)
+        OVG_HIP(hipEventCreateWithFlags(&c.sm_hmeta_done, hipEventDisableTiming));
+    OVG_HIP(hipEventRecord(c.sm_hmeta_done, c.stream));
 
     char *mb = c.sm_big.as<char>(), *sb = c.sm_gstate.as<char>();
     const BsLoop *dL = (const BsLoop *)mb;
@@ -1096,9 +1100,9 @@ int smooth_loops(Ctx &c)
                        c.x[0].as<double>(),
                        c.x[1].as<double>(), c.x[2].as<double>(), dparams(c),
                        c.params.freq_to_cutoff, c.sm_buf.as<double>());
-    std::vector<char> hmeta;
-    if (!big.empty()) OVG_TRY(smooth_big(c, big, hmeta));
-    OVG_HIP(hipStreamSynchronize(c.stream));  // hmeta is a host vector
+    // no wait here: the tables' host copy lives in the context, and the next
+    // call waits for its upload before rewriting it
+    if (!big.empty()) OVG_TRY(smooth_big(c, big, c.sm_hmeta));
     return OVG_OK;
 }
 
