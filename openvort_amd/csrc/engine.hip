@@ -432,7 +432,7 @@ extern "C" int ovg_destroy(ovg_ctx *ctx)
                    &c.loop_off, &c.loop_slots, &c.lt_buf, &c.sm_fft, &c.sm_big, &c.sm_tw,
                    &c.sm_gstate, &c.topo_stage, &c.xaos, &c.rc_room, &c.loop_open,
                    &c.dt_state, &c.dt_where, &c.dt_mark, &c.dt_pairs, &c.dt_ends, &c.dt_chg,
-                   &c.dt_cl, &c.dt_delta, &c.prem_buf};
+                   &c.dt_cl, &c.dt_delta, &c.prem_buf, &c.dt_grp};
     for (auto *b : all) b->release();
     for (int q = 0; q < 3; ++q) {
         c.x[q].release(); c.vels[q].release(); c.vs[q].release(); c.tan[q].release();

@@ -191,6 +191,7 @@ struct Ctx {
     DBuf dt_mark;                       // int per slot, 0 between passes
     DBuf dt_pairs, dt_ends, dt_chg;     // reconnection pass scratch
     DBuf dt_cl;                         // its clusters: union-find parents, flags
+    DBuf dt_grp;                        // endpoint target groups: flags, scans, cells
     DBuf dt_delta;                      // direct mode: field change of each change at each end
     DBuf prem_buf;                      // remesh by clusters: parents, owners, copies, table
     int prem_seen = 0;                  // most remesh candidates a device pass met
@@ -347,6 +348,8 @@ int radix_sort(Ctx &c, int n, int first_shift, int last_shift);
 
 // scan helper (k_tree.hip): exclusive scan of n int32 values, in -> out,
 // writes the total to *d_total when non-null.
+// Bits below the octree cell of `level` in a tuned tree key (one tangle).
+int tuned_key_cell_shift(int level);
 int launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
                           DBuf &tmp);
 

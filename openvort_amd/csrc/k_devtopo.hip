@@ -62,7 +62,6 @@ constexpr int kErrPairs = 1, kErrEnds = 2, kErrSlots = 4, kErrHeap = 8;
 constexpr int kMaxPairs = 32768, kMaxEnds = 65536;
 constexpr int kNear = 32;         // tree mode: rewirable pairs listed per endpoint
 constexpr int kHeapLds = 4096;    // remesh: top of the heap of touched slots, in LDS
-constexpr int kGroupSort = 8192;  // endpoints grouped by tree order in one LDS sort
 constexpr size_t kDeltaBytes = size_t(4) << 30;  // direct mode's field-change table
 
 template <class T>
@@ -1297,126 +1296,74 @@ __global__ void k_dt_near(int bound, int *__restrict__ state, EndRec *__restrict
     }
 }
 
-// Target groups of the endpoints for the list traversal, as the host path
-// (launch_bh_list_targets) forms them: endpoints in tree order, a new group
-// after points_per_wave targets or where two keys share fewer than the
-// group level's digits.  More endpoints than one LDS sort holds: one group
-// each.
-__global__ __launch_bounds__(1024) void k_dt_groups(
-    int bound, const int *__restrict__ state, const int *__restrict__ eslot,
-    const int *__restrict__ rank, const uint64_t *__restrict__ keys, int ppw, int L0,
-    int2 *__restrict__ grp, int *__restrict__ tgt, int *__restrict__ n_groups)
+// Target groups of the endpoints for any endpoint count, as the host path
+// (launch_bh_list_targets) forms them: the endpoints in tree order (the
+// sorted points flagged as endpoints, compacted by a scan), cut where two
+// consecutive endpoints lie in different octree cells of the group level
+// and after every points_per_wave endpoints of a cell's run.
+__global__ void k_dt_rank_flag(int n, const int *__restrict__ spts, const int *__restrict__ where,
+                               int *__restrict__ f)
 {
-    __shared__ uint64_t sk[kGroupSort];
-    const int m = min(state[DT_ENDS], bound);
-    const int t = threadIdx.x;
-    if (m > kGroupSort) {
-        for (int i = t; i < m; i += blockDim.x) {
-            tgt[i] = eslot[i];
-            grp[i] = make_int2(i, 1);
-        }
-        if (t == 0) *n_groups = m;
-        return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) f[r] = where[spts[r]] >= 0;
+}
+
+__global__ void k_dt_rank_compact(int n, const int *__restrict__ f, const int *__restrict__ pos,
+                                  const int *__restrict__ spts, const uint64_t *__restrict__ keys,
+                                  int shift, int *__restrict__ tgt, uint64_t *__restrict__ cid)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || !f[r]) return;
+    const int i = pos[r];
+    tgt[i] = spts[r];
+    // shift < 0: a group level beyond the keys' levels, every endpoint a
+    // cell of its own
+    cid[i] = shift < 0 ? (uint64_t)i : keys[r] >> shift;
+}
+
+// first index of the run of cell c in the ascending cid[0, m), and one past
+// its last
+__device__ __forceinline__ int run_lo(const uint64_t *cid, int m, uint64_t c)
+{
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cid[mid] < c) lo = mid + 1;
+        else hi = mid;
     }
-    int sz = 1;
-    while (sz < m) sz <<= 1;
-    for (int i = t; i < sz; i += blockDim.x) {
-        if (i < m) {
-            const int s = eslot[i];
-            sk[i] = ((uint64_t)(unsigned)rank[s] << 32) | (unsigned)s;
-        } else {
-            sk[i] = ~0ull;
-        }
+    return lo;
+}
+
+__device__ __forceinline__ int run_hi(const uint64_t *cid, int m, uint64_t c)
+{
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cid[mid] <= c) lo = mid + 1;
+        else hi = mid;
     }
-    __syncthreads();
-    for (int k = 2; k <= sz; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = t; i < sz; i += blockDim.x) {
-                const int p = i ^ j;
-                if (p > i) {
-                    const uint64_t a = sk[i], b = sk[p];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        sk[i] = b;
-                        sk[p] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    // A group starts where two neighbours in tree order share fewer than L0
-    // levels of one tangle's 48-bit Morton keys (3 bits a level), and every
-    // ppw targets after such a start; the sequential rule of the host path
-    // (a new group when the current one holds ppw) cuts at the same places.
-    // Each thread takes 8 consecutive targets (m <= kGroupSort = 8 * 1024).
-    __shared__ int tmax[1024], tcnt[1024];
-    const int i0 = 8 * t;
-    int seg = -1;  // last segment start at or before i
-    bool cut[8];
-    for (int q = 0; q < 8; ++q) {
-        const int i = i0 + q;
-        bool sp = false;
-        if (i < m) {
-            sp = i == 0;
-            if (!sp) {
-                const uint64_t a = keys[(int)(sk[i - 1] >> 32)], b = keys[(int)(sk[i] >> 32)];
-                const int lv = a == b ? 16 : (__clzll((long long)(a ^ b)) - 15) / 3;
-                sp = lv < L0;
-            }
-            tgt[i] = (int)(sk[i] & 0xffffffffu);
-        }
-        cut[q] = sp;
-        if (sp) seg = i;
-    }
-    // inclusive max-scan of segment starts over threads
-    tmax[t] = seg;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? tmax[t - off] : -1;
-        __syncthreads();
-        if (v > tmax[t]) tmax[t] = v;
-        __syncthreads();
-    }
-    int run = t > 0 ? tmax[t - 1] : -1;
-    int cnt = 0;
-    for (int q = 0; q < 8; ++q) {
-        const int i = i0 + q;
-        if (i >= m) {
-            cut[q] = false;
-            continue;
-        }
-        if (cut[q]) run = i;
-        cut[q] = (i - run) % ppw == 0;
-        cnt += cut[q];
-    }
-    // exclusive scan of group starts over threads
-    tcnt[t] = cnt;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? tcnt[t - off] : 0;
-        __syncthreads();
-        tcnt[t] += v;
-        __syncthreads();
-    }
-    int g = tcnt[t] - cnt;
-    for (int q = 0; q < 8; ++q) {
-        const int i = i0 + q;
-        if (i < m && cut[q]) {
-            int len = 1;
-            while (i + len < m && len < ppw) {
-                // the next group start ends this group
-                const int jn = i + len;
-                bool sp = false;
-                const uint64_t a = keys[(int)(sk[jn - 1] >> 32)], b = keys[(int)(sk[jn] >> 32)];
-                const int lv = a == b ? 16 : (__clzll((long long)(a ^ b)) - 15) / 3;
-                sp = lv < L0;
-                if (sp) break;
-                ++len;
-            }
-            grp[g++] = make_int2(i, len);
-        }
-    }
-    if (t == 1023) *n_groups = tcnt[1023];
+    return lo;
+}
+
+__global__ void k_dt_group_flag(int bound, const int *__restrict__ m_p,
+                                const uint64_t *__restrict__ cid, int ppw, int *__restrict__ gf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bound) return;
+    const int m = min(*m_p, bound);
+    gf[i] = i < m && (i - run_lo(cid, m, cid[i])) % ppw == 0;
+}
+
+__global__ void k_dt_group_emit(int bound, const int *__restrict__ m_p,
+                                const uint64_t *__restrict__ cid, const int *__restrict__ gf,
+                                const int *__restrict__ gpos, int ppw, int2 *__restrict__ grp,
+                                int *__restrict__ n_groups)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *n_groups = gpos[bound];
+    if (i >= bound || !gf[i]) return;
+    const int m = min(*m_p, bound);
+    grp[gpos[i]] = make_int2(i, min(ppw, run_hi(cid, m, cid[i]) - i));
 }
 
 struct GreedyArgs {
@@ -2504,10 +2451,27 @@ int devtopo_reconnect(Ctx &c, double t)
         OVG_TRY(T.sub_count.ensure(sizeof(int)));
         int2 *d_grp = T.sub_groups.as<int2>();
         int *d_tgt = reinterpret_cast<int *>(d_grp + ebound);
-        hipLaunchKernelGGL(k_dt_groups, dim3(1), dim3(1024), 0, c.stream, ebound, state, eslot,
-                           T.rank.as<int>(), T.keys[0].as<uint64_t>(),
-                           list_points_per_wave(c.params), list_group_level(c.params), d_grp,
-                           d_tgt, T.sub_count.as<int>());
+        // the endpoints in tree order and their target groups
+        const int nl = c.n_live;
+        const size_t s1 = ((size_t)nl + 64) & ~(size_t)63, s2 = ((size_t)ebound + 64) & ~(size_t)63;
+        OVG_TRY(c.dt_grp.ensure(sizeof(int) * (2 * s1 + 2 * s2) + sizeof(uint64_t) * s2));
+        int *gfl = c.dt_grp.as<int>(), *gpo = gfl + s1, *ef = gpo + s1, *epo = ef + s2;
+        uint64_t *cid = reinterpret_cast<uint64_t *>(epo + s2);
+        hipLaunchKernelGGL(k_dt_rank_flag, dim3(nb256(nl)), dim3(256), 0, c.stream, nl,
+                           T.sorted_pts.as<int>(), where, gfl);
+        OVG_TRY(launch_exclusive_scan(c, gfl, gpo, nl, gpo + nl, T.scan_tmp));
+        hipLaunchKernelGGL(k_dt_rank_compact, dim3(nb256(nl)), dim3(256), 0, c.stream, nl, gfl,
+                           gpo, T.sorted_pts.as<int>(), T.keys[0].as<uint64_t>(),
+                           list_group_level(c.params) > 16
+                               ? -1
+                               : tuned_key_cell_shift(list_group_level(c.params)),
+                           d_tgt, cid);
+        const int ppw = list_points_per_wave(c.params);
+        hipLaunchKernelGGL(k_dt_group_flag, dim3(nb256(ebound)), dim3(256), 0, c.stream, ebound,
+                           gpo + nl, cid, ppw, ef);
+        OVG_TRY(launch_exclusive_scan(c, ef, epo, ebound, epo + ebound, T.scan_tmp));
+        hipLaunchKernelGGL(k_dt_group_emit, dim3(nb256(ebound)), dim3(256), 0, c.stream, ebound,
+                           gpo + nl, cid, ef, epo, ppw, d_grp, T.sub_count.as<int>());
         OVG_TRY(launch_bh_list_groups(c, xv, d_grp, d_tgt, T.sub_count.as<int>(), ebound));
     } else {
         // update_velocity(tree = NULL): the FP64 direct sum (:250-251).

@@ -354,6 +354,13 @@ static void scan_level(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
 
 __global__ void k_zero_ints(int *a, int n);
 
+int tuned_key_cell_shift(int level)
+{
+    // two tuned keys share at least `level` levels (lcp_levels) exactly when
+    // they agree above this bit
+    return 64 - (63 - 3 * kTunedLevels) - 3 * level;
+}
+
 int launch_exclusive_scan(Ctx &c, const int *d_in, int *d_out, int n, int *d_total,
                           DBuf &tmp)
 {

@@ -255,3 +255,50 @@ def test_reconnect_tree_vs_direct_decisions_c2():
     assert abs(tot_t - tot_d) <= max(2, 0.1 * tot_d)
     # measured: 56 vs 57 reconnections, the same links after 9 of 10 passes
     assert same >= 8
+
+
+def test_c3_device_topology_steps_equal_host():
+    """Three steps of the benchmark's step on the full 1,386,017-point
+    tangle (reconnection with tree velocities, small loops, smoothing at
+    step 0, wrap, remesh, update_tangle, RK4, wrap), once with the topology
+    passes on the device and once on the host: the same counts, links,
+    status and positions bit for bit."""
+    t, p, dt = configs.make("c3", precision=ov.PREC_MIXED, traverse=ov.TRAVERSE_LIST,
+                            leaf_size=1)
+    p.points_per_wave = 21
+    p.group_level = 5
+    p.rec_velocity = ov.RECVEL_TREE
+    lo, hi = configs.remesh_bounds("c3")
+    out = {}
+    for mode in (ov.TOPO_HOST, ov.TOPO_DEVICE):
+        q = ov.Params.from_buffer_copy(p)
+        q.topology = mode
+        say(f"c3: 3 steps, topology {mode}")
+        with ov.Engine(q) as e:
+            e.upload(t)
+            tm, tot = 0.0, np.zeros(4, dtype=np.int64)
+            for step in range(3):
+                tot[0] += e.reconnect(tm)
+                tot[1] += e.eliminate_small_loops(5)
+                if step == 0:
+                    e.smooth()
+                e.enforce_periodic()
+                a, r = e.remesh(lo, hi)
+                tot[2] += a
+                tot[3] += r
+                e.eliminate_small_loops(5)
+                e.update_tangle(tm)
+                e.rk4_step(tm, dt)
+                e.enforce_periodic()
+                tm += dt
+            x = e.download()["vnodes"]
+            conn, st = e.download_connections()
+            out[mode] = (tot, x, conn, st, e.num_points())
+    (th, xh, ch, sh, nh), (td, xd, cd, sd, nd) = out[ov.TOPO_HOST], out[ov.TOPO_DEVICE]
+    say(f"c3 counts (rec, loops, added, removed) host {th} device {td}, points {nh} {nd}")
+    assert np.array_equal(th, td) and th[0] > 0 and th[2] > 0 and nh == nd
+    n = min(len(sh), len(sd))
+    assert np.array_equal(ch[:n], cd[:n]) and np.array_equal(sh[:n], sd[:n])
+    assert (sh[n:, 0] == -1).all() and (sd[n:, 0] == -1).all()
+    live = np.nonzero(sh[:n, 0] != -1)[0]
+    assert np.array_equal(xh[live], xd[live])
