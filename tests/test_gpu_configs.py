@@ -257,17 +257,18 @@ def test_reconnect_tree_vs_direct_decisions_c2():
     assert same >= 8
 
 
-def test_c3_device_topology_steps_equal_host():
+@pytest.mark.parametrize("rec_velocity", [ov.RECVEL_TREE, ov.RECVEL_DIRECT])
+def test_c3_device_topology_steps_equal_host(rec_velocity):
     """Three steps of the benchmark's step on the full 1,386,017-point
-    tangle (reconnection with tree velocities, small loops, smoothing at
-    step 0, wrap, remesh, update_tangle, RK4, wrap), once with the topology
-    passes on the device and once on the host: the same counts, links,
-    status and positions bit for bit."""
+    tangle (reconnection with tree or direct-sum velocities, small loops,
+    smoothing at step 0, wrap, remesh, update_tangle, RK4, wrap), once with
+    the topology passes on the device and once on the host: the same counts,
+    links, status and positions bit for bit."""
     t, p, dt = configs.make("c3", precision=ov.PREC_MIXED, traverse=ov.TRAVERSE_LIST,
                             leaf_size=1)
     p.points_per_wave = 21
     p.group_level = 5
-    p.rec_velocity = ov.RECVEL_TREE
+    p.rec_velocity = rec_velocity
     lo, hi = configs.remesh_bounds("c3")
     out = {}
     for mode in (ov.TOPO_HOST, ov.TOPO_DEVICE):
